@@ -1,0 +1,225 @@
+"""Benchmark of the fused array kernels on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--only NAME]
+
+Headline workload (one "step"): the fp64 dot product and the fused 8-op map
+chain over the same two vectors of 2^27 elements, uniform in [-1, 1), inputs
+resident in HBM.  value = algorithmic bytes of the step (dot: read 2 x 1 GiB;
+map chain: read 2 x 1 GiB + write 1 GiB) / wall time per step, in GB/s.
+The roofline object is the map-chain kernel's (the step's dominant kernel),
+timed with HIP events on the stream it runs on.  cpu_baseline is the CPU
+oracle (a C restatement of the emitted OpenMP C) on the same workload, on
+all host cores.  The other fused kernels are measured in the same run and
+reported under "kernels".
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "hydrangea-compiler_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import hydhip  # noqa: E402
+import workloads  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def timed_loop(fn, steps, warmup, events=None):
+    """Runs fn() warmup times, then `steps` times between device syncs.
+    Returns (ms_per_step, list of per-step event times in ms if events)."""
+    for _ in range(warmup):
+        fn(None)
+    hydhip.device_sync()
+    pairs = [(hydhip.Event(), hydhip.Event()) for _ in range(steps)] if events else None
+    t0 = time.perf_counter()
+    for k in range(steps):
+        fn(pairs[k] if pairs else None)
+    hydhip.device_sync()
+    t1 = time.perf_counter()
+    ev = [a.elapsed_ms(b) for a, b in pairs] if pairs else []
+    return (t1 - t0) * 1e3 / steps, ev
+
+
+def bench_dotmap(steps, warmup):
+    c = workloads.DOTMAP
+    n = c["n"]
+    xs = hydhip.splitmix_f64(n, c["seed"], 0, symmetric=True)
+    ys = hydhip.splitmix_f64(n, c["seed"], n, symmetric=True)
+    out = hydhip.DeviceArray((n,), np.float64)
+    dres = hydhip.DeviceArray((1,), np.float64)
+    hydhip.device_sync()
+
+    def step(ev):
+        hydhip.dot(xs, ys, out=dres)
+        if ev:
+            ev[0].record()
+        hydhip.map_chain(xs, ys, out=out)
+        if ev:
+            ev[1].record()
+
+    ms, ev = timed_loop(step, steps, warmup, events=True)
+    map_ms = float(np.mean(ev))
+
+    # dot alone, for its own line
+    def dot_only(evp):
+        if evp:
+            evp[0].record()
+        hydhip.dot(xs, ys, out=dres)
+        if evp:
+            evp[1].record()
+
+    _, dev_ev = timed_loop(dot_only, steps, 1, events=True)
+    dot_ms = float(np.mean(dev_ev))
+    dot_bytes = 2 * 8 * n
+    map_bytes = 3 * 8 * n
+    res = dict(
+        n=n, ms_per_step=ms, step_bytes=dot_bytes + map_bytes,
+        map_chain=dict(ms=map_ms, bytes=map_bytes, gbs=map_bytes / map_ms / 1e6),
+        dot=dict(ms=dot_ms, bytes=dot_bytes, gbs=dot_bytes / dot_ms / 1e6),
+        result=float(dres.numpy()[0]),
+    )
+    for a in (xs, ys, out):
+        a.free()
+    return res
+
+
+def bench_hist(steps, warmup):
+    c = workloads.HIST
+    d = hydhip.splitmix_i64(c["n"], c["seed"], bits=c["bits"])
+    out = hydhip.DeviceArray((c["bins"],), np.int64)
+    res = {}
+    for name, strat in (("auto", 0), ("lds", 1), ("global", 2)):
+        def step(ev, strat=strat):
+            if ev:
+                ev[0].record()
+            hydhip.weighted_histogram(d, c["bins"], strat, out=out)
+            if ev:
+                ev[1].record()
+
+        ms, ev = timed_loop(step, steps, warmup, events=True)
+        kms = float(np.mean(ev))
+        b = 8 * c["n"] + 8 * c["bins"]
+        res[name] = dict(ms_per_call=ms, kernel_ms=kms, bytes=b, gbs=b / kms / 1e6)
+    d.free()
+    return res
+
+
+def bench_jacobi(steps, warmup):
+    c = workloads.JACOBI
+    h, w, iters = c["h"], c["w"], c["iters"]
+    u = hydhip.splitmix_f64(h * w, c["seed"])
+    u.shape = (h, w)
+    b0 = hydhip.DeviceArray((h, w), np.float64)
+    b1 = hydhip.DeviceArray((h, w), np.float64)
+
+    def run(ev):
+        if ev:
+            ev[0].record()
+        hydhip.iterate_stencil2d(u, iters, buf0=b0, buf1=b1, use_graph=True)
+        if ev:
+            ev[1].record()
+
+    ms, ev = timed_loop(run, max(1, steps // 4), 1, events=True)
+    gms = float(np.mean(ev))
+    step_bytes = 2 * 8 * h * w
+    res = dict(iters=iters, graph_ms=gms, ms_per_iterate=ms, per_step_us=gms * 1e3 / iters,
+               bytes_per_step=step_bytes, gbs=step_bytes * iters / gms / 1e6)
+    for a in (u, b0, b1):
+        a.free()
+    return res
+
+
+def cpu_baseline_dotmap():
+    import pyoracle
+
+    c = workloads.DOTMAP
+    n = c["n"]
+    xs = pyoracle.splitmix_f64(n, c["seed"], 0, symmetric=True)
+    ys = pyoracle.splitmix_f64(n, c["seed"], n, symmetric=True)
+    pyoracle.map_chain(xs[:1 << 20], ys[:1 << 20])
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        pyoracle.dot(xs, ys)
+        pyoracle.map_chain(xs, ys)
+    dt = (time.perf_counter() - t0) / reps
+    return dict(value=5 * 8 * n / dt / 1e9, unit="GB/s", cores=pyoracle.num_threads(), kind="port",
+                sample="full workload (2^27 elements), dot + map chain, %d repetitions, %.3f s each" % (reps, dt))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", default="")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the fused kernels run on a single MI355X; "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    hydhip.init()
+    info = hydhip.device_info()
+    log("device", info)
+    kernels = {}
+    only = set(filter(None, args.only.split(",")))
+
+    log("dot + map chain ...")
+    dm = bench_dotmap(args.steps, args.warmup)
+    kernels["dot"] = dm["dot"]
+    kernels["map_chain"] = dm["map_chain"]
+    if not only or "hist" in only:
+        log("histogram ...")
+        kernels["weighted_histogram"] = bench_hist(args.steps, args.warmup)
+    if not only or "jacobi" in only:
+        log("jacobi ...")
+        kernels["jacobi_heat"] = bench_jacobi(args.steps, args.warmup)
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        cpu = cpu_baseline_dotmap()
+
+    value = dm["step_bytes"] / dm["ms_per_step"] / 1e6
+    mc = dm["map_chain"]
+    line = {
+        "metric": "HBM GB/s & % roofline per fused kernel; batched req/s, p50/p99 latency",
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dm["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "fp64 dot + fused 8-op map chain over xs, ys [2^27] uniform [-1,1)",
+                   "n": dm["n"], "step_bytes": dm["step_bytes"], "device": info["arch"],
+                   "cus": info["num_cus"]},
+        "roofline": {"bound": "hbm", "kernel": "map_chain_kernel", "achieved": round(mc["gbs"], 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(mc["gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": None},
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

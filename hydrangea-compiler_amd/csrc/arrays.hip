@@ -1,0 +1,304 @@
+// arrays.hip - the hyd_array_t runtime (runtime/hydrangea_runtime.h:220-456
+// of the reference) with an HBM mirror per storage block.
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <mutex>
+#include <unordered_set>
+
+#include "../../include/hyd_hip_runtime.h"
+#include "hyd_internal.hpp"
+
+extern "C" void hydhip_forget_graphs_for(const void* ptr);
+
+namespace hyd {
+namespace {
+
+constexpr uint64_t kMagic = 0x4859444849505354ULL;
+
+// The storage block handed out through hyd_array_t::storage.  The reference
+// layout {data, refcount} comes first, so code that only knows the reference
+// struct reads it unchanged.
+struct HipStorage {
+    hyd_array_storage_t base;
+    uint64_t magic = kMagic;
+    size_t bytes = 0;
+    void* dev = nullptr;           // HBM copy, allocated on first use
+    bool device_resident = false;  // HBM copy is authoritative
+    bool host_valid = true;        // host bytes match the HBM copy
+};
+
+std::mutex g_mu;
+std::unordered_set<hyd_array_storage_t*> g_storages;
+int g_host_sync = 1;
+void* g_export_stream = nullptr;
+
+[[noreturn]] void die(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    fprintf(stderr, "hydrangea: ");
+    vfprintf(stderr, fmt, ap);
+    fprintf(stderr, "\n");
+    va_end(ap);
+    exit(1);
+}
+
+HipStorage* storage_of(hyd_array_t* arr) {
+    if (!arr || !arr->storage) die("null array");
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (!g_storages.count(arr->storage)) die("array storage %p was not allocated by libhydhip", (void*)arr->storage);
+    }
+    return reinterpret_cast<HipStorage*>(arr->storage);
+}
+
+hyd_array_t* make_header(HipStorage* st, hyd_tuple_t shape, size_t elem_size) {
+    hyd_array_t* arr = (hyd_array_t*)malloc(sizeof(hyd_array_t));
+    if (!arr) die("header allocation failed");
+    arr->storage = &st->base;
+    arr->data = st->base.data;
+    arr->elem_size = elem_size;
+    arr->shape = shape;
+    return arr;
+}
+
+HipStorage* new_storage(size_t bytes) {
+    HipStorage* st = new HipStorage();
+    // calloc: zero-filled like the reference (hydrangea_runtime.h:304); large
+    // blocks come from fresh zero pages, so an unused host copy costs nothing.
+    st->base.data = (unsigned char*)calloc(bytes ? bytes : 1, 1);
+    if (!st->base.data) die("host allocation of %zu bytes failed", bytes);
+    st->base.refcount = 1;
+    st->bytes = bytes;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_storages.insert(&st->base);
+    return st;
+}
+
+int ensure_dev(HipStorage* st) {
+    if (st->dev || st->bytes == 0) return HYD_OK;
+    HYD_TRY(ensure_init());
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, st->bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(HYD_E_OOM, "hipMalloc(%zu) failed: %s", st->bytes, hipGetErrorString(e));
+    }
+    st->dev = p;
+    return HYD_OK;
+}
+
+}  // namespace
+
+// Shared with exports.hip.
+hipStream_t export_stream() { return resolve_stream(g_export_stream); }
+int host_sync_enabled() { return g_host_sync; }
+
+}  // namespace hyd
+
+using namespace hyd;
+
+extern "C" {
+
+hyd_tuple_t hyd_tuple_make(int n, ...) {
+    hyd_tuple_t t;
+    memset(&t, 0, sizeof t);
+    t.ndims = n;
+    va_list args;
+    va_start(args, n);
+    for (int i = 0; i < n && i < HYD_MAX_DIMS; i++) t.elems[i] = va_arg(args, int64_t);
+    va_end(args);
+    return t;
+}
+
+int64_t hyd_shape_size(hyd_tuple_t shape) {
+    int64_t size = 1;
+    for (int i = 0; i < shape.ndims; i++) size *= shape.elems[i];
+    return size;
+}
+
+hyd_tuple_t hyd_shape_init(hyd_tuple_t shape) {
+    hyd_tuple_t out;
+    memset(&out, 0, sizeof out);
+    if (shape.ndims <= 0) return out;
+    out.ndims = shape.ndims - 1;
+    for (int i = 0; i < out.ndims; i++) out.elems[i] = shape.elems[i];
+    return out;
+}
+
+int64_t hyd_shape_last(hyd_tuple_t shape) {
+    return shape.ndims <= 0 ? 0 : shape.elems[shape.ndims - 1];
+}
+
+hyd_tuple_t hyd_flat_to_nd(int64_t flat, hyd_tuple_t shape) {
+    hyd_tuple_t idx;
+    memset(&idx, 0, sizeof idx);
+    idx.ndims = shape.ndims;
+    int64_t rem = flat;
+    for (int i = shape.ndims - 1; i >= 0; i--) {
+        idx.elems[i] = rem % shape.elems[i];
+        rem /= shape.elems[i];
+    }
+    return idx;
+}
+
+int64_t hyd_nd_to_flat(hyd_tuple_t nd, hyd_tuple_t shape) {
+    int64_t flat = 0, stride = 1;
+    for (int i = shape.ndims - 1; i >= 0; i--) {
+        flat += nd.elems[i] * stride;
+        stride *= shape.elems[i];
+    }
+    return flat;
+}
+
+hyd_array_t* hyd_array_alloc_bytes(hyd_tuple_t shape, size_t elem_size) {
+    int64_t size = hyd_shape_size(shape);
+    if (size < 0) die("negative array size %lld", (long long)size);
+    HipStorage* st = new_storage((size_t)size * elem_size);
+    return make_header(st, shape, elem_size);
+}
+
+hyd_array_t* hyd_array_alloc(hyd_tuple_t shape) { return hyd_array_alloc_bytes(shape, sizeof(int64_t)); }
+
+hyd_array_t* hyd_array_alloc_copy(const hyd_array_t* src) {
+    hyd_array_t* s = const_cast<hyd_array_t*>(src);
+    HipStorage* ss = storage_of(s);
+    hyd_array_t* dst = hyd_array_alloc_bytes(src->shape, src->elem_size);
+    size_t bytes = (size_t)hyd_shape_size(src->shape) * src->elem_size;
+    if (ss->device_resident && !ss->host_valid) {
+        // copy HBM to HBM; the new array is device-resident as well
+        HipStorage* ds = storage_of(dst);
+        if (ensure_dev(ds) != HYD_OK) die("%s", hydhip_last_error());
+        size_t off = (size_t)(src->data - ss->base.data);
+        hipStream_t st = export_stream();
+        if (bytes && hipMemcpyAsync(ds->dev, (unsigned char*)ss->dev + off, bytes,
+                                    hipMemcpyDeviceToDevice, st) != hipSuccess)
+            die("device copy failed");
+        ds->device_resident = true;
+        ds->host_valid = false;
+        if (g_host_sync) hyd_array_sync_host(dst);
+    } else if (bytes) {
+        memcpy(dst->data, src->data, bytes);
+    }
+    return dst;
+}
+
+hyd_array_t* hyd_array_reshape_view(hyd_array_t* arr, hyd_tuple_t shape) {
+    int64_t src_size = hyd_shape_size(arr->shape);
+    int64_t dst_size = hyd_shape_size(shape);
+    if (src_size != dst_size) {
+        fprintf(stderr, "hydrangea: reshape view size mismatch (%lld elements to %lld)\n",
+                (long long)src_size, (long long)dst_size);
+        exit(1);
+    }
+    HipStorage* st = storage_of(arr);
+    st->base.refcount += 1;
+    hyd_array_t* v = make_header(st, shape, arr->elem_size);
+    v->data = arr->data;
+    return v;
+}
+
+void hyd_array_free(hyd_array_t* arr) {
+    if (arr == NULL) return;
+    HipStorage* st = storage_of(arr);
+    st->base.refcount -= 1;
+    if (st->base.refcount == 0) {
+        {
+            std::lock_guard<std::mutex> lk(g_mu);
+            g_storages.erase(&st->base);
+        }
+        if (st->dev) {
+            hydhip_forget_graphs_for(st->dev);
+            (void)hipStreamSynchronize(export_stream());
+            (void)hipFree(st->dev);
+        }
+        free(st->base.data);
+        delete st;
+    }
+    free(arr);
+}
+
+int64_t hyd_array_length(hyd_array_t* arr) { return hyd_shape_size(arr->shape); }
+int64_t hyd_array_ndims(hyd_array_t* arr) { return arr->shape.ndims; }
+int64_t hyd_array_dim(hyd_array_t* arr, int dim) { return arr->shape.elems[dim]; }
+int64_t* hyd_array_data_int64(hyd_array_t* arr) { return (int64_t*)(void*)arr->data; }
+double* hyd_array_data_f64(hyd_array_t* arr) { return (double*)(void*)arr->data; }
+
+int64_t hyd_get_env_int(const char* name) {
+    char* val = getenv(name);
+    if (!val) {
+        fprintf(stderr, "get_env_int: environment variable '%s' not set\n", name);
+        exit(1);
+    }
+    char* end;
+    long result = strtol(val, &end, 10);
+    if (*end != '\0' || val == end) {
+        fprintf(stderr, "get_env_int: cannot parse '%s' as int\n", val);
+        exit(1);
+    }
+    return result;
+}
+
+/* ---- residency ---------------------------------------------------------- */
+
+hyd_array_t* hyd_array_alloc_device(hyd_tuple_t shape, size_t elem_size) {
+    hyd_array_t* arr = hyd_array_alloc_bytes(shape, elem_size);
+    HipStorage* st = storage_of(arr);
+    if (ensure_dev(st) != HYD_OK) die("%s", hydhip_last_error());
+    if (st->bytes && fill_zero(st->dev, st->bytes, export_stream()) != HYD_OK)
+        die("%s", hydhip_last_error());
+    st->device_resident = true;
+    st->host_valid = true;  // both copies are zero
+    return arr;
+}
+
+void* hyd_array_device_data(hyd_array_t* arr) {
+    HipStorage* st = storage_of(arr);
+    if (st->bytes == 0) return nullptr;
+    if (ensure_dev(st) != HYD_OK) die("%s", hydhip_last_error());
+    if (!st->device_resident) {
+        // host copy authoritative: upload (ordered before the kernels that
+        // follow on the export stream)
+        hipStream_t s = export_stream();
+        if (hipMemcpyAsync(st->dev, st->base.data, st->bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            die("upload of %zu bytes failed", st->bytes);
+    }
+    return (unsigned char*)st->dev + (arr->data - st->base.data);
+}
+
+int hyd_array_sync_host(hyd_array_t* arr) {
+    HipStorage* st = storage_of(arr);
+    if (!st->device_resident || st->host_valid || st->bytes == 0) return HYD_OK;
+    hipStream_t s = export_stream();
+    HYD_TRY_HIP(hipMemcpyAsync(st->base.data, st->dev, st->bytes, hipMemcpyDeviceToHost, s));
+    HYD_TRY_HIP(hipStreamSynchronize(s));
+    st->host_valid = true;
+    return HYD_OK;
+}
+
+int hyd_array_mark_host_dirty(hyd_array_t* arr) {
+    HipStorage* st = storage_of(arr);
+    st->device_resident = false;
+    st->host_valid = true;
+    return HYD_OK;
+}
+
+int hyd_array_is_device_resident(hyd_array_t* arr) { return storage_of(arr)->device_resident ? 1 : 0; }
+
+void hyd_hip_set_host_sync(int on) { g_host_sync = on ? 1 : 0; }
+int hyd_hip_get_host_sync(void) { return g_host_sync; }
+void hyd_hip_set_stream(void* stream) { g_export_stream = stream; }
+
+}  // extern "C"
+
+namespace hyd {
+// A kernel just wrote the HBM copy of `arr`.
+void mark_device_written(hyd_array_t* arr) {
+    HipStorage* st = storage_of(arr);
+    st->device_resident = true;
+    st->host_valid = false;
+}
+}  // namespace hyd

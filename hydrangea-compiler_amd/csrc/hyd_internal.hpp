@@ -1,0 +1,76 @@
+// hyd_internal.hpp - shared host/device helpers of libhydhip.so (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include "../../include/hyd_hip.h"
+
+namespace hyd {
+
+// Records a formatted message for hydhip_last_error() and returns `code`.
+int set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+void clear_error();
+
+// Makes sure the runtime is initialised (device selected, library stream
+// created).  Returns HYD_OK or an error code.
+int ensure_init();
+
+// NULL -> the library's stream.
+hipStream_t resolve_stream(void* stream);
+
+struct DeviceInfo {
+    int device = 0;
+    int num_cus = 0;
+    size_t lds_per_block = 0;  // bytes usable by one workgroup
+    size_t hbm_bytes = 0;
+    char name[128] = {0};
+};
+const DeviceInfo& device_info();
+
+// Scratch memory tied to a stream: a device buffer of at least `bytes`,
+// reused by later calls on the same stream (kernels on one stream run in
+// order, so the buffer is free again when the next call's kernels start).
+// Growing it frees the old buffer after a device synchronisation.
+void* stream_workspace(hipStream_t stream, size_t bytes);
+
+// A small device buffer for scalar results of the hyd_export_* wrappers
+// (which, like the reference's, are not reentrant).
+void* scalar_slot();
+
+// Zero-fill kernel launch (graph-capture safe, unlike hipMemsetAsync).
+int fill_zero(void* dptr, size_t bytes, hipStream_t stream);
+
+}  // namespace hyd
+
+#define HYD_TRY_HIP(expr)                                                          \
+    do {                                                                           \
+        hipError_t hyd_e_ = (expr);                                                \
+        if (hyd_e_ != hipSuccess)                                                  \
+            return hyd::set_error(HYD_E_HIP, "%s failed: %s", #expr,               \
+                                  hipGetErrorString(hyd_e_));                      \
+    } while (0)
+
+#define HYD_TRY(expr)                   \
+    do {                                \
+        int hyd_rc_ = (expr);           \
+        if (hyd_rc_ != HYD_OK) return hyd_rc_; \
+    } while (0)
+
+// After a kernel launch: report launch-configuration errors.
+#define HYD_CHECK_LAUNCH() HYD_TRY_HIP(hipGetLastError())
+
+// ---- device helpers ------------------------------------------------------
+namespace hyd {
+
+__host__ __device__ inline uint64_t splitmix_at(uint64_t seed, uint64_t counter) {
+    uint64_t z = seed + (counter + 1) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace hyd

@@ -1,0 +1,234 @@
+// reduce_map.hip - fused fp64 reduction (zipwith (*.) -> reduce (+.)) and the
+// fused elementwise map chain.
+//
+// Reference shapes: reduce_generate lowering, src/Language/Hydrangea/
+// Lowering.hs:762-836 and the step `acc = acc + elem` (Lowering.hs:3517-3535),
+// emitted as `parallel for simd reduction(+:acc)` (CodegenC.hs:1306-1413);
+// map/zipwith chains as one three-address statement per op
+// (CodegenC.hs:2293-2400, operators :2469-2525, `if` as SIf :978-1001).
+//
+// CDNA4 mapping: grid-stride streaming with 16-byte loads, each thread keeps
+// four independent partial sums (fixed assignment -> deterministic), then a
+// wave64 xor-shuffle tree, an LDS tree across the workgroup's waves, one
+// partial per workgroup and a single-workgroup fixed tree over the partials.
+// Products and sums stay separate IEEE operations (-ffp-contract=off), the
+// same per-element semantics as the C; only the summation order differs,
+// which the 1e-12 relative bound covers.
+#include "hyd_exp.h"
+#include "hyd_internal.hpp"
+
+namespace hyd {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Sum over the workgroup in a fixed order; result valid in thread 0.
+template <int BLOCK>
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double part[BLOCK / 64];
+    v = wave_sum(v);
+    const int wave = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0) part[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        // pairwise over the (few) waves
+        double a[BLOCK / 64];
+#pragma unroll
+        for (int i = 0; i < BLOCK / 64; ++i) a[i] = part[i];
+#pragma unroll
+        for (int w = 1; w < BLOCK / 64; w <<= 1)
+#pragma unroll
+            for (int i = 0; i + w < BLOCK / 64; i += 2 * w) a[i] = a[i] + a[i + w];
+        s = a[0];
+    }
+    return s;
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(kBlock) void dot_partial_kernel(const double* __restrict__ xs,
+                                                             const double* __restrict__ ys,
+                                                             int64_t n, double* __restrict__ partial) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    const int64_t step = (int64_t)gridDim.x * kBlock;
+    int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (VEC) {
+        const double2* x2 = reinterpret_cast<const double2*>(xs);
+        const double2* y2 = reinterpret_cast<const double2*>(ys);
+        const int64_t npairs = n >> 1;
+        for (; p + step < npairs; p += 2 * step) {
+            double2 xa = x2[p], ya = y2[p];
+            double2 xb = x2[p + step], yb = y2[p + step];
+            double m0 = xa.x * ya.x, m1 = xa.y * ya.y;
+            double m2 = xb.x * yb.x, m3 = xb.y * yb.y;
+            a0 = a0 + m0;
+            a1 = a1 + m1;
+            a2 = a2 + m2;
+            a3 = a3 + m3;
+        }
+        for (; p < npairs; p += step) {
+            double2 xa = x2[p], ya = y2[p];
+            double m0 = xa.x * ya.x, m1 = xa.y * ya.y;
+            a0 = a0 + m0;
+            a1 = a1 + m1;
+        }
+        if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+            double m = xs[n - 1] * ys[n - 1];
+            a2 = a2 + m;
+        }
+    } else {
+        for (; p < n; p += step) {
+            double m = xs[p] * ys[p];
+            a0 = a0 + m;
+        }
+    }
+    double t = (a0 + a1) + (a2 + a3);
+    double s = block_sum<kBlock>(t);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+constexpr int kFinalBlock = 1024;
+
+__global__ __launch_bounds__(kFinalBlock) void dot_final_kernel(const double* __restrict__ partial,
+                                                                int nparts, double init,
+                                                                double* __restrict__ out) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += kFinalBlock) v = v + partial[i];
+    double s = block_sum<kFinalBlock>(v);
+    if (threadIdx.x == 0) *out = init + s;
+}
+
+// The fused chain of the map benchmark program (one statement per op, the
+// order of the source expression).
+__device__ __forceinline__ double map_chain_op(double x, double y) {
+    double t1 = 2.0 * x;
+    double t2 = t1 + y;
+    double t3 = __builtin_fabs(t2);
+    double t4 = __builtin_sqrt(t3);
+    double t5 = t4 * x;
+    double t6 = t5 - y;
+    double t7;
+    if (t6 > 0.0)
+        t7 = t6;
+    else
+        t7 = 0.0;
+    double t8 = t7 * 1.0e-3;
+    return hyd_exp(t8);
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(kBlock) void map_chain_kernel(const double* __restrict__ xs,
+                                                           const double* __restrict__ ys,
+                                                           int64_t n, double* __restrict__ out) {
+    const int64_t step = (int64_t)gridDim.x * kBlock;
+    int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (VEC) {
+        const double2* x2 = reinterpret_cast<const double2*>(xs);
+        const double2* y2 = reinterpret_cast<const double2*>(ys);
+        double2* o2 = reinterpret_cast<double2*>(out);
+        const int64_t npairs = n >> 1;
+        for (; p < npairs; p += step) {
+            double2 x = x2[p], y = y2[p];
+            double2 r;
+            r.x = map_chain_op(x.x, y.x);
+            r.y = map_chain_op(x.y, y.y);
+            __builtin_nontemporal_store(r.x, &o2[p].x);
+            __builtin_nontemporal_store(r.y, &o2[p].y);
+        }
+        if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) out[n - 1] = map_chain_op(xs[n - 1], ys[n - 1]);
+    } else {
+        for (; p < n; p += step) out[p] = map_chain_op(xs[p], ys[p]);
+    }
+}
+
+inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
+// Elementwise exp / sqrt of the map chain's math, for the parity tests of
+// the math functions on their own.
+__global__ void exp_kernel(const double* __restrict__ x, int64_t n, double* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = hyd_exp(x[i]);
+}
+
+__global__ void sqrt_kernel(const double* __restrict__ x, int64_t n, double* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = __builtin_sqrt(x[i]);
+}
+
+}  // namespace
+}  // namespace hyd
+
+using namespace hyd;
+
+// out[i] = exp(x[i]) (op 0) or sqrt(x[i]) (op 1) on the GPU.
+extern "C" int hydhip_math_f64(int op, const double* x, int64_t n, double* out, void* stream) {
+    HYD_TRY(ensure_init());
+    if (n < 0 || (n > 0 && (!x || !out)) || op < 0 || op > 1)
+        return set_error(HYD_E_INVALID, "hydhip_math_f64: bad arguments");
+    if (n == 0) return HYD_OK;
+    int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 8192));
+    if (op == 0)
+        exp_kernel<<<(unsigned)blocks, 256, 0, resolve_stream(stream)>>>(x, n, out);
+    else
+        sqrt_kernel<<<(unsigned)blocks, 256, 0, resolve_stream(stream)>>>(x, n, out);
+    HYD_CHECK_LAUNCH();
+    return HYD_OK;
+}
+
+// The same exp compiled for the host CPU (same IEEE operation sequence).
+extern "C" void hydhip_host_exp(const double* x, int64_t n, double* out) {
+    for (int64_t i = 0; i < n; ++i) out[i] = hyd_exp(x[i]);
+}
+
+using namespace hyd;
+
+extern "C" int hydhip_dot_f64(const double* xs, const double* ys, int64_t n, double init,
+                              double* out, void* stream) {
+    HYD_TRY(ensure_init());
+    if (n < 0 || !out || (n > 0 && (!xs || !ys)))
+        return set_error(HYD_E_INVALID, "hydhip_dot_f64: bad arguments");
+    hipStream_t s = resolve_stream(stream);
+    const DeviceInfo& di = device_info();
+    // Fixed partial count for a given n and device: deterministic results.
+    int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, (int64_t)kBlock * 8),
+                                                            (int64_t)di.num_cus * 8));
+    double* partial = static_cast<double*>(stream_workspace(s, (size_t)blocks * sizeof(double)));
+    if (!partial) return set_error(HYD_E_OOM, "hydhip_dot_f64: workspace allocation failed");
+    bool vec = aligned16(xs) && aligned16(ys);
+    if (vec)
+        dot_partial_kernel<true><<<(unsigned)blocks, kBlock, 0, s>>>(xs, ys, n, partial);
+    else
+        dot_partial_kernel<false><<<(unsigned)blocks, kBlock, 0, s>>>(xs, ys, n, partial);
+    HYD_CHECK_LAUNCH();
+    dot_final_kernel<<<1, kFinalBlock, 0, s>>>(partial, (int)blocks, init, out);
+    HYD_CHECK_LAUNCH();
+    return HYD_OK;
+}
+
+extern "C" int hydhip_map_chain_f64(const double* xs, const double* ys, int64_t n, double* out,
+                                    void* stream) {
+    HYD_TRY(ensure_init());
+    if (n < 0 || (n > 0 && (!xs || !ys || !out)))
+        return set_error(HYD_E_INVALID, "hydhip_map_chain_f64: bad arguments");
+    if (n == 0) return HYD_OK;
+    hipStream_t s = resolve_stream(stream);
+    const DeviceInfo& di = device_info();
+    bool vec = aligned16(xs) && aligned16(ys) && aligned16(out);
+    int64_t work = vec ? (n >> 1) : n;
+    int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(work, kBlock),
+                                                            (int64_t)di.num_cus * 16));
+    if (vec)
+        map_chain_kernel<true><<<(unsigned)blocks, kBlock, 0, s>>>(xs, ys, n, out);
+    else
+        map_chain_kernel<false><<<(unsigned)blocks, kBlock, 0, s>>>(xs, ys, n, out);
+    HYD_CHECK_LAUNCH();
+    return HYD_OK;
+}

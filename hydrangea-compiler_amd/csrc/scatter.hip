@@ -1,0 +1,282 @@
+// scatter.hip - scatter (+) into int64 bins: the weighted histogram and the
+// unfused scatter-add.
+//
+// Reference shapes: the privatized scatter of
+// src/Language/Hydrangea/CodegenC.hs:1119-1288 (per-thread private grids,
+// then a per-bin merge) and the atomic scatter of CodegenC.hs:1084-1118,
+// chosen by Parallelize.hs:466-503 / :350-419.  The combine is
+// `new = val + old` (Lowering.hs:3553-3557); integer + is associative and
+// commutative mod 2^64, so any merge order gives the emitted C's bits.
+//
+// CDNA4 mapping:
+//  * LDS:       each workgroup keeps `copies` private copies of the bins in LDS
+//               (copy = lane % copies, odd stride so copies of one bin sit in
+//               different banks), adds with 64-bit LDS atomics, then merges
+//               its copies into the global bins with one atomic per bin.
+//  * GLOBAL:    one 64-bit global atomic per element (bins too large for LDS,
+//               or too few elements to pay for a private copy).
+//  * LDS_SPLIT: bins split into LDS-sized ranges; grid.y picks the range and
+//               every workgroup streams all elements, keeping only its range.
+#include <stdio.h>
+
+#include <algorithm>
+
+#include "hyd_internal.hpp"
+
+namespace hyd {
+namespace {
+
+// route = x % bins (C truncating remainder, CodegenC.hs:2473-2474),
+// value = x * x computed mod 2^64 (the C oracle is built with -fwrapv).
+struct ModSquare {
+    int64_t bins;
+    __device__ __forceinline__ void operator()(int64_t x, int64_t& b, uint64_t& v) const {
+        b = x % bins;
+        v = (uint64_t)x * (uint64_t)x;
+    }
+};
+
+// Fused source: one int64 stream produces both route and value.
+struct FusedSrc {
+    const int64_t* __restrict__ data;
+    ModSquare fn;
+    __device__ __forceinline__ void pair(int64_t p, int64_t& b0, uint64_t& v0, int64_t& b1,
+                                         uint64_t& v1) const {
+        longlong2 x = reinterpret_cast<const longlong2*>(data)[p];
+        fn(x.x, b0, v0);
+        fn(x.y, b1, v1);
+    }
+    __device__ __forceinline__ void one(int64_t i, int64_t& b, uint64_t& v) const {
+        fn(data[i], b, v);
+    }
+};
+
+// Unfused source: separate route and value arrays.
+struct PairSrc {
+    const int64_t* __restrict__ idx;
+    const int64_t* __restrict__ vals;
+    __device__ __forceinline__ void pair(int64_t p, int64_t& b0, uint64_t& v0, int64_t& b1,
+                                         uint64_t& v1) const {
+        longlong2 i2 = reinterpret_cast<const longlong2*>(idx)[p];
+        longlong2 v2 = reinterpret_cast<const longlong2*>(vals)[p];
+        b0 = i2.x;
+        b1 = i2.y;
+        v0 = (uint64_t)v2.x;
+        v1 = (uint64_t)v2.y;
+    }
+    __device__ __forceinline__ void one(int64_t i, int64_t& b, uint64_t& v) const {
+        b = idx[i];
+        v = (uint64_t)vals[i];
+    }
+};
+
+__device__ __forceinline__ void lds_add(unsigned long long* p, uint64_t v) {
+    __hip_atomic_fetch_add(p, (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ void global_add(unsigned long long* p, uint64_t v) {
+    __hip_atomic_fetch_add(p, (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Streams the elements assigned to this workgroup (grid-stride over 16-byte
+// pairs, 4 pairs in flight per thread) and calls add(bin, value).
+template <int BLOCK, bool VEC, class Src, class Add>
+__device__ __forceinline__ void stream_elements(const Src& src, int64_t n, Add add) {
+    const int64_t step = (int64_t)gridDim.x * BLOCK;
+    int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (VEC) {
+        const int64_t npairs = n >> 1;
+        for (; p + 3 * step < npairs; p += 4 * step) {
+            int64_t b[8];
+            uint64_t v[8];
+            src.pair(p, b[0], v[0], b[1], v[1]);
+            src.pair(p + step, b[2], v[2], b[3], v[3]);
+            src.pair(p + 2 * step, b[4], v[4], b[5], v[5]);
+            src.pair(p + 3 * step, b[6], v[6], b[7], v[7]);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) add(b[k], v[k]);
+        }
+        for (; p < npairs; p += step) {
+            int64_t b0, b1;
+            uint64_t v0, v1;
+            src.pair(p, b0, v0, b1, v1);
+            add(b0, v0);
+            add(b1, v1);
+        }
+        if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+            int64_t b;
+            uint64_t v;
+            src.one(n - 1, b, v);
+            add(b, v);
+        }
+    } else {
+        for (; p < n; p += step) {
+            int64_t b;
+            uint64_t v;
+            src.one(p, b, v);
+            add(b, v);
+        }
+    }
+}
+
+template <int BLOCK, bool VEC, class Src>
+__global__ __launch_bounds__(BLOCK) void scatter_lds_kernel(Src src, int64_t n, int nbins,
+                                                            int copies, int stride,
+                                                            unsigned long long* __restrict__ dst) {
+    extern __shared__ unsigned long long lds[];
+    const int total = copies * stride;
+    for (int i = threadIdx.x; i < total; i += BLOCK) lds[i] = 0;
+    __syncthreads();
+    unsigned long long* mine = lds + (threadIdx.x % copies) * stride;
+    const uint64_t ub = (uint64_t)nbins;
+    stream_elements<BLOCK, VEC>(src, n, [&](int64_t b, uint64_t v) {
+        if ((uint64_t)b < ub) lds_add(&mine[b], v);
+    });
+    __syncthreads();
+    for (int b = threadIdx.x; b < nbins; b += BLOCK) {
+        unsigned long long s = 0;
+        for (int c = 0; c < copies; ++c) s += lds[c * stride + b];
+        if (s) global_add(&dst[b], s);
+    }
+}
+
+template <int BLOCK, bool VEC, class Src>
+__global__ __launch_bounds__(BLOCK) void scatter_split_kernel(Src src, int64_t n, int64_t nbins,
+                                                              int range,
+                                                              unsigned long long* __restrict__ dst) {
+    extern __shared__ unsigned long long lds[];
+    const int64_t lo = (int64_t)blockIdx.y * range;
+    const int64_t hi = min(lo + (int64_t)range, nbins);
+    const int width = (int)(hi - lo);
+    for (int i = threadIdx.x; i < width; i += BLOCK) lds[i] = 0;
+    __syncthreads();
+    stream_elements<BLOCK, VEC>(src, n, [&](int64_t b, uint64_t v) {
+        if ((uint64_t)(b - lo) < (uint64_t)width) lds_add(&lds[b - lo], v);
+    });
+    __syncthreads();
+    for (int b = threadIdx.x; b < width; b += BLOCK) {
+        unsigned long long s = lds[b];
+        if (s) global_add(&dst[lo + b], s);
+    }
+}
+
+template <int BLOCK, bool VEC, class Src>
+__global__ __launch_bounds__(BLOCK) void scatter_global_kernel(Src src, int64_t n, int64_t nbins,
+                                                               unsigned long long* __restrict__ dst) {
+    const uint64_t ub = (uint64_t)nbins;
+    stream_elements<BLOCK, VEC>(src, n, [&](int64_t b, uint64_t v) {
+        if ((uint64_t)b < ub) global_add(&dst[b], v);
+    });
+}
+
+constexpr int kBlock = 512;
+constexpr size_t kLdsTarget = 64 * 1024;  // per workgroup: leaves room for 2 per CU
+
+template <class Src>
+int launch_scatter(const Src& src, bool vec, int64_t n, int64_t nbins, int64_t* dst_i64,
+                   int strategy, hipStream_t s) {
+    const DeviceInfo& di = device_info();
+    unsigned long long* dst = reinterpret_cast<unsigned long long*>(dst_i64);
+    const size_t lds_cap = std::min(di.lds_per_block, (size_t)160 * 1024);
+    const int64_t per_block_elems = (int64_t)kBlock * 8;
+    int64_t want_blocks = std::max<int64_t>(1, ceil_div(n, per_block_elems));
+
+    if (strategy == HYD_SCATTER_AUTO) {
+        // Size rule: LDS bins when a private copy fits and there are >= 16
+        // elements per bin to pay for zeroing and merging it, the split form
+        // for large bin counts with many elements, else global atomics.
+        int64_t stride = nbins | 1;
+        bool fits = (size_t)stride * 8 <= lds_cap;
+        if (fits && n >= 16 * nbins)
+            strategy = HYD_SCATTER_LDS;
+        else if (!fits && n >= 8 * nbins)
+            strategy = HYD_SCATTER_LDS_SPLIT;
+        else
+            strategy = HYD_SCATTER_GLOBAL;
+    }
+
+    if (strategy == HYD_SCATTER_LDS) {
+        int stride = (int)(nbins | 1);
+        if ((size_t)stride * 8 > lds_cap)
+            return set_error(HYD_E_INVALID, "scatter: %lld bins do not fit LDS (%zu bytes)",
+                             (long long)nbins, lds_cap);
+        int copies = (int)std::max<size_t>(1, std::min<size_t>(16, kLdsTarget / ((size_t)stride * 8)));
+        size_t lds = (size_t)copies * stride * 8;
+        int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / lds));
+        int64_t blocks = std::min<int64_t>(want_blocks, (int64_t)di.num_cus * per_cu);
+        if (vec) {
+            auto k = scatter_lds_kernel<kBlock, true, Src>;
+            HYD_TRY_HIP(hipFuncSetAttribute((const void*)k,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            k<<<(unsigned)blocks, kBlock, lds, s>>>(src, n, (int)nbins, copies, stride, dst);
+        } else {
+            auto k = scatter_lds_kernel<kBlock, false, Src>;
+            HYD_TRY_HIP(hipFuncSetAttribute((const void*)k,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            k<<<(unsigned)blocks, kBlock, lds, s>>>(src, n, (int)nbins, copies, stride, dst);
+        }
+    } else if (strategy == HYD_SCATTER_LDS_SPLIT) {
+        int range = (int)std::min<int64_t>(nbins, (int64_t)(lds_cap / 8));
+        int64_t nranges = ceil_div(nbins, range);
+        if (nranges > 65535) return set_error(HYD_E_INVALID, "scatter: too many bin ranges");
+        size_t lds = (size_t)range * 8;
+        int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / lds));
+        int64_t bx = std::max<int64_t>(1, std::min<int64_t>(want_blocks,
+                                                            (int64_t)di.num_cus * per_cu / nranges));
+        dim3 grid((unsigned)bx, (unsigned)nranges);
+        if (vec) {
+            auto k = scatter_split_kernel<kBlock, true, Src>;
+            HYD_TRY_HIP(hipFuncSetAttribute((const void*)k,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            k<<<grid, kBlock, lds, s>>>(src, n, nbins, range, dst);
+        } else {
+            auto k = scatter_split_kernel<kBlock, false, Src>;
+            HYD_TRY_HIP(hipFuncSetAttribute((const void*)k,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            k<<<grid, kBlock, lds, s>>>(src, n, nbins, range, dst);
+        }
+    } else if (strategy == HYD_SCATTER_GLOBAL) {
+        int64_t blocks = std::min<int64_t>(want_blocks, (int64_t)di.num_cus * 8);
+        if (vec)
+            scatter_global_kernel<kBlock, true, Src><<<(unsigned)blocks, kBlock, 0, s>>>(src, n, nbins, dst);
+        else
+            scatter_global_kernel<kBlock, false, Src><<<(unsigned)blocks, kBlock, 0, s>>>(src, n, nbins, dst);
+    } else {
+        return set_error(HYD_E_INVALID, "scatter: unknown strategy %d", strategy);
+    }
+    HYD_CHECK_LAUNCH();
+    return HYD_OK;
+}
+
+inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
+}  // namespace
+}  // namespace hyd
+
+using namespace hyd;
+
+extern "C" int hydhip_weighted_histogram_i64(const int64_t* data, int64_t n, int64_t nbins,
+                                             int64_t* hist, int strategy, void* stream) {
+    HYD_TRY(ensure_init());
+    if (n < 0 || nbins < 1 || nbins > (int64_t)1 << 31 || !hist || (n > 0 && !data))
+        return set_error(HYD_E_INVALID,
+                         "hydhip_weighted_histogram_i64: bad arguments (n=%lld nbins=%lld)",
+                         (long long)n, (long long)nbins);
+    hipStream_t s = resolve_stream(stream);
+    // fill [nbins] 0: zero the destination (a kernel, so graph capture works).
+    HYD_TRY(fill_zero(hist, (size_t)nbins * 8, s));
+    if (n == 0) return HYD_OK;
+    FusedSrc src{data, ModSquare{nbins}};
+    return launch_scatter(src, aligned16(data), n, nbins, hist, strategy, s);
+}
+
+extern "C" int hydhip_scatter_add_i64(const int64_t* idx, const int64_t* vals, int64_t n,
+                                      int64_t* dst, int64_t ndst, int strategy, void* stream) {
+    HYD_TRY(ensure_init());
+    if (n < 0 || ndst < 1 || ndst > (int64_t)1 << 31 || !dst || (n > 0 && (!idx || !vals)))
+        return set_error(HYD_E_INVALID, "hydhip_scatter_add_i64: bad arguments");
+    if (n == 0) return HYD_OK;
+    PairSrc src{idx, vals};
+    return launch_scatter(src, aligned16(idx) && aligned16(vals), n, ndst, dst, strategy,
+                          resolve_stream(stream));
+}

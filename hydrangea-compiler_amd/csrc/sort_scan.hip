@@ -1,0 +1,621 @@
+// sort_scan.hip - int64 scans, stable LSD radix sort of (key, index) pairs,
+// gather, segmented reduce and the COO -> CSR kernels.
+//
+// Reference semantics:
+//   scan family           src/Language/Hydrangea/Lowering.hs:3563-3607
+//   segmented_reduce      Lowering.hs:874-938
+//   sort_indices          Lowering.hs:961-965 -> runtime/hydrangea_runtime.h:391-456
+//                         (stable LSD radix, 8-bit digits, passes = bytes of
+//                         the largest key, at least one)
+//   gather                Lowering.hs:1860-1898
+//   coo_sum_duplicates    Lowering.hs:966-1067 (adjacent equal (row, col)
+//                         merge; outputs sized like the input, zero tail)
+//   csr_from_sorted_coo   Lowering.hs:1068-1147 (row counts, exclusive scan,
+//                         row_ptr[nrows] = total, copy cols/vals)
+//
+// CDNA4 mapping: every scan is reduce-then-scan over 2048-element tiles
+// (tile sums, one workgroup scanning the sums, then the tiles with their
+// offsets), so no kernel ever waits on another workgroup.  The radix sort
+// runs per 8-bit digit: per-tile digit counts (LDS atomics), one scan over
+// the digit-major count table, and a stable scatter that ranks each element
+// inside its tile with wave64 ballots (a lane's rank among equal digits is
+// the popcount of the equal-digit lanes below it) plus per-wave LDS counts.
+#include <algorithm>
+
+#include "hyd_internal.hpp"
+
+namespace hyd {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kItems = 8;
+constexpr int kTile = kBlock * kItems;  // scan tile
+
+__device__ __forceinline__ int64_t wave_incl_scan(int64_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        int64_t u = __shfl_up(v, off, 64);
+        if (lane >= off) v += u;
+    }
+    return v;
+}
+
+// Exclusive scan of one value per thread across the workgroup; returns the
+// exclusive prefix and writes the block total to *total (all threads).
+__device__ __forceinline__ int64_t block_excl_scan(int64_t v, int64_t* total) {
+    __shared__ int64_t wsum[kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int64_t incl = wave_incl_scan(v);
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    int64_t before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+        int64_t s = wsum[w];
+        if (w < wave) before += s;
+        all += s;
+    }
+    __syncthreads();
+    *total = all;
+    return before + incl - v;
+}
+
+// Element e of the scanned sequence: forward a[e], reverse a[n-1-e].
+__device__ __forceinline__ int64_t seq_index(int64_t e, int64_t n, bool reverse) {
+    return reverse ? n - 1 - e : e;
+}
+
+__global__ __launch_bounds__(kBlock) void scan_tile_sums(const int64_t* __restrict__ in, int64_t n,
+                                                         bool reverse, int64_t* __restrict__ sums) {
+    const int64_t base = (int64_t)blockIdx.x * kTile;
+    int64_t v = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        int64_t e = base + (int64_t)k * kBlock + threadIdx.x;
+        if (e < n) v += in[seq_index(e, n, reverse)];
+    }
+    int64_t total;
+    (void)block_excl_scan(v, &total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// One workgroup: exclusive scan of the tile sums, in place, in chunks.
+__global__ __launch_bounds__(kBlock) void scan_sums_kernel(int64_t* __restrict__ sums, int64_t m) {
+    int64_t carry = 0;
+    for (int64_t base = 0; base < m; base += kTile) {
+        int64_t v[kItems];
+        int64_t local = 0;
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            int64_t e = base + (int64_t)threadIdx.x * kItems + k;
+            v[k] = e < m ? sums[e] : 0;
+            local += v[k];
+        }
+        int64_t total;
+        int64_t pre = block_excl_scan(local, &total) + carry;
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            int64_t e = base + (int64_t)threadIdx.x * kItems + k;
+            if (e < m) sums[e] = pre;
+            pre += v[k];
+        }
+        carry += total;
+    }
+}
+
+// Per tile: out[e] = init + (exclusive or inclusive) prefix of the sequence.
+__global__ __launch_bounds__(kBlock) void scan_apply(const int64_t* __restrict__ in, int64_t n,
+                                                     bool reverse, bool inclusive, int64_t init,
+                                                     const int64_t* __restrict__ tile_off,
+                                                     int64_t* __restrict__ out) {
+    const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kItems;
+    int64_t v[kItems];
+    int64_t local = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        int64_t e = base + k;
+        v[k] = e < n ? in[seq_index(e, n, reverse)] : 0;
+        local += v[k];
+    }
+    int64_t total;
+    int64_t acc = block_excl_scan(local, &total) + tile_off[blockIdx.x] + init;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        int64_t e = base + k;
+        if (e < n) {
+            int64_t idx = seq_index(e, n, reverse);
+            if (inclusive) {
+                acc += v[k];
+                out[idx] = acc;
+            } else {
+                out[idx] = acc;
+                acc += v[k];
+            }
+        }
+    }
+}
+
+}  // namespace
+
+// Scan over device memory with a workspace from the stream (also used by the
+// sort and the COO kernels).
+int scan_i64(const int64_t* in, int64_t n, int64_t init, int64_t* out, bool inclusive, bool reverse,
+             hipStream_t s, int64_t* tile_ws) {
+    if (n == 0) return HYD_OK;
+    int64_t tiles = ceil_div(n, kTile);
+    if (tiles > 0x7fffffff) return set_error(HYD_E_INVALID, "scan: too many elements");
+    scan_tile_sums<<<(unsigned)tiles, kBlock, 0, s>>>(in, n, reverse, tile_ws);
+    HYD_CHECK_LAUNCH();
+    scan_sums_kernel<<<1, kBlock, 0, s>>>(tile_ws, tiles);
+    HYD_CHECK_LAUNCH();
+    scan_apply<<<(unsigned)tiles, kBlock, 0, s>>>(in, n, reverse, inclusive, init, tile_ws, out);
+    HYD_CHECK_LAUNCH();
+    return HYD_OK;
+}
+
+int64_t scan_workspace_words(int64_t n) { return ceil_div(n, kTile) + 1; }
+
+namespace {
+
+// ---- radix sort -----------------------------------------------------------
+constexpr int kSortBlock = 256;
+constexpr int kSortRounds = 16;
+constexpr int kSortTile = kSortBlock * kSortRounds;  // 4096 keys per tile
+
+__global__ __launch_bounds__(kBlock) void max_key_kernel(const int64_t* __restrict__ keys, int64_t n,
+                                                         unsigned long long* __restrict__ out) {
+    // max(0, max key) as unsigned: the pass count only needs non-negative
+    // maxima (a negative maximum means one pass, like max_key = 0).
+    int64_t m = 0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock)
+        m = keys[i] > m ? keys[i] : m;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        int64_t o = __shfl_xor(m, off, 64);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0)
+        __hip_atomic_fetch_max(out, (unsigned long long)m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int digit_of(int64_t key, int shift) { return (int)((key >> shift) & 0xFF); }
+
+// counts[d * ntiles + tile] = number of keys of `tile` whose digit is d.
+__global__ __launch_bounds__(kSortBlock) void radix_count_kernel(const int64_t* __restrict__ keys,
+                                                                 int64_t n, int shift, int64_t ntiles,
+                                                                 int64_t* __restrict__ counts) {
+    __shared__ unsigned int hist[4][256];
+    for (int i = threadIdx.x; i < 4 * 256; i += kSortBlock) (&hist[0][0])[i] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kSortTile;
+    unsigned int* mine = hist[threadIdx.x >> 6];
+#pragma unroll 4
+    for (int r = 0; r < kSortRounds; ++r) {
+        int64_t e = base + (int64_t)r * kSortBlock + threadIdx.x;
+        if (e < n) atomicAdd(&mine[digit_of(keys[e], shift)], 1u);
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < 256; d += kSortBlock)
+        counts[(int64_t)d * ntiles + blockIdx.x] =
+            (int64_t)hist[0][d] + hist[1][d] + hist[2][d] + hist[3][d];
+}
+
+// Stable scatter of one tile.  idx_in == nullptr means the identity (first
+// pass of sort_indices).
+__global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
+    const int64_t* __restrict__ keys_in, const int64_t* __restrict__ idx_in, int64_t n, int shift,
+    int64_t ntiles, const int64_t* __restrict__ offsets, int64_t* __restrict__ keys_out,
+    int64_t* __restrict__ idx_out) {
+    __shared__ int64_t run[256];            // next output slot per digit
+    __shared__ unsigned int wave_cnt[4][256];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int d = threadIdx.x; d < 256; d += kSortBlock) run[d] = offsets[(int64_t)d * ntiles + blockIdx.x];
+    for (int i = threadIdx.x; i < 4 * 256; i += kSortBlock) (&wave_cnt[0][0])[i] = 0;
+    __syncthreads();
+    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const int64_t base = (int64_t)blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortRounds; ++r) {
+        const int64_t e = base + (int64_t)r * kSortBlock + threadIdx.x;
+        const bool valid = e < n;
+        int64_t key = valid ? keys_in[e] : 0;
+        int64_t idx = valid ? (idx_in ? idx_in[e] : e) : 0;
+        int d = valid ? digit_of(key, shift) : 0;
+        // lanes of this wave holding the same digit (and a valid element)
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            uint64_t m = __ballot((d >> b) & 1);
+            peers &= ((d >> b) & 1) ? m : ~m;
+        }
+        int rank_in_wave = __popcll(peers & lt_mask);
+        bool leader = valid && rank_in_wave == 0;
+        if (leader) wave_cnt[wave][d] = (unsigned)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            int64_t before = run[d];
+            for (int w = 0; w < wave; ++w) before += wave_cnt[w][d];
+            int64_t pos = before + rank_in_wave;
+            keys_out[pos] = key;
+            idx_out[pos] = idx;
+        }
+        __syncthreads();
+        for (int dd = threadIdx.x; dd < 256; dd += kSortBlock) {
+            run[dd] += (int64_t)wave_cnt[0][dd] + wave_cnt[1][dd] + wave_cnt[2][dd] + wave_cnt[3][dd];
+            wave_cnt[0][dd] = wave_cnt[1][dd] = wave_cnt[2][dd] = wave_cnt[3][dd] = 0;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void copy_i64_kernel(const int64_t* __restrict__ src, int64_t n, int64_t* __restrict__ dst) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+__global__ void iota_kernel(int64_t n, int64_t* __restrict__ dst) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = i;
+}
+
+inline unsigned grid_for(int64_t n, int block = 256, int64_t cap = 8192) {
+    int64_t g = ceil_div(n, block);
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, cap));
+}
+
+}  // namespace
+
+// Workspace layout for sort_pairs: keys x2, idx x1 (perm out is the other),
+// digit counts 256*ntiles, scan tile sums, 1 word for the max key.
+int64_t sort_workspace_words(int64_t n) {
+    int64_t ntiles = ceil_div(std::max<int64_t>(n, 1), kSortTile);
+    return 2 * n + n + 256 * ntiles + scan_workspace_words(256 * ntiles) + 8;
+}
+
+// Stable sort of keys[0..n) -> perm (indices).  `ws` holds
+// sort_workspace_words(n) words.  Passes follow the reference: the number of
+// bytes of max(0, max key), at least one (reads the max back to the host).
+int sort_indices_i64(const int64_t* keys, int64_t n, int64_t* perm, hipStream_t s, int64_t* ws,
+                     int* passes_out) {
+    if (n == 0) return HYD_OK;
+    int64_t ntiles = ceil_div(n, kSortTile);
+    int64_t* kbuf0 = ws;
+    int64_t* kbuf1 = ws + n;
+    int64_t* ibuf = ws + 2 * n;
+    int64_t* counts = ws + 3 * n;
+    int64_t* scan_ws = counts + 256 * ntiles;
+    unsigned long long* maxw =
+        reinterpret_cast<unsigned long long*>(scan_ws + scan_workspace_words(256 * ntiles));
+
+    HYD_TRY(fill_zero(maxw, 8, s));
+    max_key_kernel<<<grid_for(n, kBlock, 1024), kBlock, 0, s>>>(keys, n, maxw);
+    HYD_CHECK_LAUNCH();
+    unsigned long long maxk = 0;
+    HYD_TRY_HIP(hipMemcpyAsync(&maxk, maxw, 8, hipMemcpyDeviceToHost, s));
+    HYD_TRY_HIP(hipStreamSynchronize(s));
+    int passes = 0;
+    for (unsigned long long m = maxk; m; m >>= 8) ++passes;
+    if (passes == 0) passes = 1;
+    if (passes_out) *passes_out = passes;
+
+    // Ping-pong so that the last pass writes its indices into `perm`.
+    const int64_t* kin = keys;
+    const int64_t* iin = nullptr;
+    for (int p = 0; p < passes; ++p) {
+        bool last = p == passes - 1;
+        int64_t* kout = (p & 1) ? kbuf1 : kbuf0;
+        // indices alternate so the final pass lands in perm
+        int64_t* iout = ((passes - 1 - p) & 1) ? ibuf : perm;
+        radix_count_kernel<<<(unsigned)ntiles, kSortBlock, 0, s>>>(kin, n, p * 8, ntiles, counts);
+        HYD_CHECK_LAUNCH();
+        HYD_TRY(scan_i64(counts, 256 * ntiles, 0, counts, false, false, s, scan_ws));
+        radix_scatter_kernel<<<(unsigned)ntiles, kSortBlock, 0, s>>>(kin, iin, n, p * 8, ntiles,
+                                                                      counts, kout, iout);
+        HYD_CHECK_LAUNCH();
+        (void)last;
+        kin = kout;
+        iin = iout;
+    }
+    return HYD_OK;
+}
+
+namespace {
+
+// ---- gather, segmented reduce, COO -> CSR ----------------------------------
+
+__global__ void gather_kernel(const int64_t* __restrict__ perm, int64_t n,
+                              const int64_t* __restrict__ src, int64_t nsrc,
+                              int64_t* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t p = perm[i];
+        out[i] = (uint64_t)p < (uint64_t)nsrc ? src[p] : 0;
+    }
+}
+
+// One thread per segment for short segments; the reference's inner loop is a
+// plain sequential reduction and int64 + is order-free mod 2^64.
+__global__ void segmented_reduce_kernel(const int64_t* __restrict__ offsets, int64_t nseg,
+                                        const int64_t* __restrict__ vals, int64_t nvals, int64_t init,
+                                        int64_t* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave_id = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    // one wave per segment: lanes stride through the segment, then reduce
+    for (int64_t sg = wave_id; sg < nseg; sg += nwaves) {
+        int64_t start = offsets[sg], stop = offsets[sg + 1];
+        uint64_t acc = 0;
+        for (int64_t k = start + lane; k < stop; k += 64)
+            acc += (uint64_t)((uint64_t)k < (uint64_t)nvals ? vals[k] : 0);
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) acc += (uint64_t)__shfl_xor((long long)acc, off, 64);
+        if (lane == 0) out[sg] = (int64_t)((uint64_t)init + acc);
+    }
+}
+
+// keys[i] = rows[i] * ncols + cols[i]  (the packed row-major key of the COO
+// benchmark program, int64 wraparound).
+__global__ void pack_keys_kernel(const int64_t* __restrict__ rows, const int64_t* __restrict__ cols,
+                                 int64_t n, int64_t ncols, int64_t* __restrict__ keys) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        keys[i] = (int64_t)((uint64_t)rows[i] * (uint64_t)ncols + (uint64_t)cols[i]);
+}
+
+// head[i] = 1 when element i starts a new (row, col) run of the (optionally
+// permuted) sequence.
+__global__ void coo_heads_kernel(const int64_t* __restrict__ perm, const int64_t* __restrict__ rows,
+                                 const int64_t* __restrict__ cols, int64_t n,
+                                 int64_t* __restrict__ head) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t pi = perm ? perm[i] : i;
+        int64_t h = 1;
+        if (i > 0) {
+            int64_t pj = perm ? perm[i - 1] : i - 1;
+            h = (rows[pi] != rows[pj] || cols[pi] != cols[pj]) ? 1 : 0;
+        }
+        head[i] = h;
+    }
+}
+
+// run[i] = inclusive count of heads - 1 (the output slot of element i).
+// Heads write row/col; every element adds its value into its slot.
+__global__ void coo_merge_kernel(const int64_t* __restrict__ perm, const int64_t* __restrict__ rows,
+                                 const int64_t* __restrict__ cols, const int64_t* __restrict__ vals,
+                                 const int64_t* __restrict__ head, const int64_t* __restrict__ slot_incl,
+                                 int64_t n, int64_t* __restrict__ out_rows,
+                                 int64_t* __restrict__ out_cols, int64_t* __restrict__ out_vals) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t pi = perm ? perm[i] : i;
+        int64_t slot = slot_incl[i] - 1;
+        if (head[i]) {
+            out_rows[slot] = rows[pi];
+            out_cols[slot] = cols[pi];
+        }
+        __hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(&out_vals[slot]),
+                               (unsigned long long)vals[pi], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__global__ void row_count_kernel(const int64_t* __restrict__ rows, int64_t nnz, int64_t nrows,
+                                 int64_t* __restrict__ counts) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t r = rows[i];
+        if ((uint64_t)r < (uint64_t)nrows)
+            __hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(&counts[r]), 1ull,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// out[i] = i < nnz ? src[i] : 0 for i < len  (CSR copies cols/vals and keeps
+// the zero tail of the freshly allocated output).
+__global__ void copy_prefix_kernel(const int64_t* __restrict__ src, int64_t nnz, int64_t len,
+                                   int64_t* __restrict__ dst) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = i < nnz ? src[i] : 0;
+}
+
+__global__ void set_word_kernel(int64_t* __restrict__ dst, const int64_t* __restrict__ a,
+                                const int64_t* __restrict__ b) {
+    // *dst = *a + *b  (e.g. row_ptr[nrows] = last prefix + last count)
+    *dst = *a + *b;
+}
+
+}  // namespace
+
+int gather_i64(const int64_t* perm, int64_t n, const int64_t* src, int64_t nsrc, int64_t* out,
+               hipStream_t s) {
+    if (n == 0) return HYD_OK;
+    gather_kernel<<<grid_for(n), 256, 0, s>>>(perm, n, src, nsrc, out);
+    HYD_CHECK_LAUNCH();
+    return HYD_OK;
+}
+
+// coo_sum_duplicates over the (optionally permuted) sequence.  Outputs are
+// `len` long with a zero tail; the deduplicated count goes to *d_nnz_out
+// (device word).  ws: 2*len words.
+int coo_sum_duplicates_i64(const int64_t* perm, const int64_t* rows, const int64_t* cols,
+                           const int64_t* vals, int64_t n, int64_t len, int64_t* out_rows,
+                           int64_t* out_cols, int64_t* out_vals, int64_t* d_nnz_out, hipStream_t s,
+                           int64_t* ws) {
+    HYD_TRY(fill_zero(out_rows, (size_t)len * 8, s));
+    HYD_TRY(fill_zero(out_cols, (size_t)len * 8, s));
+    HYD_TRY(fill_zero(out_vals, (size_t)len * 8, s));
+    if (n == 0) return fill_zero(d_nnz_out, 8, s);
+    int64_t* head = ws;
+    int64_t* incl = ws + n;
+    int64_t* scan_ws = ws + 2 * n;
+    coo_heads_kernel<<<grid_for(n), 256, 0, s>>>(perm, rows, cols, n, head);
+    HYD_CHECK_LAUNCH();
+    HYD_TRY(scan_i64(head, n, 0, incl, true, false, s, scan_ws));
+    coo_merge_kernel<<<grid_for(n), 256, 0, s>>>(perm, rows, cols, vals, head, incl, n, out_rows,
+                                                 out_cols, out_vals);
+    HYD_CHECK_LAUNCH();
+    HYD_TRY_HIP(hipMemcpyAsync(d_nnz_out, incl + (n - 1), 8, hipMemcpyDeviceToDevice, s));
+    return HYD_OK;
+}
+
+int64_t coo_workspace_words(int64_t n) { return 2 * n + scan_workspace_words(n) + 8; }
+
+// csr_from_sorted_coo with nnz known on the host.  ws: nrows+1 + scan words.
+int csr_from_sorted_coo_i64(int64_t nrows, int64_t nnz, const int64_t* rows, const int64_t* cols,
+                            const int64_t* vals, int64_t len, int64_t* row_ptr, int64_t* col_idx,
+                            int64_t* out_vals, hipStream_t s, int64_t* ws) {
+    int64_t* counts = ws;
+    int64_t* scan_ws = ws + nrows + 1;
+    HYD_TRY(fill_zero(counts, (size_t)(nrows + 1) * 8, s));
+    if (nnz > 0) {
+        row_count_kernel<<<grid_for(nnz), 256, 0, s>>>(rows, nnz, nrows, counts);
+        HYD_CHECK_LAUNCH();
+    }
+    // exclusive scan over nrows+1 entries (the extra zero count makes
+    // row_ptr[nrows] the total)
+    HYD_TRY(scan_i64(counts, nrows + 1, 0, row_ptr, false, false, s, scan_ws));
+    if (len > 0) {
+        copy_prefix_kernel<<<grid_for(len), 256, 0, s>>>(cols, nnz, len, col_idx);
+        HYD_CHECK_LAUNCH();
+        copy_prefix_kernel<<<grid_for(len), 256, 0, s>>>(vals, nnz, len, out_vals);
+        HYD_CHECK_LAUNCH();
+    }
+    return HYD_OK;
+}
+
+int64_t csr_workspace_words(int64_t nrows) { return nrows + 1 + scan_workspace_words(nrows + 1) + 8; }
+
+int pack_keys_i64(const int64_t* rows, const int64_t* cols, int64_t n, int64_t ncols, int64_t* keys,
+                  hipStream_t s) {
+    if (n == 0) return HYD_OK;
+    pack_keys_kernel<<<grid_for(n), 256, 0, s>>>(rows, cols, n, ncols, keys);
+    HYD_CHECK_LAUNCH();
+    return HYD_OK;
+}
+
+int segmented_reduce_i64(const int64_t* offsets, int64_t nseg, const int64_t* vals, int64_t nvals,
+                         int64_t init, int64_t* out, hipStream_t s) {
+    if (nseg == 0) return HYD_OK;
+    segmented_reduce_kernel<<<grid_for(nseg * 64), 256, 0, s>>>(offsets, nseg, vals, nvals, init, out);
+    HYD_CHECK_LAUNCH();
+    return HYD_OK;
+}
+
+}  // namespace hyd
+
+using namespace hyd;
+
+extern "C" int hydhip_scan_i64(const int64_t* in, int64_t n, int64_t init, int64_t* out,
+                               int inclusive, int reverse, void* stream) {
+    HYD_TRY(ensure_init());
+    if (n < 0 || (n > 0 && (!in || !out))) return set_error(HYD_E_INVALID, "hydhip_scan_i64: bad arguments");
+    if (n == 0) return HYD_OK;
+    hipStream_t s = resolve_stream(stream);
+    int64_t* ws = static_cast<int64_t*>(stream_workspace(s, (size_t)scan_workspace_words(n) * 8));
+    if (!ws) return set_error(HYD_E_OOM, "hydhip_scan_i64: workspace allocation failed");
+    return scan_i64(in, n, init, out, inclusive != 0, reverse != 0, s, ws);
+}
+
+extern "C" int hydhip_sort_indices_i64(const int64_t* keys, int64_t n, int64_t* perm, void* stream) {
+    HYD_TRY(ensure_init());
+    if (n < 0 || (n > 0 && (!keys || !perm)))
+        return set_error(HYD_E_INVALID, "hydhip_sort_indices_i64: bad arguments");
+    if (n == 0) return HYD_OK;
+    hipStream_t s = resolve_stream(stream);
+    int64_t* ws = static_cast<int64_t*>(stream_workspace(s, (size_t)sort_workspace_words(n) * 8));
+    if (!ws) return set_error(HYD_E_OOM, "hydhip_sort_indices_i64: workspace allocation failed");
+    return sort_indices_i64(keys, n, perm, s, ws, nullptr);
+}
+
+extern "C" int hydhip_gather_i64(const int64_t* perm, int64_t n, const int64_t* src, int64_t nsrc,
+                                 int64_t* out, void* stream) {
+    HYD_TRY(ensure_init());
+    if (n < 0 || nsrc < 0 || (n > 0 && (!perm || !out || (nsrc > 0 && !src))))
+        return set_error(HYD_E_INVALID, "hydhip_gather_i64: bad arguments");
+    return gather_i64(perm, n, src, nsrc, out, resolve_stream(stream));
+}
+
+extern "C" int hydhip_segmented_reduce_i64(const int64_t* offsets, int64_t noffsets,
+                                           const int64_t* vals, int64_t nvals, int64_t init,
+                                           int64_t* out, void* stream) {
+    HYD_TRY(ensure_init());
+    if (noffsets < 1 || nvals < 0 || !offsets || (noffsets > 1 && !out) || (nvals > 0 && !vals))
+        return set_error(HYD_E_INVALID, "hydhip_segmented_reduce_i64: bad arguments");
+    return segmented_reduce_i64(offsets, noffsets - 1, vals, nvals, init, out, resolve_stream(stream));
+}
+
+extern "C" int hydhip_coo_sum_duplicates_i64(const int64_t* rows, const int64_t* cols,
+                                             const int64_t* vals, int64_t nnz, int64_t len,
+                                             int64_t* out_rows, int64_t* out_cols,
+                                             int64_t* out_vals, int64_t* nnz_out, void* stream) {
+    HYD_TRY(ensure_init());
+    if (nnz < 0 || len < nnz || !out_rows || !out_cols || !out_vals || !nnz_out ||
+        (nnz > 0 && (!rows || !cols || !vals)))
+        return set_error(HYD_E_INVALID, "hydhip_coo_sum_duplicates_i64: bad arguments");
+    hipStream_t s = resolve_stream(stream);
+    int64_t* ws = static_cast<int64_t*>(stream_workspace(s, (size_t)coo_workspace_words(nnz) * 8 + 64));
+    if (!ws) return set_error(HYD_E_OOM, "hydhip_coo_sum_duplicates_i64: workspace allocation failed");
+    int64_t* d_cnt = ws + coo_workspace_words(nnz);
+    HYD_TRY(coo_sum_duplicates_i64(nullptr, rows, cols, vals, nnz, len, out_rows, out_cols, out_vals,
+                                   d_cnt, s, ws));
+    HYD_TRY_HIP(hipMemcpyAsync(nnz_out, d_cnt, 8, hipMemcpyDeviceToHost, s));
+    HYD_TRY_HIP(hipStreamSynchronize(s));
+    return HYD_OK;
+}
+
+extern "C" int hydhip_csr_from_sorted_coo_i64(int64_t nrows, int64_t nnz, const int64_t* rows,
+                                              const int64_t* cols, const int64_t* vals, int64_t len,
+                                              int64_t* row_ptr, int64_t* col_idx, int64_t* out_vals,
+                                              void* stream) {
+    HYD_TRY(ensure_init());
+    if (nrows < 0 || nnz < 0 || len < nnz || !row_ptr || (len > 0 && (!col_idx || !out_vals)) ||
+        (nnz > 0 && (!rows || !cols || !vals)))
+        return set_error(HYD_E_INVALID, "hydhip_csr_from_sorted_coo_i64: bad arguments");
+    hipStream_t s = resolve_stream(stream);
+    int64_t* ws = static_cast<int64_t*>(stream_workspace(s, (size_t)csr_workspace_words(nrows) * 8));
+    if (!ws) return set_error(HYD_E_OOM, "hydhip_csr_from_sorted_coo_i64: workspace allocation failed");
+    return csr_from_sorted_coo_i64(nrows, nnz, rows, cols, vals, len, row_ptr, col_idx, out_vals, s, ws);
+}
+
+// The whole COO -> CSR program of bench/coo_csr_build/coo_csr_build.hyd:
+// packed keys, stable sort_indices, gather (fused into the duplicate merge),
+// coo_sum_duplicates, csr_from_sorted_coo.  Outputs: row_ptr[nrows+1],
+// col_idx[nnz], vals[nnz] (zero tails past *nnz_out), canonical rows/cols
+// (out_rows, out_cols, may be NULL -> workspace).
+extern "C" int hydhip_coo_to_csr_i64(int64_t nrows, int64_t ncols, int64_t nnz, const int64_t* rows,
+                                     const int64_t* cols, const int64_t* vals, int64_t* row_ptr,
+                                     int64_t* col_idx, int64_t* out_vals, int64_t* nnz_out,
+                                     void* stream) {
+    HYD_TRY(ensure_init());
+    if (nrows < 0 || ncols < 0 || nnz < 0 || !row_ptr || !nnz_out ||
+        (nnz > 0 && (!rows || !cols || !vals || !col_idx || !out_vals)))
+        return set_error(HYD_E_INVALID, "hydhip_coo_to_csr_i64: bad arguments");
+    hipStream_t s = resolve_stream(stream);
+    // workspace: keys, perm, canonical rows/cols/vals, sort/coo/csr scratch
+    int64_t sort_w = sort_workspace_words(nnz), coo_w = coo_workspace_words(nnz),
+            csr_w = csr_workspace_words(nrows);
+    int64_t scratch = std::max(sort_w, std::max(coo_w, csr_w));
+    int64_t total = 5 * nnz + scratch + 8;
+    int64_t* ws = static_cast<int64_t*>(stream_workspace(s, (size_t)total * 8));
+    if (!ws) return set_error(HYD_E_OOM, "hydhip_coo_to_csr_i64: workspace allocation failed");
+    int64_t* keys = ws;
+    int64_t* perm = ws + nnz;
+    int64_t* crow = ws + 2 * nnz;
+    int64_t* ccol = ws + 3 * nnz;
+    int64_t* cval = ws + 4 * nnz;
+    int64_t* scr = ws + 5 * nnz;
+    int64_t* d_cnt = scr + scratch;
+    HYD_TRY(pack_keys_i64(rows, cols, nnz, ncols, keys, s));
+    HYD_TRY(sort_indices_i64(keys, nnz, perm, s, scr, nullptr));
+    HYD_TRY(coo_sum_duplicates_i64(perm, rows, cols, vals, nnz, nnz, crow, ccol, cval, d_cnt, s, scr));
+    int64_t cnt = 0;
+    HYD_TRY_HIP(hipMemcpyAsync(&cnt, d_cnt, 8, hipMemcpyDeviceToHost, s));
+    HYD_TRY_HIP(hipStreamSynchronize(s));
+    *nnz_out = cnt;
+    return csr_from_sorted_coo_i64(nrows, cnt, crow, ccol, cval, nnz, row_ptr, col_idx, out_vals, s, scr);
+}

@@ -1,0 +1,245 @@
+// stencil.hip - 2-D clamp stencils (fp64) and `iterate` over them.
+//
+// Reference shapes: stencil lowering src/Language/Hydrangea/Lowering.hs:
+// 1932-1996 with split-boundary regions :2562-2661 and per-segment clamp
+// folding :2186-2213 (emitted sample: bench/stencil/out_interior.c:47-81).
+// Every region evaluates the same body on clamped neighbours, so one
+// clamped-index kernel reproduces all of them bit for bit.  `iterate` is the
+// serial outer loop of Lowering.hs:840-857 with the ping-pong buffers of
+// CFGOpt.hs:824-900: the init array is never written and each step reads the
+// previous step's output.
+//
+// CDNA4 mapping: a wave64 owns 64 consecutive columns (512-byte coalesced
+// rows); each thread walks `kRows` rows keeping north/centre/south in
+// registers, so each input row is read once per column strip; west/east come
+// from the neighbouring lanes through cross-lane moves (the two strip edges
+// load their outside neighbour).  Outputs are written with one coalesced
+// store per row.  All steps of an iterate are captured once into a hipGraph.
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <tuple>
+
+#include "hyd_internal.hpp"
+
+namespace hyd {
+namespace {
+
+// Bodies: each is the source expression of one program, one IEEE op per
+// statement in source order (fp contraction is off).
+struct HeatBody {  // a 0 0 +. alpha *. (a(-1)0 +. a 1 0 +. a 0(-1) +. a 0 1 -. 4.0 *. a 0 0)
+    double alpha;
+    __device__ __forceinline__ double operator()(double n, double s, double w, double e,
+                                                 double c) const {
+        double t1 = n + s;
+        double t2 = t1 + w;
+        double t3 = t2 + e;
+        double t4 = 4.0 * c;
+        double t5 = t3 - t4;
+        double t6 = alpha * t5;
+        return c + t6;
+    }
+};
+struct LaplacianBody {  // a(-1)0 +. a 1 0 +. a 0(-1) +. a 0 1 +. (a 0 0 *. (-4.0))
+    double alpha;
+    __device__ __forceinline__ double operator()(double n, double s, double w, double e,
+                                                 double c) const {
+        double t1 = n + s;
+        double t2 = t1 + w;
+        double t3 = t2 + e;
+        double t4 = c * -4.0;
+        return t3 + t4;
+    }
+};
+struct Jacobi4Body {  // (a(-1)0 +. a 1 0 +. a 0(-1) +. a 0 1) /. 4.0
+    double alpha;
+    __device__ __forceinline__ double operator()(double n, double s, double w, double e,
+                                                 double c) const {
+        (void)c;
+        double t1 = n + s;
+        double t2 = t1 + w;
+        double t3 = t2 + e;
+        return t3 / 4.0;
+    }
+};
+struct Avg5Body {  // (a(-1)0 +. a 1 0 +. a 0(-1) +. a 0 1 +. a 0 0) /. 5.0
+    double alpha;
+    __device__ __forceinline__ double operator()(double n, double s, double w, double e,
+                                                 double c) const {
+        double t1 = n + s;
+        double t2 = t1 + w;
+        double t3 = t2 + e;
+        double t4 = t3 + c;
+        return t4 / 5.0;
+    }
+};
+
+constexpr int kWaves = 4;   // waves per workgroup, stacked along rows
+constexpr int kRows = 16;   // rows walked by each thread
+
+__device__ __forceinline__ double shfl_up1(double v) {
+    return __shfl_up(v, 1, 64);
+}
+__device__ __forceinline__ double shfl_down1(double v) {
+    return __shfl_down(v, 1, 64);
+}
+
+template <class Body>
+__global__ __launch_bounds__(64 * kWaves) void stencil2d_clamp_kernel(const double* __restrict__ in,
+                                                                      int64_t h, int64_t w,
+                                                                      Body body,
+                                                                      double* __restrict__ out) {
+    const int lane = threadIdx.x;  // 0..63
+    const int64_t j = (int64_t)blockIdx.x * 64 + lane;
+    const int64_t r0 = ((int64_t)blockIdx.y * kWaves + threadIdx.y) * kRows;
+    if (r0 >= h) return;  // whole wave leaves together (r0 is wave-uniform)
+    const bool active = j < w;
+    const int64_t jc = active ? j : w - 1;  // inactive lanes mirror the last column
+    const int64_t jw = j - 1 < 0 ? 0 : j - 1;
+    const int64_t je = j + 1 >= w ? w - 1 : j + 1;
+    const int64_t r_end = r0 + kRows < h ? r0 + kRows : h;
+
+    const double* row_up = in + (r0 > 0 ? r0 - 1 : 0) * w;
+    double up = row_up[jc];
+    double cur = in[r0 * w + jc];
+    for (int64_t r = r0; r < r_end; ++r) {
+        const int64_t rd = r + 1 < h ? r + 1 : h - 1;
+        double down = in[rd * w + jc];
+        // neighbours along the row: lanes hold consecutive columns
+        double west = shfl_up1(cur);
+        double east = shfl_down1(cur);
+        if (lane == 0 || j - 1 < 0 || !active) west = in[r * w + (active ? jw : jc)];
+        if (lane == 63 || j + 1 >= w || !active) east = in[r * w + (active ? je : jc)];
+        double v = body(up, down, west, east, cur);
+        if (active) out[r * w + j] = v;
+        up = cur;
+        cur = down;
+    }
+}
+
+template <class Body>
+int launch_stencil(const double* in, int64_t h, int64_t w, Body body, double* out, hipStream_t s) {
+    dim3 block(64, kWaves);
+    int64_t gx = ceil_div(w, 64);
+    int64_t gy = ceil_div(h, (int64_t)kWaves * kRows);
+    if (gx > 0x7fffffff || gy > 65535)
+        return set_error(HYD_E_INVALID, "stencil: grid %lld x %lld too large", (long long)h,
+                         (long long)w);
+    stencil2d_clamp_kernel<Body><<<dim3((unsigned)gx, (unsigned)gy), block, 0, s>>>(in, h, w, body, out);
+    HYD_CHECK_LAUNCH();
+    return HYD_OK;
+}
+
+int launch_body(const double* in, int64_t h, int64_t w, int body, double alpha, double* out,
+                hipStream_t s) {
+    switch (body) {
+        case HYD_STENCIL_HEAT: return launch_stencil(in, h, w, HeatBody{alpha}, out, s);
+        case HYD_STENCIL_LAPLACIAN: return launch_stencil(in, h, w, LaplacianBody{alpha}, out, s);
+        case HYD_STENCIL_JACOBI4: return launch_stencil(in, h, w, Jacobi4Body{alpha}, out, s);
+        case HYD_STENCIL_AVG5: return launch_stencil(in, h, w, Avg5Body{alpha}, out, s);
+        default: return set_error(HYD_E_INVALID, "stencil: unknown body %d", body);
+    }
+}
+
+// hipGraph cache for iterate: one executable graph per (pointers, shape,
+// body, iters, stream).
+using GraphKey = std::tuple<const double*, double*, double*, int64_t, int64_t, int, uint64_t,
+                            int64_t, hipStream_t>;
+std::mutex g_graph_mu;
+std::map<GraphKey, hipGraphExec_t> g_graphs;
+
+int record_steps(const double* init, int64_t h, int64_t w, int body, double alpha, int64_t iters,
+                 double* buf0, double* buf1, hipStream_t s) {
+    const double* src = init;
+    for (int64_t t = 0; t < iters; ++t) {
+        double* dst = (t & 1) ? buf1 : buf0;
+        HYD_TRY(launch_body(src, h, w, body, alpha, dst, s));
+        src = dst;
+    }
+    return HYD_OK;
+}
+
+}  // namespace
+}  // namespace hyd
+
+using namespace hyd;
+
+extern "C" int hydhip_stencil2d_clamp_f64(const double* in, int64_t h, int64_t w, int body,
+                                          double alpha, double* out, void* stream) {
+    HYD_TRY(ensure_init());
+    if (h < 0 || w < 0 || ((h > 0 && w > 0) && (!in || !out)))
+        return set_error(HYD_E_INVALID, "hydhip_stencil2d_clamp_f64: bad arguments");
+    if (h == 0 || w == 0) return HYD_OK;
+    if (in == out) return set_error(HYD_E_INVALID, "hydhip_stencil2d_clamp_f64: in == out");
+    return launch_body(in, h, w, body, alpha, out, resolve_stream(stream));
+}
+
+extern "C" int hydhip_stencil2d_iterate_f64(const double* init, int64_t h, int64_t w, int body,
+                                            double alpha, int64_t iters, double* buf0,
+                                            double* buf1, int use_graph, double** result,
+                                            void* stream) {
+    HYD_TRY(ensure_init());
+    if (h < 0 || w < 0 || iters < 0 || !result ||
+        ((h > 0 && w > 0) && (!init || !buf0 || (iters > 1 && !buf1))))
+        return set_error(HYD_E_INVALID, "hydhip_stencil2d_iterate_f64: bad arguments");
+    if (buf0 == init || buf1 == init || (iters > 1 && buf0 == buf1))
+        return set_error(HYD_E_INVALID, "hydhip_stencil2d_iterate_f64: buffers must be distinct");
+    if (body < HYD_STENCIL_HEAT || body > HYD_STENCIL_AVG5)
+        return set_error(HYD_E_INVALID, "hydhip_stencil2d_iterate_f64: unknown body %d", body);
+    hipStream_t s = resolve_stream(stream);
+    *result = buf0;
+    if (h == 0 || w == 0) return HYD_OK;
+    const size_t bytes = (size_t)h * (size_t)w * sizeof(double);
+    if (iters == 0) {
+        HYD_TRY_HIP(hipMemcpyAsync(buf0, init, bytes, hipMemcpyDeviceToDevice, s));
+        return HYD_OK;
+    }
+    *result = ((iters - 1) & 1) ? buf1 : buf0;
+    if (!use_graph || iters == 1) return record_steps(init, h, w, body, alpha, iters, buf0, buf1, s);
+
+    uint64_t abits;
+    memcpy(&abits, &alpha, sizeof abits);
+    GraphKey key{init, buf0, buf1, h, w, body, abits, iters, s};
+    hipGraphExec_t exec = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_graph_mu);
+        auto it = g_graphs.find(key);
+        if (it != g_graphs.end()) exec = it->second;
+    }
+    if (!exec) {
+        hipGraph_t graph = nullptr;
+        HYD_TRY_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int rc = record_steps(init, h, w, body, alpha, iters, buf0, buf1, s);
+        hipError_t ec = hipStreamEndCapture(s, &graph);
+        if (rc != HYD_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (ec != hipSuccess)
+            return set_error(HYD_E_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(ec));
+        hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess)
+            return set_error(HYD_E_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+        std::lock_guard<std::mutex> lk(g_graph_mu);
+        g_graphs[key] = exec;
+    }
+    HYD_TRY_HIP(hipGraphLaunch(exec, s));
+    return HYD_OK;
+}
+
+// Drops cached graphs that reference `ptr` (called when a buffer is freed).
+extern "C" void hydhip_forget_graphs_for(const void* ptr) {
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    for (auto it = g_graphs.begin(); it != g_graphs.end();) {
+        const GraphKey& k = it->first;
+        if ((const void*)std::get<0>(k) == ptr || (const void*)std::get<1>(k) == ptr ||
+            (const void*)std::get<2>(k) == ptr) {
+            (void)hipGraphExecDestroy(it->second);
+            it = g_graphs.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}

@@ -1,0 +1,419 @@
+"""Python host side of libhydhip.so (ctypes).
+
+Loads hydrangea-compiler_amd/lib/libhydhip.so (built by build.py) and exposes
+  * the kernel-level C ABI (include/hyd_hip.h) over DeviceArray buffers, and
+  * the drop-in array ABI (include/hydrangea_runtime.h, hyd_exports.h): the
+    hyd_array_t / hyd_tuple_t structs and the hyd_export_* entry points.
+
+There is no CPU fallback: if the library is missing or no GPU is present,
+calls raise HydError.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libhydhip.so")
+
+HYD_OK = 0
+HYD_E_INVALID = 1
+HYD_E_OOM = 2
+HYD_E_HIP = 3
+HYD_E_NODEV = 4
+HYD_E_REJECTED = 5
+HYD_E_TIMEOUT = 6
+HYD_E_BUSY = 7
+
+SCATTER_AUTO = 0
+SCATTER_LDS = 1
+SCATTER_GLOBAL = 2
+SCATTER_LDS_SPLIT = 3
+
+STENCIL_HEAT = 0
+STENCIL_LAPLACIAN = 1
+STENCIL_JACOBI4 = 2
+STENCIL_AVG5 = 3
+
+HYD_MAX_DIMS = 8
+
+
+class HydError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("hydhip error %d: %s" % (code, msg))
+        self.code = code
+
+
+# ---- the array ABI (runtime/hydrangea_runtime.h:220-235 of the reference) ----
+class HydTuple(ctypes.Structure):
+    _fields_ = [("elems", ctypes.c_int64 * HYD_MAX_DIMS), ("ndims", ctypes.c_int)]
+
+
+class HydStorage(ctypes.Structure):
+    _fields_ = [("data", ctypes.POINTER(ctypes.c_ubyte)), ("refcount", ctypes.c_int64)]
+
+
+class HydArray(ctypes.Structure):
+    _fields_ = [
+        ("storage", ctypes.POINTER(HydStorage)),
+        ("data", ctypes.POINTER(ctypes.c_ubyte)),
+        ("elem_size", ctypes.c_size_t),
+        ("shape", HydTuple),
+    ]
+
+
+class HydCsrRecordT(ctypes.Structure):
+    """hyd_record_col_idx_ai_ncols_i_nnz_i_nrows_i_row_ptr_ai_vals_ai_t"""
+    _fields_ = [
+        ("col_idx", ctypes.POINTER(HydArray)),
+        ("ncols", ctypes.c_int64),
+        ("nnz", ctypes.c_int64),
+        ("nrows", ctypes.c_int64),
+        ("row_ptr", ctypes.POINTER(HydArray)),
+        ("vals", ctypes.POINTER(HydArray)),
+    ]
+
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_U64 = ctypes.c_uint64
+_INT = ctypes.c_int
+_D = ctypes.c_double
+_PA = ctypes.POINTER(HydArray)
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "hydhip_last_error": (ctypes.c_char_p, []),
+    "hyd_hip_init": (_INT, [ctypes.c_char_p]),
+    "hyd_hip_cleanup": (None, []),
+    "hydhip_device_info": (_INT, [ctypes.POINTER(_INT), ctypes.POINTER(_I64), ctypes.POINTER(_I64),
+                                  ctypes.c_char_p, _INT]),
+    "hydhip_malloc": (_INT, [ctypes.POINTER(_P), _I64]),
+    "hydhip_free": (_INT, [_P]),
+    "hydhip_memcpy_h2d": (_INT, [_P, _P, _I64, _P]),
+    "hydhip_memcpy_d2h": (_INT, [_P, _P, _I64, _P]),
+    "hydhip_memcpy_d2d": (_INT, [_P, _P, _I64, _P]),
+    "hydhip_stream_create": (_INT, [ctypes.POINTER(_P)]),
+    "hydhip_stream_destroy": (_INT, [_P]),
+    "hydhip_stream_sync": (_INT, [_P]),
+    "hydhip_device_sync": (_INT, []),
+    "hydhip_default_stream": (_P, []),
+    "hydhip_event_create": (_INT, [ctypes.POINTER(_P)]),
+    "hydhip_event_destroy": (_INT, [_P]),
+    "hydhip_event_record": (_INT, [_P, _P]),
+    "hydhip_event_elapsed_ms": (_INT, [_P, _P, ctypes.POINTER(ctypes.c_float)]),
+    "hydhip_fill_splitmix_i64": (_INT, [_P, _I64, _U64, _U64, _INT, _P]),
+    "hydhip_fill_splitmix_f64": (_INT, [_P, _I64, _U64, _U64, _D, _D, _P]),
+    "hydhip_weighted_histogram_i64": (_INT, [_P, _I64, _I64, _P, _INT, _P]),
+    "hydhip_scatter_add_i64": (_INT, [_P, _P, _I64, _P, _I64, _INT, _P]),
+    "hydhip_dot_f64": (_INT, [_P, _P, _I64, _D, _P, _P]),
+    "hydhip_map_chain_f64": (_INT, [_P, _P, _I64, _P, _P]),
+    "hydhip_stencil2d_clamp_f64": (_INT, [_P, _I64, _I64, _INT, _D, _P, _P]),
+    "hydhip_stencil2d_iterate_f64": (_INT, [_P, _I64, _I64, _INT, _D, _I64, _P, _P, _INT,
+                                            ctypes.POINTER(_P), _P]),
+    "hydhip_scan_i64": (_INT, [_P, _I64, _I64, _P, _INT, _INT, _P]),
+    "hydhip_sort_indices_i64": (_INT, [_P, _I64, _P, _P]),
+    "hydhip_gather_i64": (_INT, [_P, _I64, _P, _I64, _P, _P]),
+    "hydhip_segmented_reduce_i64": (_INT, [_P, _I64, _P, _I64, _I64, _P, _P]),
+    "hydhip_coo_sum_duplicates_i64": (_INT, [_P, _P, _P, _I64, _I64, _P, _P, _P,
+                                             ctypes.POINTER(_I64), _P]),
+    "hydhip_csr_from_sorted_coo_i64": (_INT, [_I64, _I64, _P, _P, _P, _I64, _P, _P, _P, _P]),
+    "hydhip_coo_to_csr_i64": (_INT, [_I64, _I64, _I64, _P, _P, _P, _P, _P, _P,
+                                     ctypes.POINTER(_I64), _P]),
+    # array ABI
+    "hyd_shape_size": (_I64, [HydTuple]),
+    "hyd_array_alloc_bytes": (_PA, [HydTuple, ctypes.c_size_t]),
+    "hyd_array_alloc": (_PA, [HydTuple]),
+    "hyd_array_alloc_copy": (_PA, [_PA]),
+    "hyd_array_alloc_device": (_PA, [HydTuple, ctypes.c_size_t]),
+    "hyd_array_reshape_view": (_PA, [_PA, HydTuple]),
+    "hyd_array_free": (None, [_PA]),
+    "hyd_array_length": (_I64, [_PA]),
+    "hyd_array_device_data": (_P, [_PA]),
+    "hyd_array_sync_host": (_INT, [_PA]),
+    "hyd_array_mark_host_dirty": (_INT, [_PA]),
+    "hyd_array_is_device_resident": (_INT, [_PA]),
+    "hyd_hip_set_host_sync": (None, [_INT]),
+    "hyd_hip_get_host_sync": (_INT, []),
+    "hyd_sort_indices": (_PA, [_PA]),
+    "hyd_export_weighted_histogram": (_PA, [_PA]),
+    "hyd_export_weighted_histogram_bins": (_PA, [_PA, _I64]),
+    "hyd_export_dot": (_D, [_PA, _PA]),
+    "hyd_export_map_chain": (_PA, [_PA, _PA]),
+    "hyd_export_heat_step": (_PA, [_PA]),
+    "hyd_export_jacobi_heat": (_PA, [_PA, _I64]),
+    "hyd_export_laplacian": (_PA, [_PA]),
+    "hyd_export_coo_to_csr": (HydCsrRecordT, [_I64, _I64, _I64, _PA, _PA, _PA]),
+}
+
+_lib = None
+
+
+def library_path():
+    return LIB_PATH
+
+
+def lib():
+    """The loaded libhydhip.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HydError(HYD_E_NODEV, "libhydhip.so not built: run hydrangea-compiler_amd/build.py")
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def _check(rc):
+    if rc != HYD_OK:
+        raise HydError(rc, lib().hydhip_last_error().decode(errors="replace"))
+
+
+def init():
+    _check(lib().hyd_hip_init(None))
+
+
+def device_info():
+    cus = _INT()
+    hbm = _I64()
+    lds = _I64()
+    name = ctypes.create_string_buffer(128)
+    _check(lib().hydhip_device_info(ctypes.byref(cus), ctypes.byref(hbm), ctypes.byref(lds), name, 128))
+    return {"num_cus": cus.value, "hbm_bytes": hbm.value, "lds_bytes": lds.value,
+            "arch": name.value.decode()}
+
+
+def sync(stream=None):
+    _check(lib().hydhip_stream_sync(stream))
+
+
+def device_sync():
+    _check(lib().hydhip_device_sync())
+
+
+class DeviceArray:
+    """A device (HBM) buffer with a numpy dtype and shape."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        p = _P()
+        _check(lib().hydhip_malloc(ctypes.byref(p), self.nbytes))
+        self.ptr = p.value
+
+    @property
+    def size(self):
+        return int(np.prod(self.shape, dtype=np.int64))
+
+    @classmethod
+    def from_numpy(cls, a, stream=None):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        if d.nbytes:
+            _check(lib().hydhip_memcpy_h2d(d.ptr, a.ctypes.data, d.nbytes, stream))
+        return d
+
+    def numpy(self, stream=None):
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            _check(lib().hydhip_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes, stream))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().hydhip_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def empty(shape, dtype):
+    return DeviceArray(shape, dtype)
+
+
+# ---- synthetic inputs ---------------------------------------------------------
+def splitmix_i64(n, seed, counter0=0, bits=64, stream=None):
+    d = DeviceArray((n,), np.int64)
+    _check(lib().hydhip_fill_splitmix_i64(d.ptr, n, seed & (2**64 - 1), counter0, 64 - bits, stream))
+    return d
+
+
+def splitmix_f64(n, seed, counter0=0, symmetric=False, stream=None):
+    d = DeviceArray((n,), np.float64)
+    scale, offset = (2.0, -1.0) if symmetric else (1.0, 0.0)
+    _check(lib().hydhip_fill_splitmix_f64(d.ptr, n, seed & (2**64 - 1), counter0, scale, offset, stream))
+    return d
+
+
+# ---- kernels --------------------------------------------------------------------
+def weighted_histogram(data, nbins, strategy=SCATTER_AUTO, out=None, stream=None):
+    out = out or DeviceArray((nbins,), np.int64)
+    _check(lib().hydhip_weighted_histogram_i64(data.ptr, data.size, nbins, out.ptr, strategy, stream))
+    return out
+
+
+def scatter_add(idx, vals, dst, strategy=SCATTER_AUTO, stream=None):
+    _check(lib().hydhip_scatter_add_i64(idx.ptr, vals.ptr, idx.size, dst.ptr, dst.size, strategy, stream))
+    return dst
+
+
+def dot(xs, ys, init=0.0, out=None, stream=None):
+    """Returns the device scalar (DeviceArray of shape (1,))."""
+    out = out or DeviceArray((1,), np.float64)
+    _check(lib().hydhip_dot_f64(xs.ptr, ys.ptr, min(xs.size, ys.size), init, out.ptr, stream))
+    return out
+
+
+def map_chain(xs, ys, out=None, stream=None):
+    n = min(xs.size, ys.size)
+    out = out or DeviceArray((n,), np.float64)
+    _check(lib().hydhip_map_chain_f64(xs.ptr, ys.ptr, n, out.ptr, stream))
+    return out
+
+
+def stencil2d(u, body=STENCIL_HEAT, alpha=0.1, out=None, stream=None):
+    h, w = u.shape
+    out = out or DeviceArray((h, w), np.float64)
+    _check(lib().hydhip_stencil2d_clamp_f64(u.ptr, h, w, body, alpha, out.ptr, stream))
+    return out
+
+
+def iterate_stencil2d(u, iters, body=STENCIL_HEAT, alpha=0.1, buf0=None, buf1=None, use_graph=True,
+                      stream=None):
+    """Returns (result_array, buf0, buf1); result_array is buf0 or buf1."""
+    h, w = u.shape
+    buf0 = buf0 or DeviceArray((h, w), np.float64)
+    buf1 = buf1 or DeviceArray((h, w), np.float64)
+    res = _P()
+    _check(lib().hydhip_stencil2d_iterate_f64(u.ptr, h, w, body, alpha, iters, buf0.ptr, buf1.ptr,
+                                              1 if use_graph else 0, ctypes.byref(res), stream))
+    return (buf0 if res.value == buf0.ptr else buf1), buf0, buf1
+
+
+def scan(a, init=0, inclusive=False, reverse=False, out=None, stream=None):
+    out = out or DeviceArray(a.shape, np.int64)
+    _check(lib().hydhip_scan_i64(a.ptr, a.size, init, out.ptr, int(inclusive), int(reverse), stream))
+    return out
+
+
+def sort_indices(keys, out=None, stream=None):
+    out = out or DeviceArray(keys.shape, np.int64)
+    _check(lib().hydhip_sort_indices_i64(keys.ptr, keys.size, out.ptr, stream))
+    return out
+
+
+def gather(perm, src, out=None, stream=None):
+    out = out or DeviceArray(perm.shape, np.int64)
+    _check(lib().hydhip_gather_i64(perm.ptr, perm.size, src.ptr, src.size, out.ptr, stream))
+    return out
+
+
+def segmented_reduce(offsets, vals, init=0, out=None, stream=None):
+    out = out or DeviceArray((max(offsets.size - 1, 0),), np.int64)
+    _check(lib().hydhip_segmented_reduce_i64(offsets.ptr, offsets.size, vals.ptr, vals.size, init,
+                                             out.ptr, stream))
+    return out
+
+
+def coo_sum_duplicates(rows, cols, vals, nnz, stream=None):
+    n = rows.size
+    o_r, o_c, o_v = (DeviceArray((n,), np.int64) for _ in range(3))
+    cnt = _I64()
+    _check(lib().hydhip_coo_sum_duplicates_i64(rows.ptr, cols.ptr, vals.ptr, nnz, n, o_r.ptr, o_c.ptr,
+                                               o_v.ptr, ctypes.byref(cnt), stream))
+    return o_r, o_c, o_v, cnt.value
+
+
+def csr_from_sorted_coo(nrows, nnz, rows, cols, vals, stream=None):
+    n = cols.size
+    rp = DeviceArray((nrows + 1,), np.int64)
+    ci = DeviceArray((n,), np.int64)
+    cv = DeviceArray((n,), np.int64)
+    _check(lib().hydhip_csr_from_sorted_coo_i64(nrows, nnz, rows.ptr, cols.ptr, vals.ptr, n, rp.ptr,
+                                                ci.ptr, cv.ptr, stream))
+    return rp, ci, cv
+
+
+def coo_to_csr(nrows, ncols, rows, cols, vals, nnz=None, stream=None):
+    nnz = rows.size if nnz is None else nnz
+    rp = DeviceArray((nrows + 1,), np.int64)
+    ci = DeviceArray((nnz,), np.int64)
+    cv = DeviceArray((nnz,), np.int64)
+    cnt = _I64()
+    _check(lib().hydhip_coo_to_csr_i64(nrows, ncols, nnz, rows.ptr, cols.ptr, vals.ptr, rp.ptr, ci.ptr,
+                                       cv.ptr, ctypes.byref(cnt), stream))
+    return rp, ci, cv, cnt.value
+
+
+# ---- events and streams ---------------------------------------------------------
+class Event:
+    def __init__(self):
+        p = _P()
+        _check(lib().hydhip_event_create(ctypes.byref(p)))
+        self.ptr = p.value
+
+    def record(self, stream=None):
+        _check(lib().hydhip_event_record(self.ptr, stream))
+
+    def elapsed_ms(self, later):
+        ms = ctypes.c_float()
+        _check(lib().hydhip_event_elapsed_ms(self.ptr, later.ptr, ctypes.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            lib().hydhip_event_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+def stream_create():
+    p = _P()
+    _check(lib().hydhip_stream_create(ctypes.byref(p)))
+    return p.value
+
+
+def stream_destroy(s):
+    _check(lib().hydhip_stream_destroy(s))
+
+
+def default_stream():
+    return lib().hydhip_default_stream()
+
+
+# ---- drop-in array ABI helpers ----------------------------------------------------
+def hyd_tuple(*dims):
+    t = HydTuple()
+    t.ndims = len(dims)
+    for i, d in enumerate(dims):
+        t.elems[i] = int(d)
+    return t
+
+
+def host_array(a):
+    """A hyd_array_t* (allocated by the library) holding a copy of numpy `a`."""
+    a = np.ascontiguousarray(a)
+    arr = lib().hyd_array_alloc_bytes(hyd_tuple(*a.shape), a.dtype.itemsize)
+    if a.nbytes:
+        ctypes.memmove(arr.contents.data, a.ctypes.data, a.nbytes)
+    return arr
+
+
+def read_array(arr, dtype):
+    """numpy copy of a hyd_array_t*'s host bytes."""
+    a = arr.contents
+    shape = tuple(a.shape.elems[i] for i in range(a.shape.ndims))
+    n = int(np.prod(shape, dtype=np.int64))
+    out = np.empty(shape, dtype)
+    if n:
+        ctypes.memmove(out.ctypes.data, a.data, n * np.dtype(dtype).itemsize)
+    return out

@@ -1,0 +1,64 @@
+/* hyd_exports.h - hyd_export_* entry points of libhydhip.so.
+ *
+ * One wrapper per exported program kernel, with the symbol name, C types and
+ * ownership the reference's --export-kernel mode generates
+ * (src/Language/Hydrangea/CodegenC.hs:372-460: "hyd_export_" ++ kernel name,
+ * hyd_array_t* for arrays, double for floats, int64_t otherwise, record
+ * structs named by recordStructName :2028-2029 with fields in sorted order).
+ * Array parameters are borrowed; returned arrays belong to the caller, who
+ * frees them with hyd_array_free (demo/gray_scott/main.cpp:93-119).  Errors
+ * print a message and exit(1), as the reference runtime does.  The source of
+ * each program is in hydrangea-compiler_amd/programs/.
+ */
+#ifndef HYDRANGEA_EXPORT_HIP_KERNELS_H
+#define HYDRANGEA_EXPORT_HIP_KERNELS_H
+
+#include "hydrangea_runtime.h"
+#include "hyd_hip_runtime.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Record returned by csr_from_sorted_coo (Lowering.hs:1088-1095). */
+typedef struct {
+  hyd_array_t* col_idx;
+  int64_t ncols;
+  int64_t nnz;
+  int64_t nrows;
+  hyd_array_t* row_ptr;
+  hyd_array_t* vals;
+} hyd_record_col_idx_ai_ncols_i_nnz_i_nrows_i_row_ptr_ai_vals_ai_t;
+
+/* Record returned by coo_sum_duplicates (Lowering.hs:996-1003). */
+typedef struct {
+  hyd_array_t* cols;
+  int64_t ncols;
+  int64_t nnz;
+  int64_t nrows;
+  hyd_array_t* rows;
+  hyd_array_t* vals;
+} hyd_record_cols_ai_ncols_i_nnz_i_nrows_i_rows_ai_vals_ai_t;
+
+/* programs/weighted_histogram.hyd */
+hyd_array_t* hyd_export_weighted_histogram(hyd_array_t* data);
+hyd_array_t* hyd_export_weighted_histogram_bins(hyd_array_t* data, int64_t bins);
+/* programs/dot.hyd */
+double hyd_export_dot(hyd_array_t* xs, hyd_array_t* ys);
+/* programs/map_chain.hyd */
+hyd_array_t* hyd_export_map_chain(hyd_array_t* xs, hyd_array_t* ys);
+/* programs/jacobi_heat.hyd */
+hyd_array_t* hyd_export_heat_step(hyd_array_t* u);
+hyd_array_t* hyd_export_jacobi_heat(hyd_array_t* u, int64_t n);
+/* programs/laplacian.hyd */
+hyd_array_t* hyd_export_laplacian(hyd_array_t* input);
+/* programs/coo_to_csr.hyd */
+hyd_record_col_idx_ai_ncols_i_nnz_i_nrows_i_row_ptr_ai_vals_ai_t
+hyd_export_coo_to_csr(int64_t nrows, int64_t ncols, int64_t nnz, hyd_array_t* rows,
+                      hyd_array_t* cols, hyd_array_t* vals);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,178 @@
+/* hyd_hip.h - kernel-level C ABI of libhydhip.so (MI355X / gfx950).
+ *
+ * These are the fused-kernel templates that the HIP backend instantiates for
+ * the loop shapes the reference's C backend emits.  Every entry point takes
+ * plain device pointers (HBM addresses) and element counts, enqueues its work
+ * on `stream` (an opaque hipStream_t; NULL selects the library's own stream)
+ * and returns a status code instead of exiting, so a host that cannot afford
+ * exit(1) (the serving loop, the test harness) can recover.  The array-level
+ * drop-in wrappers (hyd_export_*) are declared in hyd_exports.h.
+ *
+ * Integer arithmetic is two's-complement mod 2^64 (the reference's C with
+ * -fwrapv); fp64 elementwise and stencil arithmetic is IEEE binary64 with no
+ * contraction, so results equal the emitted C bit for bit; fp64 reductions
+ * use a fixed-shape tree (deterministic run to run).
+ */
+#ifndef HYD_HIP_H
+#define HYD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define HYD_OK 0
+#define HYD_E_INVALID 1   /* bad argument: null pointer, negative size, ... */
+#define HYD_E_OOM 2       /* device allocation failed */
+#define HYD_E_HIP 3       /* HIP runtime error */
+#define HYD_E_NODEV 4     /* no GPU */
+#define HYD_E_REJECTED 5  /* serving: request can never fit the HBM budget */
+#define HYD_E_TIMEOUT 6   /* serving: deadline passed */
+#define HYD_E_BUSY 7      /* serving: queue full */
+
+/* Message of the last failing call on this thread (empty if none). */
+const char* hydhip_last_error(void);
+
+/* ---- lifecycle: the hyd_metal_init/hyd_metal_cleanup analog of
+ *      src/Language/Hydrangea/CodegenMSL.hs:1219-1221 ------------------- */
+int hyd_hip_init(const char* code_object_dir_or_null);
+void hyd_hip_cleanup(void);
+/* Compute units, HBM bytes, LDS bytes per workgroup and device name. */
+int hydhip_device_info(int* num_cus, int64_t* hbm_bytes, int64_t* lds_bytes,
+                       char* name, int name_len);
+
+/* ---- memory, streams, events (plain pointers) ------------------------- */
+int hydhip_malloc(void** dptr, int64_t bytes);
+int hydhip_free(void* dptr);
+int hydhip_memcpy_h2d(void* dst, const void* src, int64_t bytes, void* stream);
+int hydhip_memcpy_d2h(void* dst, const void* src, int64_t bytes, void* stream);
+int hydhip_memcpy_d2d(void* dst, const void* src, int64_t bytes, void* stream);
+int hydhip_stream_create(void** stream);
+int hydhip_stream_destroy(void* stream);
+int hydhip_stream_sync(void* stream);
+int hydhip_device_sync(void);
+void* hydhip_default_stream(void);
+int hydhip_event_create(void** event);
+int hydhip_event_destroy(void* event);
+int hydhip_event_record(void* event, void* stream);
+int hydhip_event_elapsed_ms(void* start, void* stop, float* ms);
+
+/* ---- synthetic inputs: SplitMix64 (Vigna), counter form ---------------
+ * word(i) = mix(seed + (counter0 + i + 1) * 0x9E3779B97F4A7C15), which is the
+ * (counter0+i)-th output of the sequential generator seeded with `seed`. */
+/* dst[i] = word(i) >> shift  (uniform integers in [0, 2^(64-shift))) */
+int hydhip_fill_splitmix_i64(int64_t* dst, int64_t n, uint64_t seed,
+                             uint64_t counter0, int shift, void* stream);
+/* dst[i] = (word(i) >> 11) * 2^-53 * scale + offset, each step exact for
+ * scale in {1, 2} and offset in {0, -1}: uniform fp64 in [0,1) or [-1,1). */
+int hydhip_fill_splitmix_f64(double* dst, int64_t n, uint64_t seed,
+                             uint64_t counter0, double scale, double offset,
+                             void* stream);
+
+/* ---- scatter (+) ------------------------------------------------------
+ * Strategy: the privatization choice the reference makes in
+ * src/Language/Hydrangea/Parallelize.hs:466-503 (privatized) and
+ * :350-419 (atomic), remapped to CDNA4 memory levels. */
+#define HYD_SCATTER_AUTO 0        /* autotuner / size rule */
+#define HYD_SCATTER_LDS 1         /* LDS-private bins per workgroup, one merge */
+#define HYD_SCATTER_GLOBAL 2      /* global 64-bit atomics per element */
+#define HYD_SCATTER_LDS_SPLIT 3   /* bins split into LDS-sized ranges, one pass per range */
+
+/* Weighted histogram, the fused scatter_generate of
+ * `scatter (+) (fill [nbins] 0) (map (x % nbins) data) (map (x * x) data)`:
+ * hist[data[i] % nbins] += data[i] * data[i] (mod 2^64), hist zeroed first.
+ * Elements whose route falls outside [0, nbins) are dropped. */
+int hydhip_weighted_histogram_i64(const int64_t* data, int64_t n, int64_t nbins,
+                                  int64_t* hist, int strategy, void* stream);
+/* Unfused scatter: dst[idx[i]] += vals[i] (dst is accumulated, not zeroed). */
+int hydhip_scatter_add_i64(const int64_t* idx, const int64_t* vals, int64_t n,
+                           int64_t* dst, int64_t ndst, int strategy, void* stream);
+
+/* ---- reductions and maps (fp64) ---------------------------------------- */
+/* *out = init + sum_i xs[i]*ys[i]  (the fused reduce_generate of
+ * `reduce (+.) init (zipwith (*.) xs ys)`), fixed-shape tree. out is device. */
+int hydhip_dot_f64(const double* xs, const double* ys, int64_t n, double init,
+                   double* out, void* stream);
+/* The fused 8-op chain, out[i] = f(xs[i], ys[i]) with
+ *   t1 = 2.0*x; t2 = t1+y; t3 = |t2|; t4 = sqrt t3; t5 = t4*x; t6 = t5-y;
+ *   t7 = t6 > 0.0 ? t6 : 0.0; out = exp(t7*1.0e-3). */
+int hydhip_map_chain_f64(const double* xs, const double* ys, int64_t n,
+                         double* out, void* stream);
+
+/* The map chain's math on its own: out[i] = exp(x[i]) (op 0, the libm-exact
+ * table method of the chain) or sqrt(x[i]) (op 1), on the GPU. */
+int hydhip_math_f64(int op, const double* x, int64_t n, double* out, void* stream);
+/* The same exp on the host CPU (identical operation sequence). */
+void hydhip_host_exp(const double* x, int64_t n, double* out);
+
+/* ---- 2-D clamp stencils (fp64, row-major h x w) ------------------------ */
+#define HYD_STENCIL_HEAT 0       /* c + alpha*((((n + s) + w) + e) - 4.0*c) */
+#define HYD_STENCIL_LAPLACIAN 1  /* (((n + s) + w) + e) + c*(-4.0) */
+#define HYD_STENCIL_JACOBI4 2    /* (((n + s) + w) + e) / 4.0 */
+#define HYD_STENCIL_AVG5 3       /* ((((n + s) + w) + e) + c) / 5.0 */
+/* n = a(-1,0), s = a(1,0), w = a(0,-1), e = a(0,1), c = a(0,0), indices
+ * clamped to the grid.  alpha is used by HYD_STENCIL_HEAT only. */
+int hydhip_stencil2d_clamp_f64(const double* in, int64_t h, int64_t w, int body,
+                               double alpha, double* out, void* stream);
+/* iterate iters (stencil clamp body): ping-pong between buf0 and buf1, init is
+ * never written; *result receives buf0 or buf1 (buf0 when iters == 0, after
+ * a copy).  use_graph != 0 captures the iters launches once as a hipGraph and
+ * replays it (cached per pointer/shape/iters). */
+int hydhip_stencil2d_iterate_f64(const double* init, int64_t h, int64_t w,
+                                 int body, double alpha, int64_t iters,
+                                 double* buf0, double* buf1, int use_graph,
+                                 double** result, void* stream);
+
+/* ---- scans, sort, gather, COO -> CSR (int64) ---------------------------- */
+/* The scan family of src/Language/Hydrangea/Lowering.hs:3563-3607 with (+):
+ * exclusive: out[k] = init + sum_{j<k} a[j]; inclusive adds a[k] too;
+ * reverse runs from the end (scanr).  out may alias in. */
+int hydhip_scan_i64(const int64_t* in, int64_t n, int64_t init, int64_t* out,
+                    int inclusive, int reverse, void* stream);
+/* sort_indices (runtime/hydrangea_runtime.h:391-456): stable LSD radix sort
+ * of keys, 8-bit digits, passes = bytes of max(0, max key) (at least one);
+ * perm[k] = index of the k-th smallest key.  Reads the maximum key back to
+ * the host (one synchronisation). */
+int hydhip_sort_indices_i64(const int64_t* keys, int64_t n, int64_t* perm,
+                            void* stream);
+/* gather (Lowering.hs:1860-1898): out[i] = src[perm[i]]; an index outside
+ * [0, nsrc) yields 0. */
+int hydhip_gather_i64(const int64_t* perm, int64_t n, const int64_t* src,
+                      int64_t nsrc, int64_t* out, void* stream);
+/* segmented_reduce (+) (Lowering.hs:874-938): out[s] = init + sum of
+ * vals[offsets[s] .. offsets[s+1]), s < noffsets-1. */
+int hydhip_segmented_reduce_i64(const int64_t* offsets, int64_t noffsets,
+                                const int64_t* vals, int64_t nvals, int64_t init,
+                                int64_t* out, void* stream);
+/* coo_sum_duplicates (Lowering.hs:966-1067): merges adjacent equal
+ * (row, col) of the first nnz triplets; outputs are len long, zero tail;
+ * *nnz_out (host) receives the merged count. */
+int hydhip_coo_sum_duplicates_i64(const int64_t* rows, const int64_t* cols,
+                                  const int64_t* vals, int64_t nnz, int64_t len,
+                                  int64_t* out_rows, int64_t* out_cols,
+                                  int64_t* out_vals, int64_t* nnz_out,
+                                  void* stream);
+/* csr_from_sorted_coo (Lowering.hs:1068-1147): row_ptr[nrows+1] (exclusive
+ * scan of per-row counts of the first nnz rows, row_ptr[nrows] = total),
+ * col_idx/out_vals = first nnz of cols/vals then zeros up to len.  Rows
+ * outside [0, nrows) are not counted. */
+int hydhip_csr_from_sorted_coo_i64(int64_t nrows, int64_t nnz, const int64_t* rows,
+                                   const int64_t* cols, const int64_t* vals,
+                                   int64_t len, int64_t* row_ptr, int64_t* col_idx,
+                                   int64_t* out_vals, void* stream);
+/* The COO -> CSR build program (bench/coo_csr_build/coo_csr_build.hyd:44-62):
+ * keys = row*ncols + col, perm = sort_indices keys, gather, merge duplicates,
+ * CSR.  row_ptr[nrows+1], col_idx[nnz] and out_vals[nnz] (zero past
+ * *nnz_out); *nnz_out (host) = merged count. */
+int hydhip_coo_to_csr_i64(int64_t nrows, int64_t ncols, int64_t nnz,
+                          const int64_t* rows, const int64_t* cols,
+                          const int64_t* vals, int64_t* row_ptr, int64_t* col_idx,
+                          int64_t* out_vals, int64_t* nnz_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HYD_HIP_H */

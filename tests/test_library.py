@@ -1,0 +1,129 @@
+"""CPU checks of the C-ABI library: it loads, exports every function the
+public headers declare, keeps the reference's hyd_array_t layout, and runs
+its pure host functions.  No GPU work is launched here."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import hydhip
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INCLUDE = os.path.join(ROOT, "include")
+
+
+def declared_functions():
+    names = set()
+    for f in sorted(os.listdir(INCLUDE)):
+        if not f.endswith(".h"):
+            continue
+        text = open(os.path.join(INCLUDE, f)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", "", text)
+        for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(([^;{}]*)\)\s*;", text):
+            name = m.group(1)
+            if name in ("if", "while", "for", "sizeof", "return"):
+                continue
+            names.add(name)
+    return sorted(names)
+
+
+def test_library_loads():
+    assert os.path.exists(hydhip.LIB_PATH), "build with hydrangea-compiler_amd/build.py"
+    hydhip.lib()
+
+
+def test_every_declared_symbol_is_exported():
+    names = declared_functions()
+    assert len(names) > 40
+    lib = ctypes.CDLL(hydhip.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_every_symbol_has_a_ctypes_signature():
+    missing = [n for n in declared_functions() if n.startswith(("hydhip_", "hyd_export_", "hyd_hip_"))
+               and n not in hydhip._SIGS and n not in ("hydhip_host_exp", "hydhip_math_f64",
+                                                       "hyd_hip_set_stream")]
+    assert not missing, missing
+
+
+def test_struct_layout_matches_reference():
+    # runtime/hydrangea_runtime.h:220-235: int64 elems[8] + int ndims (padded),
+    # storage {data*, refcount}, array {storage*, data*, elem_size, shape}
+    assert ctypes.sizeof(hydhip.HydTuple) == 72
+    assert ctypes.sizeof(hydhip.HydArray) == 8 + 8 + 8 + 72
+    src = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "hydrangea_runtime.h"
+int main(void){printf("%zu %zu %zu %zu\n", sizeof(hyd_tuple_t), sizeof(hyd_array_t),
+ offsetof(hyd_array_t, shape), sizeof(hyd_array_storage_t)); return 0;}
+'''
+    tmp = "/tmp/hyd_layout_check"
+    with open(tmp + ".c", "w") as f:
+        f.write(src)
+    subprocess.run(["gcc", "-std=c11", "-I" + INCLUDE, tmp + ".c", "-o", tmp], check=True)
+    out = subprocess.run([tmp], capture_output=True, text=True, check=True).stdout.split()
+    assert out == ["72", "96", "24", "16"]
+
+
+def test_host_runtime_functions_without_gpu():
+    lib = hydhip.lib()
+    t = hydhip.hyd_tuple(3, 4)
+    assert lib.hyd_shape_size(t) == 12
+    arr = lib.hyd_array_alloc_bytes(t, 8)
+    a = arr.contents
+    assert a.elem_size == 8 and a.shape.ndims == 2 and a.storage.contents.refcount == 1
+    # calloc semantics: zero-filled
+    assert not hydhip.read_array(arr, np.int64).any()
+    v = lib.hyd_array_reshape_view(arr, hydhip.hyd_tuple(12))
+    assert a.storage.contents.refcount == 2
+    lib.hyd_array_free(v)
+    assert a.storage.contents.refcount == 1
+    lib.hyd_array_free(arr)
+
+
+def test_reshape_mismatch_exits_like_reference():
+    code = (
+        "import sys; sys.path.insert(0, %r); import hydhip; l = hydhip.lib();"
+        "a = l.hyd_array_alloc_bytes(hydhip.hyd_tuple(3), 8);"
+        "l.hyd_array_reshape_view(a, hydhip.hyd_tuple(4))" % os.path.dirname(hydhip.__file__)
+    )
+    p = subprocess.run(["python3", "-c", code], capture_output=True, text=True)
+    assert p.returncode == 1
+    assert "reshape view size mismatch (3 elements to 4)" in p.stderr
+
+
+def test_host_exp_matches_libm(orc):
+    # The map chain's exp (csrc/hyd_exp.h) compiled for the host must equal the
+    # libm exp the emitted C calls, bit for bit, over the whole range.
+    rng = np.random.default_rng(7)
+    x = np.concatenate([
+        rng.uniform(-760.0, 760.0, 1_000_000),
+        rng.uniform(0.0, 0.003, 1_000_000),        # the map chain's argument range
+        rng.uniform(-1.0, 1.0, 500_000),
+        np.frombuffer(rng.integers(0, 2**63, 500_000, dtype=np.uint64).tobytes(), np.float64),
+        np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 709.78, 709.79, -745.13, -745.14, -708.4,
+                  2.0**-54, 2.0**-55, 512.0, -512.0, 1023.9, -1074.0]),
+    ])
+    lib = hydhip.lib()
+    lib.hydhip_host_exp.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+    lib.hydhip_host_exp.restype = None
+    got = np.empty_like(x)
+    lib.hydhip_host_exp(x.ctypes.data, x.size, got.ctypes.data)
+    ref = orc.libm_exp(x)
+    same = (got.view(np.uint64) == ref.view(np.uint64)) | (np.isnan(got) & np.isnan(ref))
+    assert same.all(), x[~same][:5]
+
+
+def test_no_gpu_reports_nodev():
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU driver is present")
+    rc = hydhip.lib().hyd_hip_init(None)
+    assert rc == hydhip.HYD_E_NODEV
+    with pytest.raises(hydhip.HydError):
+        hydhip.DeviceArray((4,), np.int64)
