@@ -141,6 +141,49 @@ def bench_jacobi(steps, warmup):
     return res
 
 
+def bench_batched(steps, warmup):
+    c = workloads.BATCH
+    lens, bins = workloads.batch_instances()
+    off, hoff = hydhip.batch_offsets(lens, bins)
+    total = int(lens.sum())
+    d = hydhip.splitmix_i64(total, c["seed"], bits=c["bits"])
+    hist = hydhip.DeviceArray((int(bins.sum()),), np.int64)
+    t0 = time.perf_counter()
+    classes = hydhip.tune_batched()
+    tune_s = time.perf_counter() - t0
+    res = dict(instances=int(lens.size), elements=total, bytes=workloads.batch_algorithmic_bytes(lens, bins),
+               tuned_classes=classes, tune_s=round(tune_s, 3))
+    for name, pol in (("tuned", hydhip.BATCH_TUNED), ("lds", hydhip.BATCH_LDS), ("global", hydhip.BATCH_GLOBAL)):
+        def step(ev, pol=pol):
+            if ev:
+                ev[0].record()
+            hydhip.batched_histogram(d, off, lens, bins, hoff, hist=hist, policy=pol)
+            if ev:
+                ev[1].record()
+
+        ms, ev = timed_loop(step, max(3, steps // 4), 1, events=True)
+        res[name] = dict(ms_per_batch=ms, event_ms=float(np.mean(ev)),
+                         gbs=res["bytes"] / ms / 1e6, req_s=lens.size / ms * 1e3)
+    # per-call plain launches: one synchronous hydhip_weighted_histogram_i64 per instance
+    lib = hydhip.lib()
+    s = hydhip.default_stream()
+
+    def percall(ev):
+        for k in range(lens.size):
+            rc = lib.hydhip_weighted_histogram_i64(d.ptr + 8 * int(off[k]), int(lens[k]), int(bins[k]),
+                                                   hist.ptr + 8 * int(hoff[k]), 0, s)
+            if rc:
+                raise hydhip.HydError(rc, lib.hydhip_last_error().decode())
+            lib.hydhip_stream_sync(s)
+
+    ms, _ = timed_loop(percall, 1, 1)
+    res["per_call"] = dict(ms_per_batch=ms, req_s=lens.size / ms * 1e3)
+    res["speedup_vs_per_call"] = res["per_call"]["ms_per_batch"] / res["tuned"]["ms_per_batch"]
+    d.free()
+    hist.free()
+    return res
+
+
 def cpu_baseline_dotmap():
     import pyoracle
 
@@ -188,6 +231,10 @@ def main():
     if not only or "jacobi" in only:
         log("jacobi ...")
         kernels["jacobi_heat"] = bench_jacobi(args.steps, args.warmup)
+
+    if not only or "batch" in only:
+        log("batched histograms ...")
+        kernels["batched_histogram"] = bench_batched(args.steps, args.warmup)
 
     cpu = None
     if not args.no_cpu:

@@ -210,8 +210,10 @@ void hyd_array_free(hyd_array_t* arr) {
             g_storages.erase(&st->base);
         }
         if (st->dev) {
-            hydhip_forget_graphs_for(st->dev);
+            // drain queued work that may read this buffer (or a graph that
+            // names it) before the graph and the memory are released
             (void)hipStreamSynchronize(export_stream());
+            hydhip_forget_graphs_for(st->dev);
             (void)hipFree(st->dev);
         }
         free(st->base.data);

@@ -196,6 +196,9 @@ int hydhip_malloc(void** dptr, int64_t bytes) {
 
 int hydhip_free(void* dptr) {
     if (!dptr) return HYD_OK;
+    // Kernels queued on any stream may still read the buffer: drain them
+    // before the memory goes back to the allocator.
+    HYD_TRY_HIP(hipDeviceSynchronize());
     HYD_TRY_HIP(hipFree(dptr));
     return HYD_OK;
 }

@@ -177,7 +177,8 @@ int launch_scatter(const Src& src, bool vec, int64_t n, int64_t nbins, int64_t* 
                    int strategy, hipStream_t s) {
     const DeviceInfo& di = device_info();
     unsigned long long* dst = reinterpret_cast<unsigned long long*>(dst_i64);
-    const size_t lds_cap = std::min(di.lds_per_block, (size_t)160 * 1024);
+    // at most 128 KiB of the 160 KiB LDS per workgroup (power-of-two bin ranges)
+    const size_t lds_cap = std::min(di.lds_per_block, (size_t)128 * 1024);
     const int64_t per_block_elems = (int64_t)kBlock * 8;
     int64_t want_blocks = std::max<int64_t>(1, ceil_div(n, per_block_elems));
 

@@ -236,6 +236,10 @@ extern "C" void hydhip_forget_graphs_for(const void* ptr) {
         const GraphKey& k = it->first;
         if ((const void*)std::get<0>(k) == ptr || (const void*)std::get<1>(k) == ptr ||
             (const void*)std::get<2>(k) == ptr) {
+            // A launch of this graph may still be queued on its stream; the
+            // executable owns the kernel arguments those dispatches read, so
+            // it is destroyed only after the stream has drained.
+            (void)hipStreamSynchronize(std::get<8>(k));
             (void)hipGraphExecDestroy(it->second);
             it = g_graphs.erase(it);
         } else {

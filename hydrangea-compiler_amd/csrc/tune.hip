@@ -1,0 +1,200 @@
+// tune.hip - autotuner for the batched scatter: per (bin class, size class)
+// it times the LDS-private and the global-atomic form of the batched kernel
+// on synthetic instances of that class and keeps the faster one.  This
+// replaces the reference's fixed choices (privatize when the destination has
+// <= 262144 elements, Parallelize.hs:466-503; fixed 32x32 polyhedral tiles,
+// Polyhedral.hs:200-204) with measurements on the device at hand.
+//
+// Table file (text, one class per line: "bin_log2 n_log2 mode"), default
+// $HYD_TUNE_CACHE or $HOME/.cache/hydhip/tune_<arch>.txt.
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "hyd_internal.hpp"
+
+namespace hyd {
+
+int batched_histogram(const int64_t* data, const int64_t* off, const int64_t* len, const int64_t* bins,
+                      const int64_t* hoff, int64_t count, int64_t* hist, int policy, hipStream_t s,
+                      int64_t* out_tiles);
+
+namespace {
+
+std::mutex g_mu;
+std::map<std::pair<int, int>, int> g_table;
+bool g_loaded = false;
+
+int ilog2_ceil(int64_t v) {
+    int r = 0;
+    while (((int64_t)1 << r) < v) ++r;
+    return r;
+}
+int ilog2_floor(int64_t v) {
+    int r = 0;
+    while (v > 1) {
+        v >>= 1;
+        ++r;
+    }
+    return r;
+}
+
+std::string default_path() {
+    const char* e = getenv("HYD_TUNE_CACHE");
+    if (e && *e) return e;
+    const char* home = getenv("HOME");
+    std::string dir = std::string(home ? home : "/tmp") + "/.cache/hydhip";
+    return dir + "/tune_" + device_info().name + ".txt";
+}
+
+void mkdirs(const std::string& path) {
+    for (size_t i = 1; i < path.size(); ++i)
+        if (path[i] == '/') mkdir(path.substr(0, i).c_str(), 0755);
+}
+
+int load_locked(const std::string& path) {
+    FILE* f = fopen(path.c_str(), "r");
+    g_loaded = true;
+    if (!f) return 0;
+    int a, b, m, n = 0;
+    while (fscanf(f, "%d %d %d", &a, &b, &m) == 3) {
+        g_table[{a, b}] = m;
+        ++n;
+    }
+    fclose(f);
+    return n;
+}
+
+}  // namespace
+
+int tuned_batch_mode(int64_t n, int64_t bins) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_loaded) load_locked(default_path());
+    auto it = g_table.find({ilog2_ceil(bins), ilog2_floor(std::max<int64_t>(n, 1))});
+    return it == g_table.end() ? -1 : it->second;
+}
+
+}  // namespace hyd
+
+using namespace hyd;
+
+extern "C" {
+
+// Sets one class by hand (mode 0 = LDS, 1 = global; -1 removes it).
+int hydhip_tune_set(int64_t bins, int64_t n, int mode) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_loaded = true;
+    auto key = std::make_pair(ilog2_ceil(bins), ilog2_floor(std::max<int64_t>(n, 1)));
+    if (mode < 0)
+        g_table.erase(key);
+    else
+        g_table[key] = mode;
+    return HYD_OK;
+}
+
+int hydhip_tune_clear(void) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_table.clear();
+    g_loaded = true;
+    return HYD_OK;
+}
+
+static int tune_impl(const char* path, void* stream, int* measured);
+
+// Measures every (bins, size) class of the batched workload and stores the
+// table (path NULL -> default).  Returns the number of classes measured, or
+// minus a status code.
+int hydhip_tune_batched(const char* path, void* stream) {
+    int measured = 0;
+    int rc = tune_impl(path, stream, &measured);
+    return rc == HYD_OK ? measured : -rc;
+}
+
+static int tune_impl(const char* path, void* stream, int* measured) {
+    HYD_TRY(ensure_init());
+    hipStream_t s = resolve_stream(stream);
+    const int64_t bin_classes[] = {64, 256, 4096};
+    const int n_logs[] = {10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20};
+    const int64_t total = (int64_t)1 << 23;  // elements per measurement batch
+    int64_t* data = nullptr;
+    int64_t* hist = nullptr;
+    if (hipMalloc(&data, total * 8) != hipSuccess)
+        return set_error(HYD_E_OOM, "tune: allocation failed");
+    HYD_TRY(hydhip_fill_splitmix_i64(data, total, 0x74756e65ULL, 0, 33, s));
+    hipEvent_t e0, e1;
+    HYD_TRY_HIP(hipEventCreate(&e0));
+    HYD_TRY_HIP(hipEventCreate(&e1));
+    std::map<std::pair<int, int>, int> found;
+    for (int64_t nb : bin_classes) {
+        for (int nl : n_logs) {
+            int64_t n = (int64_t)1 << nl;
+            int64_t count = std::max<int64_t>(1, total / n);
+            std::vector<int64_t> off(count), len(count, n), bins(count, nb), hoff(count);
+            for (int64_t k = 0; k < count; ++k) {
+                off[k] = k * n;
+                hoff[k] = k * nb;
+            }
+            if (!hist) {
+                if (hipMalloc(&hist, (size_t)(total / 1024 + 1) * 4096 * 8) != hipSuccess) {
+                    (void)hipFree(data);
+                    return set_error(HYD_E_OOM, "tune: allocation failed");
+                }
+            }
+            float best[2] = {1e30f, 1e30f};
+            for (int mode = 0; mode < 2; ++mode) {
+                for (int rep = 0; rep < 4; ++rep) {
+                    HYD_TRY_HIP(hipEventRecord(e0, s));
+                    HYD_TRY(batched_histogram(data, off.data(), len.data(), bins.data(), hoff.data(),
+                                              count, hist, mode == 0 ? 1 : 2, s, nullptr));
+                    HYD_TRY_HIP(hipEventRecord(e1, s));
+                    HYD_TRY_HIP(hipEventSynchronize(e1));
+                    float ms = 0;
+                    HYD_TRY_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    if (rep > 0 && ms < best[mode]) best[mode] = ms;
+                }
+            }
+            found[{ilog2_ceil(nb), nl}] = best[0] <= best[1] ? 0 : 1;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(data);
+    (void)hipFree(hist);
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (auto& kv : found) g_table[kv.first] = kv.second;
+    g_loaded = true;
+    std::string p = path ? std::string(path) : default_path();
+    mkdirs(p);
+    FILE* f = fopen(p.c_str(), "w");
+    if (f) {
+        for (auto& kv : g_table) fprintf(f, "%d %d %d\n", kv.first.first, kv.first.second, kv.second);
+        fclose(f);
+    }
+    *measured = (int)found.size();
+    return HYD_OK;
+}
+
+// Copies the table into out[3*i..] (bin_log2, n_log2, mode); returns entries.
+int hydhip_tune_table(int* out, int max_entries) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_loaded) load_locked(default_path());
+    int i = 0;
+    for (auto& kv : g_table) {
+        if (i >= max_entries) break;
+        out[3 * i] = kv.first.first;
+        out[3 * i + 1] = kv.first.second;
+        out[3 * i + 2] = kv.second;
+        ++i;
+    }
+    return i;
+}
+
+}  // extern "C"

@@ -108,6 +108,8 @@ _SIGS = {
     "hydhip_scatter_add_i64": (_INT, [_P, _P, _I64, _P, _I64, _INT, _P]),
     "hydhip_dot_f64": (_INT, [_P, _P, _I64, _D, _P, _P]),
     "hydhip_map_chain_f64": (_INT, [_P, _P, _I64, _P, _P]),
+    "hydhip_math_f64": (_INT, [_INT, _P, _I64, _P, _P]),
+    "hydhip_host_exp": (None, [_P, _I64, _P]),
     "hydhip_stencil2d_clamp_f64": (_INT, [_P, _I64, _I64, _INT, _D, _P, _P]),
     "hydhip_stencil2d_iterate_f64": (_INT, [_P, _I64, _I64, _INT, _D, _I64, _P, _P, _INT,
                                             ctypes.POINTER(_P), _P]),
@@ -120,6 +122,11 @@ _SIGS = {
     "hydhip_csr_from_sorted_coo_i64": (_INT, [_I64, _I64, _P, _P, _P, _I64, _P, _P, _P, _P]),
     "hydhip_coo_to_csr_i64": (_INT, [_I64, _I64, _I64, _P, _P, _P, _P, _P, _P,
                                      ctypes.POINTER(_I64), _P]),
+    "hydhip_batched_histogram_i64": (_INT, [_P, _P, _P, _P, _P, _I64, _P, _INT, _P]),
+    "hydhip_tune_batched": (_INT, [ctypes.c_char_p, _P]),
+    "hydhip_tune_set": (_INT, [_I64, _I64, _INT]),
+    "hydhip_tune_clear": (_INT, []),
+    "hydhip_tune_table": (_INT, [ctypes.POINTER(_INT), _INT]),
     # array ABI
     "hyd_shape_size": (_I64, [HydTuple]),
     "hyd_array_alloc_bytes": (_PA, [HydTuple, ctypes.c_size_t]),
@@ -135,6 +142,7 @@ _SIGS = {
     "hyd_array_is_device_resident": (_INT, [_PA]),
     "hyd_hip_set_host_sync": (None, [_INT]),
     "hyd_hip_get_host_sync": (_INT, []),
+    "hyd_hip_set_stream": (None, [_P]),
     "hyd_sort_indices": (_PA, [_PA]),
     "hyd_export_weighted_histogram": (_PA, [_PA]),
     "hyd_export_weighted_histogram_bins": (_PA, [_PA, _I64]),
@@ -153,6 +161,20 @@ def library_path():
     return LIB_PATH
 
 
+class _TypedLib:
+    """Only functions with a declared signature are reachable: ctypes passes an
+    undeclared integer argument as a 32-bit int, which would truncate device
+    pointers."""
+
+    def __init__(self, cdll):
+        self._cdll = cdll
+
+    def __getattr__(self, name):
+        if name not in _SIGS:
+            raise AttributeError("libhydhip function %s has no declared ctypes signature" % name)
+        return getattr(self._cdll, name)
+
+
 def lib():
     """The loaded libhydhip.so (raises if it was not built)."""
     global _lib
@@ -164,7 +186,7 @@ def lib():
             fn = getattr(l, name)
             fn.restype = res
             fn.argtypes = args
-        _lib = l
+        _lib = _TypedLib(l)
     return _lib
 
 
@@ -352,6 +374,48 @@ def coo_to_csr(nrows, ncols, rows, cols, vals, nnz=None, stream=None):
     _check(lib().hydhip_coo_to_csr_i64(nrows, ncols, nnz, rows.ptr, cols.ptr, vals.ptr, rp.ptr, ci.ptr,
                                        cv.ptr, ctypes.byref(cnt), stream))
     return rp, ci, cv, cnt.value
+
+
+BATCH_TUNED = 0
+BATCH_LDS = 1
+BATCH_GLOBAL = 2
+
+
+def batch_offsets(lens, bins):
+    """Packed layout of a batch: element offsets and bin offsets."""
+    lens = np.ascontiguousarray(lens, np.int64)
+    bins = np.ascontiguousarray(bins, np.int64)
+    off = np.zeros(lens.size, np.int64)
+    hoff = np.zeros(bins.size, np.int64)
+    if lens.size > 1:
+        off[1:] = np.cumsum(lens)[:-1]
+        hoff[1:] = np.cumsum(bins)[:-1]
+    return off, hoff
+
+
+def batched_histogram(data, off, lens, bins, hoff, hist=None, policy=BATCH_TUNED, stream=None):
+    """Instance k: hist[hoff[k]:hoff[k]+bins[k]] = histogram of
+    data[off[k]:off[k]+lens[k]].  Descriptor arrays are host numpy arrays."""
+    off, lens, bins, hoff = (np.ascontiguousarray(a, np.int64) for a in (off, lens, bins, hoff))
+    if hist is None:
+        hist = DeviceArray((int((hoff + bins).max()) if bins.size else 0,), np.int64)
+    _check(lib().hydhip_batched_histogram_i64(data.ptr, off.ctypes.data, lens.ctypes.data,
+                                              bins.ctypes.data, hoff.ctypes.data, lens.size, hist.ptr,
+                                              policy, stream))
+    return hist
+
+
+def tune_batched(path=None, stream=None):
+    rc = lib().hydhip_tune_batched(path.encode() if path else None, stream)
+    if rc < 0:
+        _check(-rc)
+    return rc
+
+
+def tune_table():
+    buf = (_INT * (3 * 256))()
+    n = lib().hydhip_tune_table(buf, 256)
+    return [(buf[3 * i], buf[3 * i + 1], buf[3 * i + 2]) for i in range(n)]
 
 
 # ---- events and streams ---------------------------------------------------------

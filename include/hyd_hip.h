@@ -125,6 +125,26 @@ int hydhip_stencil2d_iterate_f64(const double* init, int64_t h, int64_t w,
                                  double* buf0, double* buf1, int use_graph,
                                  double** result, void* stream);
 
+/* ---- batched scatter ------------------------------------------------------
+ * Many weighted-histogram instances in one persistent launch.  Instance k
+ * reads data[off[k] .. off[k]+len[k]) and writes hist[hoff[k] ..
+ * hoff[k]+bins[k]) (zeroed first): hist[x % bins] += x*x.  The descriptor
+ * arrays off/len/bins/hoff are HOST memory (the plan is built on the host).
+ * policy: 0 = autotuned per size class, 1 = LDS bins wherever they fit,
+ * 2 = global atomics everywhere. */
+int hydhip_batched_histogram_i64(const int64_t* data, const int64_t* off,
+                                 const int64_t* len, const int64_t* bins,
+                                 const int64_t* hoff, int64_t count, int64_t* hist,
+                                 int policy, void* stream);
+/* Autotuner of the batched scatter: measures LDS vs global atomics for each
+ * (bin class, size class) and stores the table at `path` (NULL: the
+ * HYD_TUNE_CACHE variable or ~/.cache/hydhip/tune_<arch>.txt).  Returns the
+ * number of classes measured, or a negative status. */
+int hydhip_tune_batched(const char* path, void* stream);
+int hydhip_tune_set(int64_t bins, int64_t n, int mode);
+int hydhip_tune_clear(void);
+int hydhip_tune_table(int* out, int max_entries);
+
 /* ---- scans, sort, gather, COO -> CSR (int64) ---------------------------- */
 /* The scan family of src/Language/Hydrangea/Lowering.hs:3563-3607 with (+):
  * exclusive: out[k] = init + sum_{j<k} a[j]; inclusive adds a[k] too;

@@ -1,0 +1,72 @@
+"""GPU parity of the batched scatter (many histogram instances in one
+persistent launch) against the oracle, for every policy, plus the autotuner."""
+import numpy as np
+import pytest
+
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def make_batch(orc, count, lo_log, hi_log, bins_set, seed):
+    lens, bins = workloads.batch_instances(count=count, seed=seed, lo_log=lo_log, hi_log=hi_log,
+                                           bins=bins_set)
+    total = int(lens.sum())
+    data = orc.splitmix_i64(total, seed + 1, bits=31)
+    if total > 20:
+        data[::17] = -data[::17]  # negative routes are dropped
+    return data, lens, bins
+
+
+@pytest.mark.parametrize("policy", [0, 1, 2])
+@pytest.mark.parametrize("bins_set", [(64, 256, 4096, 65536), (1, 7, 100, 5000)])
+def test_batched_matches_oracle(hyd, orc, policy, bins_set):
+    print("batched policy=%d bins=%s" % (policy, bins_set), flush=True)
+    data, lens, bins = make_batch(orc, 97, 0, 15, bins_set, 1000 + policy)
+    off, hoff = hyd.batch_offsets(lens, bins)
+    d = hyd.DeviceArray.from_numpy(data)
+    got = hyd.batched_histogram(d, off, lens, bins, hoff, policy=policy).numpy()
+    ref, rhoff = orc.batched_histograms(data, off, lens, bins)
+    assert np.array_equal(rhoff, hoff)
+    assert np.array_equal(got, ref)
+
+
+def test_batched_matches_single_calls(hyd, orc):
+    data, lens, bins = make_batch(orc, 40, 8, 17, (64, 256, 4096, 65536), 77)
+    off, hoff = hyd.batch_offsets(lens, bins)
+    d = hyd.DeviceArray.from_numpy(data)
+    got = hyd.batched_histogram(d, off, lens, bins, hoff).numpy()
+    for k in range(lens.size):
+        one = orc.weighted_histogram(data[off[k]:off[k] + lens[k]], int(bins[k]))
+        assert np.array_equal(got[hoff[k]:hoff[k] + bins[k]], one), k
+
+
+def test_batched_empty_and_zero_length(hyd, orc):
+    lens = np.array([0, 5, 0, 1], np.int64)
+    bins = np.array([3, 64, 1, 4096], np.int64)
+    off, hoff = hyd.batch_offsets(lens, bins)
+    data = np.arange(6, dtype=np.int64) + 3
+    got = hyd.batched_histogram(hyd.DeviceArray.from_numpy(data), off, lens, bins, hoff).numpy()
+    ref, _ = orc.batched_histograms(data, off, lens, bins)
+    assert np.array_equal(got, ref)
+
+
+def test_batched_full_workload(hyd, orc):
+    c = workloads.BATCH
+    lens, bins = workloads.batch_instances()
+    off, hoff = hyd.batch_offsets(lens, bins)
+    total = int(lens.sum())
+    d = hyd.splitmix_i64(total, c["seed"], bits=c["bits"])
+    got = hyd.batched_histogram(d, off, lens, bins, hoff).numpy()
+    ref, _ = orc.batched_histograms(d.numpy(), off, lens, bins)
+    assert np.array_equal(got, ref)
+
+
+def test_autotuner_table_roundtrip(hyd, tmp_path):
+    path = str(tmp_path / "tune.txt")
+    n = hyd.tune_batched(path)
+    assert n == 33
+    table = hyd.tune_table()
+    assert len(table) >= 33 and all(m in (0, 1) for _, _, m in table)
+    text = open(path).read().split()
+    assert len(text) == 3 * len(table)

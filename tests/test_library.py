@@ -46,9 +46,14 @@ def test_every_declared_symbol_is_exported():
 
 def test_every_symbol_has_a_ctypes_signature():
     missing = [n for n in declared_functions() if n.startswith(("hydhip_", "hyd_export_", "hyd_hip_"))
-               and n not in hydhip._SIGS and n not in ("hydhip_host_exp", "hydhip_math_f64",
-                                                       "hyd_hip_set_stream")]
+               and n not in hydhip._SIGS]
     assert not missing, missing
+
+
+def test_untyped_calls_are_refused():
+    # an undeclared ctypes call would pass device pointers as 32-bit ints
+    with pytest.raises(AttributeError):
+        hydhip.lib().hyd_tuple_make
 
 
 def test_struct_layout_matches_reference():
@@ -111,8 +116,6 @@ def test_host_exp_matches_libm(orc):
                   2.0**-54, 2.0**-55, 512.0, -512.0, 1023.9, -1074.0]),
     ])
     lib = hydhip.lib()
-    lib.hydhip_host_exp.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
-    lib.hydhip_host_exp.restype = None
     got = np.empty_like(x)
     lib.hydhip_host_exp(x.ctypes.data, x.size, got.ctypes.data)
     ref = orc.libm_exp(x)
