@@ -95,6 +95,28 @@ def bench_dotmap(steps, warmup):
     return res
 
 
+def bench_copy(steps, warmup):
+    """Plain streaming copy of 2^27 fp64 (1 GiB read + 1 GiB write): the
+    bandwidth the fused kernels are compared with."""
+    n = 1 << 27
+    a = hydhip.splitmix_f64(n, 7)
+    b = hydhip.DeviceArray((n,), np.float64)
+    lib = hydhip.lib()
+
+    def step(ev):
+        if ev:
+            ev[0].record()
+        hydhip._check(lib.hydhip_copy_f64(a.ptr, n, b.ptr, None))
+        if ev:
+            ev[1].record()
+
+    _, ev = timed_loop(step, steps, warmup, events=True)
+    ms = float(np.mean(ev))
+    a.free()
+    b.free()
+    return dict(ms=ms, bytes=16 * n, gbs=16 * n / ms / 1e6)
+
+
 def bench_hist(steps, warmup):
     c = workloads.HIST
     d = hydhip.splitmix_i64(c["n"], c["seed"], bits=c["bits"])
@@ -225,6 +247,9 @@ def main():
     dm = bench_dotmap(args.steps, args.warmup)
     kernels["dot"] = dm["dot"]
     kernels["map_chain"] = dm["map_chain"]
+    if not only or "copy" in only:
+        log("copy reference ...")
+        kernels["copy_reference"] = bench_copy(args.steps, args.warmup)
     if not only or "hist" in only:
         log("histogram ...")
         kernels["weighted_histogram"] = bench_hist(args.steps, args.warmup)

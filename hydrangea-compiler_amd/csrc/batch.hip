@@ -3,18 +3,23 @@
 //
 // Each instance k is the program of programs/weighted_histogram.hyd with its
 // own bin count: hist_k[x % bins_k] += x*x over data_k (Parallelize.hs:466-503
-// would privatize every one of them on the CPU).  On the GPU one launch
-// serves the whole batch: the host cuts the instances into tiles (largest
-// first), the persistent workgroups pull tiles from an atomic queue counter
-// and run each tile either
-//   * LDS mode:    private bins in LDS (several copies for small bin counts),
-//                  one global atomic per touched bin at the end of the tile, or
-//   * GLOBAL mode: one 64-bit global atomic per element (bins beyond LDS, or
-//                  too few elements per bin to pay for zeroing and merging).
-// The choice per (bin class, size class) comes from the autotuner table
-// (tune.cpp) or, for classes it has not measured, a size rule.
+// privatizes every one of them on the CPU).  On the GPU the host cuts the
+// batch into tiles, largest first, and persistent workgroups pull them from an
+// atomic queue counter.  A tile is (instance, element range, bin range, mode):
+//   * OWNER:   the workgroup holds the tile's bin range in LDS (several copies
+//              for small bin counts), streams every element of the instance and
+//              writes the bin range with plain stores - no atomics, no zeroing.
+//              Instances with more bins than LDS holds get one tile per
+//              16384-bin range (each streams the instance once).
+//   * CHUNK:   like OWNER for one element chunk of a long instance; the chunks
+//              merge their LDS bins into the (pre-zeroed) output with one
+//              global atomic per touched bin.
+//   * GLOBAL:  one 64-bit global atomic per element into the pre-zeroed output.
+// Global atomics on gfx950 run at tens of G/s, so OWNER is preferred; the
+// autotuner (tune.hip) can still pick GLOBAL per (bin class, size class).
 // Integer + is order-free mod 2^64, so every schedule gives the oracle's bits.
 #include <stdio.h>
+#include <string.h>
 
 #include <algorithm>
 #include <mutex>
@@ -24,20 +29,21 @@
 
 namespace hyd {
 
-// Autotuner interface (tune.cpp).
-int tuned_batch_mode(int64_t n, int64_t bins);  // -1 if the class is untuned
+// Autotuner interface (tune.hip): 0 = LDS tiles, 1 = global atomics, -1 untuned.
+int tuned_batch_mode(int64_t n, int64_t bins);
 
 namespace {
 
-constexpr int kBlock = 512;
-constexpr int kLdsWords = 4096 + 1;           // LDS budget per workgroup (u64 words)
-constexpr int64_t kMaxLdsBins = 4096;
+enum TileMode : int32_t { kModeOwner = 0, kModeChunk = 1, kModeGlobal = 2 };
 
 struct Tile {
     int64_t inst;
-    int64_t start;
+    int64_t start;  // first element (relative to the instance)
     int64_t len;
-    int64_t mode;  // 0 = LDS, 1 = global atomics
+    int32_t lo;     // first bin of the range
+    int32_t width;  // bins in the range
+    int32_t mode;
+    int32_t pad;
 };
 
 struct InstDesc {
@@ -46,6 +52,14 @@ struct InstDesc {
     int64_t hoff;  // first bin in hist
     int64_t pad;
 };
+
+// Two launch shapes: bins <= 4096 (32 KiB LDS, 4 workgroups per CU) and bin
+// ranges of 16384 (128 KiB LDS, one 1024-thread workgroup per CU).
+constexpr int kSmallBlock = 512;
+constexpr int kSmallWords = 4097;
+constexpr int kLargeBlock = 1024;
+constexpr int kLargeWords = 16384;
+constexpr int64_t kChunk = (int64_t)1 << 18;  // elements per tile of a long instance
 
 __device__ __forceinline__ void lds_add(unsigned long long* p, uint64_t v) {
     __hip_atomic_fetch_add(p, (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -56,7 +70,7 @@ __device__ __forceinline__ void global_add(unsigned long long* p, uint64_t v) {
 
 // Calls add(x) for every element of p[0..len) handled by this workgroup's
 // threads (16-byte loads when aligned, 4 pairs in flight per thread).
-template <class Add>
+template <int BLOCK, class Add>
 __device__ __forceinline__ void tile_elements(const int64_t* __restrict__ p, int64_t len, Add add) {
     if (len <= 0) return;
     if (reinterpret_cast<uintptr_t>(p) & 15) {
@@ -67,12 +81,12 @@ __device__ __forceinline__ void tile_elements(const int64_t* __restrict__ p, int
     const longlong2* p2 = reinterpret_cast<const longlong2*>(p);
     const int64_t npairs = len >> 1;
     int64_t q = threadIdx.x;
-    for (; q + 3 * kBlock < npairs; q += 4 * kBlock) {
-        longlong2 a = p2[q], b = p2[q + kBlock], c = p2[q + 2 * kBlock], d = p2[q + 3 * kBlock];
+    for (; q + 3 * BLOCK < npairs; q += 4 * BLOCK) {
+        longlong2 a = p2[q], b = p2[q + BLOCK], c = p2[q + 2 * BLOCK], d = p2[q + 3 * BLOCK];
         add(a.x); add(a.y); add(b.x); add(b.y);
         add(c.x); add(c.y); add(d.x); add(d.y);
     }
-    for (; q < npairs; q += kBlock) {
+    for (; q < npairs; q += BLOCK) {
         longlong2 a = p2[q];
         add(a.x);
         add(a.y);
@@ -80,13 +94,14 @@ __device__ __forceinline__ void tile_elements(const int64_t* __restrict__ p, int
     if ((len & 1) && threadIdx.x == 0) add(p[len - 1]);
 }
 
-__global__ __launch_bounds__(kBlock) void batched_hist_kernel(const int64_t* __restrict__ data,
-                                                              const InstDesc* __restrict__ inst,
-                                                              const Tile* __restrict__ tiles,
-                                                              int64_t ntiles,
-                                                              unsigned int* __restrict__ counter,
-                                                              unsigned long long* __restrict__ hist) {
-    __shared__ unsigned long long lds[kLdsWords];
+template <int BLOCK, int WORDS>
+__global__ __launch_bounds__(BLOCK) void batched_hist_kernel(const int64_t* __restrict__ data,
+                                                             const InstDesc* __restrict__ inst,
+                                                             const Tile* __restrict__ tiles,
+                                                             int64_t ntiles,
+                                                             unsigned int* __restrict__ counter,
+                                                             unsigned long long* __restrict__ hist) {
+    extern __shared__ unsigned long long lds[];
     __shared__ int64_t cur;
     for (;;) {
         if (threadIdx.x == 0) cur = (int64_t)atomicAdd(counter, 1u);
@@ -99,103 +114,166 @@ __global__ __launch_bounds__(kBlock) void batched_hist_kernel(const int64_t* __r
         const int64_t nb = d.bins;
         unsigned long long* h = hist + d.hoff;
         const int64_t* src = data + d.off + tile.start;
-        if (tile.mode == 0) {
-            const int stride = (int)(nb | 1);
-            const int copies = max(1, min(16, kLdsWords / stride));
-            const int words = copies * stride;
-            for (int i = threadIdx.x; i < words; i += kBlock) lds[i] = 0;
-            __syncthreads();
-            unsigned long long* mine = lds + (threadIdx.x % copies) * stride;
-            tile_elements(src, tile.len, [&](int64_t x) {
-                int64_t b = x % nb;
-                if ((uint64_t)b < (uint64_t)nb) lds_add(&mine[b], (uint64_t)x * (uint64_t)x);
-            });
-            __syncthreads();
-            for (int b = threadIdx.x; b < nb; b += kBlock) {
-                unsigned long long s = 0;
-                for (int c = 0; c < copies; ++c) s += lds[c * stride + b];
-                if (s) global_add(&h[b], s);
-            }
-            __syncthreads();  // LDS reused by the next tile
-        } else {
-            tile_elements(src, tile.len, [&](int64_t x) {
+        if (tile.mode == kModeGlobal) {
+            tile_elements<BLOCK>(src, tile.len, [&](int64_t x) {
                 int64_t b = x % nb;
                 if ((uint64_t)b < (uint64_t)nb) global_add(&h[b], (uint64_t)x * (uint64_t)x);
             });
+            continue;
         }
+        const int64_t lo = tile.lo;
+        const int width = tile.width;
+        // copies of the range at an odd stride (a bin's copies in different
+        // banks); a range that fills LDS gets one copy at stride = width
+        // (the planner keeps width <= WORDS)
+        int stride = width | 1;
+        int copies = WORDS / stride;
+        if (copies < 1) {
+            copies = 1;
+            stride = width;
+        } else if (copies > 16) {
+            copies = 16;
+        }
+        const int words = copies * stride;
+        for (int i = threadIdx.x; i < words; i += BLOCK) lds[i] = 0;
+        __syncthreads();
+        unsigned long long* mine = lds + (threadIdx.x % copies) * stride;
+        tile_elements<BLOCK>(src, tile.len, [&](int64_t x) {
+            int64_t b = x % nb - lo;
+            if ((uint64_t)b < (uint64_t)width) lds_add(&mine[b], (uint64_t)x * (uint64_t)x);
+        });
+        __syncthreads();
+        for (int b = threadIdx.x; b < width; b += BLOCK) {
+            unsigned long long s = 0;
+            for (int c = 0; c < copies; ++c) s += lds[c * stride + b];
+            if (tile.mode == kModeOwner)
+                h[lo + b] = s;
+            else if (s)
+                global_add(&h[lo + b], s);
+        }
+        __syncthreads();  // LDS reused by the next tile
     }
 }
 
-// Size rule for classes the autotuner has not measured: LDS when the bins fit
-// and there are >= 8 elements per bin.
-int rule_mode(int64_t n, int64_t bins) { return (bins <= kMaxLdsBins && n >= 8 * bins) ? 0 : 1; }
-
-int64_t tile_len_for(int mode, int64_t bins) {
-    if (mode == 0) return std::max<int64_t>(16384, 8 * bins);
-    return 16384;
+// Zeroes hist ranges of the instances whose tiles accumulate.
+__global__ void zero_ranges_kernel(const int64_t* __restrict__ ranges, int64_t nranges,
+                                   unsigned long long* __restrict__ hist) {
+    for (int64_t r = blockIdx.x; r < nranges; r += gridDim.x) {
+        const int64_t off = ranges[2 * r], len = ranges[2 * r + 1];
+        for (int64_t i = threadIdx.x; i < len; i += blockDim.x) hist[off + i] = 0;
+    }
 }
+
+// Size rule for classes the autotuner has not measured: LDS tiles unless the
+// instance is so small next to its bins that zeroing them costs more than
+// the per-element atomics.
+int rule_mode(int64_t n, int64_t bins) { return n * 8 >= bins ? 0 : 1; }
 
 std::mutex g_plan_mu;
 
 }  // namespace
 
-// Host-side plan + launch.  Descriptor arrays are host memory; the plan is
-// uploaded into the stream workspace together with the queue counter.
 int batched_histogram(const int64_t* data, const int64_t* off, const int64_t* len,
                       const int64_t* bins, const int64_t* hoff, int64_t count, int64_t* hist,
                       int policy, hipStream_t s, int64_t* out_tiles) {
     std::vector<InstDesc> inst((size_t)count);
-    std::vector<Tile> tiles;
-    int64_t total_bins = 0;
+    std::vector<Tile> small, large;
+    std::vector<int64_t> zero;  // (hoff, bins) pairs
     for (int64_t k = 0; k < count; ++k) {
-        if (len[k] < 0 || bins[k] < 1 || off[k] < 0 || hoff[k] < 0)
+        if (len[k] < 0 || bins[k] < 1 || bins[k] > ((int64_t)1 << 31) || off[k] < 0 || hoff[k] < 0)
             return set_error(HYD_E_INVALID, "batched_histogram: bad descriptor %lld", (long long)k);
-        inst[k] = InstDesc{off[k], bins[k], hoff[k], 0};
-        total_bins = std::max(total_bins, hoff[k] + bins[k]);
+        const int64_t n = len[k], nb = bins[k];
+        inst[k] = InstDesc{off[k], nb, hoff[k], 0};
         int mode;
         if (policy == 1)
-            mode = bins[k] <= kMaxLdsBins ? 0 : 1;
+            mode = 0;
         else if (policy == 2)
             mode = 1;
         else {
-            mode = tuned_batch_mode(len[k], bins[k]);
-            if (mode < 0) mode = rule_mode(len[k], bins[k]);
-            if (bins[k] > kMaxLdsBins) mode = 1;
+            mode = tuned_batch_mode(n, nb);
+            if (mode < 0) mode = rule_mode(n, nb);
         }
-        int64_t tl = tile_len_for(mode, bins[k]);
-        for (int64_t st = 0; st < len[k]; st += tl)
-            tiles.push_back(Tile{k, st, std::min(tl, len[k] - st), mode});
+        if (mode == 1) {
+            zero.push_back(hoff[k]);
+            zero.push_back(nb);
+            for (int64_t st = 0; st < n; st += kChunk)
+                small.push_back(Tile{k, st, std::min(kChunk, n - st), 0, 0, kModeGlobal, 0});
+            continue;
+        }
+        const bool fits_small = nb <= kSmallWords - 1;
+        const int64_t range = fits_small ? nb : kLargeWords;
+        const bool chunked = n > kChunk;
+        if (chunked) {
+            zero.push_back(hoff[k]);
+            zero.push_back(nb);
+        }
+        std::vector<Tile>& dst = fits_small ? small : large;
+        for (int64_t lo = 0; lo < nb; lo += range) {
+            int32_t w = (int32_t)std::min(range, nb - lo);
+            if (!chunked) {
+                dst.push_back(Tile{k, 0, n, (int32_t)lo, w, kModeOwner, 0});
+            } else {
+                for (int64_t st = 0; st < n; st += kChunk)
+                    dst.push_back(Tile{k, st, std::min(kChunk, n - st), (int32_t)lo, w, kModeChunk, 0});
+            }
+        }
     }
-    // largest tiles first (longest-processing-time order for the queue)
-    std::stable_sort(tiles.begin(), tiles.end(), [](const Tile& a, const Tile& b) {
-        return a.len > b.len;
-    });
-    const int64_t ntiles = (int64_t)tiles.size();
-    if (out_tiles) *out_tiles = ntiles;
-    // hist: every instance's bins start at zero (fill [bins] 0)
-    HYD_TRY(fill_zero(hist, (size_t)total_bins * 8, s));
-    if (ntiles == 0) return HYD_OK;
-    size_t inst_bytes = inst.size() * sizeof(InstDesc);
-    size_t tile_bytes = tiles.size() * sizeof(Tile);
-    size_t bytes = 256 + inst_bytes + tile_bytes;
+    // largest first (longest-processing-time order for the queue)
+    auto by_len = [](const Tile& a, const Tile& b) { return a.len > b.len; };
+    std::stable_sort(small.begin(), small.end(), by_len);
+    std::stable_sort(large.begin(), large.end(), by_len);
+    if (out_tiles) *out_tiles = (int64_t)(small.size() + large.size());
+
+    const size_t inst_bytes = inst.size() * sizeof(InstDesc);
+    const size_t small_bytes = small.size() * sizeof(Tile);
+    const size_t large_bytes = large.size() * sizeof(Tile);
+    const size_t zero_bytes = zero.size() * sizeof(int64_t);
+    const size_t bytes = 256 + inst_bytes + small_bytes + large_bytes + zero_bytes;
     unsigned char* ws = static_cast<unsigned char*>(stream_workspace(s, bytes));
     if (!ws) return set_error(HYD_E_OOM, "batched_histogram: workspace allocation failed");
-    unsigned int* counter = reinterpret_cast<unsigned int*>(ws);
+    unsigned int* counters = reinterpret_cast<unsigned int*>(ws);
     InstDesc* d_inst = reinterpret_cast<InstDesc*>(ws + 256);
-    Tile* d_tiles = reinterpret_cast<Tile*>(ws + 256 + inst_bytes);
+    Tile* d_small = reinterpret_cast<Tile*>(ws + 256 + inst_bytes);
+    Tile* d_large = reinterpret_cast<Tile*>(ws + 256 + inst_bytes + small_bytes);
+    int64_t* d_zero = reinterpret_cast<int64_t*>(ws + 256 + inst_bytes + small_bytes + large_bytes);
     {
+        // one staging buffer, one copy
         std::lock_guard<std::mutex> lk(g_plan_mu);
-        HYD_TRY_HIP(hipMemcpyAsync(d_inst, inst.data(), inst_bytes, hipMemcpyHostToDevice, s));
-        HYD_TRY_HIP(hipMemcpyAsync(d_tiles, tiles.data(), tile_bytes, hipMemcpyHostToDevice, s));
-        // pageable sources: make sure the copies have consumed the vectors
-        HYD_TRY_HIP(hipStreamSynchronize(s));
+        std::vector<unsigned char> stage(bytes - 256);
+        unsigned char* q = stage.data();
+        memcpy(q, inst.data(), inst_bytes);
+        memcpy(q + inst_bytes, small.data(), small_bytes);
+        memcpy(q + inst_bytes + small_bytes, large.data(), large_bytes);
+        memcpy(q + inst_bytes + small_bytes + large_bytes, zero.data(), zero_bytes);
+        HYD_TRY_HIP(hipMemcpyAsync(ws + 256, stage.data(), stage.size(), hipMemcpyHostToDevice, s));
+        HYD_TRY_HIP(hipStreamSynchronize(s));  // pageable source: consumed before it goes away
     }
-    HYD_TRY(fill_zero(counter, 256, s));
+    HYD_TRY(fill_zero(counters, 256, s));
+    unsigned long long* h = reinterpret_cast<unsigned long long*>(hist);
     const DeviceInfo& di = device_info();
-    int64_t blocks = std::min<int64_t>(ntiles, (int64_t)di.num_cus * 4);
-    batched_hist_kernel<<<(unsigned)blocks, kBlock, 0, s>>>(
-        data, d_inst, d_tiles, ntiles, counter, reinterpret_cast<unsigned long long*>(hist));
-    HYD_CHECK_LAUNCH();
+    if (!zero.empty()) {
+        int64_t nr = (int64_t)zero.size() / 2;
+        zero_ranges_kernel<<<(unsigned)std::min<int64_t>(nr, 4096), 256, 0, s>>>(d_zero, nr, h);
+        HYD_CHECK_LAUNCH();
+    }
+    if (!small.empty()) {
+        auto k = batched_hist_kernel<kSmallBlock, kSmallWords>;
+        const int lds = kSmallWords * 8;
+        HYD_TRY_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+        int64_t blocks = std::min<int64_t>((int64_t)small.size(), (int64_t)di.num_cus * 4);
+        k<<<(unsigned)blocks, kSmallBlock, lds, s>>>(data, d_inst, d_small, (int64_t)small.size(),
+                                                     counters, h);
+        HYD_CHECK_LAUNCH();
+    }
+    if (!large.empty()) {
+        auto k = batched_hist_kernel<kLargeBlock, kLargeWords>;
+        const int lds = kLargeWords * 8;
+        HYD_TRY_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+        int64_t blocks = std::min<int64_t>((int64_t)large.size(), (int64_t)di.num_cus);
+        k<<<(unsigned)blocks, kLargeBlock, lds, s>>>(data, d_inst, d_large, (int64_t)large.size(),
+                                                     counters + 1, h);
+        HYD_CHECK_LAUNCH();
+    }
     return HYD_OK;
 }
 

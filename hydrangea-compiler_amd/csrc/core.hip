@@ -84,6 +84,24 @@ void* stream_workspace(hipStream_t stream, size_t bytes) {
     return p;
 }
 
+static std::unordered_map<hipStream_t, void*> g_zero;
+
+void* stream_zero_scratch(hipStream_t stream) {
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    auto it = g_zero.find(stream);
+    if (it != g_zero.end()) return it->second;
+    void* p = nullptr;
+    if (hipMalloc(&p, kZeroScratchBytes) != hipSuccess) return nullptr;
+    // zeroed once; every user leaves it zero again
+    if (fill_zero(p, kZeroScratchBytes, stream) != HYD_OK ||
+        hipStreamSynchronize(stream) != hipSuccess) {
+        (void)hipFree(p);
+        return nullptr;
+    }
+    g_zero[stream] = p;
+    return p;
+}
+
 static void* g_scalar = nullptr;
 
 void* scalar_slot() {
@@ -100,6 +118,8 @@ static void release_workspaces() {
     for (auto& kv : g_ws)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     g_ws.clear();
+    for (auto& kv : g_zero) (void)hipFree(kv.second);
+    g_zero.clear();
     if (g_scalar) (void)hipFree(g_scalar);
     g_scalar = nullptr;
 }
@@ -247,6 +267,11 @@ int hydhip_stream_destroy(void* stream) {
         if (it != g_ws.end()) {
             if (it->second.ptr) (void)hipFree(it->second.ptr);
             g_ws.erase(it);
+        }
+        auto z = g_zero.find(s);
+        if (z != g_zero.end()) {
+            (void)hipFree(z->second);
+            g_zero.erase(z);
         }
     }
     HYD_TRY_HIP(hipStreamDestroy(s));

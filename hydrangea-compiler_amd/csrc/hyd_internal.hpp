@@ -39,6 +39,12 @@ void* stream_workspace(hipStream_t stream, size_t bytes);
 // (which, like the reference's, are not reentrant).
 void* scalar_slot();
 
+// A per-stream block of device memory that is all zero between kernels:
+// kernels that use it (the spread merge of the scatters) clear what they
+// touched before they finish.  kZeroScratchBytes long, 256-byte aligned.
+constexpr size_t kZeroScratchBytes = (size_t)1 << 20;
+void* stream_zero_scratch(hipStream_t stream);
+
 // Zero-fill kernel launch (graph-capture safe, unlike hipMemsetAsync).
 int fill_zero(void* dptr, size_t bytes, hipStream_t stream);
 
@@ -72,5 +78,18 @@ __host__ __device__ inline uint64_t splitmix_at(uint64_t seed, uint64_t counter)
 }
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// 16-byte streaming accesses (non-temporal: read or written once).
+typedef double v2d __attribute__((ext_vector_type(2)));
+typedef long long v2i __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ v2d ld_stream2(const double* p) {
+    return __builtin_nontemporal_load(reinterpret_cast<const v2d*>(p));
+}
+__device__ __forceinline__ void st_stream2(double* p, v2d v) {
+    __builtin_nontemporal_store(v, reinterpret_cast<v2d*>(p));
+}
+__device__ __forceinline__ v2i ld_stream2(const int64_t* p) {
+    return __builtin_nontemporal_load(reinterpret_cast<const v2i*>(p));
+}
 
 }  // namespace hyd

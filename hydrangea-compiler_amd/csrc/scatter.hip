@@ -119,11 +119,20 @@ __device__ __forceinline__ void stream_elements(const Src& src, int64_t n, Add a
     }
 }
 
+// LDS-private bins.  Merge: every workgroup adds its bins into spread copy
+// (blockIdx.x % ncopy) of a zeroed scratch block, so each scratch address
+// sees gridDim/ncopy atomics instead of gridDim; the workgroup that finishes
+// last sums the copies into dst (dst = sum, or dst += sum when `accumulate`)
+// and zeroes the copies and the counter again for the next launch.
 template <int BLOCK, bool VEC, class Src>
 __global__ __launch_bounds__(BLOCK) void scatter_lds_kernel(Src src, int64_t n, int nbins,
-                                                            int copies, int stride,
+                                                            int copies, int stride, int ncopy,
+                                                            unsigned long long* __restrict__ spread,
+                                                            unsigned int* __restrict__ counter,
+                                                            int accumulate,
                                                             unsigned long long* __restrict__ dst) {
     extern __shared__ unsigned long long lds[];
+    __shared__ int is_last;
     const int total = copies * stride;
     for (int i = threadIdx.x; i < total; i += BLOCK) lds[i] = 0;
     __syncthreads();
@@ -133,11 +142,34 @@ __global__ __launch_bounds__(BLOCK) void scatter_lds_kernel(Src src, int64_t n, 
         if ((uint64_t)b < ub) lds_add(&mine[b], v);
     });
     __syncthreads();
+    unsigned long long* my_spread = spread + (size_t)(blockIdx.x % ncopy) * nbins;
     for (int b = threadIdx.x; b < nbins; b += BLOCK) {
         unsigned long long s = 0;
         for (int c = 0; c < copies; ++c) s += lds[c * stride + b];
-        if (s) global_add(&dst[b], s);
+        if (s) global_add(&my_spread[b], s);
     }
+    // publish this workgroup's adds, then count it in
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        is_last = prev == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!is_last) return;
+    // acquire: every other workgroup's spread atomics are visible to plain
+    // loads from here on; all loads of a bin are independent (issued together)
+    __threadfence();
+    const int used = (int)min((unsigned)ncopy, gridDim.x);
+    for (int b = threadIdx.x; b < nbins; b += BLOCK) {
+        unsigned long long s = 0;
+        for (int c = 0; c < used; ++c) s += spread[(size_t)c * nbins + b];
+        dst[b] = accumulate ? dst[b] + s : s;
+    }
+    __syncthreads();
+    // leave the scratch zero for the next launch (ordered by the kernel boundary)
+    for (int i = threadIdx.x; i < used * nbins; i += BLOCK) spread[i] = 0;
+    if (threadIdx.x == 0) *counter = 0u;
 }
 
 template <int BLOCK, bool VEC, class Src>
@@ -174,7 +206,7 @@ constexpr size_t kLdsTarget = 64 * 1024;  // per workgroup: leaves room for 2 pe
 
 template <class Src>
 int launch_scatter(const Src& src, bool vec, int64_t n, int64_t nbins, int64_t* dst_i64,
-                   int strategy, hipStream_t s) {
+                   int strategy, bool zero_first, hipStream_t s) {
     const DeviceInfo& di = device_info();
     unsigned long long* dst = reinterpret_cast<unsigned long long*>(dst_i64);
     // at most 128 KiB of the 160 KiB LDS per workgroup (power-of-two bin ranges)
@@ -205,17 +237,35 @@ int launch_scatter(const Src& src, bool vec, int64_t n, int64_t nbins, int64_t* 
         size_t lds = (size_t)copies * stride * 8;
         int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / lds));
         int64_t blocks = std::min<int64_t>(want_blocks, (int64_t)di.num_cus * per_cu);
+        // spread copies in the zeroed scratch: 4096 words' worth, <= 32
+        int ncopy = (int)std::max<int64_t>(1, std::min<int64_t>(32, 4096 / nbins));
+        unsigned char* zs = static_cast<unsigned char*>(stream_zero_scratch(s));
+        if (!zs) return set_error(HYD_E_OOM, "scatter: scratch allocation failed");
+        if ((size_t)ncopy * nbins * 8 > kZeroScratchBytes - 256)
+            return set_error(HYD_E_INVALID, "scatter: %lld bins exceed the merge scratch", (long long)nbins);
+        auto* spread = reinterpret_cast<unsigned long long*>(zs);
+        auto* counter = reinterpret_cast<unsigned int*>(zs + kZeroScratchBytes - 256);
+        int accumulate = zero_first ? 0 : 1;
         if (vec) {
             auto k = scatter_lds_kernel<kBlock, true, Src>;
             HYD_TRY_HIP(hipFuncSetAttribute((const void*)k,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            k<<<(unsigned)blocks, kBlock, lds, s>>>(src, n, (int)nbins, copies, stride, dst);
+            k<<<(unsigned)blocks, kBlock, lds, s>>>(src, n, (int)nbins, copies, stride, ncopy, spread,
+                                                    counter, accumulate, dst);
         } else {
             auto k = scatter_lds_kernel<kBlock, false, Src>;
             HYD_TRY_HIP(hipFuncSetAttribute((const void*)k,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            k<<<(unsigned)blocks, kBlock, lds, s>>>(src, n, (int)nbins, copies, stride, dst);
+            k<<<(unsigned)blocks, kBlock, lds, s>>>(src, n, (int)nbins, copies, stride, ncopy, spread,
+                                                    counter, accumulate, dst);
         }
+        HYD_CHECK_LAUNCH();
+        return HYD_OK;
+    }
+    // the other forms accumulate into dst
+    if (zero_first) HYD_TRY(fill_zero(dst, (size_t)nbins * 8, s));
+    if (strategy == HYD_SCATTER_LDS) {
+        // unreachable (handled above)
     } else if (strategy == HYD_SCATTER_LDS_SPLIT) {
         int range = (int)std::min<int64_t>(nbins, (int64_t)(lds_cap / 8));
         int64_t nranges = ceil_div(nbins, range);
@@ -264,11 +314,11 @@ extern "C" int hydhip_weighted_histogram_i64(const int64_t* data, int64_t n, int
                          "hydhip_weighted_histogram_i64: bad arguments (n=%lld nbins=%lld)",
                          (long long)n, (long long)nbins);
     hipStream_t s = resolve_stream(stream);
-    // fill [nbins] 0: zero the destination (a kernel, so graph capture works).
-    HYD_TRY(fill_zero(hist, (size_t)nbins * 8, s));
-    if (n == 0) return HYD_OK;
+    if (n == 0) return fill_zero(hist, (size_t)nbins * 8, s);  // fill [nbins] 0
     FusedSrc src{data, ModSquare{nbins}};
-    return launch_scatter(src, aligned16(data), n, nbins, hist, strategy, s);
+    // fill [nbins] 0 semantics: the LDS form writes every bin, the others
+    // zero the destination first (with a kernel, so graph capture works)
+    return launch_scatter(src, aligned16(data), n, nbins, hist, strategy, true, s);
 }
 
 extern "C" int hydhip_scatter_add_i64(const int64_t* idx, const int64_t* vals, int64_t n,
@@ -278,6 +328,6 @@ extern "C" int hydhip_scatter_add_i64(const int64_t* idx, const int64_t* vals, i
         return set_error(HYD_E_INVALID, "hydhip_scatter_add_i64: bad arguments");
     if (n == 0) return HYD_OK;
     PairSrc src{idx, vals};
-    return launch_scatter(src, aligned16(idx) && aligned16(vals), n, ndst, dst, strategy,
+    return launch_scatter(src, aligned16(idx) && aligned16(vals), n, ndst, dst, strategy, false,
                           resolve_stream(stream));
 }

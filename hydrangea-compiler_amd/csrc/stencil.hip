@@ -118,8 +118,71 @@ __global__ __launch_bounds__(64 * kWaves) void stencil2d_clamp_kernel(const doub
     }
 }
 
+// Two columns per lane (16-byte loads and stores): a wave covers 128
+// columns; lane l holds columns c0 = 128*bx + 2l and c0+1.  West of c0 is
+// lane l-1's second column, east of c0+1 is lane l+1's first column (the
+// strip edges load theirs); clamping at the grid edge makes a column its own
+// neighbour.  Requires an even width and 16-byte aligned rows.
+constexpr int kRows2 = 32;
+
+template <class Body>
+__global__ __launch_bounds__(64 * kWaves) void stencil2d_clamp_kernel2(const double* __restrict__ in,
+                                                                       int64_t h, int64_t w,
+                                                                       Body body,
+                                                                       double* __restrict__ out) {
+    const int lane = threadIdx.x;
+    const int64_t c0 = (int64_t)blockIdx.x * 128 + 2 * lane;
+    const int64_t r0 = ((int64_t)blockIdx.y * kWaves + threadIdx.y) * kRows2;
+    if (r0 >= h) return;  // wave-uniform
+    const bool active = c0 < w;
+    const int64_t cc = active ? c0 : w - 2;  // inactive lanes mirror the last pair
+    const int64_t r_end = r0 + kRows2 < h ? r0 + kRows2 : h;
+    const bool left_edge = c0 == 0;
+    const bool right_edge = c0 + 2 >= w;
+
+    v2d up = *reinterpret_cast<const v2d*>(in + (r0 > 0 ? r0 - 1 : 0) * w + cc);
+    v2d cur = *reinterpret_cast<const v2d*>(in + r0 * w + cc);
+    for (int64_t r = r0; r < r_end; ++r) {
+        const int64_t rd = r + 1 < h ? r + 1 : h - 1;
+        v2d down = *reinterpret_cast<const v2d*>(in + rd * w + cc);
+        double from_left = __shfl_up(cur.y, 1, 64);
+        double from_right = __shfl_down(cur.x, 1, 64);
+        double west_x, east_y;
+        if (left_edge)
+            west_x = cur.x;
+        else if (lane == 0)
+            west_x = in[r * w + cc - 1];
+        else
+            west_x = from_left;
+        if (right_edge)
+            east_y = cur.y;
+        else if (lane == 63)
+            east_y = in[r * w + cc + 2];
+        else
+            east_y = from_right;
+        v2d v;
+        v.x = body(up.x, down.x, west_x, cur.y, cur.x);
+        v.y = body(up.y, down.y, cur.x, east_y, cur.y);
+        if (active) st_stream2(out + r * w + c0, v);
+        up = cur;
+        cur = down;
+    }
+}
+
 template <class Body>
 int launch_stencil(const double* in, int64_t h, int64_t w, Body body, double* out, hipStream_t s) {
+    const bool pairs = (w % 2 == 0) && ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+    if (pairs) {
+        dim3 block(64, kWaves);
+        int64_t gx = ceil_div(w, 128);
+        int64_t gy = ceil_div(h, (int64_t)kWaves * kRows2);
+        if (gx > 0x7fffffff || gy > 65535)
+            return set_error(HYD_E_INVALID, "stencil: grid %lld x %lld too large", (long long)h,
+                             (long long)w);
+        stencil2d_clamp_kernel2<Body><<<dim3((unsigned)gx, (unsigned)gy), block, 0, s>>>(in, h, w, body, out);
+        HYD_CHECK_LAUNCH();
+        return HYD_OK;
+    }
     dim3 block(64, kWaves);
     int64_t gx = ceil_div(w, 64);
     int64_t gy = ceil_div(h, (int64_t)kWaves * kRows);

@@ -121,7 +121,8 @@ int hydhip_tune_batched(const char* path, void* stream) {
 static int tune_impl(const char* path, void* stream, int* measured) {
     HYD_TRY(ensure_init());
     hipStream_t s = resolve_stream(stream);
-    const int64_t bin_classes[] = {64, 256, 4096};
+    const int64_t bin_classes[] = {64, 256, 4096, 65536};
+    const int64_t hist_words = (int64_t)1 << 26;  // 512 MiB of output per measurement
     const int n_logs[] = {10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20};
     const int64_t total = (int64_t)1 << 23;  // elements per measurement batch
     int64_t* data = nullptr;
@@ -136,14 +137,14 @@ static int tune_impl(const char* path, void* stream, int* measured) {
     for (int64_t nb : bin_classes) {
         for (int nl : n_logs) {
             int64_t n = (int64_t)1 << nl;
-            int64_t count = std::max<int64_t>(1, total / n);
+            int64_t count = std::max<int64_t>(1, std::min<int64_t>(total / n, hist_words / nb));
             std::vector<int64_t> off(count), len(count, n), bins(count, nb), hoff(count);
             for (int64_t k = 0; k < count; ++k) {
                 off[k] = k * n;
                 hoff[k] = k * nb;
             }
             if (!hist) {
-                if (hipMalloc(&hist, (size_t)(total / 1024 + 1) * 4096 * 8) != hipSuccess) {
+                if (hipMalloc(&hist, (size_t)hist_words * 8) != hipSuccess) {
                     (void)hipFree(data);
                     return set_error(HYD_E_OOM, "tune: allocation failed");
                 }

@@ -106,6 +106,9 @@ int hydhip_map_chain_f64(const double* xs, const double* ys, int64_t n,
 int hydhip_math_f64(int op, const double* x, int64_t n, double* out, void* stream);
 /* The same exp on the host CPU (identical operation sequence). */
 void hydhip_host_exp(const double* x, int64_t n, double* out);
+/* Streaming copy dst[i] = src[i] (n even, 16-byte aligned): the bandwidth
+ * reference for the fused kernels. */
+int hydhip_copy_f64(const double* src, int64_t n, double* dst, void* stream);
 
 /* ---- 2-D clamp stencils (fp64, row-major h x w) ------------------------ */
 #define HYD_STENCIL_HEAT 0       /* c + alpha*((((n + s) + w) + e) - 4.0*c) */

@@ -41,6 +41,21 @@ def test_batched_matches_single_calls(hyd, orc):
         assert np.array_equal(got[hoff[k]:hoff[k] + bins[k]], one), k
 
 
+@pytest.mark.parametrize("policy", [0, 1, 2])
+def test_batched_chunks_and_bin_ranges(hyd, orc, policy):
+    # long instances (several element chunks), bin counts beyond one LDS range
+    # (16384) that are not a multiple of it, odd offsets (unaligned starts)
+    lens = np.array([(1 << 18) + 5, 3 << 18, 100, 7, (1 << 18) + 1, 40_001], np.int64)
+    bins = np.array([40_000, 64, 4096, 65536, 65536, 16_385], np.int64)
+    off, hoff = hyd.batch_offsets(lens, bins)
+    data = orc.splitmix_i64(int(lens.sum()), 4242, bits=31)
+    data[::11] = -data[::11]
+    d = hyd.DeviceArray.from_numpy(data)
+    got = hyd.batched_histogram(d, off, lens, bins, hoff, policy=policy).numpy()
+    ref, _ = orc.batched_histograms(data, off, lens, bins)
+    assert np.array_equal(got, ref)
+
+
 def test_batched_empty_and_zero_length(hyd, orc):
     lens = np.array([0, 5, 0, 1], np.int64)
     bins = np.array([3, 64, 1, 4096], np.int64)
@@ -65,8 +80,8 @@ def test_batched_full_workload(hyd, orc):
 def test_autotuner_table_roundtrip(hyd, tmp_path):
     path = str(tmp_path / "tune.txt")
     n = hyd.tune_batched(path)
-    assert n == 33
+    assert n == 44  # 4 bin classes x 11 size classes
     table = hyd.tune_table()
-    assert len(table) >= 33 and all(m in (0, 1) for _, _, m in table)
+    assert len(table) >= 44 and all(m in (0, 1) for _, _, m in table)
     text = open(path).read().split()
     assert len(text) == 3 * len(table)
