@@ -206,6 +206,98 @@ def bench_batched(steps, warmup):
     return res
 
 
+def bench_serve(count=1500, budget=8 << 30, workers=4, load=0.7):
+    """Served mixed stream vs plain per-call launches on the same requests.
+
+    plain: one synchronous hydhip_run_request_plain per request (hipMalloc'd
+    outputs, plain launches, synchronise, hipFree), back to back.
+    served (saturation): every request submitted at t=0; req/s = count / span.
+    served (Poisson): arrivals at `load` x the saturation rate; p50/p99 of
+    done - submit (server clock)."""
+    import servemix
+
+    reqs = workloads.serve_mix(count)
+    pools = servemix.Pools(reqs)
+    hreqs = [pools.request(r) for r in reqs]
+    alg_bytes = sum(workloads.request_bytes_algorithmic(r) for r in reqs)
+
+    # warm both paths (code objects, workspaces)
+    for r in hreqs[:8]:
+        hydhip.free_plain(hydhip.run_plain(r))
+
+    t0 = time.perf_counter()
+    for r in hreqs:
+        hydhip.free_plain(hydhip.run_plain(r))
+    plain_s = time.perf_counter() - t0
+
+    def drain(srv, outstanding, lat, deadline_s, block_ms):
+        got = srv.next_done(block_ms)
+        while got is not None:
+            rid, res = got
+            if res.status != hydhip.HYD_OK:
+                raise hydhip.HydError(res.status, "served request failed")
+            lat.append(res.done_ms - res.submit_ms)
+            srv.release(rid)
+            outstanding.discard(rid)
+            got = srv.next_done(0.0)
+
+    srv = hydhip.Server(budget, workers)
+    try:
+        warm = [srv.submit(r) for r in hreqs[:8]]
+        for rid in warm:
+            srv.wait(rid)
+            srv.release(rid)
+        while srv.next_done(0.0) is not None:
+            pass
+        # saturation: everything at once
+        lat_sat = []
+        outstanding = set()
+        t0 = time.perf_counter()
+        for r in hreqs:
+            outstanding.add(srv.submit(r))
+            drain(srv, outstanding, lat_sat, None, 0.0)
+        while outstanding:
+            drain(srv, outstanding, lat_sat, None, 1000.0)
+        sat_s = time.perf_counter() - t0
+        st_sat = srv.stats()
+        # Poisson arrivals at a fraction of the saturation rate
+        rate = load * count / sat_s
+        rng = np.random.default_rng(workloads.SERVE["seed"] & 0xFFFFFFFF)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+        lat = []
+        outstanding = set()
+        t0 = time.perf_counter()
+        for k, r in enumerate(hreqs):
+            while True:
+                wait_s = arrivals[k] - (time.perf_counter() - t0)
+                if wait_s <= 0:
+                    break
+                drain(srv, outstanding, lat, None, min(wait_s * 1e3, 1.0))
+            outstanding.add(srv.submit(r))
+        while outstanding:
+            drain(srv, outstanding, lat, None, 1000.0)
+        poisson_s = time.perf_counter() - t0
+        st = srv.stats()
+    finally:
+        srv.close()
+        pools.free()
+    kinds = {k: sum(1 for r in reqs if r["kind"] == k) for k in ("stencil", "hist", "dot", "coo")}
+    return dict(
+        requests=count, mix=kinds, hbm_budget=budget, workers=workers,
+        algorithmic_bytes=alg_bytes,
+        plain=dict(seconds=plain_s, req_s=count / plain_s),
+        served_saturation=dict(seconds=sat_s, req_s=count / sat_s, gbs=alg_bytes / sat_s / 1e9,
+                               p50_ms=float(np.percentile(lat_sat, 50)),
+                               p99_ms=float(np.percentile(lat_sat, 99)),
+                               peak_hbm_bytes=st_sat["peak_in_use"], jobs=st_sat["jobs"],
+                               batched_jobs=st_sat["batched_jobs"]),
+        served_poisson=dict(rate_req_s=rate, seconds=poisson_s, req_s=count / poisson_s,
+                            p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99))),
+        speedup_vs_plain=plain_s / sat_s,
+        totals=dict(completed=st["completed"], failed=st["failed"], rejected=st["rejected"]),
+    )
+
+
 def cpu_baseline_dotmap():
     import pyoracle
 
@@ -260,6 +352,10 @@ def main():
     if not only or "batch" in only:
         log("batched histograms ...")
         kernels["batched_histogram"] = bench_batched(args.steps, args.warmup)
+
+    if not only or "serve" in only:
+        log("served mix ...")
+        kernels["served_mix"] = bench_serve()
 
     cpu = None
     if not args.no_cpu:

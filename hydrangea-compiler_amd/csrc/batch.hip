@@ -47,9 +47,9 @@ struct Tile {
 };
 
 struct InstDesc {
-    int64_t off;   // first element in data
+    const int64_t* src;       // the instance's elements
+    unsigned long long* dst;  // its bins
     int64_t bins;
-    int64_t hoff;  // first bin in hist
     int64_t pad;
 };
 
@@ -95,12 +95,10 @@ __device__ __forceinline__ void tile_elements(const int64_t* __restrict__ p, int
 }
 
 template <int BLOCK, int WORDS>
-__global__ __launch_bounds__(BLOCK) void batched_hist_kernel(const int64_t* __restrict__ data,
-                                                             const InstDesc* __restrict__ inst,
+__global__ __launch_bounds__(BLOCK) void batched_hist_kernel(const InstDesc* __restrict__ inst,
                                                              const Tile* __restrict__ tiles,
                                                              int64_t ntiles,
-                                                             unsigned int* __restrict__ counter,
-                                                             unsigned long long* __restrict__ hist) {
+                                                             unsigned int* __restrict__ counter) {
     extern __shared__ unsigned long long lds[];
     __shared__ int64_t cur;
     for (;;) {
@@ -112,8 +110,8 @@ __global__ __launch_bounds__(BLOCK) void batched_hist_kernel(const int64_t* __re
         const Tile tile = tiles[t];
         const InstDesc d = inst[tile.inst];
         const int64_t nb = d.bins;
-        unsigned long long* h = hist + d.hoff;
-        const int64_t* src = data + d.off + tile.start;
+        unsigned long long* h = d.dst;
+        const int64_t* src = d.src + tile.start;
         if (tile.mode == kModeGlobal) {
             tile_elements<BLOCK>(src, tile.len, [&](int64_t x) {
                 int64_t b = x % nb;
@@ -155,12 +153,13 @@ __global__ __launch_bounds__(BLOCK) void batched_hist_kernel(const int64_t* __re
     }
 }
 
-// Zeroes hist ranges of the instances whose tiles accumulate.
-__global__ void zero_ranges_kernel(const int64_t* __restrict__ ranges, int64_t nranges,
-                                   unsigned long long* __restrict__ hist) {
+// Zeroes the bins of the instances whose tiles accumulate ((pointer, length)
+// pairs).
+__global__ void zero_ranges_kernel(const int64_t* __restrict__ ranges, int64_t nranges) {
     for (int64_t r = blockIdx.x; r < nranges; r += gridDim.x) {
-        const int64_t off = ranges[2 * r], len = ranges[2 * r + 1];
-        for (int64_t i = threadIdx.x; i < len; i += blockDim.x) hist[off + i] = 0;
+        unsigned long long* p = reinterpret_cast<unsigned long long*>(ranges[2 * r]);
+        const int64_t len = ranges[2 * r + 1];
+        for (int64_t i = threadIdx.x; i < len; i += blockDim.x) p[i] = 0;
     }
 }
 
@@ -173,17 +172,19 @@ std::mutex g_plan_mu;
 
 }  // namespace
 
-int batched_histogram(const int64_t* data, const int64_t* off, const int64_t* len,
-                      const int64_t* bins, const int64_t* hoff, int64_t count, int64_t* hist,
-                      int policy, hipStream_t s, int64_t* out_tiles) {
+// Instance k: src[k][0..len[k]) -> dst[k][0..bins[k]).  All arrays are host
+// arrays of device pointers / sizes.
+int batched_histogram_ptrs(const int64_t* const* src, const int64_t* len, const int64_t* bins,
+                           int64_t* const* dst, int64_t count, int policy, hipStream_t s,
+                           int64_t* out_tiles) {
     std::vector<InstDesc> inst((size_t)count);
     std::vector<Tile> small, large;
-    std::vector<int64_t> zero;  // (hoff, bins) pairs
+    std::vector<int64_t> zero;  // (dst pointer, bins) pairs
     for (int64_t k = 0; k < count; ++k) {
-        if (len[k] < 0 || bins[k] < 1 || bins[k] > ((int64_t)1 << 31) || off[k] < 0 || hoff[k] < 0)
+        if (len[k] < 0 || bins[k] < 1 || bins[k] > ((int64_t)1 << 31) || !dst[k] || (len[k] > 0 && !src[k]))
             return set_error(HYD_E_INVALID, "batched_histogram: bad descriptor %lld", (long long)k);
         const int64_t n = len[k], nb = bins[k];
-        inst[k] = InstDesc{off[k], nb, hoff[k], 0};
+        inst[k] = InstDesc{src[k], reinterpret_cast<unsigned long long*>(dst[k]), nb, 0};
         int mode;
         if (policy == 1)
             mode = 0;
@@ -194,7 +195,7 @@ int batched_histogram(const int64_t* data, const int64_t* off, const int64_t* le
             if (mode < 0) mode = rule_mode(n, nb);
         }
         if (mode == 1) {
-            zero.push_back(hoff[k]);
+            zero.push_back((int64_t)reinterpret_cast<uintptr_t>(dst[k]));
             zero.push_back(nb);
             for (int64_t st = 0; st < n; st += kChunk)
                 small.push_back(Tile{k, st, std::min(kChunk, n - st), 0, 0, kModeGlobal, 0});
@@ -204,7 +205,7 @@ int batched_histogram(const int64_t* data, const int64_t* off, const int64_t* le
         const int64_t range = fits_small ? nb : kLargeWords;
         const bool chunked = n > kChunk;
         if (chunked) {
-            zero.push_back(hoff[k]);
+            zero.push_back((int64_t)reinterpret_cast<uintptr_t>(dst[k]));
             zero.push_back(nb);
         }
         std::vector<Tile>& dst = fits_small ? small : large;
@@ -249,11 +250,10 @@ int batched_histogram(const int64_t* data, const int64_t* off, const int64_t* le
         HYD_TRY_HIP(hipStreamSynchronize(s));  // pageable source: consumed before it goes away
     }
     HYD_TRY(fill_zero(counters, 256, s));
-    unsigned long long* h = reinterpret_cast<unsigned long long*>(hist);
     const DeviceInfo& di = device_info();
     if (!zero.empty()) {
         int64_t nr = (int64_t)zero.size() / 2;
-        zero_ranges_kernel<<<(unsigned)std::min<int64_t>(nr, 4096), 256, 0, s>>>(d_zero, nr, h);
+        zero_ranges_kernel<<<(unsigned)std::min<int64_t>(nr, 4096), 256, 0, s>>>(d_zero, nr);
         HYD_CHECK_LAUNCH();
     }
     if (!small.empty()) {
@@ -261,8 +261,8 @@ int batched_histogram(const int64_t* data, const int64_t* off, const int64_t* le
         const int lds = kSmallWords * 8;
         HYD_TRY_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
         int64_t blocks = std::min<int64_t>((int64_t)small.size(), (int64_t)di.num_cus * 4);
-        k<<<(unsigned)blocks, kSmallBlock, lds, s>>>(data, d_inst, d_small, (int64_t)small.size(),
-                                                     counters, h);
+        k<<<(unsigned)blocks, kSmallBlock, lds, s>>>(d_inst, d_small, (int64_t)small.size(),
+                                                     counters);
         HYD_CHECK_LAUNCH();
     }
     if (!large.empty()) {
@@ -270,11 +270,26 @@ int batched_histogram(const int64_t* data, const int64_t* off, const int64_t* le
         const int lds = kLargeWords * 8;
         HYD_TRY_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
         int64_t blocks = std::min<int64_t>((int64_t)large.size(), (int64_t)di.num_cus);
-        k<<<(unsigned)blocks, kLargeBlock, lds, s>>>(data, d_inst, d_large, (int64_t)large.size(),
-                                                     counters + 1, h);
+        k<<<(unsigned)blocks, kLargeBlock, lds, s>>>(d_inst, d_large, (int64_t)large.size(),
+                                                     counters + 1);
         HYD_CHECK_LAUNCH();
     }
     return HYD_OK;
+}
+
+// Packed layout: instance k reads data[off[k] ..] and writes hist[hoff[k] ..].
+int batched_histogram(const int64_t* data, const int64_t* off, const int64_t* len,
+                      const int64_t* bins, const int64_t* hoff, int64_t count, int64_t* hist,
+                      int policy, hipStream_t s, int64_t* out_tiles) {
+    std::vector<const int64_t*> src((size_t)count);
+    std::vector<int64_t*> dst((size_t)count);
+    for (int64_t k = 0; k < count; ++k) {
+        if (off[k] < 0 || hoff[k] < 0)
+            return set_error(HYD_E_INVALID, "batched_histogram: bad offsets at %lld", (long long)k);
+        src[k] = data + off[k];
+        dst[k] = hist + hoff[k];
+    }
+    return batched_histogram_ptrs(src.data(), len, bins, dst.data(), count, policy, s, out_tiles);
 }
 
 }  // namespace hyd

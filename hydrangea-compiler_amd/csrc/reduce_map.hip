@@ -14,6 +14,10 @@
 // Products and sums stay separate IEEE operations (-ffp-contract=off), the
 // same per-element semantics as the C; only the summation order differs,
 // which the 1e-12 relative bound covers.
+#include <string.h>
+
+#include <vector>
+
 #include "hyd_exp.h"
 #include "hyd_internal.hpp"
 
@@ -215,6 +219,108 @@ extern "C" int hydhip_math_f64(int op, const double* x, int64_t n, double* out, 
         sqrt_kernel<<<(unsigned)blocks, 256, 0, resolve_stream(stream)>>>(x, n, out);
     HYD_CHECK_LAUNCH();
     return HYD_OK;
+}
+
+// ---- batched dot ------------------------------------------------------------
+// Request r is cut into fixed tiles of kDotTile elements; tile (r, t) is summed
+// by one workgroup in a fixed order and request r's result is init plus its
+// tile partials added in tile order.  The bits of a request therefore do not
+// depend on the other requests of the batch.
+namespace hyd {
+namespace {
+constexpr int64_t kDotTile = 65536;
+
+struct DotTile {
+    const double* xs;
+    const double* ys;
+    int64_t len;
+    int64_t pad;
+};
+
+__global__ __launch_bounds__(kBlock) void batched_dot_tiles(const DotTile* __restrict__ tiles,
+                                                           double* __restrict__ partial) {
+    const DotTile t = tiles[blockIdx.x];
+    double a0 = 0.0, a1 = 0.0;
+    const bool vec = ((reinterpret_cast<uintptr_t>(t.xs) | reinterpret_cast<uintptr_t>(t.ys)) & 15) == 0;
+    if (vec) {
+        const int64_t npairs = t.len >> 1;
+        for (int64_t p = threadIdx.x; p < npairs; p += kBlock) {
+            v2d x = ld_stream2(t.xs + 2 * p), y = ld_stream2(t.ys + 2 * p);
+            double m0 = x.x * y.x, m1 = x.y * y.y;
+            a0 = a0 + m0;
+            a1 = a1 + m1;
+        }
+        if ((t.len & 1) && threadIdx.x == 0) {
+            double m = t.xs[t.len - 1] * t.ys[t.len - 1];
+            a0 = a0 + m;
+        }
+    } else {
+        for (int64_t i = threadIdx.x; i < t.len; i += kBlock) {
+            double m = t.xs[i] * t.ys[i];
+            a0 = a0 + m;
+        }
+    }
+    double s = block_sum<kBlock>(a0 + a1);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ void batched_dot_final(const double* __restrict__ partial, const int64_t* __restrict__ first,
+                                  int64_t count, double init, double* const* __restrict__ outs) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= count) return;
+    double acc = init;
+    for (int64_t t = first[r]; t < first[r + 1]; ++t) acc = acc + partial[t];
+    *outs[r] = acc;
+}
+}  // namespace
+
+// Host-side plan + launches; xs/ys/outs/lens are host arrays of device
+// pointers and sizes.
+int batched_dot(const double* const* xs, const double* const* ys, const int64_t* lens, int64_t count,
+                double init, double* const* outs, hipStream_t s) {
+    std::vector<DotTile> tiles;
+    std::vector<int64_t> first((size_t)count + 1);
+    for (int64_t r = 0; r < count; ++r) {
+        if (lens[r] < 0 || !outs[r] || (lens[r] > 0 && (!xs[r] || !ys[r])))
+            return set_error(HYD_E_INVALID, "batched_dot: bad request %lld", (long long)r);
+        first[r] = (int64_t)tiles.size();
+        for (int64_t st = 0; st < lens[r]; st += kDotTile)
+            tiles.push_back(DotTile{xs[r] + st, ys[r] + st, std::min(kDotTile, lens[r] - st), 0});
+    }
+    first[count] = (int64_t)tiles.size();
+    const size_t tb = tiles.size() * sizeof(DotTile), fb = first.size() * 8, ob = (size_t)count * 8;
+    const size_t pb = tiles.size() * 8;
+    unsigned char* ws = static_cast<unsigned char*>(stream_workspace(s, tb + fb + ob + pb + 64));
+    if (!ws) return set_error(HYD_E_OOM, "batched_dot: workspace allocation failed");
+    std::vector<unsigned char> stage(tb + fb + ob);
+    memcpy(stage.data(), tiles.data(), tb);
+    memcpy(stage.data() + tb, first.data(), fb);
+    memcpy(stage.data() + tb + fb, outs, ob);
+    HYD_TRY_HIP(hipMemcpyAsync(ws, stage.data(), stage.size(), hipMemcpyHostToDevice, s));
+    HYD_TRY_HIP(hipStreamSynchronize(s));  // pageable staging consumed
+    auto* d_tiles = reinterpret_cast<DotTile*>(ws);
+    auto* d_first = reinterpret_cast<int64_t*>(ws + tb);
+    auto* d_outs = reinterpret_cast<double* const*>(ws + tb + fb);
+    auto* d_part = reinterpret_cast<double*>(ws + ((tb + fb + ob + 15) & ~(size_t)15));
+    if (!tiles.empty()) {
+        batched_dot_tiles<<<(unsigned)tiles.size(), kBlock, 0, s>>>(d_tiles, d_part);
+        HYD_CHECK_LAUNCH();
+    }
+    batched_dot_final<<<(unsigned)ceil_div(count, 256), 256, 0, s>>>(d_part, d_first, count, init, d_outs);
+    HYD_CHECK_LAUNCH();
+    return HYD_OK;
+}
+
+}  // namespace hyd
+
+extern "C" int hydhip_batched_dot_f64(const double* const* xs, const double* const* ys,
+                                      const int64_t* lens, int64_t count, double init,
+                                      double* const* outs, void* stream) {
+    HYD_TRY(ensure_init());
+    if (count < 0 || (count > 0 && (!xs || !ys || !lens || !outs)))
+        return set_error(HYD_E_INVALID, "hydhip_batched_dot_f64: bad arguments");
+    if (count == 0) return HYD_OK;
+    return batched_dot(xs, ys, lens, count, init, outs, resolve_stream(stream));
 }
 
 // dst[i] = src[i], i < n (n even, 16-byte aligned pointers).

@@ -508,6 +508,13 @@ int segmented_reduce_i64(const int64_t* offsets, int64_t nseg, const int64_t* va
 
 }  // namespace hyd
 
+namespace hyd {
+int64_t coo_to_csr_workspace_words(int64_t nnz, int64_t nrows);
+int coo_to_csr_ws(int64_t nrows, int64_t ncols, int64_t nnz, const int64_t* rows, const int64_t* cols,
+                  const int64_t* vals, int64_t* row_ptr, int64_t* col_idx, int64_t* out_vals,
+                  int64_t* nnz_out, hipStream_t s, int64_t* ws);
+}  // namespace hyd
+
 using namespace hyd;
 
 extern "C" int hydhip_scan_i64(const int64_t* in, int64_t n, int64_t init, int64_t* out,
@@ -596,13 +603,25 @@ extern "C" int hydhip_coo_to_csr_i64(int64_t nrows, int64_t ncols, int64_t nnz, 
         (nnz > 0 && (!rows || !cols || !vals || !col_idx || !out_vals)))
         return set_error(HYD_E_INVALID, "hydhip_coo_to_csr_i64: bad arguments");
     hipStream_t s = resolve_stream(stream);
-    // workspace: keys, perm, canonical rows/cols/vals, sort/coo/csr scratch
-    int64_t sort_w = sort_workspace_words(nnz), coo_w = coo_workspace_words(nnz),
-            csr_w = csr_workspace_words(nrows);
-    int64_t scratch = std::max(sort_w, std::max(coo_w, csr_w));
-    int64_t total = 5 * nnz + scratch + 8;
-    int64_t* ws = static_cast<int64_t*>(stream_workspace(s, (size_t)total * 8));
+    int64_t* ws = static_cast<int64_t*>(
+        stream_workspace(s, (size_t)coo_to_csr_workspace_words(nnz, nrows) * 8));
     if (!ws) return set_error(HYD_E_OOM, "hydhip_coo_to_csr_i64: workspace allocation failed");
+    return coo_to_csr_ws(nrows, ncols, nnz, rows, cols, vals, row_ptr, col_idx, out_vals, nnz_out, s, ws);
+}
+
+namespace hyd {
+// Words of workspace coo_to_csr_ws needs: keys, perm, canonical
+// rows/cols/vals, and the largest of the sort / merge / CSR scratch.
+int64_t coo_to_csr_workspace_words(int64_t nnz, int64_t nrows) {
+    int64_t scratch = std::max(sort_workspace_words(nnz),
+                               std::max(coo_workspace_words(nnz), csr_workspace_words(nrows)));
+    return 5 * nnz + scratch + 8;
+}
+
+int coo_to_csr_ws(int64_t nrows, int64_t ncols, int64_t nnz, const int64_t* rows, const int64_t* cols,
+                  const int64_t* vals, int64_t* row_ptr, int64_t* col_idx, int64_t* out_vals,
+                  int64_t* nnz_out, hipStream_t s, int64_t* ws) {
+    int64_t scratch = coo_to_csr_workspace_words(nnz, nrows) - 5 * nnz - 8;
     int64_t* keys = ws;
     int64_t* perm = ws + nnz;
     int64_t* crow = ws + 2 * nnz;
@@ -619,3 +638,4 @@ extern "C" int hydhip_coo_to_csr_i64(int64_t nrows, int64_t ncols, int64_t nnz, 
     *nnz_out = cnt;
     return csr_from_sorted_coo_i64(nrows, cnt, crow, ccol, cval, nnz, row_ptr, col_idx, out_vals, s, scr);
 }
+}  // namespace hyd

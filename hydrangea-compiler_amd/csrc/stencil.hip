@@ -169,6 +169,77 @@ __global__ __launch_bounds__(64 * kWaves) void stencil2d_clamp_kernel2(const dou
     }
 }
 
+// Temporal blocking: `steps` (<= kTK) stencil steps in one launch.  A
+// workgroup owns a kTT x kTT output tile and loads it with a kTK-cell halo
+// (kTL x kTL, clamped to the grid) into LDS; step s recomputes the region
+// shrunk by s cells (grid edges do not shrink: clamping keeps them exact),
+// ping-ponging between two LDS buffers.  Every cell is computed from exactly
+// the neighbour values of the previous step with the same operations, so the
+// result equals `steps` single-step launches bit for bit; HBM sees one read
+// of the haloed tile and one write of the tile per `steps` steps.
+constexpr int kTK = 8;                // fused steps
+constexpr int kTT = 48;               // output tile edge
+constexpr int kTL = kTT + 2 * kTK;    // loaded tile edge (64: one wave per row)
+constexpr int kTBlock = 256;
+
+template <class Body>
+__global__ __launch_bounds__(kTBlock) void stencil2d_temporal_kernel(const double* __restrict__ in,
+                                                                     int64_t h, int64_t w, int steps,
+                                                                     Body body,
+                                                                     double* __restrict__ out) {
+    extern __shared__ double sm[];
+    double* buf0 = sm;
+    double* buf1 = sm + kTL * kTL;
+    const int64_t r0 = (int64_t)blockIdx.y * kTT, c0 = (int64_t)blockIdx.x * kTT;
+    const int64_t gr = r0 - kTK, gc = c0 - kTK;  // global coordinates of local (0, 0)
+    for (int idx = threadIdx.x; idx < kTL * kTL; idx += kTBlock) {
+        const int64_t i = gr + idx / kTL, j = gc + idx % kTL;
+        if (i >= 0 && i < h && j >= 0 && j < w) buf0[idx] = in[i * w + j];
+    }
+    __syncthreads();
+    for (int s = 1; s <= steps; ++s) {
+        const double* src = (s & 1) ? buf0 : buf1;
+        double* dst = (s & 1) ? buf1 : buf0;
+        const int64_t ilo = gr + s > 0 ? gr + s : 0;
+        const int64_t ihi = r0 + kTT + kTK - s < h ? r0 + kTT + kTK - s : h;
+        const int64_t jlo = gc + s > 0 ? gc + s : 0;
+        const int64_t jhi = c0 + kTT + kTK - s < w ? c0 + kTT + kTK - s : w;
+        for (int idx = threadIdx.x; idx < kTL * kTL; idx += kTBlock) {
+            const int li = idx / kTL, lj = idx % kTL;
+            const int64_t i = gr + li, j = gc + lj;
+            if (i < ilo || i >= ihi || j < jlo || j >= jhi) continue;
+            // clamped neighbours, as local indices (all inside the previous region)
+            const int ln = (int)((i > 0 ? i - 1 : 0) - gr);
+            const int ls = (int)((i + 1 < h ? i + 1 : h - 1) - gr);
+            const int lw = (int)((j > 0 ? j - 1 : 0) - gc);
+            const int le = (int)((j + 1 < w ? j + 1 : w - 1) - gc);
+            dst[idx] = body(src[ln * kTL + lj], src[ls * kTL + lj], src[li * kTL + lw],
+                            src[li * kTL + le], src[idx]);
+        }
+        __syncthreads();
+    }
+    const double* res = (steps & 1) ? buf1 : buf0;
+    const int64_t rend = r0 + kTT < h ? r0 + kTT : h, cend = c0 + kTT < w ? c0 + kTT : w;
+    for (int idx = threadIdx.x; idx < kTL * kTL; idx += kTBlock) {
+        const int64_t i = gr + idx / kTL, j = gc + idx % kTL;
+        if (i >= r0 && i < rend && j >= c0 && j < cend) out[i * w + j] = res[idx];
+    }
+}
+
+template <class Body>
+int launch_temporal(const double* in, int64_t h, int64_t w, int steps, Body body, double* out,
+                    hipStream_t s) {
+    int64_t gx = ceil_div(w, kTT), gy = ceil_div(h, kTT);
+    if (gx > 0x7fffffff || gy > 65535)
+        return set_error(HYD_E_INVALID, "stencil: grid %lld x %lld too large", (long long)h, (long long)w);
+    const int lds = 2 * kTL * kTL * (int)sizeof(double);
+    auto k = stencil2d_temporal_kernel<Body>;
+    HYD_TRY_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+    k<<<dim3((unsigned)gx, (unsigned)gy), kTBlock, lds, s>>>(in, h, w, steps, body, out);
+    HYD_CHECK_LAUNCH();
+    return HYD_OK;
+}
+
 template <class Body>
 int launch_stencil(const double* in, int64_t h, int64_t w, Body body, double* out, hipStream_t s) {
     const bool pairs = (w % 2 == 0) && ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
@@ -212,12 +283,31 @@ using GraphKey = std::tuple<const double*, double*, double*, int64_t, int64_t, i
 std::mutex g_graph_mu;
 std::map<GraphKey, hipGraphExec_t> g_graphs;
 
+// `steps` consecutive steps from src into dst (one launch).
+int launch_steps(const double* src, int64_t h, int64_t w, int body, double alpha, int steps, double* dst,
+                 hipStream_t s) {
+    if (steps == 1) return launch_body(src, h, w, body, alpha, dst, s);
+    switch (body) {
+        case HYD_STENCIL_HEAT: return launch_temporal(src, h, w, steps, HeatBody{alpha}, dst, s);
+        case HYD_STENCIL_LAPLACIAN: return launch_temporal(src, h, w, steps, LaplacianBody{alpha}, dst, s);
+        case HYD_STENCIL_JACOBI4: return launch_temporal(src, h, w, steps, Jacobi4Body{alpha}, dst, s);
+        case HYD_STENCIL_AVG5: return launch_temporal(src, h, w, steps, Avg5Body{alpha}, dst, s);
+        default: return set_error(HYD_E_INVALID, "stencil: unknown body %d", body);
+    }
+}
+
+// Passes of up to kTK fused steps, ping-ponging between buf0 and buf1; the
+// result is in the buffer of the last pass.
+int64_t num_passes(int64_t iters) { return (iters + kTK - 1) / kTK; }
+
 int record_steps(const double* init, int64_t h, int64_t w, int body, double alpha, int64_t iters,
                  double* buf0, double* buf1, hipStream_t s) {
     const double* src = init;
-    for (int64_t t = 0; t < iters; ++t) {
-        double* dst = (t & 1) ? buf1 : buf0;
-        HYD_TRY(launch_body(src, h, w, body, alpha, dst, s));
+    int64_t p = 0;
+    for (int64_t t = 0; t < iters; t += kTK, ++p) {
+        const int steps = (int)std::min<int64_t>(kTK, iters - t);
+        double* dst = (p & 1) ? buf1 : buf0;
+        HYD_TRY(launch_steps(src, h, w, body, alpha, steps, dst, s));
         src = dst;
     }
     return HYD_OK;
@@ -258,7 +348,7 @@ extern "C" int hydhip_stencil2d_iterate_f64(const double* init, int64_t h, int64
         HYD_TRY_HIP(hipMemcpyAsync(buf0, init, bytes, hipMemcpyDeviceToDevice, s));
         return HYD_OK;
     }
-    *result = ((iters - 1) & 1) ? buf1 : buf0;
+    *result = ((num_passes(iters) - 1) & 1) ? buf1 : buf0;
     if (!use_graph || iters == 1) return record_steps(init, h, w, body, alpha, iters, buf0, buf1, s);
 
     uint64_t abits;

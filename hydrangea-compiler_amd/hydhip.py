@@ -74,6 +74,31 @@ class HydCsrRecordT(ctypes.Structure):
     ]
 
 
+class HydRequest(ctypes.Structure):
+    """hyd_request_t (include/hyd_serve.h)"""
+    _fields_ = [("kind", ctypes.c_int32), ("pad", ctypes.c_int32), ("n", ctypes.c_int64),
+                ("h", ctypes.c_int64), ("w", ctypes.c_int64), ("iters", ctypes.c_int64),
+                ("bins", ctypes.c_int64), ("nrows", ctypes.c_int64), ("ncols", ctypes.c_int64),
+                ("in0", ctypes.c_void_p), ("in1", ctypes.c_void_p), ("in2", ctypes.c_void_p),
+                ("deadline_ms", ctypes.c_double)]
+
+
+class HydResult(ctypes.Structure):
+    """hyd_result_t"""
+    _fields_ = [("status", ctypes.c_int32), ("done", ctypes.c_int32), ("out0", ctypes.c_void_p),
+                ("out1", ctypes.c_void_p), ("out2", ctypes.c_void_p), ("count", ctypes.c_int64),
+                ("submit_ms", ctypes.c_double), ("admit_ms", ctypes.c_double),
+                ("start_ms", ctypes.c_double), ("done_ms", ctypes.c_double),
+                ("bytes", ctypes.c_int64), ("batch_id", ctypes.c_int64)]
+
+
+class HydServerStats(ctypes.Structure):
+    """hyd_server_stats_t"""
+    _fields_ = [(name, ctypes.c_int64) for name in (
+        "budget", "in_use", "peak_in_use", "submitted", "admitted", "completed", "failed", "rejected",
+        "timed_out", "queued", "running", "jobs", "batched_jobs", "batched_requests")]
+
+
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
 _U64 = ctypes.c_uint64
@@ -128,6 +153,21 @@ _SIGS = {
     "hydhip_tune_set": (_INT, [_I64, _I64, _INT]),
     "hydhip_tune_clear": (_INT, []),
     "hydhip_tune_table": (_INT, [ctypes.POINTER(_INT), _INT]),
+    "hydhip_batched_dot_f64": (_INT, [_P, _P, _P, _I64, _D, _P, _P]),
+    # serving
+    "hyd_server_create": (_P, [_I64, _INT, _INT]),
+    "hyd_server_destroy": (None, [_P]),
+    "hyd_server_submit": (_INT, [_P, ctypes.POINTER(HydRequest), ctypes.POINTER(_I64)]),
+    "hyd_server_poll": (_INT, [_P, _I64, ctypes.POINTER(HydResult)]),
+    "hyd_server_wait": (_INT, [_P, _I64, _D, ctypes.POINTER(HydResult)]),
+    "hyd_server_wait_all": (_INT, [_P, _D]),
+    "hyd_server_next_done": (_INT, [_P, _D, ctypes.POINTER(_I64), ctypes.POINTER(HydResult)]),
+    "hyd_server_release": (_INT, [_P, _I64]),
+    "hyd_server_stats": (_INT, [_P, ctypes.POINTER(HydServerStats)]),
+    "hyd_server_inject_failures": (_INT, [_P, _I64]),
+    "hyd_request_bytes": (_I64, [ctypes.POINTER(HydRequest)]),
+    "hydhip_run_request_plain": (_INT, [ctypes.POINTER(HydRequest), ctypes.POINTER(HydResult)]),
+    "hydhip_result_free_plain": (_INT, [ctypes.POINTER(HydResult)]),
     # array ABI
     "hyd_shape_size": (_I64, [HydTuple]),
     "hyd_array_alloc_bytes": (_PA, [HydTuple, ctypes.c_size_t]),
@@ -417,6 +457,113 @@ def tune_table():
     buf = (_INT * (3 * 256))()
     n = lib().hydhip_tune_table(buf, 256)
     return [(buf[3 * i], buf[3 * i + 1], buf[3 * i + 2]) for i in range(n)]
+
+
+def batched_dot(xs_list, ys_list, init=0.0, stream=None):
+    """Deterministic per-request dots in one launch; returns a DeviceArray of results."""
+    count = len(xs_list)
+    outs = DeviceArray((max(count, 1),), np.float64)
+    xp = (ctypes.c_void_p * count)(*[x.ptr for x in xs_list])
+    yp = (ctypes.c_void_p * count)(*[y.ptr for y in ys_list])
+    op = (ctypes.c_void_p * count)(*[outs.ptr + 8 * i for i in range(count)])
+    lens = np.array([min(x.size, y.size) for x, y in zip(xs_list, ys_list)], np.int64)
+    _check(lib().hydhip_batched_dot_f64(xp, yp, lens.ctypes.data, count, init, op, stream))
+    return outs
+
+
+# ---- serving -------------------------------------------------------------------------
+REQ_STENCIL = 0
+REQ_HIST = 1
+REQ_DOT = 2
+REQ_COO = 3
+SERVE_NO_BATCH = 1
+SERVE_FAKE = 2
+
+
+def request(kind, n=0, h=0, w=0, iters=0, bins=0, nrows=0, ncols=0, in0=None, in1=None, in2=None,
+            deadline_ms=0.0):
+    return HydRequest(kind, 0, n, h, w, iters, bins, nrows, ncols, in0, in1, in2, deadline_ms)
+
+
+class Server:
+    """Continuous-batching server (include/hyd_serve.h)."""
+
+    def __init__(self, budget_bytes, workers=4, flags=0):
+        self.ptr = lib().hyd_server_create(budget_bytes, workers, flags)
+        if not self.ptr:
+            raise HydError(HYD_E_INVALID, lib().hydhip_last_error().decode(errors="replace"))
+
+    def submit(self, req):
+        rid = _I64()
+        rc = lib().hyd_server_submit(self.ptr, ctypes.byref(req), ctypes.byref(rid))
+        if rc != HYD_OK:
+            raise HydError(rc, lib().hydhip_last_error().decode(errors="replace"))
+        return rid.value
+
+    def poll(self, rid):
+        res = HydResult()
+        rc = lib().hyd_server_poll(self.ptr, rid, ctypes.byref(res))
+        return (res if rc == HYD_OK else None)
+
+    def wait(self, rid, timeout_ms=60_000.0):
+        res = HydResult()
+        rc = lib().hyd_server_wait(self.ptr, rid, timeout_ms, ctypes.byref(res))
+        if rc != HYD_OK:
+            raise HydError(rc, "request %d not finished" % rid)
+        return res
+
+    def next_done(self, timeout_ms=0.0):
+        """(id, result) of the next finished request, or None on timeout."""
+        rid = _I64()
+        res = HydResult()
+        rc = lib().hyd_server_next_done(self.ptr, timeout_ms, ctypes.byref(rid), ctypes.byref(res))
+        return (rid.value, res) if rc == HYD_OK else None
+
+    def wait_all(self, timeout_ms=60_000.0):
+        return lib().hyd_server_wait_all(self.ptr, timeout_ms) == HYD_OK
+
+    def release(self, rid):
+        _check(lib().hyd_server_release(self.ptr, rid))
+
+    def stats(self):
+        st = HydServerStats()
+        _check(lib().hyd_server_stats(self.ptr, ctypes.byref(st)))
+        return {name: getattr(st, name) for name, _ in HydServerStats._fields_}
+
+    def inject_failures(self, njobs):
+        _check(lib().hyd_server_inject_failures(self.ptr, njobs))
+
+    def close(self):
+        if self.ptr:
+            lib().hyd_server_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def request_bytes(req):
+    return lib().hyd_request_bytes(ctypes.byref(req))
+
+
+def run_plain(req):
+    res = HydResult()
+    _check(lib().hydhip_run_request_plain(ctypes.byref(req), ctypes.byref(res)))
+    return res
+
+
+def free_plain(res):
+    _check(lib().hydhip_result_free_plain(ctypes.byref(res)))
+
+
+def copy_out(ptr, count, dtype):
+    out = np.empty(count, dtype)
+    if count:
+        _check(lib().hydhip_memcpy_d2h(out.ctypes.data, ptr, out.nbytes, None))
+    return out
 
 
 # ---- events and streams ---------------------------------------------------------

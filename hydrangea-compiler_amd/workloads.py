@@ -58,6 +58,43 @@ def batch_instances(count=BATCH["count"], seed=BATCH["seed"], lo_log=10, hi_log=
     return lens, nb
 
 
+def serve_mix(count, seed=SERVE["seed"], max_side_log=12, max_hist_log=24, max_dot_log=26,
+              nnz_lo=1e5, nnz_hi=1e7):
+    """The served mix: 40% Jacobi stencils (side floor(2^(8+4u)), 10-100
+    steps), 30% 64-bin histograms (n = floor(2^(12+12u))), 20% fp64 dots
+    (n = floor(2^(14+12u))), 10% COO->CSR builds over 2^20 x 2^20
+    (nnz log-uniform in [1e5, 1e7]).  Upper ends can be lowered for tests."""
+    reqs = []
+    for k in range(count):
+        c = 4 * k
+        u = unit(seed, c)
+        v = unit(seed, c + 1)
+        w = unit(seed, c + 2)
+        if u < 0.4:
+            side = int(math.floor(2.0 ** (8 + (max_side_log - 8) * v)))
+            steps = 10 + int(w * 91) if w < 1 else 100
+            reqs.append(dict(kind="stencil", side=side, iters=min(steps, 100)))
+        elif u < 0.7:
+            reqs.append(dict(kind="hist", n=int(math.floor(2.0 ** (12 + (max_hist_log - 12) * v))), bins=64))
+        elif u < 0.9:
+            reqs.append(dict(kind="dot", n=int(math.floor(2.0 ** (14 + (max_dot_log - 14) * v)))))
+        else:
+            nnz = int(nnz_lo * (nnz_hi / nnz_lo) ** v)
+            reqs.append(dict(kind="coo", nnz=nnz, nrows=1 << 20, ncols=1 << 20))
+    return reqs
+
+
+def request_bytes_algorithmic(r):
+    """Compulsory HBM bytes of one served request."""
+    if r["kind"] == "stencil":
+        return 16 * r["side"] ** 2 * r["iters"]
+    if r["kind"] == "hist":
+        return 8 * r["n"] + 8 * r["bins"]
+    if r["kind"] == "dot":
+        return 16 * r["n"]
+    return 40 * r["nnz"] + 8 * (r["nrows"] + 1)
+
+
 def batch_algorithmic_bytes(lens, bins):
     """Compulsory HBM traffic: read 8 B per element, write 8 B per bin."""
     return int(8 * np.sum(lens) + 8 * np.sum(bins))

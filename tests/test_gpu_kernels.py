@@ -223,7 +223,16 @@ def test_stencil_known_answers(hyd):
     assert bits_equal(hyd.stencil2d(dev(hyd, c), hyd.STENCIL_HEAT, 0.1).numpy(), c)
 
 
-@pytest.mark.parametrize("iters", [0, 1, 2, 5])
+@pytest.mark.parametrize("shape", SHAPES + [(49, 97), (96, 96), (200, 47)])
+@pytest.mark.parametrize("body", [0, 1, 2, 3])
+def test_iterate_temporal_blocks(hyd, orc, shape, body):
+    # 11 steps = one fused pass of 8 and one of 3, on tile-unaligned shapes
+    u = orc.splitmix_f64(shape[0] * shape[1], 351 + body + shape[1]).reshape(shape)
+    res, _, _ = hyd.iterate_stencil2d(dev(hyd, u), 11, body=body, alpha=0.1, use_graph=False)
+    assert bits_equal(res.numpy(), orc.iterate_stencil2d(u, 11, body, 0.1))
+
+
+@pytest.mark.parametrize("iters", [0, 1, 2, 5, 8, 9, 17])
 @pytest.mark.parametrize("graph", [False, True])
 def test_iterate(hyd, orc, iters, graph):
     u = orc.splitmix_f64(70 * 135, 401 + iters).reshape(70, 135)
