@@ -50,7 +50,7 @@ struct InstDesc {
     const int64_t* src;       // the instance's elements
     unsigned long long* dst;  // its bins
     int64_t bins;
-    int64_t pad;
+    int64_t pow2;             // bins - 1 for a power-of-two bin count, else 0
 };
 
 // Two launch shapes: bins <= 4096 (32 KiB LDS, 4 workgroups per CU) and bin
@@ -114,7 +114,7 @@ __global__ __launch_bounds__(BLOCK) void batched_hist_kernel(const InstDesc* __r
         const int64_t* src = d.src + tile.start;
         if (tile.mode == kModeGlobal) {
             tile_elements<BLOCK>(src, tile.len, [&](int64_t x) {
-                int64_t b = x % nb;
+                int64_t b = c_mod(x, nb, d.pow2);
                 if ((uint64_t)b < (uint64_t)nb) global_add(&h[b], (uint64_t)x * (uint64_t)x);
             });
             continue;
@@ -137,7 +137,7 @@ __global__ __launch_bounds__(BLOCK) void batched_hist_kernel(const InstDesc* __r
         __syncthreads();
         unsigned long long* mine = lds + (threadIdx.x % copies) * stride;
         tile_elements<BLOCK>(src, tile.len, [&](int64_t x) {
-            int64_t b = x % nb - lo;
+            int64_t b = c_mod(x, nb, d.pow2) - lo;
             if ((uint64_t)b < (uint64_t)width) lds_add(&mine[b], (uint64_t)x * (uint64_t)x);
         });
         __syncthreads();
@@ -184,7 +184,7 @@ int batched_histogram_ptrs(const int64_t* const* src, const int64_t* len, const 
         if (len[k] < 0 || bins[k] < 1 || bins[k] > ((int64_t)1 << 31) || !dst[k] || (len[k] > 0 && !src[k]))
             return set_error(HYD_E_INVALID, "batched_histogram: bad descriptor %lld", (long long)k);
         const int64_t n = len[k], nb = bins[k];
-        inst[k] = InstDesc{src[k], reinterpret_cast<unsigned long long*>(dst[k]), nb, 0};
+        inst[k] = InstDesc{src[k], reinterpret_cast<unsigned long long*>(dst[k]), nb, pow2_mask(nb)};
         int mode;
         if (policy == 1)
             mode = 0;

@@ -79,6 +79,21 @@ __host__ __device__ inline uint64_t splitmix_at(uint64_t seed, uint64_t counter)
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// C's truncating remainder x % m (CodegenC.hs:2473-2474) for m >= 1, fast:
+// a mask when m is a power of two (pow2 = m - 1, else 0), 32-bit division
+// when x and m fit in 32 bits, the 64-bit division otherwise.  Negative x
+// gives the C result (in (-m, 0]).
+__device__ __forceinline__ int64_t c_mod(int64_t x, int64_t m, int64_t pow2) {
+    if (pow2) {
+        const int64_t r = x & pow2;
+        return (x < 0 && r) ? r - m : r;
+    }
+    if ((uint64_t)(x + 0x80000000ll) < 0x100000000ull && m <= 0x7fffffff)
+        return (int64_t)((int32_t)x % (int32_t)m);
+    return x % m;
+}
+inline int64_t pow2_mask(int64_t m) { return (m > 0 && (m & (m - 1)) == 0) ? m - 1 : 0; }
+
 // 16-byte streaming accesses (non-temporal: read or written once).
 typedef double v2d __attribute__((ext_vector_type(2)));
 typedef long long v2i __attribute__((ext_vector_type(2)));

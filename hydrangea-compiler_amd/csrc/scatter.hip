@@ -30,8 +30,9 @@ namespace {
 // value = x * x computed mod 2^64 (the C oracle is built with -fwrapv).
 struct ModSquare {
     int64_t bins;
+    int64_t pow2;  // bins - 1 when bins is a power of two, else 0
     __device__ __forceinline__ void operator()(int64_t x, int64_t& b, uint64_t& v) const {
-        b = x % bins;
+        b = c_mod(x, bins, pow2);
         v = (uint64_t)x * (uint64_t)x;
     }
 };
@@ -304,7 +305,7 @@ extern "C" int hydhip_weighted_histogram_i64(const int64_t* data, int64_t n, int
                          (long long)n, (long long)nbins);
     hipStream_t s = resolve_stream(stream);
     if (n == 0) return fill_zero(hist, (size_t)nbins * 8, s);  // fill [nbins] 0
-    FusedSrc src{data, ModSquare{nbins}};
+    FusedSrc src{data, ModSquare{nbins, pow2_mask(nbins)}};
     // fill [nbins] 0 semantics: the LDS form writes every bin, the others
     // zero the destination first (with a kernel, so graph capture works)
     return launch_scatter(src, aligned16(data), n, nbins, hist, strategy, true, s);

@@ -171,58 +171,69 @@ __global__ __launch_bounds__(64 * kWaves) void stencil2d_clamp_kernel2(const dou
 
 // Temporal blocking: `steps` (<= kTK) stencil steps in one launch.  A
 // workgroup owns a kTT x kTT output tile and loads it with a kTK-cell halo
-// (kTL x kTL, clamped to the grid) into LDS; step s recomputes the region
-// shrunk by s cells (grid edges do not shrink: clamping keeps them exact),
-// ping-ponging between two LDS buffers.  Every cell is computed from exactly
-// the neighbour values of the previous step with the same operations, so the
-// result equals `steps` single-step launches bit for bit; HBM sees one read
-// of the haloed tile and one write of the tile per `steps` steps.
+// (a 64 x 64 box, lanes = columns, each of the 4 waves holding 16 rows in
+// registers).  Each step recomputes the whole box: north/south come from the
+// wave's own registers (the rows just outside its strip are exchanged
+// through LDS), west/east from the neighbouring lanes; clamping applies at
+// the grid edges.  The box edges go stale by one cell per step, but the
+// region still exact after step s is the box shrunk by s cells (not shrunk at
+// grid edges), so after kTK steps the output tile holds exactly the values
+// of kTK single-step launches - same operations on the same neighbour
+// values, bit for bit.  HBM sees one read of the haloed box and one write of
+// the tile per `steps` steps.
 constexpr int kTK = 8;                // fused steps
 constexpr int kTT = 48;               // output tile edge
-constexpr int kTL = kTT + 2 * kTK;    // loaded tile edge (64: one wave per row)
-constexpr int kTBlock = 256;
+constexpr int kTL = kTT + 2 * kTK;    // box edge: 64 = one column per lane
+constexpr int kTRows = 16;            // rows per wave
+constexpr int kTWaves = kTL / kTRows; // 4
+constexpr int kTBlock = 64 * kTWaves;
 
 template <class Body>
 __global__ __launch_bounds__(kTBlock) void stencil2d_temporal_kernel(const double* __restrict__ in,
                                                                      int64_t h, int64_t w, int steps,
                                                                      Body body,
                                                                      double* __restrict__ out) {
-    extern __shared__ double sm[];
-    double* buf0 = sm;
-    double* buf1 = sm + kTL * kTL;
+    // [step parity][wave][first/last row of the strip][lane]
+    __shared__ double halo[2][kTWaves][2][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int64_t r0 = (int64_t)blockIdx.y * kTT, c0 = (int64_t)blockIdx.x * kTT;
-    const int64_t gr = r0 - kTK, gc = c0 - kTK;  // global coordinates of local (0, 0)
-    for (int idx = threadIdx.x; idx < kTL * kTL; idx += kTBlock) {
-        const int64_t i = gr + idx / kTL, j = gc + idx % kTL;
-        if (i >= 0 && i < h && j >= 0 && j < w) buf0[idx] = in[i * w + j];
+    const int64_t c = c0 - kTK + lane;                    // this lane's column
+    const int64_t rbase = r0 - kTK + (int64_t)wv * kTRows;  // this wave's first row
+    const bool col_in = c >= 0 && c < w;
+    double v[kTRows];
+#pragma unroll
+    for (int k = 0; k < kTRows; ++k) {
+        const int64_t r = rbase + k;
+        v[k] = (col_in && r >= 0 && r < h) ? in[r * w + c] : 0.0;
     }
-    __syncthreads();
-    for (int s = 1; s <= steps; ++s) {
-        const double* src = (s & 1) ? buf0 : buf1;
-        double* dst = (s & 1) ? buf1 : buf0;
-        const int64_t ilo = gr + s > 0 ? gr + s : 0;
-        const int64_t ihi = r0 + kTT + kTK - s < h ? r0 + kTT + kTK - s : h;
-        const int64_t jlo = gc + s > 0 ? gc + s : 0;
-        const int64_t jhi = c0 + kTT + kTK - s < w ? c0 + kTT + kTK - s : w;
-        for (int idx = threadIdx.x; idx < kTL * kTL; idx += kTBlock) {
-            const int li = idx / kTL, lj = idx % kTL;
-            const int64_t i = gr + li, j = gc + lj;
-            if (i < ilo || i >= ihi || j < jlo || j >= jhi) continue;
-            // clamped neighbours, as local indices (all inside the previous region)
-            const int ln = (int)((i > 0 ? i - 1 : 0) - gr);
-            const int ls = (int)((i + 1 < h ? i + 1 : h - 1) - gr);
-            const int lw = (int)((j > 0 ? j - 1 : 0) - gc);
-            const int le = (int)((j + 1 < w ? j + 1 : w - 1) - gc);
-            dst[idx] = body(src[ln * kTL + lj], src[ls * kTL + lj], src[li * kTL + lw],
-                            src[li * kTL + le], src[idx]);
+    const bool west_edge = c == 0, east_edge = c == w - 1;
+    for (int s = 0; s < steps; ++s) {
+        const int p = s & 1;
+        halo[p][wv][0][lane] = v[0];
+        halo[p][wv][1][lane] = v[kTRows - 1];
+        __syncthreads();  // (parity buffers: one barrier per step suffices)
+        double prev = wv > 0 ? halo[p][wv - 1][1][lane] : v[0];           // row above the strip
+        const double dn = wv < kTWaves - 1 ? halo[p][wv + 1][0][lane] : v[kTRows - 1];
+#pragma unroll
+        for (int k = 0; k < kTRows; ++k) {
+            const int64_t r = rbase + k;
+            const double cur = v[k];
+            const double below = (k + 1 < kTRows) ? v[k + 1] : dn;
+            const double north = r == 0 ? cur : prev;
+            const double south = r == h - 1 ? cur : below;
+            double west = __shfl_up(cur, 1, 64);
+            double east = __shfl_down(cur, 1, 64);
+            if (west_edge) west = cur;
+            if (east_edge) east = cur;
+            prev = cur;
+            v[k] = body(north, south, west, east, cur);
         }
-        __syncthreads();
     }
-    const double* res = (steps & 1) ? buf1 : buf0;
-    const int64_t rend = r0 + kTT < h ? r0 + kTT : h, cend = c0 + kTT < w ? c0 + kTT : w;
-    for (int idx = threadIdx.x; idx < kTL * kTL; idx += kTBlock) {
-        const int64_t i = gr + idx / kTL, j = gc + idx % kTL;
-        if (i >= r0 && i < rend && j >= c0 && j < cend) out[i * w + j] = res[idx];
+    const bool col_out = c >= c0 && c < c0 + kTT && c < w;
+#pragma unroll
+    for (int k = 0; k < kTRows; ++k) {
+        const int64_t r = rbase + k;
+        if (col_out && r >= r0 && r < r0 + kTT && r < h) out[r * w + c] = v[k];
     }
 }
 
@@ -232,10 +243,8 @@ int launch_temporal(const double* in, int64_t h, int64_t w, int steps, Body body
     int64_t gx = ceil_div(w, kTT), gy = ceil_div(h, kTT);
     if (gx > 0x7fffffff || gy > 65535)
         return set_error(HYD_E_INVALID, "stencil: grid %lld x %lld too large", (long long)h, (long long)w);
-    const int lds = 2 * kTL * kTL * (int)sizeof(double);
-    auto k = stencil2d_temporal_kernel<Body>;
-    HYD_TRY_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-    k<<<dim3((unsigned)gx, (unsigned)gy), kTBlock, lds, s>>>(in, h, w, steps, body, out);
+    stencil2d_temporal_kernel<Body><<<dim3((unsigned)gx, (unsigned)gy), kTBlock, 0, s>>>(in, h, w, steps,
+                                                                                       body, out);
     HYD_CHECK_LAUNCH();
     return HYD_OK;
 }

@@ -62,6 +62,19 @@ def test_weighted_histogram(hyd, orc, n, nbins, strategy):
     assert bits_equal(got, orc.weighted_histogram(data, nbins))
 
 
+@pytest.mark.parametrize("nbins", [3, 64, 1000, 65536])
+def test_weighted_histogram_wide_values(hyd, orc, nbins):
+    # 63-bit and negative values: the 64-bit remainder path and wrapping squares
+    data = orc.splitmix_i64(300_001, 0x77 + nbins, bits=63)
+    data[::3] = -data[::3]
+    data[::5] = data[::5] >> 40
+    for strategy in STRATS:
+        if strategy == 1 and nbins > 16384:
+            continue
+        got = hyd.weighted_histogram(dev(hyd, data), nbins, strategy).numpy()
+        assert bits_equal(got, orc.weighted_histogram(data, nbins)), strategy
+
+
 def test_weighted_histogram_misaligned(hyd, orc):
     # an 8-byte offset view takes the scalar-load path
     n = 100_001
