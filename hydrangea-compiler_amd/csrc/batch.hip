@@ -50,7 +50,7 @@ struct InstDesc {
     const int64_t* src;       // the instance's elements
     unsigned long long* dst;  // its bins
     int64_t bins;
-    int64_t pow2;             // bins - 1 for a power-of-two bin count, else 0
+    ModConst mod;             // route = x % bins
 };
 
 // Two launch shapes: bins <= 4096 (32 KiB LDS, 4 workgroups per CU) and bin
@@ -112,11 +112,17 @@ __global__ __launch_bounds__(BLOCK) void batched_hist_kernel(const InstDesc* __r
         const int64_t nb = d.bins;
         unsigned long long* h = d.dst;
         const int64_t* src = d.src + tile.start;
+        // the remainder's form is chosen per tile, never per element (a branch
+        // per element would serialise the loads of the streaming loop)
+        const bool pow2 = d.mod.mask != 0;
         if (tile.mode == kModeGlobal) {
-            tile_elements<BLOCK>(src, tile.len, [&](int64_t x) {
-                int64_t b = c_mod(x, nb, d.pow2);
+            auto add = [&](int64_t b, int64_t x) {
                 if ((uint64_t)b < (uint64_t)nb) global_add(&h[b], (uint64_t)x * (uint64_t)x);
-            });
+            };
+            if (pow2)
+                tile_elements<BLOCK>(src, tile.len, [&](int64_t x) { add(c_mod_pow2(x, d.mod), x); });
+            else
+                tile_elements<BLOCK>(src, tile.len, [&](int64_t x) { add(c_mod(x, d.mod), x); });
             continue;
         }
         const int64_t lo = tile.lo;
@@ -136,10 +142,14 @@ __global__ __launch_bounds__(BLOCK) void batched_hist_kernel(const InstDesc* __r
         for (int i = threadIdx.x; i < words; i += BLOCK) lds[i] = 0;
         __syncthreads();
         unsigned long long* mine = lds + (threadIdx.x % copies) * stride;
-        tile_elements<BLOCK>(src, tile.len, [&](int64_t x) {
-            int64_t b = c_mod(x, nb, d.pow2) - lo;
+        auto add = [&](int64_t b, int64_t x) {
+            b -= lo;
             if ((uint64_t)b < (uint64_t)width) lds_add(&mine[b], (uint64_t)x * (uint64_t)x);
-        });
+        };
+        if (pow2)
+            tile_elements<BLOCK>(src, tile.len, [&](int64_t x) { add(c_mod_pow2(x, d.mod), x); });
+        else
+            tile_elements<BLOCK>(src, tile.len, [&](int64_t x) { add(c_mod(x, d.mod), x); });
         __syncthreads();
         for (int b = threadIdx.x; b < width; b += BLOCK) {
             unsigned long long s = 0;
@@ -184,7 +194,7 @@ int batched_histogram_ptrs(const int64_t* const* src, const int64_t* len, const 
         if (len[k] < 0 || bins[k] < 1 || bins[k] > ((int64_t)1 << 31) || !dst[k] || (len[k] > 0 && !src[k]))
             return set_error(HYD_E_INVALID, "batched_histogram: bad descriptor %lld", (long long)k);
         const int64_t n = len[k], nb = bins[k];
-        inst[k] = InstDesc{src[k], reinterpret_cast<unsigned long long*>(dst[k]), nb, pow2_mask(nb)};
+        inst[k] = InstDesc{src[k], reinterpret_cast<unsigned long long*>(dst[k]), nb, make_mod_const(nb)};
         int mode;
         if (policy == 1)
             mode = 0;

@@ -97,18 +97,12 @@ HYD_HD static inline double hyd_exp(double x) {
     const double c4 = 0x1.55555cf172b91p-5;
     const double c5 = 0x1.1111167a4d017p-7;
 
-    uint32_t abstop = (uint32_t)(hyd_as_u64(x) >> 52) & 0x7ffu;
     /* 0x3c9 = top12(2^-54), 0x408 = top12(512), 0x409 = top12(1024) */
-    if (abstop - 0x3c9u >= 0x408u - 0x3c9u) {
-        if (abstop - 0x3c9u >= 0x80000000u) return 1.0 + x; /* tiny |x| */
-        if (abstop >= 0x409u) {
-            if (hyd_as_u64(x) == 0xfff0000000000000ull) return 0.0; /* -inf */
-            if (abstop >= 0x7ffu) return 1.0 + x;                  /* +inf, nan */
-            if (hyd_as_u64(x) >> 63) return 0.0;                    /* underflow */
-            return hyd_as_double(0x7ff0000000000000ull);            /* overflow */
-        }
-        abstop = 0; /* 512 <= |x| < 1024: rescaled below */
-    }
+    const uint32_t abstop = (uint32_t)(hyd_as_u64(x) >> 52) & 0x7ffu;
+    /* The main path runs for every input (its integer steps stay in range
+     * for any x); the cases it does not cover replace its result afterwards:
+     * tiny |x| by a select (no branch: on the GPU about half the lanes of the
+     * map chain take it), |x| >= 512, inf and nan by a branch that is rare. */
     double kd = HYD_FMA(inv_ln2_n, x, shift);
     uint64_t ki = hyd_as_u64(kd);
     kd -= shift;
@@ -119,9 +113,18 @@ HYD_HD static inline double hyd_exp(double x) {
     uint64_t sbits = hyd_exp_tab(idx + 1) + top;
     double r2 = r * r;
     double tmp = HYD_FMA(r2 * r2, HYD_FMA(r, c5, c4), HYD_FMA(r2, HYD_FMA(r, c3, c2), tail + r));
-    if (abstop == 0) return hyd_exp_special(tmp, sbits, ki);
     double scale = hyd_as_double(sbits);
-    return HYD_FMA(scale, tmp, scale);
+    double y = HYD_FMA(scale, tmp, scale);
+    const double tiny = 1.0 + x;
+    y = abstop < 0x3c9u ? tiny : y; /* tiny |x| */
+    if (abstop >= 0x408u) {
+        if (abstop == 0x408u) return hyd_exp_special(tmp, sbits, ki); /* 512 <= |x| < 1024 */
+        if (hyd_as_u64(x) == 0xfff0000000000000ull) return 0.0;      /* -inf */
+        if (abstop >= 0x7ffu) return 1.0 + x;                       /* +inf, nan */
+        if (hyd_as_u64(x) >> 63) return 0.0;                         /* underflow */
+        return hyd_as_double(0x7ff0000000000000ull);                 /* overflow */
+    }
+    return y;
 }
 
 #endif /* HYD_EXP_H */

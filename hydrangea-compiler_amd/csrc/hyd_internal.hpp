@@ -79,20 +79,51 @@ __host__ __device__ inline uint64_t splitmix_at(uint64_t seed, uint64_t counter)
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
-// C's truncating remainder x % m (CodegenC.hs:2473-2474) for m >= 1, fast:
-// a mask when m is a power of two (pow2 = m - 1, else 0), 32-bit division
-// when x and m fit in 32 bits, the 64-bit division otherwise.  Negative x
-// gives the C result (in (-m, 0]).
-__device__ __forceinline__ int64_t c_mod(int64_t x, int64_t m, int64_t pow2) {
-    if (pow2) {
-        const int64_t r = x & pow2;
-        return (x < 0 && r) ? r - m : r;
-    }
-    if ((uint64_t)(x + 0x80000000ll) < 0x100000000ull && m <= 0x7fffffff)
-        return (int64_t)((int32_t)x % (int32_t)m);
-    return x % m;
+// C's truncating remainder x % m (CodegenC.hs:2473-2474) for an invariant
+// divisor m >= 1, without a division or a branch: unsigned division of |x| by
+// multiplication with a precomputed magic number (Granlund & Montgomery,
+// "Division by invariant integers using multiplication", PLDI 1994, fig. 4.1:
+// l = ceil(log2 m), magic = floor(2^64 (2^l - m) / m) + 1, sh1 = min(l, 1),
+// sh2 = max(l - 1, 0); q = (t + ((n - t) >> sh1)) >> sh2 with t = mulhi(magic, n),
+// exact for every 0 <= n < 2^64), then the sign of x (result in (-m, m)).
+// A branch per element would keep the compiler from issuing the loads of a
+// streaming loop ahead of the arithmetic, so callers pick the form once per
+// kernel or tile: c_mod_pow2 (a mask) when `mask` is set (m a power of two,
+// m >= 2), else c_mod.
+struct ModConst {
+    uint64_t m;
+    uint64_t magic;
+    uint32_t sh1, sh2;
+    uint64_t mask;  // m - 1 when m >= 2 is a power of two, else 0
+};
+
+inline ModConst make_mod_const(int64_t m) {
+    ModConst k;
+    k.m = (uint64_t)m;
+    k.mask = (m >= 2 && (m & (m - 1)) == 0) ? (uint64_t)m - 1 : 0;
+    uint32_t l = 0;
+    while (l < 64 && ((uint64_t)1 << l) < k.m) ++l;
+    const unsigned __int128 num = ((unsigned __int128)(((unsigned __int128)1 << l) - k.m)) << 64;
+    k.magic = (uint64_t)(num / k.m) + 1;
+    k.sh1 = l < 1 ? l : 1;
+    k.sh2 = l > 0 ? l - 1 : 0;
+    return k;
 }
-inline int64_t pow2_mask(int64_t m) { return (m > 0 && (m & (m - 1)) == 0) ? m - 1 : 0; }
+
+__host__ __device__ __forceinline__ int64_t c_mod(int64_t x, const ModConst& k) {
+    const uint64_t n = x < 0 ? 0 - (uint64_t)x : (uint64_t)x;
+    const uint64_t t = (uint64_t)(((unsigned __int128)k.magic * n) >> 64);
+    const uint64_t q = (t + ((n - t) >> k.sh1)) >> k.sh2;
+    const uint64_t r = n - q * k.m;
+    return x < 0 ? -(int64_t)r : (int64_t)r;
+}
+
+// x % m for m = k.mask + 1 a power of two (two's complement: the low bits,
+// moved into (-m, 0] for a negative x with a nonzero remainder).
+__host__ __device__ __forceinline__ int64_t c_mod_pow2(int64_t x, const ModConst& k) {
+    const int64_t r = x & (int64_t)k.mask;
+    return (x < 0 && r) ? r - (int64_t)k.m : r;
+}
 
 // 16-byte streaming accesses (non-temporal: read or written once).
 typedef double v2d __attribute__((ext_vector_type(2)));

@@ -55,54 +55,65 @@ __device__ __forceinline__ double block_sum(double v) {
     return s;
 }
 
-template <bool VEC>
+// Launch shape of the streaming kernels: a few waves per CU, each thread with
+// STREAM_DEPTH 16-byte pairs (per array) in flight.  Measured on MI355X, HBM
+// efficiency falls as the number of concurrent waves grows (more open DRAM
+// pages), so the grid is sized in workgroups per CU, not by the data.
+constexpr int kStreamDepth = 4;
+constexpr int kDotPerCU = 1;
+constexpr int kMapPerCU = 8;
+constexpr int kMapDepth = 2;
+constexpr int kCopyPerCU = 1;
+
+// Grid for `work` pairs: at most `per_cu` workgroups per CU, at least one.
+inline int64_t stream_grid(int64_t work, int depth, int per_cu) {
+    return std::max<int64_t>(1, std::min<int64_t>(ceil_div(work, (int64_t)kBlock * depth),
+                                                  (int64_t)device_info().num_cus * per_cu));
+}
+
+template <bool VEC, int D>
 __global__ __launch_bounds__(kBlock) void dot_partial_kernel(const double* __restrict__ xs,
                                                              const double* __restrict__ ys,
                                                              int64_t n, double* __restrict__ partial) {
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    // four partial sums per thread; pair k of an iteration feeds a[2k % 4],
+    // a[(2k + 1) % 4] (fixed assignment: deterministic)
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
     const int64_t step = (int64_t)gridDim.x * kBlock;
     int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (VEC) {
-        const double2* x2 = reinterpret_cast<const double2*>(xs);
-        const double2* y2 = reinterpret_cast<const double2*>(ys);
         const int64_t npairs = n >> 1;
-        // 8 independent 16-byte loads in flight per thread
-        for (; p + 3 * step < npairs; p += 4 * step) {
-            v2d xa = ld_stream2(xs + 2 * p), ya = ld_stream2(ys + 2 * p);
-            v2d xb = ld_stream2(xs + 2 * (p + step)), yb = ld_stream2(ys + 2 * (p + step));
-            v2d xc = ld_stream2(xs + 2 * (p + 2 * step)), yc = ld_stream2(ys + 2 * (p + 2 * step));
-            v2d xd = ld_stream2(xs + 2 * (p + 3 * step)), yd = ld_stream2(ys + 2 * (p + 3 * step));
-            double m0 = xa.x * ya.x, m1 = xa.y * ya.y;
-            double m2 = xb.x * yb.x, m3 = xb.y * yb.y;
-            a0 = a0 + m0;
-            a1 = a1 + m1;
-            a2 = a2 + m2;
-            a3 = a3 + m3;
-            double m4 = xc.x * yc.x, m5 = xc.y * yc.y;
-            double m6 = xd.x * yd.x, m7 = xd.y * yd.y;
-            a0 = a0 + m4;
-            a1 = a1 + m5;
-            a2 = a2 + m6;
-            a3 = a3 + m7;
+        for (; p + (D - 1) * step < npairs; p += D * step) {
+            v2d x[D], y[D];
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                x[k] = ld_stream2(xs + 2 * (p + k * step));
+                y[k] = ld_stream2(ys + 2 * (p + k * step));
+            }
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const double m0 = x[k].x * y[k].x, m1 = x[k].y * y[k].y;
+                a[(2 * k) & 3] = a[(2 * k) & 3] + m0;
+                a[(2 * k + 1) & 3] = a[(2 * k + 1) & 3] + m1;
+            }
         }
         for (; p < npairs; p += step) {
-            double2 xa = x2[p], ya = y2[p];
-            double m0 = xa.x * ya.x, m1 = xa.y * ya.y;
-            a0 = a0 + m0;
-            a1 = a1 + m1;
+            const v2d x = ld_stream2(xs + 2 * p), y = ld_stream2(ys + 2 * p);
+            const double m0 = x.x * y.x, m1 = x.y * y.y;
+            a[0] = a[0] + m0;
+            a[1] = a[1] + m1;
         }
         if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
-            double m = xs[n - 1] * ys[n - 1];
-            a2 = a2 + m;
+            const double m = xs[n - 1] * ys[n - 1];
+            a[2] = a[2] + m;
         }
     } else {
         for (; p < n; p += step) {
-            double m = xs[p] * ys[p];
-            a0 = a0 + m;
+            const double m = xs[p] * ys[p];
+            a[0] = a[0] + m;
         }
     }
-    double t = (a0 + a1) + (a2 + a3);
-    double s = block_sum<kBlock>(t);
+    const double t = (a[0] + a[1]) + (a[2] + a[3]);
+    const double s = block_sum<kBlock>(t);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
@@ -135,7 +146,20 @@ __device__ __forceinline__ double map_chain_op(double x, double y) {
     return hyd_exp(t8);
 }
 
-template <bool VEC>
+template <int D>
+__device__ __forceinline__ void load_pairs(const double* __restrict__ xs, const double* __restrict__ ys,
+                                           int64_t p, int64_t step, v2d* x, v2d* y) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        x[k] = ld_stream2(xs + 2 * (p + k * step));
+        y[k] = ld_stream2(ys + 2 * (p + k * step));
+    }
+}
+
+// The chain's math (a correctly rounded sqrt, the table exp) is heavy enough
+// that this kernel wants more waves per CU than the pure streaming kernels:
+// measured best at 8 workgroups per CU with 2 pairs in flight per thread.
+template <bool VEC, int D>
 __global__ __launch_bounds__(kBlock) void map_chain_kernel(const double* __restrict__ xs,
                                                            const double* __restrict__ ys,
                                                            int64_t n, double* __restrict__ out) {
@@ -143,20 +167,20 @@ __global__ __launch_bounds__(kBlock) void map_chain_kernel(const double* __restr
     int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (VEC) {
         const int64_t npairs = n >> 1;
-        // two pairs per iteration: four 16-byte loads in flight before the math
-        for (; p + step < npairs; p += 2 * step) {
-            v2d xa = ld_stream2(xs + 2 * p), ya = ld_stream2(ys + 2 * p);
-            v2d xb = ld_stream2(xs + 2 * (p + step)), yb = ld_stream2(ys + 2 * (p + step));
-            v2d ra, rb;
-            ra.x = map_chain_op(xa.x, ya.x);
-            ra.y = map_chain_op(xa.y, ya.y);
-            rb.x = map_chain_op(xb.x, yb.x);
-            rb.y = map_chain_op(xb.y, yb.y);
-            st_stream2(out + 2 * p, ra);
-            st_stream2(out + 2 * (p + step), rb);
+        // D pairs per iteration: 2D 16-byte loads in flight before the math
+        for (; p + (D - 1) * step < npairs; p += D * step) {
+            v2d x[D], y[D];
+            load_pairs<D>(xs, ys, p, step, x, y);
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                v2d r;
+                r.x = map_chain_op(x[k].x, y[k].x);
+                r.y = map_chain_op(x[k].y, y[k].y);
+                st_stream2(out + 2 * (p + k * step), r);
+            }
         }
         for (; p < npairs; p += step) {
-            v2d x = ld_stream2(xs + 2 * p), y = ld_stream2(ys + 2 * p);
+            const v2d x = ld_stream2(xs + 2 * p), y = ld_stream2(ys + 2 * p);
             v2d r;
             r.x = map_chain_op(x.x, y.x);
             r.y = map_chain_op(x.y, y.y);
@@ -172,17 +196,17 @@ inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 
 
 // Plain streaming copy: the bandwidth reference the fused kernels are read
 // against (and the calibration kernel for the HBM byte counters).
+template <int D>
 __global__ __launch_bounds__(kBlock) void copy_kernel(const double* __restrict__ src, int64_t npairs,
                                                       double* __restrict__ dst) {
     const int64_t step = (int64_t)gridDim.x * kBlock;
     int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    for (; p + 3 * step < npairs; p += 4 * step) {
-        v2d a = ld_stream2(src + 2 * p), b = ld_stream2(src + 2 * (p + step));
-        v2d c = ld_stream2(src + 2 * (p + 2 * step)), d = ld_stream2(src + 2 * (p + 3 * step));
-        st_stream2(dst + 2 * p, a);
-        st_stream2(dst + 2 * (p + step), b);
-        st_stream2(dst + 2 * (p + 2 * step), c);
-        st_stream2(dst + 2 * (p + 3 * step), d);
+    for (; p + (D - 1) * step < npairs; p += D * step) {
+        v2d a[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) a[k] = ld_stream2(src + 2 * (p + k * step));
+#pragma unroll
+        for (int k = 0; k < D; ++k) st_stream2(dst + 2 * (p + k * step), a[k]);
     }
     for (; p < npairs; p += step) st_stream2(dst + 2 * p, ld_stream2(src + 2 * p));
 }
@@ -329,10 +353,9 @@ extern "C" int hydhip_copy_f64(const double* src, int64_t n, double* dst, void* 
     if (n < 0 || (n & 1) || (n > 0 && (!src || !dst)) || !aligned16(src) || !aligned16(dst))
         return set_error(HYD_E_INVALID, "hydhip_copy_f64: needs an even count and 16-byte aligned pointers");
     if (n == 0) return HYD_OK;
-    int64_t npairs = n >> 1;
-    int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(npairs, kBlock * 4),
-                                                            (int64_t)device_info().num_cus * 8));
-    copy_kernel<<<(unsigned)blocks, kBlock, 0, resolve_stream(stream)>>>(src, npairs, dst);
+    const int64_t npairs = n >> 1;
+    const int64_t blocks = stream_grid(npairs, kStreamDepth, kCopyPerCU);
+    copy_kernel<kStreamDepth><<<(unsigned)blocks, kBlock, 0, resolve_stream(stream)>>>(src, npairs, dst);
     HYD_CHECK_LAUNCH();
     return HYD_OK;
 }
@@ -350,17 +373,15 @@ extern "C" int hydhip_dot_f64(const double* xs, const double* ys, int64_t n, dou
     if (n < 0 || !out || (n > 0 && (!xs || !ys)))
         return set_error(HYD_E_INVALID, "hydhip_dot_f64: bad arguments");
     hipStream_t s = resolve_stream(stream);
-    const DeviceInfo& di = device_info();
     // Fixed partial count for a given n and device: deterministic results.
-    int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, (int64_t)kBlock * 8),
-                                                            (int64_t)di.num_cus * 8));
+    const bool vec = aligned16(xs) && aligned16(ys);
+    const int64_t blocks = stream_grid(vec ? (n >> 1) : n, vec ? kStreamDepth : 1, kDotPerCU);
     double* partial = static_cast<double*>(stream_workspace(s, (size_t)blocks * sizeof(double)));
     if (!partial) return set_error(HYD_E_OOM, "hydhip_dot_f64: workspace allocation failed");
-    bool vec = aligned16(xs) && aligned16(ys);
     if (vec)
-        dot_partial_kernel<true><<<(unsigned)blocks, kBlock, 0, s>>>(xs, ys, n, partial);
+        dot_partial_kernel<true, kStreamDepth><<<(unsigned)blocks, kBlock, 0, s>>>(xs, ys, n, partial);
     else
-        dot_partial_kernel<false><<<(unsigned)blocks, kBlock, 0, s>>>(xs, ys, n, partial);
+        dot_partial_kernel<false, kStreamDepth><<<(unsigned)blocks, kBlock, 0, s>>>(xs, ys, n, partial);
     HYD_CHECK_LAUNCH();
     dot_final_kernel<<<1, kFinalBlock, 0, s>>>(partial, (int)blocks, init, out);
     HYD_CHECK_LAUNCH();
@@ -374,15 +395,12 @@ extern "C" int hydhip_map_chain_f64(const double* xs, const double* ys, int64_t 
         return set_error(HYD_E_INVALID, "hydhip_map_chain_f64: bad arguments");
     if (n == 0) return HYD_OK;
     hipStream_t s = resolve_stream(stream);
-    const DeviceInfo& di = device_info();
-    bool vec = aligned16(xs) && aligned16(ys) && aligned16(out);
-    int64_t work = vec ? (n >> 1) : n;
-    int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(work, kBlock),
-                                                            (int64_t)di.num_cus * 16));
+    const bool vec = aligned16(xs) && aligned16(ys) && aligned16(out);
+    const int64_t blocks = stream_grid(vec ? (n >> 1) : n, vec ? kMapDepth : 1, kMapPerCU);
     if (vec)
-        map_chain_kernel<true><<<(unsigned)blocks, kBlock, 0, s>>>(xs, ys, n, out);
+        map_chain_kernel<true, kMapDepth><<<(unsigned)blocks, kBlock, 0, s>>>(xs, ys, n, out);
     else
-        map_chain_kernel<false><<<(unsigned)blocks, kBlock, 0, s>>>(xs, ys, n, out);
+        map_chain_kernel<false, kMapDepth><<<(unsigned)blocks, kBlock, 0, s>>>(xs, ys, n, out);
     HYD_CHECK_LAUNCH();
     return HYD_OK;
 }

@@ -28,19 +28,21 @@ namespace {
 
 // route = x % bins (C truncating remainder, CodegenC.hs:2473-2474),
 // value = x * x computed mod 2^64 (the C oracle is built with -fwrapv).
+// POW2: bins is a power of two >= 2 (mask form of the remainder).
+template <bool POW2>
 struct ModSquare {
-    int64_t bins;
-    int64_t pow2;  // bins - 1 when bins is a power of two, else 0
+    ModConst mod;
     __device__ __forceinline__ void operator()(int64_t x, int64_t& b, uint64_t& v) const {
-        b = c_mod(x, bins, pow2);
+        b = POW2 ? c_mod_pow2(x, mod) : c_mod(x, mod);
         v = (uint64_t)x * (uint64_t)x;
     }
 };
 
 // Fused source: one int64 stream produces both route and value.
+template <bool POW2>
 struct FusedSrc {
     const int64_t* __restrict__ data;
-    ModSquare fn;
+    ModSquare<POW2> fn;
     __device__ __forceinline__ void pair(int64_t p, int64_t& b0, uint64_t& v0, int64_t& b1,
                                          uint64_t& v1) const {
         longlong2 x = reinterpret_cast<const longlong2*>(data)[p];
@@ -80,22 +82,24 @@ __device__ __forceinline__ void global_add(unsigned long long* p, uint64_t v) {
 }
 
 // Streams the elements assigned to this workgroup (grid-stride over 16-byte
-// pairs, 4 pairs in flight per thread) and calls add(bin, value).
+// pairs, kDepth pairs in flight per thread: with one 512-thread workgroup
+// per CU that is 64 KiB of loads in flight per CU) and calls add(bin, value).
+constexpr int kDepth = 8;
+
 template <int BLOCK, bool VEC, class Src, class Add>
 __device__ __forceinline__ void stream_elements(const Src& src, int64_t n, Add add) {
     const int64_t step = (int64_t)gridDim.x * BLOCK;
     int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (VEC) {
         const int64_t npairs = n >> 1;
-        for (; p + 3 * step < npairs; p += 4 * step) {
-            int64_t b[8];
-            uint64_t v[8];
-            src.pair(p, b[0], v[0], b[1], v[1]);
-            src.pair(p + step, b[2], v[2], b[3], v[3]);
-            src.pair(p + 2 * step, b[4], v[4], b[5], v[5]);
-            src.pair(p + 3 * step, b[6], v[6], b[7], v[7]);
+        for (; p + (kDepth - 1) * step < npairs; p += kDepth * step) {
+            int64_t b[2 * kDepth];
+            uint64_t v[2 * kDepth];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) add(b[k], v[k]);
+            for (int k = 0; k < kDepth; ++k)
+                src.pair(p + k * step, b[2 * k], v[2 * k], b[2 * k + 1], v[2 * k + 1]);
+#pragma unroll
+            for (int k = 0; k < 2 * kDepth; ++k) add(b[k], v[k]);
         }
         for (; p < npairs; p += step) {
             int64_t b0, b1;
@@ -305,10 +309,14 @@ extern "C" int hydhip_weighted_histogram_i64(const int64_t* data, int64_t n, int
                          (long long)n, (long long)nbins);
     hipStream_t s = resolve_stream(stream);
     if (n == 0) return fill_zero(hist, (size_t)nbins * 8, s);  // fill [nbins] 0
-    FusedSrc src{data, ModSquare{nbins, pow2_mask(nbins)}};
     // fill [nbins] 0 semantics: the LDS form writes every bin, the others
     // zero the destination first (with a kernel, so graph capture works)
-    return launch_scatter(src, aligned16(data), n, nbins, hist, strategy, true, s);
+    const ModConst mod = make_mod_const(nbins);
+    if (mod.mask)
+        return launch_scatter(FusedSrc<true>{data, ModSquare<true>{mod}}, aligned16(data), n, nbins,
+                              hist, strategy, true, s);
+    return launch_scatter(FusedSrc<false>{data, ModSquare<false>{mod}}, aligned16(data), n, nbins,
+                          hist, strategy, true, s);
 }
 
 extern "C" int hydhip_scatter_add_i64(const int64_t* idx, const int64_t* vals, int64_t n,
@@ -320,4 +328,13 @@ extern "C" int hydhip_scatter_add_i64(const int64_t* idx, const int64_t* vals, i
     PairSrc src{idx, vals};
     return launch_scatter(src, aligned16(idx) && aligned16(vals), n, ndst, dst, strategy, false,
                           resolve_stream(stream));
+}
+
+// The remainder the kernels route with, compiled for the host CPU.
+extern "C" int hydhip_host_mod_i64(const int64_t* x, int64_t n, int64_t m, int64_t* out) {
+    if (n < 0 || m < 1 || (n > 0 && (!x || !out)))
+        return set_error(HYD_E_INVALID, "hydhip_host_mod_i64: bad arguments");
+    const ModConst k = make_mod_const(m);
+    for (int64_t i = 0; i < n; ++i) out[i] = k.mask ? c_mod_pow2(x[i], k) : c_mod(x[i], k);
+    return HYD_OK;
 }

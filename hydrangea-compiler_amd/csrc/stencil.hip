@@ -169,24 +169,104 @@ __global__ __launch_bounds__(64 * kWaves) void stencil2d_clamp_kernel2(const dou
     }
 }
 
+// Whole-wave lane shifts (DPP wave_shr:1 / wave_shl:1, two 32-bit moves per
+// double): lane i receives lane i-1's value (from_lower_lane) or lane i+1's
+// (from_upper_lane); the lane without a source receives 0 (bound_ctrl).
+__device__ __forceinline__ double dpp_shift(double v, int ctrl_is_shr) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    int lo = (int)(unsigned)b, hi = (int)(unsigned)(b >> 32);
+    if (ctrl_is_shr) {
+        lo = __builtin_amdgcn_mov_dpp(lo, 0x138, 0xf, 0xf, true);
+        hi = __builtin_amdgcn_mov_dpp(hi, 0x138, 0xf, 0xf, true);
+    } else {
+        lo = __builtin_amdgcn_mov_dpp(lo, 0x130, 0xf, 0xf, true);
+        hi = __builtin_amdgcn_mov_dpp(hi, 0x130, 0xf, 0xf, true);
+    }
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double from_lower_lane(double v) { return dpp_shift(v, 1); }
+__device__ __forceinline__ double from_upper_lane(double v) { return dpp_shift(v, 0); }
+
 // Temporal blocking: `steps` (<= kTK) stencil steps in one launch.  A
-// workgroup owns a kTT x kTT output tile and loads it with a kTK-cell halo
-// (a 64 x 64 box, lanes = columns, each of the 4 waves holding 16 rows in
-// registers).  Each step recomputes the whole box: north/south come from the
-// wave's own registers (the rows just outside its strip are exchanged
-// through LDS), west/east from the neighbouring lanes; clamping applies at
-// the grid edges.  The box edges go stale by one cell per step, but the
-// region still exact after step s is the box shrunk by s cells (not shrunk at
-// grid edges), so after kTK steps the output tile holds exactly the values
-// of kTK single-step launches - same operations on the same neighbour
-// values, bit for bit.  HBM sees one read of the haloed box and one write of
-// the tile per `steps` steps.
-constexpr int kTK = 8;                // fused steps
-constexpr int kTT = 48;               // output tile edge
-constexpr int kTL = kTT + 2 * kTK;    // box edge: 64 = one column per lane
-constexpr int kTRows = 16;            // rows per wave
-constexpr int kTWaves = kTL / kTRows; // 4
+// workgroup owns a kTR x kTC output tile and loads it with a kTK-cell halo:
+// a 64-row x 128-column box, lane l holding columns 2l and 2l+1 of the box,
+// each of the 4 waves 16 rows, all in registers.  Each step recomputes the
+// whole box: north/south come from the wave's own registers (the rows just
+// outside its strip are exchanged through LDS), west/east from the pair
+// itself or the neighbouring lane (DPP); clamping applies at the grid edges
+// (EDGE workgroups only - interior boxes touch no grid edge).  The box edges
+// go stale by one cell per step, but the region still exact after step s is
+// the box shrunk by s cells (not shrunk at grid edges), so after kTK steps the
+// output tile holds exactly the values of kTK single-step launches: the same
+// operations on the same neighbour values, bit for bit.  HBM sees one read of
+// the haloed box and one write of the tile per `steps` steps.
+constexpr int kTK = 8;                  // fused steps
+constexpr int kTRows = 16;              // rows per wave
+constexpr int kTWaves = 4;
+constexpr int kTBoxR = kTRows * kTWaves;  // 64
+constexpr int kTBoxC = 128;               // 2 columns per lane
+constexpr int kTR = kTBoxR - 2 * kTK;     // 48 output rows
+constexpr int kTC = kTBoxC - 2 * kTK;     // 112 output columns
 constexpr int kTBlock = 64 * kTWaves;
+
+template <class Body, bool EDGE>
+__device__ __forceinline__ void temporal_box(const double* __restrict__ in, int64_t h, int64_t w,
+                                             int steps, Body body, double* __restrict__ out,
+                                             int64_t r0, int64_t c0, v2d (*halo)[kTWaves][2][64]) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t c = c0 - kTK + 2 * lane;                    // this lane's first column (even)
+    const int64_t rbase = r0 - kTK + (int64_t)wv * kTRows;  // this wave's first row
+    v2d v[kTRows];
+#pragma unroll
+    for (int k = 0; k < kTRows; ++k) {
+        const int64_t r = rbase + k;
+        if (EDGE) {
+            const bool row_in = r >= 0 && r < h;
+            v[k].x = (row_in && c >= 0 && c < w) ? in[r * w + c] : 0.0;
+            v[k].y = (row_in && c + 1 >= 0 && c + 1 < w) ? in[r * w + c + 1] : 0.0;
+        } else {
+            v[k].x = in[r * w + c];
+            v[k].y = in[r * w + c + 1];
+        }
+    }
+    // grid-edge flags (EDGE only): x-cell at column c, y-cell at column c + 1
+    const bool x_west_edge = c == 0, x_east_edge = c == w - 1, y_east_edge = c + 1 == w - 1;
+    for (int s = 0; s < steps; ++s) {
+        const int p = s & 1;
+        halo[p][wv][0][lane] = v[0];
+        halo[p][wv][1][lane] = v[kTRows - 1];
+        __syncthreads();  // parity-double-buffered: one barrier per step
+        v2d prev = wv > 0 ? halo[p][wv - 1][1][lane] : v[0];  // row above the strip
+        const v2d dn = wv < kTWaves - 1 ? halo[p][wv + 1][0][lane] : v[kTRows - 1];
+#pragma unroll
+        for (int k = 0; k < kTRows; ++k) {
+            const v2d cur = v[k];
+            v2d north = prev, south = (k + 1 < kTRows) ? v[k + 1] : dn;
+            double west_x = from_lower_lane(cur.y), east_x = cur.y;
+            double west_y = cur.x, east_y = from_upper_lane(cur.x);
+            if (EDGE) {
+                const int64_t r = rbase + k;
+                if (r == 0) north = cur;
+                if (r == h - 1) south = cur;
+                if (x_west_edge) west_x = cur.x;
+                if (x_east_edge) east_x = cur.x;
+                if (y_east_edge) east_y = cur.y;
+            }
+            prev = cur;
+            v[k].x = body(north.x, south.x, west_x, east_x, cur.x);
+            v[k].y = body(north.y, south.y, west_y, east_y, cur.y);
+        }
+    }
+    const bool col_out = c >= c0 && c < c0 + kTC;
+#pragma unroll
+    for (int k = 0; k < kTRows; ++k) {
+        const int64_t r = rbase + k;
+        if (col_out && r >= r0 && r < r0 + kTR && r < h) {
+            if (c < w) out[r * w + c] = v[k].x;
+            if (c + 1 < w) out[r * w + c + 1] = v[k].y;
+        }
+    }
+}
 
 template <class Body>
 __global__ __launch_bounds__(kTBlock) void stencil2d_temporal_kernel(const double* __restrict__ in,
@@ -194,53 +274,21 @@ __global__ __launch_bounds__(kTBlock) void stencil2d_temporal_kernel(const doubl
                                                                      Body body,
                                                                      double* __restrict__ out) {
     // [step parity][wave][first/last row of the strip][lane]
-    __shared__ double halo[2][kTWaves][2][64];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int64_t r0 = (int64_t)blockIdx.y * kTT, c0 = (int64_t)blockIdx.x * kTT;
-    const int64_t c = c0 - kTK + lane;                    // this lane's column
-    const int64_t rbase = r0 - kTK + (int64_t)wv * kTRows;  // this wave's first row
-    const bool col_in = c >= 0 && c < w;
-    double v[kTRows];
-#pragma unroll
-    for (int k = 0; k < kTRows; ++k) {
-        const int64_t r = rbase + k;
-        v[k] = (col_in && r >= 0 && r < h) ? in[r * w + c] : 0.0;
-    }
-    const bool west_edge = c == 0, east_edge = c == w - 1;
-    for (int s = 0; s < steps; ++s) {
-        const int p = s & 1;
-        halo[p][wv][0][lane] = v[0];
-        halo[p][wv][1][lane] = v[kTRows - 1];
-        __syncthreads();  // (parity buffers: one barrier per step suffices)
-        double prev = wv > 0 ? halo[p][wv - 1][1][lane] : v[0];           // row above the strip
-        const double dn = wv < kTWaves - 1 ? halo[p][wv + 1][0][lane] : v[kTRows - 1];
-#pragma unroll
-        for (int k = 0; k < kTRows; ++k) {
-            const int64_t r = rbase + k;
-            const double cur = v[k];
-            const double below = (k + 1 < kTRows) ? v[k + 1] : dn;
-            const double north = r == 0 ? cur : prev;
-            const double south = r == h - 1 ? cur : below;
-            double west = __shfl_up(cur, 1, 64);
-            double east = __shfl_down(cur, 1, 64);
-            if (west_edge) west = cur;
-            if (east_edge) east = cur;
-            prev = cur;
-            v[k] = body(north, south, west, east, cur);
-        }
-    }
-    const bool col_out = c >= c0 && c < c0 + kTT && c < w;
-#pragma unroll
-    for (int k = 0; k < kTRows; ++k) {
-        const int64_t r = rbase + k;
-        if (col_out && r >= r0 && r < r0 + kTT && r < h) out[r * w + c] = v[k];
-    }
+    __shared__ v2d halo[2][kTWaves][2][64];
+    const int64_t r0 = (int64_t)blockIdx.y * kTR, c0 = (int64_t)blockIdx.x * kTC;
+    // interior: every box cell is inside the grid and off its edge rows/columns
+    const bool interior = r0 - kTK >= 1 && r0 - kTK + kTBoxR <= h - 1 && c0 - kTK >= 1 &&
+                          c0 - kTK + kTBoxC <= w - 1;
+    if (interior)
+        temporal_box<Body, false>(in, h, w, steps, body, out, r0, c0, halo);
+    else
+        temporal_box<Body, true>(in, h, w, steps, body, out, r0, c0, halo);
 }
 
 template <class Body>
 int launch_temporal(const double* in, int64_t h, int64_t w, int steps, Body body, double* out,
                     hipStream_t s) {
-    int64_t gx = ceil_div(w, kTT), gy = ceil_div(h, kTT);
+    int64_t gx = ceil_div(w, kTC), gy = ceil_div(h, kTR);
     if (gx > 0x7fffffff || gy > 65535)
         return set_error(HYD_E_INVALID, "stencil: grid %lld x %lld too large", (long long)h, (long long)w);
     stencil2d_temporal_kernel<Body><<<dim3((unsigned)gx, (unsigned)gy), kTBlock, 0, s>>>(in, h, w, steps,

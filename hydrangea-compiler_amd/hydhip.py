@@ -135,6 +135,7 @@ _SIGS = {
     "hydhip_map_chain_f64": (_INT, [_P, _P, _I64, _P, _P]),
     "hydhip_math_f64": (_INT, [_INT, _P, _I64, _P, _P]),
     "hydhip_host_exp": (None, [_P, _I64, _P]),
+    "hydhip_host_mod_i64": (_INT, [_P, _I64, _I64, _P]),
     "hydhip_copy_f64": (_INT, [_P, _I64, _P, _P]),
     "hydhip_stencil2d_clamp_f64": (_INT, [_P, _I64, _I64, _INT, _D, _P, _P]),
     "hydhip_stencil2d_iterate_f64": (_INT, [_P, _I64, _I64, _INT, _D, _I64, _P, _P, _INT,

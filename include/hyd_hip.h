@@ -106,6 +106,9 @@ int hydhip_map_chain_f64(const double* xs, const double* ys, int64_t n,
 int hydhip_math_f64(int op, const double* x, int64_t n, double* out, void* stream);
 /* The same exp on the host CPU (identical operation sequence). */
 void hydhip_host_exp(const double* x, int64_t n, double* out);
+/* The histograms' route x % m (C truncating remainder, m >= 1) by the
+ * division-free method the kernels use, run on the host CPU: out[i] = x[i] % m. */
+int hydhip_host_mod_i64(const int64_t* x, int64_t n, int64_t m, int64_t* out);
 /* Streaming copy dst[i] = src[i] (n even, 16-byte aligned): the bandwidth
  * reference for the fused kernels. */
 int hydhip_copy_f64(const double* src, int64_t n, double* dst, void* stream);

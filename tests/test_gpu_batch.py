@@ -56,6 +56,21 @@ def test_batched_chunks_and_bin_ranges(hyd, orc, policy):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("policy", [0, 1, 2])
+def test_batched_wide_values(hyd, orc, policy):
+    # 63-bit values, half negated: routes through both remainder forms (mask
+    # for power-of-two bin counts, multiply-shift otherwise) over the full range
+    lens = np.array([5000, 70_001, 3, 20_000, 1 << 17, 999], np.int64)
+    bins = np.array([64, 1000, 7, 65536, 40_000, (1 << 20) + 3], np.int64)
+    off, hoff = hyd.batch_offsets(lens, bins)
+    data = orc.splitmix_i64(int(lens.sum()), 9191, bits=63)
+    data[::2] = -data[::2]
+    d = hyd.DeviceArray.from_numpy(data)
+    got = hyd.batched_histogram(d, off, lens, bins, hoff, policy=policy).numpy()
+    ref, _ = orc.batched_histograms(data, off, lens, bins)
+    assert np.array_equal(got, ref)
+
+
 def test_batched_empty_and_zero_length(hyd, orc):
     lens = np.array([0, 5, 0, 1], np.int64)
     bins = np.array([3, 64, 1, 4096], np.int64)

@@ -123,6 +123,42 @@ def test_host_exp_matches_libm(orc):
     assert same.all(), x[~same][:5]
 
 
+def c_rem(x, m):
+    """C's truncating remainder (sign of the dividend), numpy int64."""
+    r = np.abs(x) % m  # |INT64_MIN| wraps to itself; handled below
+    r = np.where(x == np.iinfo(np.int64).min, (-(x + m)) % m, r)
+    return np.where(x < 0, -r, r)
+
+
+def test_host_mod_matches_c_remainder():
+    # The histograms route with a division-free remainder (multiply by a
+    # precomputed magic number); it must equal C's `%` for every int64 and
+    # every divisor the kernels accept (1 <= m <= 2^31).
+    rng = np.random.default_rng(11)
+    lo, hi = np.iinfo(np.int64).min, np.iinfo(np.int64).max
+    edge = np.array([0, 1, -1, 2, -2, 63, 64, 65, -63, -64, -65, lo, lo + 1, hi, hi - 1,
+                     2**31 - 1, 2**31, -2**31, -2**31 - 1, 2**32, 2**32 + 1, -2**32,
+                     2**53 + 1, -(2**53) - 1, 2**62, -(2**62)], np.int64)
+    xs = np.concatenate([edge, rng.integers(lo, hi, 200_000, dtype=np.int64, endpoint=True),
+                         rng.integers(-2**20, 2**20, 50_000, dtype=np.int64)])
+    divisors = [1, 2, 3, 5, 7, 10, 63, 64, 65, 100, 255, 256, 1000, 4095, 4096, 4097, 65535,
+                65536, 65537, 1 << 20, (1 << 20) + 7, 2**31 - 1, 2**31]
+    divisors += [int(v) for v in rng.integers(2, 2**31, 20)]
+    lib = hydhip.lib()
+    got = np.empty_like(xs)
+    for m in divisors:
+        assert lib.hydhip_host_mod_i64(xs.ctypes.data, xs.size, m, got.ctypes.data) == hydhip.HYD_OK
+        ref = c_rem(xs, m)
+        bad = got != ref
+        assert not bad.any(), (m, xs[bad][:4], got[bad][:4], ref[bad][:4])
+    # exact Python big-integer check of the edge values
+    for m in (3, 64, 2**31 - 1, 2**31):
+        lib.hydhip_host_mod_i64(edge.ctypes.data, edge.size, m, got.ctypes.data)
+        for x, g in zip(edge.tolist(), got[:edge.size].tolist()):
+            r = abs(x) % m
+            assert g == (-r if x < 0 else r), (x, m, g)
+
+
 def test_no_gpu_reports_nodev():
     if os.path.exists("/dev/kfd"):
         pytest.skip("a GPU driver is present")
