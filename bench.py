@@ -36,6 +36,22 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel_prefix, elements):
+    """HBM bytes per launch of a kernel as recorded by the rocprofv3 counter
+    passes in profiles/hbm_traffic.json (FETCH_SIZE and WRITE_SIZE per
+    dispatch, scaled by the copy-kernel calibration), or None when that
+    record is absent or was taken on a workload of another size."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        rec = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    for name, k in rec.get("kernels", {}).items():
+        if name.startswith("void " + kernel_prefix) or name.startswith(kernel_prefix):
+            return k["traffic_bytes"] if k.get("elements") == elements else None
+    return None
+
+
 def timed_loop(fn, steps, warmup, events=None):
     """Runs fn() warmup times, then `steps` times between device syncs.
     Returns (ms_per_step, list of per-step event times in ms if events)."""
@@ -382,7 +398,7 @@ def main():
                    "cus": info["num_cus"]},
         "roofline": {"bound": "hbm", "kernel": "map_chain_kernel", "achieved": round(mc["gbs"], 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(mc["gbs"] / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+                     "traffic": measured_traffic("map_chain_kernel<true", workloads.DOTMAP["n"])},
         "cpu_baseline": cpu,
         "kernels": kernels,
     }

@@ -56,9 +56,10 @@ __device__ __forceinline__ double block_sum(double v) {
 }
 
 // Launch shape of the streaming kernels: a few waves per CU, each thread with
-// STREAM_DEPTH 16-byte pairs (per array) in flight.  Measured on MI355X, HBM
-// efficiency falls as the number of concurrent waves grows (more open DRAM
-// pages), so the grid is sized in workgroups per CU, not by the data.
+// kStreamDepth 16-byte pairs (per array) in flight.  Measured on MI355X, the
+// HBM rate of a read or copy stream falls as the number of concurrent waves
+// grows (dot: 7.3 TB/s at one 256-thread workgroup per CU, 5.8 TB/s at eight),
+// so the grid is sized in workgroups per CU, not by the data.
 constexpr int kStreamDepth = 4;
 constexpr int kDotPerCU = 1;
 constexpr int kMapPerCU = 8;

@@ -152,15 +152,27 @@ __global__ __launch_bounds__(BLOCK) void scatter_lds_kernel(Src src, int64_t n, 
 }
 
 // dst[b] = (accumulate ? dst[b] : 0) + sum of the `used` spread copies of bin
-// b; the copies are zeroed for the next launch.
-__global__ void spread_fold_kernel(unsigned long long* __restrict__ spread, int used, int nbins,
-                                   int accumulate, unsigned long long* __restrict__ dst) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nbins) return;
-    unsigned long long s = 0;
-    for (int c = 0; c < used; ++c) s += spread[(size_t)c * nbins + b];
-    for (int c = 0; c < used; ++c) spread[(size_t)c * nbins + b] = 0;
-    dst[b] = accumulate ? dst[b] + s : s;
+// b; the copies are zeroed for the next launch.  One thread per (bin, copy):
+// `lanes` (a power of two >= used, <= 64) consecutive lanes hold the copies of
+// one bin and add them with a shuffle butterfly, so the whole fold costs one
+// memory round trip instead of `used` dependent ones.
+constexpr int kFoldBlock = 256;
+
+__global__ __launch_bounds__(kFoldBlock) void spread_fold_kernel(unsigned long long* __restrict__ spread,
+                                                                 int used, int lanes, int nbins,
+                                                                 int accumulate,
+                                                                 unsigned long long* __restrict__ dst) {
+    const int64_t t = (int64_t)blockIdx.x * kFoldBlock + threadIdx.x;
+    const int64_t b = t / lanes;
+    const int c = (int)(t % lanes);
+    unsigned long long v = 0;
+    if (b < nbins && c < used) {
+        unsigned long long* p = spread + (size_t)c * nbins + b;
+        v = *p;
+        *p = 0;
+    }
+    for (int off = lanes >> 1; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (c == 0 && b < nbins) dst[b] = accumulate ? dst[b] + v : v;
 }
 
 template <int BLOCK, bool VEC, class Src>
@@ -251,8 +263,10 @@ int launch_scatter(const Src& src, bool vec, int64_t n, int64_t nbins, int64_t* 
         }
         HYD_CHECK_LAUNCH();
         const int used = (int)std::min<int64_t>(ncopy, blocks);
-        spread_fold_kernel<<<(unsigned)ceil_div(nbins, 256), 256, 0, s>>>(spread, used, (int)nbins,
-                                                                         accumulate, dst);
+        int lanes = 1;
+        while (lanes < used) lanes <<= 1;  // <= 32
+        spread_fold_kernel<<<(unsigned)ceil_div(nbins * lanes, kFoldBlock), kFoldBlock, 0, s>>>(
+            spread, used, lanes, (int)nbins, accumulate, dst);
         HYD_CHECK_LAUNCH();
         return HYD_OK;
     }
