@@ -4,19 +4,30 @@
 // Each instance k is the program of programs/weighted_histogram.hyd with its
 // own bin count: hist_k[x % bins_k] += x*x over data_k (Parallelize.hs:466-503
 // privatizes every one of them on the CPU).  On the GPU the host cuts the
-// batch into tiles, largest first, and persistent workgroups pull them from an
-// atomic queue counter.  A tile is (instance, element range, bin range, mode):
+// batch into tiles and one persistent launch (one 1024-thread workgroup per
+// CU, 128 KiB of LDS each) works through them.  A tile is (instance, element
+// range, bin range, mode):
 //   * OWNER:   the workgroup holds the tile's bin range in LDS (several copies
 //              for small bin counts), streams every element of the instance and
 //              writes the bin range with plain stores - no atomics, no zeroing.
 //              Instances with more bins than LDS holds get one tile per
-//              16384-bin range (each streams the instance once).
-//   * CHUNK:   like OWNER for one element chunk of a long instance; the chunks
-//              merge their LDS bins into the (pre-zeroed) output with one
-//              global atomic per touched bin.
-//   * GLOBAL:  one 64-bit global atomic per element into the pre-zeroed output.
-// Global atomics on gfx950 run at tens of G/s, so OWNER is preferred; the
-// autotuner (tune.hip) can still pick GLOBAL per (bin class, size class).
+//              16384-bin range; each of them streams the instance.
+//   * GLOBAL:  one 64-bit global atomic per element of an element chunk into
+//              the pre-zeroed output (the autotuner, tune.hip, can pick it per
+//              (bin class, size class); global atomics on gfx950 run at tens of
+//              G/s, so OWNER is the default).
+// Scheduling: every workgroup has its own queue, planned on the host
+// longest-first (LPT).  Measured on MI355X, device-scope atomics on one
+// address serialise at close to a microsecond each, so a shared queue counter
+// costs more than the tiles; per-workgroup counters are contention-free, and
+// each workgroup claims its next tile while it works on the current one.
+// Workgroup i of the launch starts on XCD i mod 8 (the dispatcher deals
+// workgroups round-robin over the XCDs): the range tiles of one instance, which
+// all stream the same elements, go to queues of workgroups on one XCD, so they
+// run side by side and share its L2 - the instance comes from HBM about once
+// instead of once per range.  A workgroup whose queue is empty takes tiles from
+// the other queues (its own XCD first) and leaves when all are empty.  A wrong
+// guess of the placement costs locality, never correctness.
 // Integer + is order-free mod 2^64, so every schedule gives the oracle's bits.
 #include <stdio.h>
 #include <string.h>
@@ -34,32 +45,26 @@ int tuned_batch_mode(int64_t n, int64_t bins);
 
 namespace {
 
-enum TileMode : int32_t { kModeOwner = 0, kModeChunk = 1, kModeGlobal = 2 };
+enum TileMode : int32_t { kModeOwner = 0, kModeGlobal = 2 };
 
+// Self-contained: one load gives a workgroup everything about its tile.
 struct Tile {
-    int64_t inst;
-    int64_t start;  // first element (relative to the instance)
-    int64_t len;
-    int32_t lo;     // first bin of the range
-    int32_t width;  // bins in the range
+    const int64_t* src;       // first element of the tile
+    unsigned long long* dst;  // the instance's bins
+    int64_t len;              // elements
+    int64_t bins;
+    ModConst mod;             // route = x % bins
+    int32_t lo;               // first bin of the range
+    int32_t width;            // bins in the range
     int32_t mode;
     int32_t pad;
 };
 
-struct InstDesc {
-    const int64_t* src;       // the instance's elements
-    unsigned long long* dst;  // its bins
-    int64_t bins;
-    ModConst mod;             // route = x % bins
-};
-
-// Two launch shapes: bins <= 4096 (32 KiB LDS, 4 workgroups per CU) and bin
-// ranges of 16384 (128 KiB LDS, one 1024-thread workgroup per CU).
-constexpr int kSmallBlock = 512;
-constexpr int kSmallWords = 4097;
-constexpr int kLargeBlock = 1024;
-constexpr int kLargeWords = 16384;
-constexpr int64_t kChunk = (int64_t)1 << 18;  // elements per tile of a long instance
+constexpr int kBlock = 1024;
+constexpr int kWords = 16384;                 // LDS words per workgroup: one bin range
+constexpr int kXcds = 8;                      // XCDs of the MI355X
+constexpr int64_t kChunk = (int64_t)1 << 18;  // elements per GLOBAL tile
+constexpr int64_t kTileOverhead = 4096;       // fixed cost of a tile, in elements
 
 __device__ __forceinline__ void lds_add(unsigned long long* p, uint64_t v) {
     __hip_atomic_fetch_add(p, (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -68,70 +73,158 @@ __device__ __forceinline__ void global_add(unsigned long long* p, uint64_t v) {
     __hip_atomic_fetch_add(p, (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Element pointers come out of the instance descriptors, so the compiler
+// cannot tell they are global memory; without the cast it emits flat loads,
+// which share a wait counter with the LDS atomics and serialise the loop.
+typedef const __attribute__((address_space(1))) v2i GlobalPair;
+typedef const __attribute__((address_space(1))) int64_t GlobalElem;
+
 // Calls add(x) for every element of p[0..len) handled by this workgroup's
 // threads (16-byte loads when aligned, 4 pairs in flight per thread).
 template <int BLOCK, class Add>
-__device__ __forceinline__ void tile_elements(const int64_t* __restrict__ p, int64_t len, Add add) {
+__device__ __forceinline__ void tile_elements(const int64_t* p_generic, int64_t len, Add add) {
     if (len <= 0) return;
-    if (reinterpret_cast<uintptr_t>(p) & 15) {
+    GlobalElem* p = (GlobalElem*)p_generic;
+    if (reinterpret_cast<uintptr_t>(p_generic) & 15) {
         if (threadIdx.x == 0) add(p[0]);
         ++p;
         --len;
     }
-    const longlong2* p2 = reinterpret_cast<const longlong2*>(p);
+    GlobalPair* p2 = (GlobalPair*)p;
     const int64_t npairs = len >> 1;
     int64_t q = threadIdx.x;
-    for (; q + 3 * BLOCK < npairs; q += 4 * BLOCK) {
-        longlong2 a = p2[q], b = p2[q + BLOCK], c = p2[q + 2 * BLOCK], d = p2[q + 3 * BLOCK];
-        add(a.x); add(a.y); add(b.x); add(b.y);
-        add(c.x); add(c.y); add(d.x); add(d.y);
+    // software pipelined over two register sets (no copies between them, so
+    // the wait before the arithmetic on one set leaves the other's loads in
+    // flight): the next four pairs load while the current eight elements are
+    // routed and added
+    v2i A[4], B[4];
+    auto load = [&](v2i* r, int64_t at) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) r[k] = p2[at + k * BLOCK];
+    };
+    auto run = [&](const v2i* r) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            add(r[k].x);
+            add(r[k].y);
+        }
+    };
+    // The next set is loaded unconditionally (on the last round from the
+    // current set's own, valid, addresses; the values are then unused) so
+    // that the wait before the arithmetic counts only the current set.
+    if (q + 3 * BLOCK < npairs) {
+        load(A, q);
+        for (;;) {
+            const bool nb = q + 7 * BLOCK < npairs;  // B = the four pairs after A
+            load(B, nb ? q + 4 * BLOCK : q);
+            __builtin_amdgcn_sched_barrier(0);
+            run(A);
+            q += 4 * BLOCK;
+            if (!nb) break;
+            const bool na = q + 7 * BLOCK < npairs;  // A = the four pairs after B
+            load(A, na ? q + 4 * BLOCK : q);
+            __builtin_amdgcn_sched_barrier(0);
+            run(B);
+            q += 4 * BLOCK;
+            if (!na) break;
+        }
     }
     for (; q < npairs; q += BLOCK) {
-        longlong2 a = p2[q];
+        const v2i a = p2[q];
         add(a.x);
         add(a.y);
     }
     if ((len & 1) && threadIdx.x == 0) add(p[len - 1]);
 }
 
-template <int BLOCK, int WORDS>
-__global__ __launch_bounds__(BLOCK) void batched_hist_kernel(const InstDesc* __restrict__ inst,
-                                                             const Tile* __restrict__ tiles,
-                                                             int64_t ntiles,
-                                                             unsigned int* __restrict__ counter) {
+// Claims the next tile of queue v: its index, or -1 when the queue is empty.
+__device__ __forceinline__ int64_t claim(const int64_t* __restrict__ qseg, unsigned int* counters, int v) {
+    const int64_t b = qseg[v], e = qseg[v + 1];
+    if (b >= e) return -1;
+    const int64_t i = (int64_t)atomicAdd(&counters[v], 1u);
+    return b + i < e ? b + i : -1;
+}
+
+// Run by the 64 lanes of wave 0 once the workgroup's own queue is empty: the
+// lanes check up to 64 queues at a time for tiles left (own XCD first, then
+// all; each workgroup starts at a different queue so the thieves spread out)
+// and lane 0 claims from the first one found.  Every failed claim means the
+// queue ran out meanwhile (counters only grow), so the search ends.  Returns
+// the claimed tile (in every lane) or -1 when all queues are empty.
+__device__ int64_t steal(const int64_t* __restrict__ qseg, unsigned int* counters, int nq) {
+    const int lane = threadIdx.x;
+    const int home = (int)(blockIdx.x % kXcds);
+    for (int pass = 0; pass < 2; ++pass) {
+        const int span = pass == 0 ? (nq - home + kXcds - 1) / kXcds : nq;
+        const int rot = pass == 0 ? (int)(blockIdx.x / kXcds) : (int)blockIdx.x;
+        for (;;) {
+            int found = -1;
+            for (int base = 0; base < span && found < 0; base += 64) {
+                const int j = base + lane;
+                const int jj = span > 0 ? (j + rot) % span : 0;
+                const int v = pass == 0 ? home + kXcds * jj : jj;
+                bool has = false;
+                if (j < span && v < nq) {
+                    const int64_t taken = (int64_t)__hip_atomic_load(&counters[v], __ATOMIC_RELAXED,
+                                                                     __HIP_MEMORY_SCOPE_AGENT);
+                    has = taken < qseg[v + 1] - qseg[v];
+                }
+                const unsigned long long m = __ballot(has);
+                if (m) found = __shfl(v, __ffsll((unsigned long long)m) - 1, 64);
+            }
+            if (found < 0) break;
+            int64_t t = -1;
+            if (lane == 0) t = claim(qseg, counters, found);
+            t = __shfl(t, 0, 64);
+            if (t >= 0) return t;
+        }
+    }
+    return -1;
+}
+
+__global__ __launch_bounds__(kBlock) void batched_hist_kernel(const Tile* __restrict__ tiles,
+                                                             const int64_t* __restrict__ qseg,
+                                                             unsigned int* __restrict__ counters,
+                                                             int nq) {
     extern __shared__ unsigned long long lds[];
     __shared__ int64_t cur;
+    const int me = (int)blockIdx.x;
+    int64_t next = -1;  // thread 0: the claim made while the previous tile ran
+    if (threadIdx.x == 0) next = claim(qseg, counters, me);
     for (;;) {
-        if (threadIdx.x == 0) cur = (int64_t)atomicAdd(counter, 1u);
+        if (threadIdx.x < 64) {  // wave 0
+            int64_t t = __shfl(next, 0, 64);
+            if (t < 0) t = steal(qseg, counters, nq);
+            if (threadIdx.x == 0) cur = t;
+        }
         __syncthreads();
         const int64_t t = cur;
         __syncthreads();  // everyone has read `cur` before thread 0 may overwrite it
-        if (t >= ntiles) break;  // uniform: every wave of the workgroup leaves
+        if (t < 0) break;  // uniform: every wave of the workgroup leaves
+        if (threadIdx.x == 0) next = claim(qseg, counters, me);  // overlaps this tile
         const Tile tile = tiles[t];
-        const InstDesc d = inst[tile.inst];
-        const int64_t nb = d.bins;
-        unsigned long long* h = d.dst;
-        const int64_t* src = d.src + tile.start;
+        const int64_t nb = tile.bins;
+        unsigned long long* h = tile.dst;
         // the remainder's form is chosen per tile, never per element (a branch
         // per element would serialise the loads of the streaming loop)
-        const bool pow2 = d.mod.mask != 0;
+        const bool pow2 = tile.mod.mask != 0;
         if (tile.mode == kModeGlobal) {
             auto add = [&](int64_t b, int64_t x) {
                 if ((uint64_t)b < (uint64_t)nb) global_add(&h[b], (uint64_t)x * (uint64_t)x);
             };
             if (pow2)
-                tile_elements<BLOCK>(src, tile.len, [&](int64_t x) { add(c_mod_pow2(x, d.mod), x); });
+                tile_elements<kBlock>(tile.src, tile.len, [&](int64_t x) { add(c_mod_pow2(x, tile.mod), x); });
             else
-                tile_elements<BLOCK>(src, tile.len, [&](int64_t x) { add(c_mod(x, d.mod), x); });
+                tile_elements<kBlock>(tile.src, tile.len, [&](int64_t x) { add(c_mod(x, tile.mod), x); });
             continue;
         }
         const int64_t lo = tile.lo;
         const int width = tile.width;
         // copies of the range at an odd stride (a bin's copies in different
         // banks); a range that fills LDS gets one copy at stride = width
-        // (the planner keeps width <= WORDS)
+        // (the planner keeps width <= kWords)
         int stride = width | 1;
-        int copies = WORDS / stride;
+        int copies = kWords / stride;
         if (copies < 1) {
             copies = 1;
             stride = width;
@@ -139,25 +232,30 @@ __global__ __launch_bounds__(BLOCK) void batched_hist_kernel(const InstDesc* __r
             copies = 16;
         }
         const int words = copies * stride;
-        for (int i = threadIdx.x; i < words; i += BLOCK) lds[i] = 0;
+        for (int i = threadIdx.x; i < words; i += kBlock) lds[i] = 0;
         __syncthreads();
         unsigned long long* mine = lds + (threadIdx.x % copies) * stride;
-        auto add = [&](int64_t b, int64_t x) {
-            b -= lo;
-            if ((uint64_t)b < (uint64_t)width) lds_add(&mine[b], (uint64_t)x * (uint64_t)x);
-        };
-        if (pow2)
-            tile_elements<BLOCK>(src, tile.len, [&](int64_t x) { add(c_mod_pow2(x, d.mod), x); });
-        else
-            tile_elements<BLOCK>(src, tile.len, [&](int64_t x) { add(c_mod(x, d.mod), x); });
+        if (pow2) {
+            // x % m for m = 2^k <= 2^31 in 32-bit arithmetic: the low bits r,
+            // kept when x >= 0 or r == 0 (a negative x with r != 0 has the
+            // negative C remainder r - m: not a bin)
+            const uint32_t mask = (uint32_t)tile.mod.mask, lo32 = (uint32_t)lo, w32 = (uint32_t)width;
+            tile_elements<kBlock>(tile.src, tile.len, [&](int64_t x) {
+                const uint32_t r = (uint32_t)x & mask;
+                const uint32_t i = r - lo32;
+                if ((x >= 0 || r == 0) && i < w32) lds_add(&mine[i], (uint64_t)x * (uint64_t)x);
+            });
+        } else {
+            tile_elements<kBlock>(tile.src, tile.len, [&](int64_t x) {
+                const int64_t b = c_mod(x, tile.mod) - lo;
+                if ((uint64_t)b < (uint64_t)width) lds_add(&mine[b], (uint64_t)x * (uint64_t)x);
+            });
+        }
         __syncthreads();
-        for (int b = threadIdx.x; b < width; b += BLOCK) {
+        for (int b = threadIdx.x; b < width; b += kBlock) {
             unsigned long long s = 0;
             for (int c = 0; c < copies; ++c) s += lds[c * stride + b];
-            if (tile.mode == kModeOwner)
-                h[lo + b] = s;
-            else if (s)
-                global_add(&h[lo + b], s);
+            h[lo + b] = s;
         }
         __syncthreads();  // LDS reused by the next tile
     }
@@ -178,7 +276,15 @@ __global__ void zero_ranges_kernel(const int64_t* __restrict__ ranges, int64_t n
 // the per-element atomics.
 int rule_mode(int64_t n, int64_t bins) { return n * 8 >= bins ? 0 : 1; }
 
-std::mutex g_plan_mu;
+struct Group {  // tiles that should run side by side on one XCD (an instance's ranges)
+    int64_t cost;
+    int64_t first, count;  // in the flat tile list
+};
+
+std::once_flag g_attr_once;
+hipError_t g_attr_err = hipSuccess;
+
+int64_t tile_cost(const Tile& t) { return t.len + (t.mode == kModeOwner ? t.width : 0) + kTileOverhead; }
 
 }  // namespace
 
@@ -187,14 +293,15 @@ std::mutex g_plan_mu;
 int batched_histogram_ptrs(const int64_t* const* src, const int64_t* len, const int64_t* bins,
                            int64_t* const* dst, int64_t count, int policy, hipStream_t s,
                            int64_t* out_tiles) {
-    std::vector<InstDesc> inst((size_t)count);
-    std::vector<Tile> small, large;
+    std::vector<Tile> flat;
+    std::vector<Group> groups;
     std::vector<int64_t> zero;  // (dst pointer, bins) pairs
     for (int64_t k = 0; k < count; ++k) {
         if (len[k] < 0 || bins[k] < 1 || bins[k] > ((int64_t)1 << 31) || !dst[k] || (len[k] > 0 && !src[k]))
             return set_error(HYD_E_INVALID, "batched_histogram: bad descriptor %lld", (long long)k);
         const int64_t n = len[k], nb = bins[k];
-        inst[k] = InstDesc{src[k], reinterpret_cast<unsigned long long*>(dst[k]), nb, make_mod_const(nb)};
+        const ModConst mod = make_mod_const(nb);
+        unsigned long long* d = reinterpret_cast<unsigned long long*>(dst[k]);
         int mode;
         if (policy == 1)
             mode = 0;
@@ -204,86 +311,85 @@ int batched_histogram_ptrs(const int64_t* const* src, const int64_t* len, const 
             mode = tuned_batch_mode(n, nb);
             if (mode < 0) mode = rule_mode(n, nb);
         }
+        const int64_t first = (int64_t)flat.size();
         if (mode == 1) {
             zero.push_back((int64_t)reinterpret_cast<uintptr_t>(dst[k]));
             zero.push_back(nb);
-            for (int64_t st = 0; st < n; st += kChunk)
-                small.push_back(Tile{k, st, std::min(kChunk, n - st), 0, 0, kModeGlobal, 0});
+            for (int64_t st = 0; st < n; st += kChunk) {
+                flat.push_back(Tile{src[k] + st, d, std::min(kChunk, n - st), nb, mod, 0, 0, kModeGlobal, 0});
+                groups.push_back(Group{tile_cost(flat.back()), (int64_t)flat.size() - 1, 1});
+            }
             continue;
         }
-        const bool fits_small = nb <= kSmallWords - 1;
-        const int64_t range = fits_small ? nb : kLargeWords;
-        const bool chunked = n > kChunk;
-        if (chunked) {
-            zero.push_back((int64_t)reinterpret_cast<uintptr_t>(dst[k]));
-            zero.push_back(nb);
+        int64_t cost = 0;
+        for (int64_t lo = 0; lo < nb; lo += kWords) {
+            const int32_t w = (int32_t)std::min<int64_t>(kWords, nb - lo);
+            flat.push_back(Tile{src[k], d, n, nb, mod, (int32_t)lo, w, kModeOwner, 0});
+            cost += tile_cost(flat.back());
         }
-        std::vector<Tile>& dst = fits_small ? small : large;
-        for (int64_t lo = 0; lo < nb; lo += range) {
-            int32_t w = (int32_t)std::min(range, nb - lo);
-            if (!chunked) {
-                dst.push_back(Tile{k, 0, n, (int32_t)lo, w, kModeOwner, 0});
-            } else {
-                for (int64_t st = 0; st < n; st += kChunk)
-                    dst.push_back(Tile{k, st, std::min(kChunk, n - st), (int32_t)lo, w, kModeChunk, 0});
-            }
+        groups.push_back(Group{cost, first, (int64_t)flat.size() - first});
+    }
+    const int nq = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)flat.size(), device_info().num_cus));
+    // LPT: groups longest first, each onto the least-loaded XCD, each of its
+    // tiles onto the least-loaded queue of that XCD (queue q runs on XCD q % 8)
+    std::stable_sort(groups.begin(), groups.end(),
+                     [](const Group& a, const Group& b) { return a.cost > b.cost; });
+    const int nx = std::min(kXcds, nq);
+    std::vector<int64_t> xload((size_t)nx, 0), qload((size_t)nq, 0);
+    std::vector<std::vector<int64_t>> qtiles((size_t)nq);
+    for (const Group& g : groups) {
+        int x = 0;
+        for (int j = 1; j < nx; ++j)
+            if (xload[j] < xload[x]) x = j;
+        for (int64_t i = 0; i < g.count; ++i) {
+            const int64_t ti = g.first + i;
+            int q = x;
+            for (int c = x + kXcds; c < nq; c += kXcds)
+                if (qload[c] < qload[q]) q = c;
+            qtiles[q].push_back(ti);
+            const int64_t tc = tile_cost(flat[(size_t)ti]);
+            qload[q] += tc;
+            xload[x] += tc;
         }
     }
-    // largest first (longest-processing-time order for the queue)
-    auto by_len = [](const Tile& a, const Tile& b) { return a.len > b.len; };
-    std::stable_sort(small.begin(), small.end(), by_len);
-    std::stable_sort(large.begin(), large.end(), by_len);
-    if (out_tiles) *out_tiles = (int64_t)(small.size() + large.size());
+    std::vector<int64_t> qseg((size_t)nq + 1);
+    for (int q = 0; q < nq; ++q) qseg[q + 1] = qseg[q] + (int64_t)qtiles[q].size();
+    if (out_tiles) *out_tiles = (int64_t)flat.size();
 
-    const size_t inst_bytes = inst.size() * sizeof(InstDesc);
-    const size_t small_bytes = small.size() * sizeof(Tile);
-    const size_t large_bytes = large.size() * sizeof(Tile);
+    // descriptor block: counters (zeroed), queue boundaries, tiles, zero list
+    const size_t cnt_bytes = ((size_t)nq * 4 + 255) & ~(size_t)255;
+    const size_t seg_bytes = (((size_t)nq + 1) * 8 + 255) & ~(size_t)255;
+    const size_t tile_bytes = flat.size() * sizeof(Tile);
     const size_t zero_bytes = zero.size() * sizeof(int64_t);
-    const size_t bytes = 256 + inst_bytes + small_bytes + large_bytes + zero_bytes;
+    const size_t bytes = cnt_bytes + seg_bytes + tile_bytes + zero_bytes;
     unsigned char* ws = static_cast<unsigned char*>(stream_workspace(s, bytes));
     if (!ws) return set_error(HYD_E_OOM, "batched_histogram: workspace allocation failed");
+    HYD_TRY(stage_h2d(s, ws, bytes, [&](unsigned char* q) {
+        memset(q, 0, cnt_bytes);
+        memcpy(q + cnt_bytes, qseg.data(), qseg.size() * 8);
+        Tile* t = reinterpret_cast<Tile*>(q + cnt_bytes + seg_bytes);
+        for (int qq = 0; qq < nq; ++qq)
+            for (int64_t ti : qtiles[qq]) *t++ = flat[(size_t)ti];
+        memcpy(q + cnt_bytes + seg_bytes + tile_bytes, zero.data(), zero_bytes);
+    }));
     unsigned int* counters = reinterpret_cast<unsigned int*>(ws);
-    InstDesc* d_inst = reinterpret_cast<InstDesc*>(ws + 256);
-    Tile* d_small = reinterpret_cast<Tile*>(ws + 256 + inst_bytes);
-    Tile* d_large = reinterpret_cast<Tile*>(ws + 256 + inst_bytes + small_bytes);
-    int64_t* d_zero = reinterpret_cast<int64_t*>(ws + 256 + inst_bytes + small_bytes + large_bytes);
-    {
-        // one staging buffer, one copy
-        std::lock_guard<std::mutex> lk(g_plan_mu);
-        std::vector<unsigned char> stage(bytes - 256);
-        unsigned char* q = stage.data();
-        memcpy(q, inst.data(), inst_bytes);
-        memcpy(q + inst_bytes, small.data(), small_bytes);
-        memcpy(q + inst_bytes + small_bytes, large.data(), large_bytes);
-        memcpy(q + inst_bytes + small_bytes + large_bytes, zero.data(), zero_bytes);
-        HYD_TRY_HIP(hipMemcpyAsync(ws + 256, stage.data(), stage.size(), hipMemcpyHostToDevice, s));
-        HYD_TRY_HIP(hipStreamSynchronize(s));  // pageable source: consumed before it goes away
-    }
-    HYD_TRY(fill_zero(counters, 256, s));
-    const DeviceInfo& di = device_info();
+    const int64_t* d_seg = reinterpret_cast<const int64_t*>(ws + cnt_bytes);
+    const Tile* d_tiles = reinterpret_cast<const Tile*>(ws + cnt_bytes + seg_bytes);
+    const int64_t* d_zero = reinterpret_cast<const int64_t*>(ws + cnt_bytes + seg_bytes + tile_bytes);
     if (!zero.empty()) {
-        int64_t nr = (int64_t)zero.size() / 2;
+        const int64_t nr = (int64_t)zero.size() / 2;
         zero_ranges_kernel<<<(unsigned)std::min<int64_t>(nr, 4096), 256, 0, s>>>(d_zero, nr);
         HYD_CHECK_LAUNCH();
     }
-    if (!small.empty()) {
-        auto k = batched_hist_kernel<kSmallBlock, kSmallWords>;
-        const int lds = kSmallWords * 8;
-        HYD_TRY_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-        int64_t blocks = std::min<int64_t>((int64_t)small.size(), (int64_t)di.num_cus * 4);
-        k<<<(unsigned)blocks, kSmallBlock, lds, s>>>(d_inst, d_small, (int64_t)small.size(),
-                                                     counters);
-        HYD_CHECK_LAUNCH();
-    }
-    if (!large.empty()) {
-        auto k = batched_hist_kernel<kLargeBlock, kLargeWords>;
-        const int lds = kLargeWords * 8;
-        HYD_TRY_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-        int64_t blocks = std::min<int64_t>((int64_t)large.size(), (int64_t)di.num_cus);
-        k<<<(unsigned)blocks, kLargeBlock, lds, s>>>(d_inst, d_large, (int64_t)large.size(),
-                                                     counters + 1);
-        HYD_CHECK_LAUNCH();
-    }
+    if (flat.empty()) return HYD_OK;
+    const int lds = kWords * 8;
+    std::call_once(g_attr_once, [&] {
+        g_attr_err = hipFuncSetAttribute((const void*)batched_hist_kernel,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    });
+    HYD_TRY_HIP(g_attr_err);
+    batched_hist_kernel<<<(unsigned)nq, kBlock, lds, s>>>(d_tiles, d_seg, counters, nq);
+    HYD_CHECK_LAUNCH();
     return HYD_OK;
 }
 

@@ -4,6 +4,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <functional>
 #include <mutex>
 #include <unordered_map>
 
@@ -113,7 +114,70 @@ void* scalar_slot() {
     return g_scalar;
 }
 
+// Pinned staging for host-to-device copies of launch descriptors: two pinned
+// buffers per stream, used alternately.  A buffer is refilled only after the
+// copy that last read it has completed (an event recorded right after that
+// copy), so the host can plan the next call while the GPU still runs this one
+// and no call has to wait for its own stream to drain.
+namespace {
+struct Staging {
+    std::mutex mu;
+    void* host[2] = {nullptr, nullptr};
+    size_t bytes[2] = {0, 0};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool pending[2] = {false, false};
+    int next = 0;
+};
+std::mutex g_stage_mu;
+std::unordered_map<hipStream_t, Staging*> g_stage;
+}  // namespace
+
+int stage_h2d(hipStream_t s, void* dev, size_t bytes, const std::function<void(unsigned char*)>& fill) {
+    Staging* st;
+    {
+        std::lock_guard<std::mutex> lk(g_stage_mu);
+        Staging*& slot = g_stage[s];
+        if (!slot) slot = new Staging();
+        st = slot;
+    }
+    std::lock_guard<std::mutex> lk(st->mu);
+    const int k = st->next;
+    st->next ^= 1;
+    if (st->pending[k]) {
+        HYD_TRY_HIP(hipEventSynchronize(st->ev[k]));
+        st->pending[k] = false;
+    }
+    if (st->bytes[k] < bytes) {
+        if (st->host[k]) HYD_TRY_HIP(hipHostFree(st->host[k]));
+        st->host[k] = nullptr;
+        st->bytes[k] = 0;
+        const size_t want = bytes < (64u << 10) ? (64u << 10) : bytes;
+        HYD_TRY_HIP(hipHostMalloc(&st->host[k], want, hipHostMallocDefault));
+        st->bytes[k] = want;
+    }
+    if (!st->ev[k]) HYD_TRY_HIP(hipEventCreateWithFlags(&st->ev[k], hipEventDisableTiming));
+    fill(static_cast<unsigned char*>(st->host[k]));
+    HYD_TRY_HIP(hipMemcpyAsync(dev, st->host[k], bytes, hipMemcpyHostToDevice, s));
+    HYD_TRY_HIP(hipEventRecord(st->ev[k], s));
+    st->pending[k] = true;
+    return HYD_OK;
+}
+
+static void release_staging() {
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    for (auto& kv : g_stage) {
+        Staging* st = kv.second;
+        for (int k = 0; k < 2; ++k) {
+            if (st->ev[k]) (void)hipEventDestroy(st->ev[k]);
+            if (st->host[k]) (void)hipHostFree(st->host[k]);
+        }
+        delete st;
+    }
+    g_stage.clear();
+}
+
 static void release_workspaces() {
+    release_staging();
     std::lock_guard<std::mutex> lk(g_ws_mu);
     for (auto& kv : g_ws)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);

@@ -5,6 +5,8 @@
 #include <stdint.h>
 #include <stddef.h>
 
+#include <functional>
+
 #include "../../include/hyd_hip.h"
 
 namespace hyd {
@@ -44,6 +46,11 @@ void* scalar_slot();
 // touched before they finish.  kZeroScratchBytes long, 256-byte aligned.
 constexpr size_t kZeroScratchBytes = (size_t)1 << 20;
 void* stream_zero_scratch(hipStream_t stream);
+
+// Copies `bytes` of descriptors, written by fill() into a pinned host buffer,
+// to dev on stream s (asynchronously; the buffer is reused only once the copy
+// that read it has completed).  Not for use under graph capture.
+int stage_h2d(hipStream_t s, void* dev, size_t bytes, const std::function<void(unsigned char*)>& fill);
 
 // Zero-fill kernel launch (graph-capture safe, unlike hipMemsetAsync).
 int fill_zero(void* dptr, size_t bytes, hipStream_t stream);

@@ -317,12 +317,11 @@ int batched_dot(const double* const* xs, const double* const* ys, const int64_t*
     const size_t pb = tiles.size() * 8;
     unsigned char* ws = static_cast<unsigned char*>(stream_workspace(s, tb + fb + ob + pb + 64));
     if (!ws) return set_error(HYD_E_OOM, "batched_dot: workspace allocation failed");
-    std::vector<unsigned char> stage(tb + fb + ob);
-    memcpy(stage.data(), tiles.data(), tb);
-    memcpy(stage.data() + tb, first.data(), fb);
-    memcpy(stage.data() + tb + fb, outs, ob);
-    HYD_TRY_HIP(hipMemcpyAsync(ws, stage.data(), stage.size(), hipMemcpyHostToDevice, s));
-    HYD_TRY_HIP(hipStreamSynchronize(s));  // pageable staging consumed
+    HYD_TRY(stage_h2d(s, ws, tb + fb + ob, [&](unsigned char* q) {
+        memcpy(q, tiles.data(), tb);
+        memcpy(q + tb, first.data(), fb);
+        memcpy(q + tb + fb, outs, ob);
+    }));
     auto* d_tiles = reinterpret_cast<DotTile*>(ws);
     auto* d_first = reinterpret_cast<int64_t*>(ws + tb);
     auto* d_outs = reinterpret_cast<double* const*>(ws + tb + fb);

@@ -33,7 +33,14 @@ template <bool POW2>
 struct ModSquare {
     ModConst mod;
     __device__ __forceinline__ void operator()(int64_t x, int64_t& b, uint64_t& v) const {
-        b = POW2 ? c_mod_pow2(x, mod) : c_mod(x, mod);
+        if (POW2) {
+            // m = 2^k <= 2^31: the low bits r in 32-bit arithmetic; a negative x
+            // with r != 0 has the negative C remainder r - m, which is no bin
+            const uint32_t r = (uint32_t)x & (uint32_t)mod.mask;
+            b = (x >= 0 || r == 0) ? (int64_t)r : -1;
+        } else {
+            b = c_mod(x, mod);
+        }
         v = (uint64_t)x * (uint64_t)x;
     }
 };
