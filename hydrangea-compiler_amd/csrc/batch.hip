@@ -57,7 +57,7 @@ struct Tile {
     int32_t lo;               // first bin of the range
     int32_t width;            // bins in the range
     int32_t mode;
-    int32_t pad;
+    int32_t once;             // the tile is the only reader of its elements
 };
 
 constexpr int kBlock = 1024;
@@ -65,6 +65,7 @@ constexpr int kWords = 16384;                 // LDS words per workgroup: one bi
 constexpr int kXcds = 8;                      // XCDs of the MI355X
 constexpr int64_t kChunk = (int64_t)1 << 18;  // elements per GLOBAL tile
 constexpr int64_t kTileOverhead = 4096;       // fixed cost of a tile, in elements
+constexpr int kTeam = 4;                      // workgroups that stream one instance together
 
 __device__ __forceinline__ void lds_add(unsigned long long* p, uint64_t v) {
     __hip_atomic_fetch_add(p, (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -81,7 +82,11 @@ typedef const __attribute__((address_space(1))) int64_t GlobalElem;
 
 // Calls add(x) for every element of p[0..len) handled by this workgroup's
 // threads (16-byte loads when aligned, 4 pairs in flight per thread).
-template <int BLOCK, class Add>
+// ONCE: the elements are read by this tile only - non-temporal loads (measured
+// on MI355X: plain loads of a read-once stream keep the CU's L1 allocating and
+// stall the address path for about half the cycles).  Otherwise plain loads,
+// so that the other range tiles of the instance find the elements in L2.
+template <int BLOCK, bool ONCE, class Add>
 __device__ __forceinline__ void tile_elements(const int64_t* p_generic, int64_t len, Add add) {
     if (len <= 0) return;
     GlobalElem* p = (GlobalElem*)p_generic;
@@ -100,7 +105,12 @@ __device__ __forceinline__ void tile_elements(const int64_t* p_generic, int64_t 
     v2i A[4], B[4];
     auto load = [&](v2i* r, int64_t at) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) r[k] = p2[at + k * BLOCK];
+        for (int k = 0; k < 4; ++k) {
+            if (ONCE)
+                r[k] = __builtin_nontemporal_load(&p2[at + k * BLOCK]);
+            else
+                r[k] = p2[at + k * BLOCK];
+        }
     };
     auto run = [&](const v2i* r) {
 #pragma unroll
@@ -213,9 +223,9 @@ __global__ __launch_bounds__(kBlock) void batched_hist_kernel(const Tile* __rest
                 if ((uint64_t)b < (uint64_t)nb) global_add(&h[b], (uint64_t)x * (uint64_t)x);
             };
             if (pow2)
-                tile_elements<kBlock>(tile.src, tile.len, [&](int64_t x) { add(c_mod_pow2(x, tile.mod), x); });
+                tile_elements<kBlock, true>(tile.src, tile.len, [&](int64_t x) { add(c_mod_pow2(x, tile.mod), x); });
             else
-                tile_elements<kBlock>(tile.src, tile.len, [&](int64_t x) { add(c_mod(x, tile.mod), x); });
+                tile_elements<kBlock, true>(tile.src, tile.len, [&](int64_t x) { add(c_mod(x, tile.mod), x); });
             continue;
         }
         const int64_t lo = tile.lo;
@@ -240,16 +250,24 @@ __global__ __launch_bounds__(kBlock) void batched_hist_kernel(const Tile* __rest
             // kept when x >= 0 or r == 0 (a negative x with r != 0 has the
             // negative C remainder r - m: not a bin)
             const uint32_t mask = (uint32_t)tile.mod.mask, lo32 = (uint32_t)lo, w32 = (uint32_t)width;
-            tile_elements<kBlock>(tile.src, tile.len, [&](int64_t x) {
+            auto f = [&](int64_t x) {
                 const uint32_t r = (uint32_t)x & mask;
                 const uint32_t i = r - lo32;
                 if ((x >= 0 || r == 0) && i < w32) lds_add(&mine[i], (uint64_t)x * (uint64_t)x);
-            });
+            };
+            if (tile.once)
+                tile_elements<kBlock, true>(tile.src, tile.len, f);
+            else
+                tile_elements<kBlock, false>(tile.src, tile.len, f);
         } else {
-            tile_elements<kBlock>(tile.src, tile.len, [&](int64_t x) {
+            auto f = [&](int64_t x) {
                 const int64_t b = c_mod(x, tile.mod) - lo;
                 if ((uint64_t)b < (uint64_t)width) lds_add(&mine[b], (uint64_t)x * (uint64_t)x);
-            });
+            };
+            if (tile.once)
+                tile_elements<kBlock, true>(tile.src, tile.len, f);
+            else
+                tile_elements<kBlock, false>(tile.src, tile.len, f);
         }
         __syncthreads();
         for (int b = threadIdx.x; b < width; b += kBlock) {
@@ -316,44 +334,86 @@ int batched_histogram_ptrs(const int64_t* const* src, const int64_t* len, const 
             zero.push_back((int64_t)reinterpret_cast<uintptr_t>(dst[k]));
             zero.push_back(nb);
             for (int64_t st = 0; st < n; st += kChunk) {
-                flat.push_back(Tile{src[k] + st, d, std::min(kChunk, n - st), nb, mod, 0, 0, kModeGlobal, 0});
+                flat.push_back(Tile{src[k] + st, d, std::min(kChunk, n - st), nb, mod, 0, 0, kModeGlobal, 1});
                 groups.push_back(Group{tile_cost(flat.back()), (int64_t)flat.size() - 1, 1});
             }
             continue;
         }
         int64_t cost = 0;
+        const int32_t once = nb <= kWords ? 1 : 0;  // one range tile reads the elements
         for (int64_t lo = 0; lo < nb; lo += kWords) {
             const int32_t w = (int32_t)std::min<int64_t>(kWords, nb - lo);
-            flat.push_back(Tile{src[k], d, n, nb, mod, (int32_t)lo, w, kModeOwner, 0});
+            flat.push_back(Tile{src[k], d, n, nb, mod, (int32_t)lo, w, kModeOwner, once});
             cost += tile_cost(flat.back());
         }
         groups.push_back(Group{cost, first, (int64_t)flat.size() - first});
     }
     const int nq = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)flat.size(), device_info().num_cus));
-    // LPT: groups longest first, each onto the least-loaded XCD, each of its
-    // tiles onto the least-loaded queue of that XCD (queue q runs on XCD q % 8)
-    std::stable_sort(groups.begin(), groups.end(),
-                     [](const Group& a, const Group& b) { return a.cost > b.cost; });
-    const int nx = std::min(kXcds, nq);
-    std::vector<int64_t> xload((size_t)nx, 0), qload((size_t)nq, 0);
+    // Gang scheduling: teams of kTeam workgroups on one XCD (queue q runs on
+    // XCD q % 8) get aligned queues - job j hands each member one tile: the
+    // range tiles of one instance (which stream the same elements and so share
+    // them through L2 only while they run side by side), or a bundle of
+    // single tiles of similar length.  Jobs go longest first onto the
+    // least-loaded team (LPT).  Batches too small for full teams use teams of
+    // one workgroup.
     std::vector<std::vector<int64_t>> qtiles((size_t)nq);
-    for (const Group& g : groups) {
-        int x = 0;
-        for (int j = 1; j < nx; ++j)
-            if (xload[j] < xload[x]) x = j;
-        for (int64_t i = 0; i < g.count; ++i) {
-            const int64_t ti = g.first + i;
-            int q = x;
-            for (int c = x + kXcds; c < nq; c += kXcds)
-                if (qload[c] < qload[q]) q = c;
-            qtiles[q].push_back(ti);
-            const int64_t tc = tile_cost(flat[(size_t)ti]);
-            qload[q] += tc;
-            xload[x] += tc;
+    {
+        const int per_x = nq / kXcds;
+        const int team = (nq % kXcds == 0 && per_x >= kTeam && per_x % kTeam == 0) ? kTeam : 1;
+        std::vector<std::vector<int>> teams;  // member queues
+        for (int x = 0; x < std::min(kXcds, nq); ++x)
+            for (int q0 = x; q0 < nq; q0 += kXcds * team) {
+                std::vector<int> m;
+                for (int i = 0; i < team && q0 + kXcds * i < nq; ++i) m.push_back(q0 + kXcds * i);
+                teams.push_back(m);
+            }
+        struct Job {
+            int64_t cost;
+            std::vector<int64_t> tiles;
+        };
+        std::vector<Job> jobs;
+        std::vector<int64_t> singles;
+        for (const Group& g : groups) {
+            if (g.count == 1) {
+                singles.push_back(g.first);
+                continue;
+            }
+            for (int64_t i = 0; i < g.count; i += team) {
+                Job jb{0, {}};
+                for (int64_t k = i; k < std::min<int64_t>(g.count, i + team); ++k) {
+                    jb.tiles.push_back(g.first + k);
+                    jb.cost = std::max(jb.cost, tile_cost(flat[(size_t)(g.first + k)]));
+                }
+                jobs.push_back(std::move(jb));
+            }
+        }
+        std::stable_sort(singles.begin(), singles.end(), [&](int64_t a, int64_t b) {
+            return tile_cost(flat[(size_t)a]) > tile_cost(flat[(size_t)b]);
+        });
+        for (size_t i = 0; i < singles.size(); i += (size_t)team) {
+            Job jb{0, {}};
+            for (size_t k = i; k < std::min(singles.size(), i + (size_t)team); ++k) {
+                jb.tiles.push_back(singles[k]);
+                jb.cost = std::max(jb.cost, tile_cost(flat[(size_t)singles[k]]));
+            }
+            jobs.push_back(std::move(jb));
+        }
+        std::stable_sort(jobs.begin(), jobs.end(), [](const Job& a, const Job& b) { return a.cost > b.cost; });
+        std::vector<int64_t> tload(teams.size(), 0);
+        for (const Job& jb : jobs) {
+            size_t t = 0;
+            for (size_t u = 1; u < teams.size(); ++u)
+                if (tload[u] < tload[t]) t = u;
+            for (size_t k = 0; k < jb.tiles.size() && k < teams[t].size(); ++k)
+                qtiles[(size_t)teams[t][k]].push_back(jb.tiles[k]);
+            tload[t] += jb.cost;
         }
     }
     std::vector<int64_t> qseg((size_t)nq + 1);
     for (int q = 0; q < nq; ++q) qseg[q + 1] = qseg[q] + (int64_t)qtiles[q].size();
+    if (qseg[(size_t)nq] != (int64_t)flat.size())
+        return set_error(HYD_E_INVALID, "batched_histogram: planner placed %lld of %zu tiles",
+                         (long long)qseg[(size_t)nq], flat.size());
     if (out_tiles) *out_tiles = (int64_t)flat.size();
 
     // descriptor block: counters (zeroed), queue boundaries, tiles, zero list

@@ -45,14 +45,25 @@ struct ModSquare {
     }
 };
 
+// 16-byte load of a pair: non-temporal when the stream is read once (plain
+// loads of a read-once stream keep the CU's L1 allocating and stall its
+// address path - measured on MI355X), plain when several workgroups read
+// the same elements and should find them in L2.
+template <bool ONCE>
+__device__ __forceinline__ v2i load_pair(const int64_t* p) {
+    if (ONCE) return ld_stream2(p);
+    return *reinterpret_cast<const v2i*>(p);
+}
+
 // Fused source: one int64 stream produces both route and value.
-template <bool POW2>
+template <bool POW2, bool ONCE = true>
 struct FusedSrc {
     const int64_t* __restrict__ data;
     ModSquare<POW2> fn;
+    FusedSrc<POW2, false> shared() const { return FusedSrc<POW2, false>{data, fn}; }
     __device__ __forceinline__ void pair(int64_t p, int64_t& b0, uint64_t& v0, int64_t& b1,
                                          uint64_t& v1) const {
-        longlong2 x = reinterpret_cast<const longlong2*>(data)[p];
+        const v2i x = load_pair<ONCE>(data + 2 * p);
         fn(x.x, b0, v0);
         fn(x.y, b1, v1);
     }
@@ -62,13 +73,14 @@ struct FusedSrc {
 };
 
 // Unfused source: separate route and value arrays.
+template <bool ONCE = true>
 struct PairSrc {
     const int64_t* __restrict__ idx;
     const int64_t* __restrict__ vals;
+    PairSrc<false> shared() const { return PairSrc<false>{idx, vals}; }
     __device__ __forceinline__ void pair(int64_t p, int64_t& b0, uint64_t& v0, int64_t& b1,
                                          uint64_t& v1) const {
-        longlong2 i2 = reinterpret_cast<const longlong2*>(idx)[p];
-        longlong2 v2 = reinterpret_cast<const longlong2*>(vals)[p];
+        const v2i i2 = load_pair<ONCE>(idx + 2 * p), v2 = load_pair<ONCE>(vals + 2 * p);
         b0 = i2.x;
         b1 = i2.y;
         v0 = (uint64_t)v2.x;
@@ -290,16 +302,19 @@ int launch_scatter(const Src& src, bool vec, int64_t n, int64_t nbins, int64_t* 
         int64_t bx = std::max<int64_t>(1, std::min<int64_t>(want_blocks,
                                                             (int64_t)di.num_cus * per_cu / nranges));
         dim3 grid((unsigned)bx, (unsigned)nranges);
+        // every bin range streams all elements: plain loads, shared through L2
+        auto ssrc = src.shared();
+        using SSrc = decltype(ssrc);
         if (vec) {
-            auto k = scatter_split_kernel<kBlock, true, Src>;
+            auto k = scatter_split_kernel<kBlock, true, SSrc>;
             HYD_TRY_HIP(hipFuncSetAttribute((const void*)k,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            k<<<grid, kBlock, lds, s>>>(src, n, nbins, range, dst);
+            k<<<grid, kBlock, lds, s>>>(ssrc, n, nbins, range, dst);
         } else {
-            auto k = scatter_split_kernel<kBlock, false, Src>;
+            auto k = scatter_split_kernel<kBlock, false, SSrc>;
             HYD_TRY_HIP(hipFuncSetAttribute((const void*)k,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            k<<<grid, kBlock, lds, s>>>(src, n, nbins, range, dst);
+            k<<<grid, kBlock, lds, s>>>(ssrc, n, nbins, range, dst);
         }
     } else if (strategy == HYD_SCATTER_GLOBAL) {
         int64_t blocks = std::min<int64_t>(want_blocks, (int64_t)di.num_cus * 8);
@@ -346,7 +361,7 @@ extern "C" int hydhip_scatter_add_i64(const int64_t* idx, const int64_t* vals, i
     if (n < 0 || ndst < 1 || ndst > (int64_t)1 << 31 || !dst || (n > 0 && (!idx || !vals)))
         return set_error(HYD_E_INVALID, "hydhip_scatter_add_i64: bad arguments");
     if (n == 0) return HYD_OK;
-    PairSrc src{idx, vals};
+    PairSrc<> src{idx, vals};
     return launch_scatter(src, aligned16(idx) && aligned16(vals), n, ndst, dst, strategy, false,
                           resolve_stream(stream));
 }
