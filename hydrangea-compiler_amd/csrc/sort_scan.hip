@@ -382,29 +382,76 @@ __global__ void coo_heads_kernel(const int64_t* __restrict__ perm, const int64_t
     }
 }
 
-// run[i] = inclusive count of heads - 1 (the output slot of element i).
-// Heads write row/col; every element adds its value into its slot.
-__global__ void coo_merge_kernel(const int64_t* __restrict__ perm, const int64_t* __restrict__ rows,
-                                 const int64_t* __restrict__ cols, const int64_t* __restrict__ vals,
-                                 const int64_t* __restrict__ head, const int64_t* __restrict__ slot_incl,
-                                 int64_t n, int64_t* __restrict__ out_rows,
-                                 int64_t* __restrict__ out_cols, int64_t* __restrict__ out_vals) {
+// Duplicates are merged without atomics: with P the inclusive prefix sum of
+// the (permuted) values and slot(i) = inclusive count of heads - 1, the run
+// ending at element i sums to P[i] - P[end of the previous run] (exact in
+// wrapping int64 arithmetic, like the reference's run loop).  Heads write
+// row/col into their slot; run ends record P into end_p[slot].
+__global__ void coo_run_ends_kernel(const int64_t* __restrict__ perm, const int64_t* __restrict__ rows,
+                                    const int64_t* __restrict__ cols, const int64_t* __restrict__ head,
+                                    const int64_t* __restrict__ slot_incl, const int64_t* __restrict__ psum,
+                                    int64_t n, int64_t* __restrict__ out_rows,
+                                    int64_t* __restrict__ out_cols, int64_t* __restrict__ end_p) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        int64_t pi = perm ? perm[i] : i;
-        int64_t slot = slot_incl[i] - 1;
+        const int64_t slot = slot_incl[i] - 1;
         if (head[i]) {
+            const int64_t pi = perm ? perm[i] : i;
             out_rows[slot] = rows[pi];
             out_cols[slot] = cols[pi];
         }
-        __hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(&out_vals[slot]),
-                               (unsigned long long)vals[pi], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+        if (i == n - 1 || head[i + 1]) end_p[slot] = psum[i];
     }
 }
 
+// out_vals[k] = end_p[k] - end_p[k - 1] for the runs k < *count.
+__global__ void coo_run_sums_kernel(const int64_t* __restrict__ end_p, const int64_t* __restrict__ count,
+                                    int64_t n, int64_t* __restrict__ out_vals) {
+    const int64_t cnt = *count;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n && k < cnt;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t prev = k ? (uint64_t)end_p[k - 1] : 0;
+        out_vals[k] = (int64_t)((uint64_t)end_p[k] - prev);
+    }
+}
+
+// flag = 1 when rows[0..n) is not non-decreasing (benign race: all write 1).
+__global__ void unsorted_flag_kernel(const int64_t* __restrict__ rows, int64_t n, int64_t* flag) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x + 1; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        if (rows[i - 1] > rows[i]) *flag = 1;
+}
+
+// Sorted rows (flag 0): the first and the last element of each row's run
+// record first[r] = i and last1[r] = i + 1, so the row's count is
+// last1[r] - first[r] (0 - 0 for a row with no elements).  No atomics: each
+// entry has exactly one writer.
+__global__ void row_bounds_kernel(const int64_t* __restrict__ rows, int64_t nnz, int64_t nrows,
+                                  const int64_t* __restrict__ flag, int64_t* __restrict__ first,
+                                  int64_t* __restrict__ last1) {
+    if (*flag) return;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = rows[i];
+        if ((uint64_t)r >= (uint64_t)nrows) continue;
+        if (i == 0 || rows[i - 1] != r) first[r] = i;
+        if (i == nnz - 1 || rows[i + 1] != r) last1[r] = i + 1;
+    }
+}
+
+__global__ void row_count_from_bounds_kernel(const int64_t* __restrict__ first,
+                                             const int64_t* __restrict__ last1, int64_t nrows,
+                                             const int64_t* __restrict__ flag, int64_t* __restrict__ counts) {
+    if (*flag) return;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows;
+         r += (int64_t)gridDim.x * blockDim.x)
+        counts[r] = last1[r] - first[r];
+}
+
+// Rows in any order (flag 1): one global atomic per element.
 __global__ void row_count_kernel(const int64_t* __restrict__ rows, int64_t nnz, int64_t nrows,
-                                 int64_t* __restrict__ counts) {
+                                 const int64_t* __restrict__ flag, int64_t* __restrict__ counts) {
+    if (!*flag) return;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz;
          i += (int64_t)gridDim.x * blockDim.x) {
         int64_t r = rows[i];
@@ -441,7 +488,7 @@ int gather_i64(const int64_t* perm, int64_t n, const int64_t* src, int64_t nsrc,
 
 // coo_sum_duplicates over the (optionally permuted) sequence.  Outputs are
 // `len` long with a zero tail; the deduplicated count goes to *d_nnz_out
-// (device word).  ws: 2*len words.
+// (device word).  ws: coo_workspace_words(n) words.
 int coo_sum_duplicates_i64(const int64_t* perm, const int64_t* rows, const int64_t* cols,
                            const int64_t* vals, int64_t n, int64_t len, int64_t* out_rows,
                            int64_t* out_cols, int64_t* out_vals, int64_t* d_nnz_out, hipStream_t s,
@@ -452,28 +499,52 @@ int coo_sum_duplicates_i64(const int64_t* perm, const int64_t* rows, const int64
     if (n == 0) return fill_zero(d_nnz_out, 8, s);
     int64_t* head = ws;
     int64_t* incl = ws + n;
-    int64_t* scan_ws = ws + 2 * n;
+    int64_t* gv = ws + 2 * n;    // values in sequence order
+    int64_t* psum = ws + 3 * n;  // their inclusive prefix sums
+    int64_t* end_p = ws + 4 * n;
+    int64_t* scan_ws = ws + 5 * n;
     coo_heads_kernel<<<grid_for(n), 256, 0, s>>>(perm, rows, cols, n, head);
     HYD_CHECK_LAUNCH();
     HYD_TRY(scan_i64(head, n, 0, incl, true, false, s, scan_ws));
-    coo_merge_kernel<<<grid_for(n), 256, 0, s>>>(perm, rows, cols, vals, head, incl, n, out_rows,
-                                                 out_cols, out_vals);
+    const int64_t* seq_vals = vals;
+    if (perm) {
+        gather_kernel<<<grid_for(n), 256, 0, s>>>(perm, n, vals, n, gv);
+        HYD_CHECK_LAUNCH();
+        seq_vals = gv;
+    }
+    HYD_TRY(scan_i64(seq_vals, n, 0, psum, true, false, s, scan_ws));
+    coo_run_ends_kernel<<<grid_for(n), 256, 0, s>>>(perm, rows, cols, head, incl, psum, n, out_rows,
+                                                    out_cols, end_p);
+    HYD_CHECK_LAUNCH();
+    coo_run_sums_kernel<<<grid_for(n), 256, 0, s>>>(end_p, incl + (n - 1), n, out_vals);
     HYD_CHECK_LAUNCH();
     HYD_TRY_HIP(hipMemcpyAsync(d_nnz_out, incl + (n - 1), 8, hipMemcpyDeviceToDevice, s));
     return HYD_OK;
 }
 
-int64_t coo_workspace_words(int64_t n) { return 2 * n + scan_workspace_words(n) + 8; }
+int64_t coo_workspace_words(int64_t n) { return 5 * n + scan_workspace_words(n) + 8; }
 
-// csr_from_sorted_coo with nnz known on the host.  ws: nrows+1 + scan words.
+// csr_from_sorted_coo with nnz known on the host.  Row counts without atomics
+// when the rows are sorted (the usual case: the COO pipeline sorts by
+// row-major key), one atomic per element otherwise; a device flag picks the
+// path, so there is no host round trip.  ws: csr_workspace_words(nrows).
 int csr_from_sorted_coo_i64(int64_t nrows, int64_t nnz, const int64_t* rows, const int64_t* cols,
                             const int64_t* vals, int64_t len, int64_t* row_ptr, int64_t* col_idx,
                             int64_t* out_vals, hipStream_t s, int64_t* ws) {
-    int64_t* counts = ws;
-    int64_t* scan_ws = ws + nrows + 1;
-    HYD_TRY(fill_zero(counts, (size_t)(nrows + 1) * 8, s));
-    if (nnz > 0) {
-        row_count_kernel<<<grid_for(nnz), 256, 0, s>>>(rows, nnz, nrows, counts);
+    int64_t* counts = ws;                       // nrows + 1
+    int64_t* first = ws + (nrows + 1);          // nrows
+    int64_t* last1 = ws + 2 * nrows + 1;        // nrows
+    int64_t* flag = ws + 3 * nrows + 1;         // 1
+    int64_t* scan_ws = ws + 3 * nrows + 2;
+    HYD_TRY(fill_zero(ws, (size_t)(3 * nrows + 2) * 8, s));
+    if (nnz > 0 && nrows > 0) {
+        unsorted_flag_kernel<<<grid_for(nnz), 256, 0, s>>>(rows, nnz, flag);
+        HYD_CHECK_LAUNCH();
+        row_bounds_kernel<<<grid_for(nnz), 256, 0, s>>>(rows, nnz, nrows, flag, first, last1);
+        HYD_CHECK_LAUNCH();
+        row_count_from_bounds_kernel<<<grid_for(nrows), 256, 0, s>>>(first, last1, nrows, flag, counts);
+        HYD_CHECK_LAUNCH();
+        row_count_kernel<<<grid_for(nnz), 256, 0, s>>>(rows, nnz, nrows, flag, counts);
         HYD_CHECK_LAUNCH();
     }
     // exclusive scan over nrows+1 entries (the extra zero count makes
@@ -488,7 +559,7 @@ int csr_from_sorted_coo_i64(int64_t nrows, int64_t nnz, const int64_t* rows, con
     return HYD_OK;
 }
 
-int64_t csr_workspace_words(int64_t nrows) { return nrows + 1 + scan_workspace_words(nrows + 1) + 8; }
+int64_t csr_workspace_words(int64_t nrows) { return 3 * nrows + 2 + scan_workspace_words(nrows + 1) + 8; }
 
 int pack_keys_i64(const int64_t* rows, const int64_t* cols, int64_t n, int64_t ncols, int64_t* keys,
                   hipStream_t s) {
