@@ -151,6 +151,16 @@ int hydhip_tune_set(int64_t bins, int64_t n, int mode);
 int hydhip_tune_clear(void);
 int hydhip_tune_table(int* out, int max_entries);
 
+/* Many dot products in one launch: *outs[r] = init + sum_i xs[r][i] * ys[r][i]
+ * for requests r < count (the zipwith -> reduce of programs/dot.hyd, one
+ * instance per request).  xs, ys, lens and outs are HOST arrays of device
+ * pointers and lengths.  Each request is cut into fixed 65536-element tiles
+ * summed in a fixed order, so its bits do not depend on the other requests
+ * of the batch (results within 1e-12 relative of the serial sum). */
+int hydhip_batched_dot_f64(const double* const* xs, const double* const* ys,
+                           const int64_t* lens, int64_t count, double init,
+                           double* const* outs, void* stream);
+
 /* ---- scans, sort, gather, COO -> CSR (int64) ---------------------------- */
 /* The scan family of src/Language/Hydrangea/Lowering.hs:3563-3607 with (+):
  * exclusive: out[k] = init + sum_{j<k} a[j]; inclusive adds a[k] too;
