@@ -222,7 +222,7 @@ def bench_batched(steps, warmup):
     return res
 
 
-def bench_serve(count=1500, budget=8 << 30, workers=4, load=0.7):
+def bench_serve(count=1500, budget=8 << 30, workers=8, load=0.7):
     """Served mixed stream vs plain per-call launches on the same requests.
 
     plain: one synchronous hydhip_run_request_plain per request (hipMalloc'd

@@ -54,14 +54,6 @@ HYD_HD static inline uint64_t hyd_as_u64(double d) {
     return c.u;
 }
 
-HYD_HD static inline uint64_t hyd_exp_tab(uint64_t i) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return hyd_exp_tab_dev[i];
-#else
-    return hyd_exp_tab_host[i];
-#endif
-}
-
 /* |x| >= 512 (or the scale would leave the normal range): rescale to keep the
  * intermediate in range, with the rounding care the subnormal results need. */
 HYD_HD static inline double hyd_exp_special(double tmp, uint64_t sbits, uint64_t ki) {
@@ -87,7 +79,19 @@ HYD_HD static inline double hyd_exp_special(double tmp, uint64_t sbits, uint64_t
     return 0x1p-1022 * y;
 }
 
+/* The exp with its table passed in (on the GPU a copy in LDS: the lookup then
+ * takes the LDS path instead of competing with a kernel's streaming loads). */
+HYD_HD static inline double hyd_exp_with(double x, const uint64_t* tab);
+
 HYD_HD static inline double hyd_exp(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return hyd_exp_with(x, hyd_exp_tab_dev);
+#else
+    return hyd_exp_with(x, hyd_exp_tab_host);
+#endif
+}
+
+HYD_HD static inline double hyd_exp_with(double x, const uint64_t* tab) {
     const double inv_ln2_n = 0x1.71547652b82fep0 * 128.0;
     const double neg_ln2_hi_n = -0x1.62e42fefa0000p-8;
     const double neg_ln2_lo_n = -0x1.cf79abc9e3b3ap-47;
@@ -109,8 +113,8 @@ HYD_HD static inline double hyd_exp(double x) {
     double r = HYD_FMA(kd, neg_ln2_lo_n, HYD_FMA(kd, neg_ln2_hi_n, x));
     uint64_t idx = 2 * (ki % 128);
     uint64_t top = ki << 45;
-    double tail = hyd_as_double(hyd_exp_tab(idx));
-    uint64_t sbits = hyd_exp_tab(idx + 1) + top;
+    double tail = hyd_as_double(tab[idx]);
+    uint64_t sbits = tab[idx + 1] + top;
     double r2 = r * r;
     double tmp = HYD_FMA(r2 * r2, HYD_FMA(r, c5, c4), HYD_FMA(r2, HYD_FMA(r, c3, c2), tail + r));
     double scale = hyd_as_double(sbits);

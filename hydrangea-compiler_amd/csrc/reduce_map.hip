@@ -62,8 +62,8 @@ __device__ __forceinline__ double block_sum(double v) {
 // so the grid is sized in workgroups per CU, not by the data.
 constexpr int kStreamDepth = 4;
 constexpr int kDotPerCU = 1;
-constexpr int kMapPerCU = 8;
-constexpr int kMapDepth = 2;
+constexpr int kMapPerCU = 1;
+constexpr int kMapDepth = 6;
 constexpr int kCopyPerCU = 1;
 
 // Grid for `work` pairs: at most `per_cu` workgroups per CU, at least one.
@@ -131,7 +131,7 @@ __global__ __launch_bounds__(kFinalBlock) void dot_final_kernel(const double* __
 
 // The fused chain of the map benchmark program (one statement per op, the
 // order of the source expression).
-__device__ __forceinline__ double map_chain_op(double x, double y) {
+__device__ __forceinline__ double map_chain_op(double x, double y, const uint64_t* exp_tab) {
     double t1 = 2.0 * x;
     double t2 = t1 + y;
     double t3 = __builtin_fabs(t2);
@@ -144,7 +144,7 @@ __device__ __forceinline__ double map_chain_op(double x, double y) {
     else
         t7 = 0.0;
     double t8 = t7 * 1.0e-3;
-    return hyd_exp(t8);
+    return hyd_exp_with(t8, exp_tab);
 }
 
 template <int D>
@@ -157,13 +157,19 @@ __device__ __forceinline__ void load_pairs(const double* __restrict__ xs, const 
     }
 }
 
-// The chain's math (a correctly rounded sqrt, the table exp) is heavy enough
-// that this kernel wants more waves per CU than the pure streaming kernels:
-// measured best at 8 workgroups per CU with 2 pairs in flight per thread.
+// One workgroup per CU with 6 pairs per array in flight (measured best on
+// MI355X: 6.4 TB/s).  The exp table is copied into LDS: with the lookups as
+// global loads they shared the L1 path with the stream, and the kernel then
+// needed 8 workgroups per CU to reach 5.4 TB/s.
 template <bool VEC, int D>
 __global__ __launch_bounds__(kBlock) void map_chain_kernel(const double* __restrict__ xs,
                                                            const double* __restrict__ ys,
                                                            int64_t n, double* __restrict__ out) {
+    // the exp table in LDS: its per-element lookups then stay off the path
+    // of the streaming loads
+    __shared__ uint64_t exp_tab[256];
+    for (int i = threadIdx.x; i < 256; i += kBlock) exp_tab[i] = hyd_exp_tab_dev[i];
+    __syncthreads();
     const int64_t step = (int64_t)gridDim.x * kBlock;
     int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (VEC) {
@@ -175,21 +181,21 @@ __global__ __launch_bounds__(kBlock) void map_chain_kernel(const double* __restr
 #pragma unroll
             for (int k = 0; k < D; ++k) {
                 v2d r;
-                r.x = map_chain_op(x[k].x, y[k].x);
-                r.y = map_chain_op(x[k].y, y[k].y);
+                r.x = map_chain_op(x[k].x, y[k].x, exp_tab);
+                r.y = map_chain_op(x[k].y, y[k].y, exp_tab);
                 st_stream2(out + 2 * (p + k * step), r);
             }
         }
         for (; p < npairs; p += step) {
             const v2d x = ld_stream2(xs + 2 * p), y = ld_stream2(ys + 2 * p);
             v2d r;
-            r.x = map_chain_op(x.x, y.x);
-            r.y = map_chain_op(x.y, y.y);
+            r.x = map_chain_op(x.x, y.x, exp_tab);
+            r.y = map_chain_op(x.y, y.y, exp_tab);
             st_stream2(out + 2 * p, r);
         }
-        if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) out[n - 1] = map_chain_op(xs[n - 1], ys[n - 1]);
+        if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) out[n - 1] = map_chain_op(xs[n - 1], ys[n - 1], exp_tab);
     } else {
-        for (; p < n; p += step) out[p] = map_chain_op(xs[p], ys[p]);
+        for (; p < n; p += step) out[p] = map_chain_op(xs[p], ys[p], exp_tab);
     }
 }
 
