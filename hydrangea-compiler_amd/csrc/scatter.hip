@@ -92,6 +92,30 @@ struct PairSrc {
     }
 };
 
+// Guarded unfused source (scatter_guarded, Lowering.hs:1432-1517): element i
+// contributes vals[i] at idx[i] when guard[i] != 0 (bools are int64 in the
+// emitted C); a false guard routes to -1, which every form drops.
+template <bool ONCE = true>
+struct GuardedPairSrc {
+    const int64_t* __restrict__ idx;
+    const int64_t* __restrict__ vals;
+    const int64_t* __restrict__ guard;
+    GuardedPairSrc<false> shared() const { return GuardedPairSrc<false>{idx, vals, guard}; }
+    __device__ __forceinline__ void pair(int64_t p, int64_t& b0, uint64_t& v0, int64_t& b1,
+                                         uint64_t& v1) const {
+        const v2i i2 = load_pair<ONCE>(idx + 2 * p), v2 = load_pair<ONCE>(vals + 2 * p),
+                  g2 = load_pair<ONCE>(guard + 2 * p);
+        b0 = g2.x ? i2.x : -1;
+        b1 = g2.y ? i2.y : -1;
+        v0 = (uint64_t)v2.x;
+        v1 = (uint64_t)v2.y;
+    }
+    __device__ __forceinline__ void one(int64_t i, int64_t& b, uint64_t& v) const {
+        b = guard[i] ? idx[i] : -1;
+        v = (uint64_t)vals[i];
+    }
+};
+
 __device__ __forceinline__ void lds_add(unsigned long long* p, uint64_t v) {
     __hip_atomic_fetch_add(p, (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
@@ -364,6 +388,41 @@ extern "C" int hydhip_scatter_add_i64(const int64_t* idx, const int64_t* vals, i
     PairSrc<> src{idx, vals};
     return launch_scatter(src, aligned16(idx) && aligned16(vals), n, ndst, dst, strategy, false,
                           resolve_stream(stream));
+}
+
+// Guarded scatter (+) into dst (which holds the defaults): dst[idx[i]] +=
+// vals[i] where guard[i] != 0; guard NULL: every element.  int64 + is
+// order-free mod 2^64, so the privatized forms give the serial loop's bits.
+extern "C" int hydhip_scatter_guarded_add_i64(const int64_t* idx, const int64_t* vals,
+                                              const int64_t* guard, int64_t n, int64_t* dst,
+                                              int64_t ndst, int strategy, void* stream) {
+    HYD_TRY(ensure_init());
+    if (n < 0 || ndst < 1 || ndst > (int64_t)1 << 31 || !dst || (n > 0 && (!idx || !vals)))
+        return set_error(HYD_E_INVALID, "hydhip_scatter_guarded_add_i64: bad arguments");
+    if (n == 0) return HYD_OK;
+    hipStream_t s = resolve_stream(stream);
+    if (!guard)
+        return launch_scatter(PairSrc<>{idx, vals}, aligned16(idx) && aligned16(vals), n, ndst, dst,
+                              strategy, false, s);
+    return launch_scatter(GuardedPairSrc<>{idx, vals, guard},
+                          aligned16(idx) && aligned16(vals) && aligned16(guard), n, ndst, dst, strategy,
+                          false, s);
+}
+
+// Scatter chain (EScatterChain, Lowering.hs:1622-1858) with (+) on int64: the
+// phases accumulate into one buffer in turn.  idx/vals/guard/lens are host
+// arrays of device pointers and lengths (guard[k] NULL: phase k unguarded).
+extern "C" int hydhip_scatter_chain_add_i64(int64_t nphases, const int64_t* const* idx,
+                                            const int64_t* const* vals, const int64_t* const* guard,
+                                            const int64_t* lens, int64_t* dst, int64_t ndst,
+                                            int strategy, void* stream) {
+    HYD_TRY(ensure_init());
+    if (nphases < 0 || (nphases > 0 && (!idx || !vals || !lens)))
+        return set_error(HYD_E_INVALID, "hydhip_scatter_chain_add_i64: bad arguments");
+    for (int64_t k = 0; k < nphases; ++k)
+        HYD_TRY(hydhip_scatter_guarded_add_i64(idx[k], vals[k], guard ? guard[k] : nullptr, lens[k], dst,
+                                               ndst, strategy, stream));
+    return HYD_OK;
 }
 
 // The remainder the kernels route with, compiled for the host CPU.

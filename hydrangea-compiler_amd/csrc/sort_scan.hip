@@ -470,6 +470,43 @@ __global__ void copy_prefix_kernel(const int64_t* __restrict__ src, int64_t nnz,
         dst[i] = i < nnz ? src[i] : 0;
 }
 
+// ---- order-exact fp64 scatter (+.) ------------------------------------------
+// fp64 + is not associative, so the bits of a scatter (+.) depend on the order
+// in which each bin receives its values: the reference's loop goes in element
+// order (Lowering.hs:1432-1517, serial `for i`).  The GPU reproduces that
+// order: a stable sort of the contributions by bin keeps element order within
+// a bin, and the first element of each bin's run folds the run in that order.
+
+// keys[i] = the bin of element i, or ndst when it does not contribute (a
+// false guard or a route outside [0, ndst)): such elements sort last.
+__global__ void scatter_keys_kernel(const int64_t* __restrict__ idx, const int64_t* __restrict__ guard,
+                                    int64_t n, int64_t ndst, int64_t* __restrict__ keys) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = idx[i];
+        keys[i] = ((!guard || guard[i]) && b >= 0 && b < ndst) ? b : ndst;
+    }
+}
+
+// skeys: the keys in sorted order, perm: the stable sort's permutation.
+// old = dst[b]; new = val + old, element by element (the combine write of
+// Lowering.hs:3553-3557, operands in the reference's order).
+__global__ void ordered_fold_kernel(const int64_t* __restrict__ skeys, const int64_t* __restrict__ perm,
+                                    const double* __restrict__ vals, int64_t n, int64_t ndst,
+                                    double* __restrict__ dst) {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = skeys[j];
+        if (b >= ndst || (j > 0 && skeys[j - 1] == b)) continue;
+        double acc = dst[b];
+        for (int64_t t = j; t < n && skeys[t] == b; ++t) {
+            const double v = vals[perm[t]];
+            acc = v + acc;
+        }
+        dst[b] = acc;
+    }
+}
+
 __global__ void set_word_kernel(int64_t* __restrict__ dst, const int64_t* __restrict__ a,
                                 const int64_t* __restrict__ b) {
     // *dst = *a + *b  (e.g. row_ptr[nrows] = last prefix + last count)
@@ -560,6 +597,23 @@ int csr_from_sorted_coo_i64(int64_t nrows, int64_t nnz, const int64_t* rows, con
 }
 
 int64_t csr_workspace_words(int64_t nrows) { return 3 * nrows + 2 + scan_workspace_words(nrows + 1) + 8; }
+
+// Order-exact scatter (+.) of vals (n, element order) with bin keys (ndst =
+// dropped) into dst.  ws: scatter_ordered_workspace_words(n).
+int scatter_ordered_add_f64(const int64_t* keys, const double* vals, int64_t n, int64_t ndst, double* dst,
+                            hipStream_t s, int64_t* ws) {
+    if (n == 0) return HYD_OK;
+    int64_t* perm = ws;
+    int64_t* skeys = ws + n;
+    HYD_TRY(sort_indices_i64(keys, n, perm, s, ws + 2 * n, nullptr));
+    gather_kernel<<<grid_for(n), 256, 0, s>>>(perm, n, keys, n, skeys);
+    HYD_CHECK_LAUNCH();
+    ordered_fold_kernel<<<grid_for(n), 256, 0, s>>>(skeys, perm, vals, n, ndst, dst);
+    HYD_CHECK_LAUNCH();
+    return HYD_OK;
+}
+
+int64_t scatter_ordered_workspace_words(int64_t n) { return 2 * n + sort_workspace_words(n) + 8; }
 
 int pack_keys_i64(const int64_t* rows, const int64_t* cols, int64_t n, int64_t ncols, int64_t* keys,
                   hipStream_t s) {
@@ -710,3 +764,60 @@ int coo_to_csr_ws(int64_t nrows, int64_t ncols, int64_t nnz, const int64_t* rows
     return csr_from_sorted_coo_i64(nrows, cnt, crow, ccol, cval, nnz, row_ptr, col_idx, out_vals, s, scr);
 }
 }  // namespace hyd
+
+// Guarded scatter (+.) on fp64, bit-exact to the reference's serial loop:
+// dst (holding the defaults) receives vals[i] at idx[i] where guard[i] != 0
+// (guard NULL: every element), bin by bin in element order.
+extern "C" int hydhip_scatter_guarded_add_f64(const int64_t* idx, const double* vals,
+                                              const int64_t* guard, int64_t n, double* dst,
+                                              int64_t ndst, void* stream) {
+    HYD_TRY(ensure_init());
+    if (n < 0 || ndst < 1 || ndst > (int64_t)1 << 31 || !dst || (n > 0 && (!idx || !vals)))
+        return set_error(HYD_E_INVALID, "hydhip_scatter_guarded_add_f64: bad arguments");
+    if (n == 0) return HYD_OK;
+    hipStream_t s = resolve_stream(stream);
+    int64_t* ws = static_cast<int64_t*>(
+        stream_workspace(s, (size_t)(n + scatter_ordered_workspace_words(n)) * 8));
+    if (!ws) return set_error(HYD_E_OOM, "hydhip_scatter_guarded_add_f64: workspace allocation failed");
+    int64_t* keys = ws;
+    scatter_keys_kernel<<<grid_for(n), 256, 0, s>>>(idx, guard, n, ndst, keys);
+    HYD_CHECK_LAUNCH();
+    return scatter_ordered_add_f64(keys, vals, n, ndst, dst, s, ws + n);
+}
+
+// Scatter chain (EScatterChain, Lowering.hs:1622-1858) with (+.) on fp64: the
+// phases' contributions concatenated in phase order, then one order-exact
+// scatter - per bin, phase 0's values in element order, then phase 1's, ...
+// as the reference's phase loops produce.  idx/vals/guard/lens are host
+// arrays of device pointers and lengths (guard[k] NULL: unguarded phase).
+extern "C" int hydhip_scatter_chain_add_f64(int64_t nphases, const int64_t* const* idx,
+                                            const double* const* vals, const int64_t* const* guard,
+                                            const int64_t* lens, double* dst, int64_t ndst, void* stream) {
+    HYD_TRY(ensure_init());
+    if (nphases < 0 || ndst < 1 || ndst > (int64_t)1 << 31 || !dst ||
+        (nphases > 0 && (!idx || !vals || !lens)))
+        return set_error(HYD_E_INVALID, "hydhip_scatter_chain_add_f64: bad arguments");
+    int64_t total = 0;
+    for (int64_t k = 0; k < nphases; ++k) {
+        if (lens[k] < 0 || (lens[k] > 0 && (!idx[k] || !vals[k])))
+            return set_error(HYD_E_INVALID, "hydhip_scatter_chain_add_f64: bad phase %lld", (long long)k);
+        total += lens[k];
+    }
+    if (total == 0) return HYD_OK;
+    hipStream_t s = resolve_stream(stream);
+    int64_t* ws = static_cast<int64_t*>(
+        stream_workspace(s, (size_t)(2 * total + scatter_ordered_workspace_words(total)) * 8));
+    if (!ws) return set_error(HYD_E_OOM, "hydhip_scatter_chain_add_f64: workspace allocation failed");
+    int64_t* keys = ws;
+    double* cvals = reinterpret_cast<double*>(ws + total);
+    int64_t off = 0;
+    for (int64_t k = 0; k < nphases; ++k) {
+        const int64_t m = lens[k];
+        if (m == 0) continue;
+        scatter_keys_kernel<<<grid_for(m), 256, 0, s>>>(idx[k], guard ? guard[k] : nullptr, m, ndst, keys + off);
+        HYD_CHECK_LAUNCH();
+        HYD_TRY_HIP(hipMemcpyAsync(cvals + off, vals[k], (size_t)m * 8, hipMemcpyDeviceToDevice, s));
+        off += m;
+    }
+    return scatter_ordered_add_f64(keys, cvals, total, ndst, dst, s, ws + 2 * total);
+}

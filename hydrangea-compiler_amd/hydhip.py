@@ -131,6 +131,10 @@ _SIGS = {
     "hydhip_fill_splitmix_f64": (_INT, [_P, _I64, _U64, _U64, _D, _D, _P]),
     "hydhip_weighted_histogram_i64": (_INT, [_P, _I64, _I64, _P, _INT, _P]),
     "hydhip_scatter_add_i64": (_INT, [_P, _P, _I64, _P, _I64, _INT, _P]),
+    "hydhip_scatter_guarded_add_i64": (_INT, [_P, _P, _P, _I64, _P, _I64, _INT, _P]),
+    "hydhip_scatter_chain_add_i64": (_INT, [_I64, _P, _P, _P, _P, _P, _I64, _INT, _P]),
+    "hydhip_scatter_guarded_add_f64": (_INT, [_P, _P, _P, _I64, _P, _I64, _P]),
+    "hydhip_scatter_chain_add_f64": (_INT, [_I64, _P, _P, _P, _P, _P, _I64, _P]),
     "hydhip_dot_f64": (_INT, [_P, _P, _I64, _D, _P, _P]),
     "hydhip_map_chain_f64": (_INT, [_P, _P, _I64, _P, _P]),
     "hydhip_math_f64": (_INT, [_INT, _P, _I64, _P, _P]),
@@ -327,6 +331,34 @@ def weighted_histogram(data, nbins, strategy=SCATTER_AUTO, out=None, stream=None
 
 def scatter_add(idx, vals, dst, strategy=SCATTER_AUTO, stream=None):
     _check(lib().hydhip_scatter_add_i64(idx.ptr, vals.ptr, idx.size, dst.ptr, dst.size, strategy, stream))
+    return dst
+
+
+def scatter_guarded_add(idx, vals, guard, dst, strategy=SCATTER_AUTO, stream=None):
+    """dst (holding the defaults) += vals at idx where guard != 0 (guard None:
+    everywhere).  int64 dst: (+); float64 dst: order-exact (+.)."""
+    g = guard.ptr if guard is not None else None
+    if dst.dtype == np.float64:
+        _check(lib().hydhip_scatter_guarded_add_f64(idx.ptr, vals.ptr, g, idx.size, dst.ptr, dst.size,
+                                                    stream))
+    else:
+        _check(lib().hydhip_scatter_guarded_add_i64(idx.ptr, vals.ptr, g, idx.size, dst.ptr, dst.size,
+                                                    strategy, stream))
+    return dst
+
+
+def scatter_chain_add(phases, dst, strategy=SCATTER_AUTO, stream=None):
+    """phases: list of (idx, vals, guard or None) DeviceArrays, applied in turn."""
+    k = len(phases)
+    ip = (ctypes.c_void_p * max(k, 1))(*[p[0].ptr for p in phases])
+    vp = (ctypes.c_void_p * max(k, 1))(*[p[1].ptr for p in phases])
+    gp = (ctypes.c_void_p * max(k, 1))(*[(p[2].ptr if p[2] is not None else None) for p in phases])
+    lens = np.array([p[0].size for p in phases] or [0], np.int64)
+    if dst.dtype == np.float64:
+        _check(lib().hydhip_scatter_chain_add_f64(k, ip, vp, gp, lens.ctypes.data, dst.ptr, dst.size, stream))
+    else:
+        _check(lib().hydhip_scatter_chain_add_i64(k, ip, vp, gp, lens.ctypes.data, dst.ptr, dst.size,
+                                                  strategy, stream))
     return dst
 
 

@@ -86,6 +86,31 @@ int hydhip_fill_splitmix_f64(double* dst, int64_t n, uint64_t seed,
  * Elements whose route falls outside [0, nbins) are dropped. */
 int hydhip_weighted_histogram_i64(const int64_t* data, int64_t n, int64_t nbins,
                                   int64_t* hist, int strategy, void* stream);
+/* Guarded scatter (+) on int64, scatter_guarded of
+ * src/Language/Hydrangea/Lowering.hs:1432-1517: dst holds the defaults and
+ * receives dst[idx[i]] += vals[i] for every i with guard[i] != 0 (guard NULL:
+ * every i).  Routes outside [0, ndst) are dropped. */
+int hydhip_scatter_guarded_add_i64(const int64_t* idx, const int64_t* vals,
+                                   const int64_t* guard, int64_t n, int64_t* dst,
+                                   int64_t ndst, int strategy, void* stream);
+/* Scatter chain (+) on int64 (EScatterChain, Lowering.hs:1622-1858): phase k
+ * is a guarded scatter of lens[k] elements into the same dst, in turn.
+ * idx/vals/guard/lens are HOST arrays of device pointers and lengths;
+ * guard (or guard[k]) NULL means unguarded. */
+int hydhip_scatter_chain_add_i64(int64_t nphases, const int64_t* const* idx,
+                                 const int64_t* const* vals, const int64_t* const* guard,
+                                 const int64_t* lens, int64_t* dst, int64_t ndst,
+                                 int strategy, void* stream);
+/* The same on fp64 ((+.)), bit-exact to the reference's serial loops: per
+ * bin, the values arrive in element order (chains: phase by phase), which a
+ * stable sort by bin reproduces before an in-order fold per bin. */
+int hydhip_scatter_guarded_add_f64(const int64_t* idx, const double* vals,
+                                   const int64_t* guard, int64_t n, double* dst,
+                                   int64_t ndst, void* stream);
+int hydhip_scatter_chain_add_f64(int64_t nphases, const int64_t* const* idx,
+                                 const double* const* vals, const int64_t* const* guard,
+                                 const int64_t* lens, double* dst, int64_t ndst,
+                                 void* stream);
 /* Unfused scatter: dst[idx[i]] += vals[i] (dst is accumulated, not zeroed). */
 int hydhip_scatter_add_i64(const int64_t* idx, const int64_t* vals, int64_t n,
                            int64_t* dst, int64_t ndst, int strategy, void* stream);

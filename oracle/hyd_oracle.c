@@ -97,6 +97,49 @@ void oracle_scatter_add_privatized(const int64_t* idx, const int64_t* vals, int6
     free(grids);
 }
 
+/* Guarded scatter (+), Lowering.hs:1432-1517: dst starts as the defaults
+ * (the caller's copy); for i in order: if (guard[i]) { old = dst[idx];
+ * new = val + old; dst[idx] = new; }.  A guard is true when nonzero (bools
+ * are int64 in the emitted C, CodegenC.hs:2016).  guard == NULL: every
+ * element contributes (plain scatter). */
+void oracle_scatter_guarded_add_i64(const int64_t* idx, const int64_t* vals, const int64_t* guard,
+                                    int64_t n, int64_t* dst, int64_t ndst) {
+    for (int64_t i = 0; i < n; i++) {
+        if (guard && !guard[i]) continue;
+        int64_t b = idx[i];
+        if (b < 0 || b >= ndst) continue; /* proven in range by the reference's types */
+        int64_t old = dst[b];
+        int64_t nw = vals[i] + old;
+        dst[b] = nw;
+    }
+}
+
+/* The same loop over fp64 values ((+.)): order-dependent, so this serial
+ * index order is the contract the GPU reproduces bit for bit. */
+void oracle_scatter_guarded_add_f64(const int64_t* idx, const double* vals, const int64_t* guard,
+                                    int64_t n, double* dst, int64_t ndst) {
+    for (int64_t i = 0; i < n; i++) {
+        if (guard && !guard[i]) continue;
+        int64_t b = idx[i];
+        if (b < 0 || b >= ndst) continue;
+        double old = dst[b];
+        double nw = vals[i] + old;
+        dst[b] = nw;
+    }
+}
+
+/* Scatter chain (EScatterChain, Lowering.hs:1622-1858): the phases run one
+ * after another into the same buffer, each as the (guarded) loop above.
+ * Phase k: idx + off[k], vals + off[k], guard + off[k] (or no guard when
+ * has_guard[k] == 0), len[k] elements. */
+void oracle_scatter_chain_add_f64(const int64_t* idx, const double* vals, const int64_t* guard,
+                                  const int64_t* off, const int64_t* len, const int64_t* has_guard,
+                                  int64_t nphases, double* dst, int64_t ndst) {
+    for (int64_t k = 0; k < nphases; k++)
+        oracle_scatter_guarded_add_f64(idx + off[k], vals + off[k], has_guard[k] ? guard + off[k] : NULL,
+                                       len[k], dst, ndst);
+}
+
 /* The fused weighted histogram (scatter_generate after fusion rule 6,
  * Fusion.hs:2010-2030): route = x % nbins (C truncating %), value = x * x,
  * both recomputed from data[i]; destination fill [nbins] 0; privatized. */

@@ -23,6 +23,9 @@ _SIGS = {
     "oracle_num_threads": (_INT, []),
     "oracle_scatter_add_serial": (None, [_P, _P, _I64, _P, _I64]),
     "oracle_scatter_add_privatized": (None, [_P, _P, _I64, _P, _I64]),
+    "oracle_scatter_guarded_add_i64": (None, [_P, _P, _P, _I64, _P, _I64]),
+    "oracle_scatter_guarded_add_f64": (None, [_P, _P, _P, _I64, _P, _I64]),
+    "oracle_scatter_chain_add_f64": (None, [_P, _P, _P, _P, _P, _P, _I64, _P, _I64]),
     "oracle_weighted_histogram": (None, [_P, _I64, _I64, _P]),
     "oracle_weighted_histogram_serial": (None, [_P, _I64, _I64, _P]),
     "oracle_batched_histograms": (None, [_P, _P, _P, _P, _P, _I64, _P]),
@@ -95,6 +98,40 @@ def scatter_add(idx, vals, dst, privatized=True):
     f = lib().oracle_scatter_add_privatized if privatized else lib().oracle_scatter_add_serial
     f(_p(idx), _p(vals), idx.size, _p(dst), dst.size)
     return dst
+
+
+def scatter_guarded_add(idx, vals, guard, dst):
+    """Serial guarded scatter (+) / (+.) into a copy of dst (the defaults);
+    guard None: unguarded.  fp64 when vals/dst are float64."""
+    idx = _i64(idx)
+    g = None if guard is None else _i64(guard)
+    if np.asarray(dst).dtype == np.float64:
+        vals = np.ascontiguousarray(vals, np.float64)
+        out = np.ascontiguousarray(dst, np.float64).copy()
+        lib().oracle_scatter_guarded_add_f64(_p(idx), _p(vals), None if g is None else _p(g), idx.size,
+                                             _p(out), out.size)
+    else:
+        vals = _i64(vals)
+        out = _i64(dst).copy()
+        lib().oracle_scatter_guarded_add_i64(_p(idx), _p(vals), None if g is None else _p(g), idx.size,
+                                             _p(out), out.size)
+    return out
+
+
+def scatter_chain_add_f64(phases, dst):
+    """phases: list of (idx, vals, guard or None); serial, phase by phase."""
+    idx = _i64(np.concatenate([np.asarray(p[0], np.int64) for p in phases]) if phases else np.zeros(0))
+    vals = np.ascontiguousarray(np.concatenate([np.asarray(p[1], np.float64) for p in phases])
+                                if phases else np.zeros(0), np.float64)
+    guard = _i64(np.concatenate([np.asarray(p[2] if p[2] is not None else np.ones(len(p[0])), np.int64)
+                                 for p in phases]) if phases else np.zeros(0))
+    lens = _i64([len(p[0]) for p in phases])
+    off = _i64(np.concatenate([[0], np.cumsum(lens)[:-1]]) if phases else np.zeros(0))
+    has = _i64([0 if p[2] is None else 1 for p in phases])
+    out = np.ascontiguousarray(dst, np.float64).copy()
+    lib().oracle_scatter_chain_add_f64(_p(idx), _p(vals), _p(guard), _p(off), _p(lens), _p(has),
+                                       len(phases), _p(out), out.size)
+    return out
 
 
 def weighted_histogram(data, nbins, serial=False):

@@ -68,6 +68,13 @@ def cases():
         [202, 210, 218, 226], derived=True)
     add("scatter_guarded_even_only", "scatter", "test/Language/Hydrangea/EndToEndSpec.hs:636-642",
         dict(idx=[0, 1, 2, 3], vals=[1, 7, 13, 19], ndst=4), [1, 7, 13, 19], derived=True)
+    # the guarded form itself: src = iota 8, route x/2, weight 3x+1, keep x/2 == (x+1)/2
+    # (true for even x), defaults fill [4] 0
+    add("scatter_guarded_fused_example", "scatter_guarded",
+        "examples/scatter_guarded_fused.hyd:1-5; test/Language/Hydrangea/EndToEndSpec.hs:636-642",
+        dict(idx=[x // 2 for x in range(8)], vals=[3 * x + 1 for x in range(8)],
+             guard=[1 if x // 2 == (x + 1) // 2 else 0 for x in range(8)], defaults=[0, 0, 0, 0]),
+        [1, 7, 13, 19], derived=True)
     add("scatter_parallel_privatized_example", "scatter", "examples/scatter_parallel_privatized.hyd:1-9",
         dict(idx=[i // 8 for i in range(64)], vals=[1] * 64, ndst=8), [8] * 8, derived=True)
     add("scatter_parallel_direct_example", "scatter", "examples/scatter_parallel_direct.hyd:1-8",

@@ -88,6 +88,76 @@ def test_scatter_known(orc, case, privatized):
     assert got.tolist() == case["expected"]
 
 
+def test_scatter_guarded_known(orc):
+    (c,) = golden("scatter_guarded")
+    i = c["inputs"]
+    got = orc.scatter_guarded_add(i["idx"], i["vals"], i["guard"], np.array(i["defaults"], np.int64))
+    assert got.tolist() == c["expected"]
+    # the same answer in fp64 ((+.) form)
+    got = orc.scatter_guarded_add(i["idx"], np.array(i["vals"], np.float64), i["guard"],
+                                  np.array(i["defaults"], np.float64))
+    assert got.tolist() == [float(v) for v in c["expected"]]
+
+
+def test_scatter_guarded_matches_filtered_scatter(orc):
+    # guard g == plain scatter of the elements where g holds; defaults carry through
+    rng = np.random.default_rng(21)
+    n, ndst = 5000, 37
+    idx = rng.integers(-3, ndst + 3, n)
+    vals = rng.integers(-2**40, 2**40, n)
+    guard = rng.integers(0, 3, n)  # nonzero = true
+    defaults = rng.integers(-1000, 1000, ndst)
+    got = orc.scatter_guarded_add(idx, vals, guard, defaults)
+    keep = guard != 0
+    ref = defaults.copy()
+    ok = keep & (idx >= 0) & (idx < ndst)
+    np.add.at(ref, idx[ok], vals[ok])
+    assert np.array_equal(got, ref)
+    assert np.array_equal(orc.scatter_guarded_add(idx, vals, None, defaults),
+                          orc.scatter_add(idx, vals, defaults, privatized=False))
+
+
+def test_scatter_guarded_f64_is_serial_order(orc):
+    # (+.) is not associative: the contract is the serial index order per bin.
+    # 1e16 + 1 rounds back to 1e16, so ((0 + 1e16) + 1) + -1e16 = 0 whereas the
+    # order 1, 1e16, -1e16 gives 0 too but 1e16, -1e16, 1 gives 1.
+    idx = [0, 0, 0, 1, 1, 1]
+    vals = [1e16, 1.0, -1e16, 1e16, -1e16, 1.0]
+    got = orc.scatter_guarded_add(idx, np.array(vals), None, np.zeros(2))
+    assert got.tolist() == [0.0, 1.0]
+    # pure-Python serial loop over random data with heavy collisions
+    rng = np.random.default_rng(5)
+    n, ndst = 3000, 11
+    idx = rng.integers(0, ndst, n)
+    vals = rng.standard_normal(n) * 10.0 ** rng.integers(-8, 9, n)
+    guard = rng.integers(0, 2, n)
+    dst = rng.standard_normal(ndst)
+    ref = dst.copy()
+    for i in range(n):
+        if guard[i]:
+            ref[idx[i]] = vals[i] + ref[idx[i]]
+    got = orc.scatter_guarded_add(idx, vals, guard, dst)
+    assert got.view(np.uint64).tolist() == ref.view(np.uint64).tolist()
+
+
+def test_scatter_chain_f64_is_phase_order(orc):
+    rng = np.random.default_rng(6)
+    ndst = 7
+    phases = []
+    for k in range(3):
+        m = 400 + 100 * k
+        phases.append((rng.integers(0, ndst, m), rng.standard_normal(m) * 1e8,
+                       None if k == 1 else rng.integers(0, 2, m)))
+    dst = rng.standard_normal(ndst)
+    ref = dst.copy()
+    for idx, vals, guard in phases:
+        for i in range(len(idx)):
+            if guard is None or guard[i]:
+                ref[idx[i]] = vals[i] + ref[idx[i]]
+    got = orc.scatter_chain_add_f64(phases, dst)
+    assert got.view(np.uint64).tolist() == ref.view(np.uint64).tolist()
+
+
 def test_weighted_histogram_closed_form(orc):
     (c,) = golden("weighted_histogram")
     i = c["inputs"]
