@@ -133,6 +133,41 @@ def bench_copy(steps, warmup):
     return dict(ms=ms, bytes=16 * n, gbs=16 * n / ms / 1e6)
 
 
+def bench_guarded(steps, warmup):
+    """Guarded scatters on the reference's voxel benchmark shapes: int64 (+)
+    rasterization (5M points) and the order-exact fp64 (+.) trilinear splat
+    (1M points = 8M contributions), both into 64^3 voxels.  Algorithmic bytes:
+    idx + vals + guard per element read, the voxel grid read and written."""
+    res = {}
+    for name, gen, cfg, dtype in (("voxel_rasterization_i64", workloads.voxel_rasterization, workloads.VOXEL,
+                                   np.int64),
+                                  ("voxel_trilinear_splat_f64", workloads.voxel_trilinear_splat,
+                                   workloads.SPLAT, np.float64)):
+        idx, vals, guard, nvox = gen(**cfg)
+        di = hydhip.DeviceArray.from_numpy(idx)
+        dv = hydhip.DeviceArray.from_numpy(vals.astype(dtype))
+        dg = hydhip.DeviceArray.from_numpy(guard)
+        dst = hydhip.DeviceArray((nvox,), dtype)
+        zeros = hydhip.DeviceArray.from_numpy(np.zeros(nvox, dtype))
+
+        def step(ev):
+            hydhip.copy_device(zeros, dst)  # fill [nvox] 0
+            if ev:
+                ev[0].record()
+            hydhip.scatter_guarded_add(di, dv, dg, dst)
+            if ev:
+                ev[1].record()
+
+        ms, ev = timed_loop(step, steps, warmup, events=True)
+        kms = float(np.mean(ev))
+        b = 24 * idx.size + 16 * nvox
+        res[name] = dict(elements=int(idx.size), kept=int(guard.sum()), voxels=nvox, ms=kms,
+                         bytes=b, gbs=b / kms / 1e6, melem_s=idx.size / kms / 1e3)
+        for a in (di, dv, dg, dst, zeros):
+            a.free()
+    return res
+
+
 def bench_hist(steps, warmup):
     c = workloads.HIST
     d = hydhip.splitmix_i64(c["n"], c["seed"], bits=c["bits"])
@@ -361,6 +396,9 @@ def main():
     if not only or "hist" in only:
         log("histogram ...")
         kernels["weighted_histogram"] = bench_hist(args.steps, args.warmup)
+    if not only or "guarded" in only:
+        log("guarded scatters ...")
+        kernels["scatter_guarded"] = bench_guarded(args.steps, args.warmup)
     if not only or "jacobi" in only:
         log("jacobi ...")
         kernels["jacobi_heat"] = bench_jacobi(args.steps, args.warmup)

@@ -488,22 +488,124 @@ __global__ void scatter_keys_kernel(const int64_t* __restrict__ idx, const int64
     }
 }
 
-// skeys: the keys in sorted order, perm: the stable sort's permutation.
-// old = dst[b]; new = val + old, element by element (the combine write of
-// Lowering.hs:3553-3557, operands in the reference's order).
-__global__ void ordered_fold_kernel(const int64_t* __restrict__ skeys, const int64_t* __restrict__ perm,
-                                    const double* __restrict__ vals, int64_t n, int64_t ndst,
-                                    double* __restrict__ dst) {
+// keep[i] = 1 when element i contributes (its key is a bin < ndst).
+__global__ void keep_flags_kernel(const int64_t* __restrict__ keys, int64_t n, int64_t ndst,
+                                  int64_t* __restrict__ keep) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        keep[i] = keys[i] < ndst ? 1 : 0;
+}
+
+// Order-preserving compaction of the contributing elements: their keys and
+// their positions in the original sequence.
+__global__ void compact_kernel(const int64_t* __restrict__ keys, const int64_t* __restrict__ incl,
+                               int64_t n, int64_t ndst, int64_t* __restrict__ ckeys,
+                               int64_t* __restrict__ cpos) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        if (keys[i] >= ndst) continue;
+        const int64_t j = incl[i] - 1;
+        ckeys[j] = keys[i];
+        cpos[j] = i;
+    }
+}
+
+// out[j] = src[pos[perm[j]]] (the values, bit-copied, in sorted order).
+__global__ void gather2_kernel(const int64_t* __restrict__ perm, const int64_t* __restrict__ pos, int64_t m,
+                               const int64_t* __restrict__ src, int64_t* __restrict__ out) {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m;
+         j += (int64_t)gridDim.x * blockDim.x)
+        out[j] = src[pos[perm[j]]];
+}
+
+// Runs of equal bins in the sorted keys (bins < ndst only): head[j] = 1 at
+// the first element of a run.
+__global__ void run_heads_kernel(const int64_t* __restrict__ skeys, int64_t n, int64_t ndst,
+                                 int64_t* __restrict__ head) {
     for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n;
          j += (int64_t)gridDim.x * blockDim.x) {
         const int64_t b = skeys[j];
-        if (b >= ndst || (j > 0 && skeys[j - 1] == b)) continue;
+        head[j] = (b < ndst && (j == 0 || skeys[j - 1] != b)) ? 1 : 0;
+    }
+}
+
+// With r = (inclusive count of heads up to j) - 1: the first element of run r
+// records its start, the last its end.
+__global__ void run_bounds_kernel(const int64_t* __restrict__ skeys, const int64_t* __restrict__ incl,
+                                  int64_t n, int64_t ndst, int64_t* __restrict__ start,
+                                  int64_t* __restrict__ end) {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = skeys[j];
+        if (b >= ndst) continue;
+        const int64_t r = incl[j] - 1;
+        if (j == 0 || skeys[j - 1] != b) start[r] = j;
+        if (j == n - 1 || skeys[j + 1] != b) end[r] = j + 1;
+    }
+}
+
+// Folds the runs: old = dst[b]; new = val + old for the run's values in order
+// (the combine write of Lowering.hs:3553-3557, operands in the reference's
+// order), the values already gathered into sorted order.  Short runs: one
+// thread each (fold_short_runs_kernel).  A long run (more than kLongRun
+// values - a grid whose points hit few voxels) gets a whole wave
+// (fold_long_runs_kernel): 64 coalesced loads per step with the next step's
+// loads in flight, and the 64 adds in order on the broadcast values.  The sum
+// stays sequential either way, so the bits are the serial loop's.
+constexpr int64_t kLongRun = 32;
+
+__global__ void fold_short_runs_kernel(const int64_t* __restrict__ skeys, const double* __restrict__ svals,
+                                       const int64_t* __restrict__ start, const int64_t* __restrict__ end,
+                                       const int64_t* __restrict__ nruns, double* __restrict__ dst) {
+    const int64_t R = *nruns;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < R;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t a = start[r], e = end[r];
+        if (e - a > kLongRun) continue;
+        const int64_t b = skeys[a];
         double acc = dst[b];
-        for (int64_t t = j; t < n && skeys[t] == b; ++t) {
-            const double v = vals[perm[t]];
+        for (int64_t t = a; t < e; ++t) {
+            const double v = svals[t];
             acc = v + acc;
         }
         dst[b] = acc;
+    }
+}
+
+__device__ __forceinline__ double lane_value(double v, int l) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// One wave per run index; waves whose run is short leave at once.
+__global__ __launch_bounds__(256) void fold_long_runs_kernel(const int64_t* __restrict__ skeys,
+                                                             const double* __restrict__ svals,
+                                                             const int64_t* __restrict__ start,
+                                                             const int64_t* __restrict__ end,
+                                                             const int64_t* __restrict__ nruns,
+                                                             double* __restrict__ dst) {
+    const int64_t R = *nruns;
+    const int lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
+    for (int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64; r < R; r += waves) {
+        const int64_t a = start[r], e = end[r];  // uniform across the wave
+        if (e - a <= kLongRun) continue;
+        const int64_t b = skeys[a];
+        double acc = dst[b];
+        double v = a + lane < e ? svals[a + lane] : 0.0;
+        for (int64_t t0 = a; t0 < e; t0 += 64) {
+            const int64_t tn = t0 + 64;
+            const double vn = tn + lane < e ? svals[tn + lane] : 0.0;  // next step in flight
+            const int cnt = e - t0 < 64 ? (int)(e - t0) : 64;
+            for (int i = 0; i < cnt; ++i) {
+                const double x = lane_value(v, i);
+                acc = x + acc;
+            }
+            v = vn;
+        }
+        if (lane == 0) dst[b] = acc;
     }
 }
 
@@ -603,17 +705,48 @@ int64_t csr_workspace_words(int64_t nrows) { return 3 * nrows + 2 + scan_workspa
 int scatter_ordered_add_f64(const int64_t* keys, const double* vals, int64_t n, int64_t ndst, double* dst,
                             hipStream_t s, int64_t* ws) {
     if (n == 0) return HYD_OK;
-    int64_t* perm = ws;
-    int64_t* skeys = ws + n;
-    HYD_TRY(sort_indices_i64(keys, n, perm, s, ws + 2 * n, nullptr));
-    gather_kernel<<<grid_for(n), 256, 0, s>>>(perm, n, keys, n, skeys);
+    int64_t* keep = ws;            // n, later the run heads
+    int64_t* incl = ws + n;        // n
+    int64_t* ckeys = ws + 2 * n;   // m: the contributing keys, element order
+    int64_t* cpos = ws + 3 * n;    // m: their positions
+    int64_t* perm = ws + 4 * n;    // m
+    int64_t* skeys = ws + 5 * n;   // m
+    int64_t* svals = ws + 6 * n;   // m: the values' bits in sorted order
+    int64_t* start = ws + 7 * n;   // runs
+    int64_t* end = ws + 8 * n;     // runs
+    int64_t* scratch = ws + 9 * n; // sort, then scans
+    // only the contributing elements are sorted (the guard often drops most)
+    keep_flags_kernel<<<grid_for(n), 256, 0, s>>>(keys, n, ndst, keep);
     HYD_CHECK_LAUNCH();
-    ordered_fold_kernel<<<grid_for(n), 256, 0, s>>>(skeys, perm, vals, n, ndst, dst);
+    HYD_TRY(scan_i64(keep, n, 0, incl, true, false, s, scratch));
+    compact_kernel<<<grid_for(n), 256, 0, s>>>(keys, incl, n, ndst, ckeys, cpos);
+    HYD_CHECK_LAUNCH();
+    int64_t m = 0;
+    HYD_TRY_HIP(hipMemcpyAsync(&m, incl + (n - 1), 8, hipMemcpyDeviceToHost, s));
+    HYD_TRY_HIP(hipStreamSynchronize(s));
+    if (m == 0) return HYD_OK;
+    HYD_TRY(sort_indices_i64(ckeys, m, perm, s, scratch, nullptr));
+    gather_kernel<<<grid_for(m), 256, 0, s>>>(perm, m, ckeys, m, skeys);
+    HYD_CHECK_LAUNCH();
+    gather2_kernel<<<grid_for(m), 256, 0, s>>>(perm, cpos, m, reinterpret_cast<const int64_t*>(vals), svals);
+    HYD_CHECK_LAUNCH();
+    int64_t* head = keep;
+    run_heads_kernel<<<grid_for(m), 256, 0, s>>>(skeys, m, ndst, head);
+    HYD_CHECK_LAUNCH();
+    HYD_TRY(scan_i64(head, m, 0, incl, true, false, s, scratch));
+    run_bounds_kernel<<<grid_for(m), 256, 0, s>>>(skeys, incl, m, ndst, start, end);
+    HYD_CHECK_LAUNCH();
+    const double* sv = reinterpret_cast<const double*>(svals);
+    fold_short_runs_kernel<<<grid_for(m), 256, 0, s>>>(skeys, sv, start, end, incl + (m - 1), dst);
+    HYD_CHECK_LAUNCH();
+    fold_long_runs_kernel<<<grid_for(m * 64 / kLongRun), 256, 0, s>>>(skeys, sv, start, end, incl + (m - 1), dst);
     HYD_CHECK_LAUNCH();
     return HYD_OK;
 }
 
-int64_t scatter_ordered_workspace_words(int64_t n) { return 2 * n + sort_workspace_words(n) + 8; }
+int64_t scatter_ordered_workspace_words(int64_t n) {
+    return 9 * n + std::max(sort_workspace_words(n), scan_workspace_words(n)) + 8;
+}
 
 int pack_keys_i64(const int64_t* rows, const int64_t* cols, int64_t n, int64_t ncols, int64_t* keys,
                   hipStream_t s) {

@@ -362,6 +362,14 @@ def scatter_chain_add(phases, dst, strategy=SCATTER_AUTO, stream=None):
     return dst
 
 
+def copy_device(src, dst, stream=None):
+    """dst <- src (device to device, same byte size), stream-ordered."""
+    if src.nbytes != dst.nbytes:
+        raise ValueError("copy_device: sizes differ")
+    _check(lib().hydhip_memcpy_d2d(dst.ptr, src.ptr, src.nbytes, stream))
+    return dst
+
+
 def dot(xs, ys, init=0.0, out=None, stream=None):
     """Returns the device scalar (DeviceArray of shape (1,))."""
     out = out or DeviceArray((1,), np.float64)

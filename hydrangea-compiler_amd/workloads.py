@@ -98,3 +98,47 @@ def request_bytes_algorithmic(r):
 def batch_algorithmic_bytes(lens, bins):
     """Compulsory HBM traffic: read 8 B per element, write 8 B per bin."""
     return int(8 * np.sum(lens) + 8 * np.sum(bins))
+
+
+# Guarded scatters: the reference's voxel benchmarks (bench/voxel_rasterization,
+# bench/voxel_trilinear_splat) with their default grid 64^3 and keep period 3,
+# their route/weight/keep functions materialised as arrays.
+VOXEL = dict(points=5_000_000, nx=64, ny=64, nz=64, keep_period=3)
+SPLAT = dict(points=1_000_000, nx=64, ny=64, nz=64, keep_period=3)
+
+
+def voxel_rasterization(points, nx, ny, nz, keep_period):
+    """route/weight/keep of bench/voxel_rasterization/voxel_rasterization.hyd:
+    imod x m = x - (x / m) * m; all operands non-negative, so C division is
+    floor division.  Returns (idx, vals, guard) int64 and nvox."""
+    p = np.arange(points, dtype=np.int64)
+    x = (p * 17 + 3) % nx
+    y = (p * 29 + 5) % ny
+    z = (p * 43 + 7) % nz
+    route = ((z * ny) + y) * nx + x
+    weight = ((x + 1) * (y + 2)) + (z * 3) + 1
+    keep = ((p // keep_period) * keep_period == p).astype(np.int64)
+    return route, weight, keep, nx * ny * nz
+
+
+def voxel_trilinear_splat(points, nx, ny, nz, keep_period):
+    """route_of/weight_of/keep_of of bench/voxel_trilinear_splat/
+    voxel_trilinear_splat.hyd over 8 corner contributions per point; the
+    weight is float_of(base * wx * wy * wz) /. 64.0 (exact in fp64 here).
+    Returns (idx int64, vals fp64, guard int64) and nvox."""
+    i = np.arange(points * 8, dtype=np.int64)
+    p = i // 8
+    c = i % 8
+    bx, by, bz = c % 2, (c // 2) % 2, (c // 4) % 2
+    x = (p * 17 + 3 + bx) % nx
+    y = (p * 29 + 5 + by) % ny
+    z = (p * 43 + 7 + bz) % nz
+    route = ((z * ny) + y) * nx + x
+    fx, fy, fz = (p * 5 + 1) % 4, (p * 7 + 2) % 4, (p * 11 + 3) % 4
+    wx = (1 - bx) * (4 - fx) + bx * fx
+    wy = (1 - by) * (4 - fy) + by * fy
+    wz = (1 - bz) * (4 - fz) + bz * fz
+    base = (p * 3) + 1
+    weight = (base * wx * wy * wz).astype(np.float64) / 64.0
+    keep = ((p // keep_period) * keep_period == p).astype(np.int64)
+    return route, weight, keep, nx * ny * nz

@@ -131,3 +131,17 @@ def test_chain_f64_phase_order(hyd, orc):
                           dst)
     ref = orc.scatter_chain_add_f64(phases, defaults)
     assert np.array_equal(bits(dst.numpy()), bits(ref))
+
+
+def test_voxel_benchmarks_full_size(hyd, orc):
+    # the bench workloads themselves, bit-exact against the serial loops
+    import workloads
+    for gen, cfg, dtype in ((workloads.voxel_rasterization, workloads.VOXEL, np.int64),
+                            (workloads.voxel_trilinear_splat, workloads.SPLAT, np.float64)):
+        idx, vals, guard, nvox = gen(**cfg)
+        vals = vals.astype(dtype)
+        print("voxel %s points=%d contributions=%d" % (dtype.__name__, cfg["points"], idx.size), flush=True)
+        dst = dev(hyd, np.zeros(nvox, dtype))
+        hyd.scatter_guarded_add(dev(hyd, idx), dev(hyd, vals), dev(hyd, guard), dst)
+        ref = orc.scatter_guarded_add(idx, vals, guard, np.zeros(nvox, dtype))
+        assert np.array_equal(bits(dst.numpy()), bits(ref))
