@@ -125,9 +125,10 @@ __device__ __forceinline__ void global_add(unsigned long long* p, uint64_t v) {
 }
 
 // Streams the elements assigned to this workgroup (grid-stride over 16-byte
-// pairs, kDepth pairs in flight per thread: with one 512-thread workgroup
-// per CU that is 64 KiB of loads in flight per CU) and calls add(bin, value).
-constexpr int kDepth = 8;
+// pairs, kDepth pairs in flight per thread: with one 512-thread workgroup per
+// CU that is 32 KiB of loads in flight per CU - measured on MI355X as good as
+// deeper pipelines) and calls add(bin, value).
+constexpr int kDepth = 4;
 
 template <int BLOCK, bool VEC, class Src, class Add>
 __device__ __forceinline__ void stream_elements(const Src& src, int64_t n, Add add) {
