@@ -1,0 +1,874 @@
+// csv.hip - CSV text <-> int64 / binary64 arrays on the GPU: the read_array /
+// read_array_float / write_array / write_array_float runtime of the
+// reference (runtime/hydrangea_runtime.h:549-594, runtime/hyd_write_csv.c),
+// with the number conversions of csv_num.h.
+//
+// Parse (text in HBM -> values):
+//   1. count: each workgroup stages 8 KiB of text (plus halos) in LDS; a word
+//      is a maximal run of non-gap characters (gap = ',' or isspace).  The
+//      thread whose 32-byte slice holds a word's first character checks the
+//      gap before it and scans the word's numbers (a word holds several when
+//      they abut: "1-2" is 1 then -2).  Per tile: the number count and the
+//      first failure (local index).
+//   2. an exclusive scan of the tile counts gives each tile's first element.
+//   3. values: the same scan again, converting each number and storing
+//      element g < count.  A decimal the 128-bit table cannot round with
+//      certainty is marked in a bitmap, with its text position in its slot.
+//   4. exact: big-integer rounding of the marked elements (rare).
+//   5. status: the first failure over all tiles, or, when fewer numbers than
+//      elements exist, what the reference's loop reports at the end.
+//
+// Format (values -> text in HBM): per element the digits (fp64: 17
+// significant, table path, exact path for undecided roundings), per
+// 1024-element block the text length, a scan of the block lengths, then each
+// block writes its text into LDS and from there to HBM with 16-byte stores.
+//
+// The gap rule between two numbers follows the reference's loop: after an
+// element it skips ',', ' ', '\n', '\r' (fgetc), then fscanf skips isspace,
+// so a gap is [, \n\r]* followed by [ \t\n\v\f\r]*: a ',' after a '\t', '\v'
+// or '\f' fails the next element, and before the first element only
+// isspace is skipped.
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+
+#include "../../include/hyd_hip_runtime.h"
+#include "csv_num.h"
+#include "hyd_internal.hpp"
+
+namespace hyd {
+int scan_i64(const int64_t* in, int64_t n, int64_t init, int64_t* out, bool inclusive, bool reverse,
+             hipStream_t s, int64_t* tile_ws);
+int64_t scan_workspace_words(int64_t n);
+hipStream_t export_stream();
+int host_sync_enabled();
+void mark_device_written(hyd_array_t* arr);
+
+namespace {
+
+using namespace csv;
+
+constexpr int kPThreads = 256;
+constexpr int kWaves = kPThreads / 64;
+constexpr int kSlice = 32;                          // text bytes per thread
+constexpr int kTile = kPThreads * kSlice;           // 8 KiB per workgroup
+constexpr int kHaloLo = 64, kHaloHi = 448;          // LDS beyond the tile
+constexpr int kBuf = kHaloLo + kTile + kHaloHi;     // 8704 bytes, a multiple of 16
+constexpr int kChunks = kBuf / 16;
+
+// ---- block helpers --------------------------------------------------------
+__device__ __forceinline__ int64_t block_excl_scan(int64_t v, int64_t* total, int64_t* sh) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int64_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int64_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) sh[wid] = x;
+    __syncthreads();
+    int64_t off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        const int64_t t = sh[w];
+        if (w < wid) off += t;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + x - v;
+}
+
+__device__ __forceinline__ int64_t block_min(int64_t v, int64_t* sh) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const int64_t y = __shfl_xor(v, d, 64);
+        v = y < v ? y : v;
+    }
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    int64_t m = sh[0];
+#pragma unroll
+    for (int w = 1; w < kWaves; ++w) m = sh[w] < m ? sh[w] : m;
+    __syncthreads();
+    return m;
+}
+
+// Stages text[base, base + kBuf) into LDS (bytes outside [0, n) are left
+// unused: the accessor checks the range first).
+template <bool ALIGNED>
+__device__ __forceinline__ void stage_tile(const unsigned char* __restrict__ text, int64_t n, int64_t base,
+                                           unsigned char* buf) {
+    for (int j = threadIdx.x; j < kChunks; j += kPThreads) {
+        const int64_t p = base + 16 * (int64_t)j;
+        if (ALIGNED && p >= 0 && p + 16 <= n) {
+            *reinterpret_cast<uint4*>(buf + 16 * j) = *reinterpret_cast<const uint4*>(text + p);
+        } else {
+            for (int b = 0; b < 16; ++b)
+                if (p + b >= 0 && p + b < n) buf[16 * j + b] = text[p + b];
+        }
+    }
+    __syncthreads();
+}
+
+// Backward check of the gap that ends at p - 1 (p: a word's first char).
+template <class At>
+__device__ __forceinline__ bool gap_ok(const At& at, int64_t p) {
+    bool comma_seen = false;  // a ',' lies between here and the word
+    for (int64_t i = p - 1;; --i) {
+        const int c = at(i);
+        if (c < 0) return !comma_seen;  // start of text: only isspace allowed
+        if (!is_gap(c)) return true;    // end of the previous word
+        if (c == ',') comma_seen = true;
+        else if (comma_seen && !is_sep(c)) return false;  // '\t' '\v' '\f' before a ','
+    }
+}
+
+struct NullSink {
+    int64_t qadj = 0;
+    __device__ __forceinline__ void digit(int, bool, bool) {}
+};
+
+template <bool FLOAT, class At>
+__device__ __forceinline__ int64_t token_len(const At& at, int64_t p) {
+    if (FLOAT) {
+        NullSink s;
+        return scan_f64(at, p, s).len;
+    }
+    return scan_i64(at, p, (int64_t*)nullptr);
+}
+
+// Walks the words that start in this thread's slice.  Emit(p_token) is called
+// for each number in order; returns the count, and in *err the count at the
+// first failure (-1: none).  Stops at the first failure.
+template <bool FLOAT, class At, class Emit>
+__device__ __forceinline__ int64_t walk_slice(const At& at, int64_t s0, int64_t n, int64_t* err,
+                                              const Emit& emit) {
+    int64_t cnt = 0;
+    *err = -1;
+    int prev = at(s0 - 1);
+    for (int k = 0; k < kSlice; ++k) {
+        const int64_t p = s0 + k;
+        if (p >= n) break;
+        const int c = at(p);
+        const bool start = !is_gap(c) && (prev < 0 || is_gap(prev));
+        prev = c;
+        if (!start) continue;
+        if (!gap_ok(at, p)) {
+            *err = cnt;
+            return cnt;
+        }
+        int64_t q = p;
+        for (;;) {
+            const int64_t len = token_len<FLOAT>(at, q);
+            if (len < 0) {
+                *err = cnt;
+                return cnt;
+            }
+            emit(q, cnt);
+            ++cnt;
+            q += len;
+            const int c2 = at(q);
+            if (c2 < 0 || is_gap(c2)) break;
+        }
+    }
+    return cnt;
+}
+
+#define HYD_TILE_ACCESSOR                                                     \
+    __shared__ __attribute__((aligned(16))) unsigned char buf[kBuf];          \
+    const int64_t t0 = (int64_t)blockIdx.x * kTile;                           \
+    const int64_t base = t0 - kHaloLo;                                        \
+    stage_tile<ALIGNED>(text, n, base, buf);                                  \
+    auto at = [&](int64_t p) -> int {                                         \
+        if (p < 0 || p >= n) return -1;                                       \
+        const int64_t o = p - base;                                           \
+        if (o >= 0 && o < kBuf) return buf[o];                                \
+        return text[p];                                                       \
+    };                                                                        \
+    const int64_t s0 = t0 + (int64_t)threadIdx.x * kSlice;
+
+template <bool FLOAT, bool ALIGNED>
+__global__ __launch_bounds__(kPThreads) void csv_count_kernel(const unsigned char* __restrict__ text, int64_t n,
+                                                              int64_t* __restrict__ tile_cnt,
+                                                              int64_t* __restrict__ tile_err) {
+    HYD_TILE_ACCESSOR
+    __shared__ int64_t sh[kWaves];
+    int64_t err;
+    const int64_t cnt = walk_slice<FLOAT>(at, s0, n, &err, [](int64_t, int64_t) {});
+    int64_t total;
+    const int64_t off = block_excl_scan(cnt, &total, sh);
+    const int64_t e = block_min(err >= 0 ? off + err : INT64_MAX, sh);
+    if (threadIdx.x == 0) {
+        tile_cnt[blockIdx.x] = total;
+        tile_err[blockIdx.x] = e == INT64_MAX ? -1 : e;
+    }
+}
+
+template <bool FLOAT, bool ALIGNED>
+__global__ __launch_bounds__(kPThreads) void csv_value_kernel(const unsigned char* __restrict__ text, int64_t n,
+                                                              const int64_t* __restrict__ tile_base,
+                                                              int64_t count, int64_t* __restrict__ out,
+                                                              uint32_t* __restrict__ exact_bits,
+                                                              unsigned long long* __restrict__ exact_cnt) {
+    HYD_TILE_ACCESSOR
+    __shared__ int64_t sh[kWaves];
+    int64_t err;
+    const int64_t cnt = walk_slice<FLOAT>(at, s0, n, &err, [](int64_t, int64_t) {});
+    int64_t total;
+    const int64_t g0 = tile_base[blockIdx.x] + block_excl_scan(cnt, &total, sh);
+    if (g0 >= count || cnt == 0) return;
+    walk_slice<FLOAT>(at, s0, n, &err, [&](int64_t p, int64_t j) {
+        const int64_t g = g0 + j;
+        if (g >= count) return;
+        if (FLOAT) {
+            const FloatTok t = scan_f64_fast(at, p);
+            uint64_t bits;
+            if (tok_to_bits(t, &bits)) {
+                out[g] = (int64_t)bits;
+            } else {
+                out[g] = p;  // the exact kernel rereads the text there
+                atomicOr(&exact_bits[g >> 5], 1u << (g & 31));
+                atomicAdd(exact_cnt, 1ULL);
+            }
+        } else {
+            int64_t v;
+            scan_i64(at, p, &v);
+            out[g] = v;
+        }
+    });
+}
+
+// Exact rounding of the marked decimals (few; runs on a small grid).
+__global__ __launch_bounds__(64) void csv_exact_kernel(const unsigned char* __restrict__ text, int64_t n,
+                                                       int64_t count, const uint32_t* __restrict__ exact_bits,
+                                                       const unsigned long long* __restrict__ exact_cnt,
+                                                       int64_t* __restrict__ out) {
+    if (*exact_cnt == 0) return;
+    auto at = [&](int64_t p) -> int { return (p < 0 || p >= n) ? -1 : text[p]; };
+    const int64_t words = (count + 31) >> 5;
+    for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words;
+         w += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t b = exact_bits[w];
+        while (b) {
+            const int bit = __builtin_ctz(b);
+            b &= b - 1;
+            const int64_t g = w * 32 + bit;
+            const int64_t p = out[g];
+            const FloatTok t = scan_f64_fast(at, p);
+            uint64_t cand;
+            tok_to_bits(t, &cand);
+            Big D, L, R;
+            out[g] = (int64_t)dec_exact_bits(at, p, cand, D, L, R);
+        }
+    }
+}
+
+// status[0] = first failing element (-1: none below count), status[1] = kind,
+// status[2] = numbers found (capped by the first failure), status[3] = exact.
+__global__ __launch_bounds__(kPThreads) void csv_status_kernel(const unsigned char* __restrict__ text, int64_t n,
+                                                               int64_t count, int64_t tiles,
+                                                               const int64_t* __restrict__ tile_cnt,
+                                                               const int64_t* __restrict__ tile_err,
+                                                               const int64_t* __restrict__ tile_base,
+                                                               const unsigned long long* __restrict__ exact_cnt,
+                                                               int64_t* __restrict__ status) {
+    __shared__ int64_t sh[kWaves];
+    int64_t e = INT64_MAX;
+    for (int64_t t = threadIdx.x; t < tiles; t += kPThreads)
+        if (tile_err[t] >= 0) {
+            const int64_t g = tile_base[t] + tile_err[t];
+            e = g < e ? g : e;
+        }
+    e = block_min(e, sh);
+    const int64_t found = tiles ? tile_base[tiles - 1] + tile_cnt[tiles - 1] : 0;
+    int64_t kind = 0;
+    if (e != INT64_MAX) {
+        kind = HYDHIP_CSV_PARSE_ERROR;
+    } else if (found < count) {
+        e = found;
+        if (found == 0) {
+            kind = HYDHIP_CSV_PARSE_ERROR;  // the first fscanf meets the end
+        } else {
+            // the text after the last number is all gap (no word failed):
+            // the separator loop reaches the end unless a '\t' '\v' '\f'
+            // stops it, after which fscanf meets the end or a ','.  Scan
+            // back from the end to the last non-gap character.
+            int64_t bad = 0;
+            for (int64_t c0 = n - 1; c0 >= 0; c0 -= kPThreads) {
+                const int64_t p = c0 - threadIdx.x;
+                const int c = p >= 0 ? text[p] : ',';
+                const bool word = p >= 0 && !is_gap(c);
+                const bool tab = p >= 0 && is_gap(c) && !is_sep(c);
+                const int64_t last = -block_min(word ? -p : 1, sh);  // -1: none in this chunk
+                bad |= -block_min((tab && p > last) ? -1 : 0, sh);
+                if (last >= 0) break;
+            }
+            kind = bad ? HYDHIP_CSV_PARSE_ERROR : HYDHIP_CSV_TOO_FEW;
+        }
+    }
+    if (e >= count) {
+        e = -1;
+        kind = 0;
+    }
+    if (threadIdx.x == 0) {
+        status[0] = e;
+        status[1] = kind;
+        status[2] = found;
+        status[3] = (int64_t)*exact_cnt;
+    }
+}
+
+// ---- format ---------------------------------------------------------------
+constexpr int kFThreads = 256;
+constexpr int kFPer = 4;
+constexpr int kFBlock = kFThreads * kFPer;  // elements per workgroup
+constexpr int kMaxText = 25;               // "-1.2345678901234567e-308" + separator
+constexpr int kStage = kFBlock * kMaxText + 32;
+
+struct Dig {
+    uint64_t N;  // 17 digits; the bits for zero, inf, nan
+    int32_t X;
+    int32_t flags;  // 1 = special (zero, inf, nan), 2 = negative
+};
+
+__global__ __launch_bounds__(256) void fmt_digits_kernel(const uint64_t* __restrict__ vals, int64_t n,
+                                                         Dig* __restrict__ dig, uint32_t* __restrict__ exact_bits,
+                                                         unsigned long long* __restrict__ exact_cnt) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t b = vals[i];
+        Dig d;
+        if (special_len(b)) {
+            d.N = b;
+            d.X = 0;
+            d.flags = 1;
+        } else {
+            uint64_t N;
+            int X;
+            if (!f64_to_dec17(b & ~kSignBit, &N, &X)) {
+                atomicOr(&exact_bits[i >> 5], 1u << (i & 31));
+                atomicAdd(exact_cnt, 1ULL);
+            }
+            d.N = N;
+            d.X = X;
+            d.flags = (b >> 63) ? 2 : 0;
+        }
+        dig[i] = d;
+    }
+}
+
+__global__ __launch_bounds__(64) void fmt_exact_kernel(const uint64_t* __restrict__ vals, int64_t n,
+                                                       Dig* __restrict__ dig, const uint32_t* __restrict__ exact_bits,
+                                                       const unsigned long long* __restrict__ exact_cnt) {
+    if (*exact_cnt == 0) return;
+    const int64_t words = (n + 31) >> 5;
+    for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words;
+         w += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t b = exact_bits[w];
+        while (b) {
+            const int bit = __builtin_ctz(b);
+            b &= b - 1;
+            const int64_t i = w * 32 + bit;
+            Big D, L, R;
+            uint64_t N;
+            int X;
+            dec17_exact(vals[i] & ~kSignBit, dig[i].N, dig[i].X, &N, &X, D, L, R);
+            dig[i].N = N;
+            dig[i].X = X;
+        }
+    }
+}
+
+template <bool FLOAT>
+__device__ __forceinline__ int elem_len(const uint64_t* vals, const Dig* dig, int64_t i) {
+    if (FLOAT) {
+        const Dig d = dig[i];
+        if (d.flags & 1) return special_len(d.N);
+        return g17_len(g17_make(d.flags & 2, d.N, d.X));
+    }
+    return i64_len((int64_t)vals[i]);
+}
+
+// Row position of element i: ',' after it, or '\n' at the end of a row.
+__device__ __forceinline__ char sep_after(int64_t i, int64_t row_len) {
+    return (i % row_len) == row_len - 1 ? '\n' : ',';
+}
+
+template <bool FLOAT>
+__global__ __launch_bounds__(kFThreads) void fmt_len_kernel(const uint64_t* __restrict__ vals,
+                                                            const Dig* __restrict__ dig, int64_t n,
+                                                            int64_t* __restrict__ block_len) {
+    __shared__ int64_t sh[kWaves];
+    const int64_t i0 = (int64_t)blockIdx.x * kFBlock + threadIdx.x * kFPer;
+    int64_t len = 0;
+#pragma unroll
+    for (int k = 0; k < kFPer; ++k)
+        if (i0 + k < n) len += elem_len<FLOAT>(vals, dig, i0 + k) + 1;
+    int64_t total;
+    block_excl_scan(len, &total, sh);
+    if (threadIdx.x == 0) block_len[blockIdx.x] = total;
+}
+
+template <bool FLOAT>
+__global__ __launch_bounds__(kFThreads) void fmt_write_kernel(const uint64_t* __restrict__ vals,
+                                                              const Dig* __restrict__ dig, int64_t n, int64_t row_len,
+                                                              const int64_t* __restrict__ block_off,
+                                                              unsigned char* __restrict__ text) {
+    __shared__ int64_t sh[kWaves];
+    __shared__ __attribute__((aligned(16))) unsigned char stage[kStage];
+    const int64_t i0 = (int64_t)blockIdx.x * kFBlock + threadIdx.x * kFPer;
+    int lens[kFPer];
+    int64_t len = 0;
+#pragma unroll
+    for (int k = 0; k < kFPer; ++k) {
+        lens[k] = i0 + k < n ? elem_len<FLOAT>(vals, dig, i0 + k) + 1 : 0;
+        len += lens[k];
+    }
+    int64_t total;
+    const int64_t off = block_excl_scan(len, &total, sh);
+    const int64_t gbase = block_off[blockIdx.x];
+    // LDS byte j holds text[gbase - lead + j], with text + gbase - lead 16-byte aligned
+    const int lead = (int)((reinterpret_cast<uintptr_t>(text) + (uintptr_t)gbase) & 15);
+    auto put = [&](int64_t pos, char ch) { stage[lead + pos] = (unsigned char)ch; };
+    int64_t pos = off;
+#pragma unroll
+    for (int k = 0; k < kFPer; ++k) {
+        const int64_t i = i0 + k;
+        if (i < n) {
+            if (FLOAT) {
+                const Dig d = dig[i];
+                if (d.flags & 1) special_write(d.N, put, pos);
+                else g17_write(g17_make(d.flags & 2, d.N, d.X), put, pos);
+            } else {
+                i64_write((int64_t)vals[i], put, pos);
+            }
+            put(pos + lens[k] - 1, sep_after(i, row_len));
+            pos += lens[k];
+        }
+    }
+    __syncthreads();
+    // LDS -> HBM: whole 16-byte lines inside [gbase, gbase + total) as one
+    // store each, the partial first and last lines byte by byte
+    const int64_t end = gbase + total;
+    const int64_t a0 = gbase - lead;  // 16-aligned
+    const int64_t lines = (end - a0 + 15) >> 4;
+    for (int64_t L = threadIdx.x; L < lines; L += kFThreads) {
+        const int64_t a = a0 + 16 * L;
+        if (a >= gbase && a + 16 <= end) {
+            *reinterpret_cast<uint4*>(text + a) = *reinterpret_cast<const uint4*>(stage + 16 * L);
+        } else {
+            for (int b = 0; b < 16; ++b)
+                if (a + b >= gbase && a + b < end) text[a + b] = stage[16 * L + b];
+        }
+    }
+}
+
+int64_t bitmap_words64(int64_t count) { return ((count + 31) / 32 + 1) / 2 + 1; }
+
+}  // namespace
+
+// ---- host side ------------------------------------------------------------
+int csv_parse(const unsigned char* text, int64_t nbytes, int64_t count, bool is_float, int64_t* out,
+              hydhip_csv_status_t* st, hipStream_t s) {
+    st->error_element = -1;
+    st->error_kind = HYDHIP_CSV_OK;
+    st->exact_path = 0;
+    st->numbers_found = 0;
+    if (count == 0) return HYD_OK;
+    if (nbytes == 0) {
+        st->error_element = 0;
+        st->error_kind = HYDHIP_CSV_PARSE_ERROR;
+        return HYD_OK;
+    }
+    const int64_t tiles = ceil_div(nbytes, kTile);
+    if (tiles > 0x7fffffff) return set_error(HYD_E_INVALID, "csv: text too large");
+    const int64_t bw = bitmap_words64(count);
+    const int64_t words = 3 * tiles + scan_workspace_words(tiles) + bw + 8;
+    int64_t* ws = static_cast<int64_t*>(stream_workspace(s, (size_t)words * 8));
+    if (!ws) return set_error(HYD_E_OOM, "csv: workspace of %lld words", (long long)words);
+    int64_t* tile_cnt = ws;
+    int64_t* tile_err = tile_cnt + tiles;
+    int64_t* tile_base = tile_err + tiles;
+    int64_t* scan_ws = tile_base + tiles;
+    int64_t* status = scan_ws + scan_workspace_words(tiles);       // 4 words
+    unsigned long long* exact_cnt = reinterpret_cast<unsigned long long*>(status + 4);
+    uint32_t* exact_bits = reinterpret_cast<uint32_t*>(status + 8);
+    HYD_TRY(fill_zero(status, (size_t)(8 + bw) * 8, s));
+    const bool aligned = (reinterpret_cast<uintptr_t>(text) & 15) == 0;
+    const unsigned g = (unsigned)tiles;
+#define HYD_PARSE_LAUNCH(F, A)                                                                          \
+    do {                                                                                                \
+        csv_count_kernel<F, A><<<g, kPThreads, 0, s>>>(text, nbytes, tile_cnt, tile_err);               \
+        HYD_CHECK_LAUNCH();                                                                             \
+        HYD_TRY(scan_i64(tile_cnt, tiles, 0, tile_base, false, false, s, scan_ws));                    \
+        csv_value_kernel<F, A><<<g, kPThreads, 0, s>>>(text, nbytes, tile_base, count, out, exact_bits, \
+                                                       exact_cnt);                                      \
+        HYD_CHECK_LAUNCH();                                                                             \
+    } while (0)
+    if (is_float) {
+        if (aligned) HYD_PARSE_LAUNCH(true, true);
+        else HYD_PARSE_LAUNCH(true, false);
+        csv_exact_kernel<<<64, 64, 0, s>>>(text, nbytes, count, exact_bits, exact_cnt, out);
+        HYD_CHECK_LAUNCH();
+    } else {
+        if (aligned) HYD_PARSE_LAUNCH(false, true);
+        else HYD_PARSE_LAUNCH(false, false);
+    }
+#undef HYD_PARSE_LAUNCH
+    csv_status_kernel<<<1, kPThreads, 0, s>>>(text, nbytes, count, tiles, tile_cnt, tile_err, tile_base,
+                                              exact_cnt, status);
+    HYD_CHECK_LAUNCH();
+    int64_t h[4];
+    HYD_TRY_HIP(hipMemcpyAsync(h, status, sizeof h, hipMemcpyDeviceToHost, s));
+    HYD_TRY_HIP(hipStreamSynchronize(s));
+    st->error_element = h[0];
+    st->error_kind = (int32_t)h[1];
+    st->numbers_found = h[2];
+    st->exact_path = h[3];
+    return HYD_OK;
+}
+
+int csv_format(const uint64_t* vals, int64_t count, int64_t row_len, bool is_float, unsigned char* text,
+               int64_t capacity, int64_t* nbytes, int64_t* exact, hipStream_t s) {
+    *nbytes = 0;
+    if (exact) *exact = 0;
+    if (count == 0) return HYD_OK;
+    if (row_len <= 0) return set_error(HYD_E_INVALID, "csv: row length %lld", (long long)row_len);
+    const int64_t blocks = ceil_div(count, kFBlock);
+    if (blocks > 0x7fffffff) return set_error(HYD_E_INVALID, "csv: too many values");
+    const int64_t bw = bitmap_words64(count);
+    const int64_t dig_words = is_float ? 2 * count : 0;
+    const int64_t words = 2 * blocks + scan_workspace_words(blocks) + 8 + bw + dig_words;
+    int64_t* ws = static_cast<int64_t*>(stream_workspace(s, (size_t)words * 8));
+    if (!ws) return set_error(HYD_E_OOM, "csv: workspace of %lld words", (long long)words);
+    int64_t* block_len = ws;
+    int64_t* block_off = block_len + blocks;
+    int64_t* scan_ws = block_off + blocks;
+    int64_t* misc = scan_ws + scan_workspace_words(blocks);  // [0] exact count
+    unsigned long long* exact_cnt = reinterpret_cast<unsigned long long*>(misc);
+    uint32_t* exact_bits = reinterpret_cast<uint32_t*>(misc + 8);
+    Dig* dig = reinterpret_cast<Dig*>(misc + 8 + bw);
+    const unsigned g = (unsigned)blocks;
+    if (is_float) {
+        HYD_TRY(fill_zero(misc, (size_t)(8 + bw) * 8, s));
+        const int64_t dg = std::min<int64_t>(ceil_div(count, 256), (int64_t)device_info().num_cus * 64);
+        fmt_digits_kernel<<<(unsigned)dg, 256, 0, s>>>(vals, count, dig, exact_bits, exact_cnt);
+        HYD_CHECK_LAUNCH();
+        fmt_exact_kernel<<<64, 64, 0, s>>>(vals, count, dig, exact_bits, exact_cnt);
+        HYD_CHECK_LAUNCH();
+        fmt_len_kernel<true><<<g, kFThreads, 0, s>>>(vals, dig, count, block_len);
+    } else {
+        fmt_len_kernel<false><<<g, kFThreads, 0, s>>>(vals, dig, count, block_len);
+    }
+    HYD_CHECK_LAUNCH();
+    HYD_TRY(scan_i64(block_len, blocks, 0, block_off, false, false, s, scan_ws));
+    int64_t h[3];
+    HYD_TRY_HIP(hipMemcpyAsync(&h[0], block_off + blocks - 1, 8, hipMemcpyDeviceToHost, s));
+    HYD_TRY_HIP(hipMemcpyAsync(&h[1], block_len + blocks - 1, 8, hipMemcpyDeviceToHost, s));
+    HYD_TRY_HIP(hipMemcpyAsync(&h[2], misc, 8, hipMemcpyDeviceToHost, s));
+    HYD_TRY_HIP(hipStreamSynchronize(s));
+    const int64_t total = h[0] + h[1];
+    if (exact) *exact = is_float ? h[2] : 0;
+    if (total > capacity)
+        return set_error(HYD_E_INVALID, "csv: text needs %lld bytes, capacity %lld", (long long)total,
+                         (long long)capacity);
+    if (is_float) fmt_write_kernel<true><<<g, kFThreads, 0, s>>>(vals, dig, count, row_len, block_off, text);
+    else fmt_write_kernel<false><<<g, kFThreads, 0, s>>>(vals, dig, count, row_len, block_off, text);
+    HYD_CHECK_LAUNCH();
+    *nbytes = total;
+    return HYD_OK;
+}
+
+// ---- file I/O (pinned double buffers) -------------------------------------
+namespace {
+
+constexpr size_t kIoChunk = (size_t)64 << 20;
+
+struct D2HBuffers {
+    std::mutex mu;
+    void* host[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+D2HBuffers g_d2h;
+
+[[noreturn]] void die_hip(const char* what) {
+    fprintf(stderr, "%s: %s\n", what, hydhip_last_error());
+    exit(1);
+}
+
+// Streams dev[0, bytes) to f through two pinned buffers (the copy of chunk
+// i + 1 overlaps the fwrite of chunk i).
+int d2h_write(FILE* f, const unsigned char* dev, int64_t bytes, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_d2h.mu);
+    for (int k = 0; k < 2; ++k) {
+        if (!g_d2h.host[k]) HYD_TRY_HIP(hipHostMalloc(&g_d2h.host[k], kIoChunk, hipHostMallocDefault));
+        if (!g_d2h.ev[k]) HYD_TRY_HIP(hipEventCreateWithFlags(&g_d2h.ev[k], hipEventDisableTiming));
+    }
+    const int64_t chunks = ceil_div(bytes, (int64_t)kIoChunk);
+    auto issue = [&](int64_t c) -> int {
+        const int64_t off = c * (int64_t)kIoChunk;
+        const int64_t len = std::min<int64_t>((int64_t)kIoChunk, bytes - off);
+        HYD_TRY_HIP(hipMemcpyAsync(g_d2h.host[c & 1], dev + off, (size_t)len, hipMemcpyDeviceToHost, s));
+        HYD_TRY_HIP(hipEventRecord(g_d2h.ev[c & 1], s));
+        return HYD_OK;
+    };
+    if (chunks > 0) HYD_TRY(issue(0));
+    for (int64_t c = 0; c < chunks; ++c) {
+        if (c + 1 < chunks) HYD_TRY(issue(c + 1));
+        HYD_TRY_HIP(hipEventSynchronize(g_d2h.ev[c & 1]));
+        const int64_t off = c * (int64_t)kIoChunk;
+        const int64_t len = std::min<int64_t>((int64_t)kIoChunk, bytes - off);
+        if (fwrite(g_d2h.host[c & 1], 1, (size_t)len, f) != (size_t)len)
+            return set_error(HYD_E_INVALID, "csv: short write");
+    }
+    return HYD_OK;
+}
+
+// Device copy of a file's bytes (a buffer kept for the next call).
+struct TextBuffer {
+    std::mutex mu;
+    unsigned char* dev = nullptr;
+    size_t bytes = 0;
+};
+TextBuffer g_text;
+
+int text_buffer(size_t bytes, unsigned char** out) {
+    if (g_text.bytes < bytes) {
+        if (g_text.dev) HYD_TRY_HIP(hipFree(g_text.dev));
+        g_text.dev = nullptr;
+        g_text.bytes = 0;
+        const size_t want = bytes < ((size_t)1 << 20) ? ((size_t)1 << 20) : bytes;
+        HYD_TRY_HIP(hipMalloc(&g_text.dev, want));
+        g_text.bytes = want;
+    }
+    *out = g_text.dev;
+    return HYD_OK;
+}
+
+hyd_array_t* read_csv_file(const char* filename, hyd_tuple_t shape, bool is_float, const char* who) {
+    if (ensure_init() != HYD_OK) die_hip(who);
+    hyd_array_t* arr = hyd_array_alloc_device(shape, sizeof(int64_t));
+    const int64_t count = hyd_shape_size(shape);
+    FILE* f = fopen(filename, "r");
+    if (!f) {
+        fprintf(stderr, "%s: cannot open '%s'\n", who, filename);
+        exit(1);
+    }
+    hipStream_t s = export_stream();
+    std::lock_guard<std::mutex> lk(g_text.mu);
+    unsigned char* dev = nullptr;
+    int64_t nbytes = 0;
+    off_t end = -1;
+    if (fseeko(f, 0, SEEK_END) == 0) end = ftello(f);
+    if (end >= 0 && fseeko(f, 0, SEEK_SET) == 0) {
+        // a regular file: read straight into pinned chunks on their way to HBM
+        nbytes = (int64_t)end;
+        if (text_buffer((size_t)nbytes + 16, &dev) != HYD_OK) die_hip(who);
+        for (int64_t off = 0; off < nbytes; off += (int64_t)kIoChunk) {
+            const size_t len = (size_t)std::min<int64_t>((int64_t)kIoChunk, nbytes - off);
+            size_t got = 0;
+            if (stage_h2d(s, dev + off, len, [&](unsigned char* b) { got = fread(b, 1, len, f); }) != HYD_OK)
+                die_hip(who);
+            if (got != len) {
+                fprintf(stderr, "%s: cannot read '%s'\n", who, filename);
+                exit(1);
+            }
+        }
+    } else {
+        // a pipe or similar: read it all, then copy
+        clearerr(f);
+        std::string all;
+        char tmp[1 << 16];
+        size_t got;
+        while ((got = fread(tmp, 1, sizeof tmp, f)) > 0) all.append(tmp, got);
+        nbytes = (int64_t)all.size();
+        if (text_buffer((size_t)nbytes + 16, &dev) != HYD_OK) die_hip(who);
+        if (nbytes && (hipMemcpyAsync(dev, all.data(), all.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+                       hipStreamSynchronize(s) != hipSuccess))
+            die_hip(who);
+    }
+    fclose(f);
+    hydhip_csv_status_t st;
+    if (csv_parse(dev, nbytes, count, is_float, static_cast<int64_t*>(hyd_array_device_data(arr)), &st, s) !=
+        HYD_OK)
+        die_hip(who);
+    if (st.error_kind == HYDHIP_CSV_TOO_FEW) {
+        fprintf(stderr, "%s: too few values\n", who);
+        exit(1);
+    }
+    if (st.error_kind == HYDHIP_CSV_PARSE_ERROR) {
+        fprintf(stderr, "%s: parse error at element %lld\n", who, (long long)st.error_element);
+        exit(1);
+    }
+    mark_device_written(arr);
+    if (host_sync_enabled() && hyd_array_sync_host(arr) != HYD_OK) die_hip(who);
+    return arr;
+}
+
+int64_t write_csv_file(hyd_array_t* arr, const char* filename, bool is_float) {
+    FILE* f = fopen(filename, "w");
+    if (!f) {
+        perror("fopen");
+        exit(1);
+    }
+    const int ndims = arr->shape.ndims;
+    const int64_t size = hyd_shape_size(arr->shape);
+    const int64_t rows = ndims == 2 ? arr->shape.elems[0] : 1;
+    const int64_t row_len = ndims == 2 ? arr->shape.elems[1] : size;
+    if (size == 0) {
+        for (int64_t i = 0; i < rows; ++i) fputc('\n', f);  // empty rows
+    } else {
+        if (ensure_init() != HYD_OK) die_hip("write_array");
+        hipStream_t s = export_stream();
+        const uint64_t* vals = static_cast<const uint64_t*>(hyd_array_device_data(arr));
+        std::lock_guard<std::mutex> lk(g_text.mu);
+        const int64_t cap = size * kMaxText;
+        unsigned char* dev = nullptr;
+        int64_t nbytes = 0;
+        if (text_buffer((size_t)cap, &dev) != HYD_OK ||
+            csv_format(vals, size, row_len, is_float, dev, cap, &nbytes, nullptr, s) != HYD_OK ||
+            d2h_write(f, dev, nbytes, s) != HYD_OK)
+            die_hip("write_array");
+    }
+    fclose(f);
+    return 0;
+}
+
+}  // namespace
+}  // namespace hyd
+
+// ---- C ABI ----------------------------------------------------------------
+using namespace hyd;
+
+extern "C" int hydhip_csv_parse_i64(const void* text, int64_t nbytes, int64_t count, int64_t* out,
+                                    hydhip_csv_status_t* status, void* stream) {
+    clear_error();
+    HYD_TRY(ensure_init());
+    if (nbytes < 0 || count < 0 || !status || (count && !out) || (nbytes && !text))
+        return set_error(HYD_E_INVALID, "csv_parse_i64: bad arguments");
+    return csv_parse(static_cast<const unsigned char*>(text), nbytes, count, false, out, status,
+                     resolve_stream(stream));
+}
+
+extern "C" int hydhip_csv_parse_f64(const void* text, int64_t nbytes, int64_t count, double* out,
+                                    hydhip_csv_status_t* status, void* stream) {
+    clear_error();
+    HYD_TRY(ensure_init());
+    if (nbytes < 0 || count < 0 || !status || (count && !out) || (nbytes && !text))
+        return set_error(HYD_E_INVALID, "csv_parse_f64: bad arguments");
+    return csv_parse(static_cast<const unsigned char*>(text), nbytes, count, true, reinterpret_cast<int64_t*>(out),
+                     status, resolve_stream(stream));
+}
+
+extern "C" int64_t hydhip_csv_text_bound(int64_t count) { return count < 0 ? 0 : count * kMaxText; }
+
+extern "C" int hydhip_csv_format_i64(const int64_t* vals, int64_t count, int64_t row_len, void* text,
+                                     int64_t capacity, int64_t* nbytes, void* stream) {
+    clear_error();
+    HYD_TRY(ensure_init());
+    if (count < 0 || !nbytes || (count && (!vals || !text)))
+        return set_error(HYD_E_INVALID, "csv_format_i64: bad arguments");
+    return csv_format(reinterpret_cast<const uint64_t*>(vals), count, row_len, false,
+                      static_cast<unsigned char*>(text), capacity, nbytes, nullptr, resolve_stream(stream));
+}
+
+extern "C" int hydhip_csv_format_f64(const double* vals, int64_t count, int64_t row_len, void* text,
+                                     int64_t capacity, int64_t* nbytes, int64_t* exact_path, void* stream) {
+    clear_error();
+    HYD_TRY(ensure_init());
+    if (count < 0 || !nbytes || (count && (!vals || !text)))
+        return set_error(HYD_E_INVALID, "csv_format_f64: bad arguments");
+    return csv_format(reinterpret_cast<const uint64_t*>(vals), count, row_len, true, static_cast<unsigned char*>(text),
+                      capacity, nbytes, exact_path, resolve_stream(stream));
+}
+
+// Drop-in runtime functions (runtime/hydrangea_runtime.h:549-594 and
+// runtime/hyd_write_csv.c:24-59): same names, arguments, messages and exit(1).
+extern "C" hyd_array_t* hyd_read_array_csv(const char* filename, hyd_tuple_t shape) {
+    return read_csv_file(filename, shape, false, "read_array");
+}
+
+extern "C" hyd_array_t* hyd_read_float_array_csv(const char* filename, hyd_tuple_t shape) {
+    return read_csv_file(filename, shape, true, "read_array_float");
+}
+
+extern "C" int64_t hyd_write_array_csv(hyd_array_t* arr, const char* filename) {
+    return write_csv_file(arr, filename, false);
+}
+
+extern "C" int64_t hyd_write_array_csv_float(hyd_array_t* arr, const char* filename) {
+    return write_csv_file(arr, filename, true);
+}
+
+// ---- CPU test hooks: the same conversions, compiled for the host ------------
+// fscanf-like: skips isspace, then reads one number from s[0, len).  Returns 1
+// (value in *value, characters used in *consumed), 0 (no number) or -1 (only
+// whitespace); *exact = 1 when the big-integer path rounded it.  flags & 1
+// sends every decimal through the big-integer path (to test that path).
+extern "C" int hydhip_host_csv_scan(const char* s, int64_t len, int is_float, int flags, void* value,
+                                    int64_t* consumed, int* exact) {
+    auto at = [&](int64_t p) -> int { return (p < 0 || p >= len) ? -1 : (unsigned char)s[p]; };
+    int64_t p = 0;
+    while (p < len && is_ws(at(p))) ++p;
+    *exact = 0;
+    if (p >= len) return -1;
+    if (!is_float) {
+        const int64_t l = scan_i64(at, p, static_cast<int64_t*>(value));
+        if (l < 0) return 0;
+        *consumed = p + l;
+        return 1;
+    }
+    const FloatTok t = scan_f64_fast(at, p);
+    if (t.len < 0) return 0;
+    uint64_t bits;
+    const bool decided = tok_to_bits(t, &bits);
+    if (!decided || ((flags & 1) && t.kind == kDec && t.w != 0)) {
+        Big* big = new Big[3];
+        bits = dec_exact_bits(at, p, bits, big[0], big[1], big[2]);
+        delete[] big;
+        *exact = 1;
+    }
+    memcpy(value, &bits, 8);
+    *consumed = p + t.len;
+    return 1;
+}
+
+// "%.17g" / "%lld" text of each value (no separator) at out + k * stride;
+// returns how many fp64 values took the big-integer path.  flags & 1 sends
+// every finite nonzero fp64 value through it.
+extern "C" int64_t hydhip_host_csv_format(const void* vals, int64_t n, int is_float, int flags, char* out,
+                                          int64_t stride, int32_t* lens) {
+    int64_t exact = 0;
+    Big* big = new Big[3];
+    for (int64_t k = 0; k < n; ++k) {
+        char* o = out + k * stride;
+        auto put = [&](int64_t pos, char ch) { o[pos] = ch; };
+        uint64_t b;
+        memcpy(&b, static_cast<const char*>(vals) + 8 * k, 8);
+        if (!is_float) {
+            lens[k] = i64_write((int64_t)b, put, 0);
+            continue;
+        }
+        if (const int sl = special_len(b)) {
+            special_write(b, put, 0);
+            lens[k] = sl;
+            continue;
+        }
+        uint64_t N;
+        int X;
+        const bool decided = (flags & 1) ? f64_to_dec17<true>(b & ~kSignBit, &N, &X)
+                                         : f64_to_dec17(b & ~kSignBit, &N, &X);
+        if (!decided) {
+            dec17_exact(b & ~kSignBit, N, X, &N, &X, big[0], big[1], big[2]);
+            ++exact;
+        }
+        lens[k] = g17_write(g17_make(b >> 63, N, X), put, 0);
+    }
+    delete[] big;
+    return exact;
+}

@@ -92,6 +92,12 @@ class HydResult(ctypes.Structure):
                 ("bytes", ctypes.c_int64), ("batch_id", ctypes.c_int64)]
 
 
+class CsvStatus(ctypes.Structure):
+    _fields_ = [("error_element", ctypes.c_int64), ("error_kind", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("numbers_found", ctypes.c_int64),
+                ("exact_path", ctypes.c_int64)]
+
+
 class HydServerStats(ctypes.Structure):
     """hyd_server_stats_t"""
     _fields_ = [(name, ctypes.c_int64) for name in (
@@ -159,6 +165,15 @@ _SIGS = {
     "hydhip_tune_clear": (_INT, []),
     "hydhip_tune_table": (_INT, [ctypes.POINTER(_INT), _INT]),
     "hydhip_batched_dot_f64": (_INT, [_P, _P, _P, _I64, _D, _P, _P]),
+    "hydhip_csv_parse_i64": (_INT, [_P, _I64, _I64, _P, ctypes.POINTER(CsvStatus), _P]),
+    "hydhip_csv_parse_f64": (_INT, [_P, _I64, _I64, _P, ctypes.POINTER(CsvStatus), _P]),
+    "hydhip_csv_text_bound": (_I64, [_I64]),
+    "hydhip_csv_format_i64": (_INT, [_P, _I64, _I64, _P, _I64, ctypes.POINTER(_I64), _P]),
+    "hydhip_csv_format_f64": (_INT, [_P, _I64, _I64, _P, _I64, ctypes.POINTER(_I64),
+                                     ctypes.POINTER(_I64), _P]),
+    "hydhip_host_csv_scan": (_INT, [ctypes.c_char_p, _I64, _INT, _INT, _P, ctypes.POINTER(_I64),
+                                    ctypes.POINTER(_INT)]),
+    "hydhip_host_csv_format": (_I64, [_P, _I64, _INT, _INT, _P, _I64, _P]),
     # serving
     "hyd_server_create": (_P, [_I64, _INT, _INT]),
     "hyd_server_destroy": (None, [_P]),
@@ -190,6 +205,10 @@ _SIGS = {
     "hyd_hip_get_host_sync": (_INT, []),
     "hyd_hip_set_stream": (None, [_P]),
     "hyd_sort_indices": (_PA, [_PA]),
+    "hyd_read_array_csv": (_PA, [ctypes.c_char_p, HydTuple]),
+    "hyd_read_float_array_csv": (_PA, [ctypes.c_char_p, HydTuple]),
+    "hyd_write_array_csv": (_I64, [_PA, ctypes.c_char_p]),
+    "hyd_write_array_csv_float": (_I64, [_PA, ctypes.c_char_p]),
     "hyd_export_weighted_histogram": (_PA, [_PA]),
     "hyd_export_weighted_histogram_bins": (_PA, [_PA, _I64]),
     "hyd_export_dot": (_D, [_PA, _PA]),
@@ -637,6 +656,72 @@ def stream_create():
 
 def stream_destroy(s):
     _check(lib().hydhip_stream_destroy(s))
+
+
+# ---- CSV text <-> arrays ------------------------------------------------------
+CSV_OK, CSV_PARSE_ERROR, CSV_TOO_FEW = 0, 1, 2
+
+
+def csv_parse(text, count, is_float, out=None, stream=None):
+    """Parses `text` (a DeviceArray of bytes, or bytes to upload) as the
+    reference's read_array / read_array_float loop reads a file.  Returns
+    (DeviceArray of `count` values, CsvStatus)."""
+    if not isinstance(text, DeviceArray):
+        text = DeviceArray.from_numpy(np.frombuffer(bytes(text), np.uint8))
+    dtype = np.float64 if is_float else np.int64
+    out = out if out is not None else DeviceArray((count,), dtype)
+    st = CsvStatus()
+    fn = lib().hydhip_csv_parse_f64 if is_float else lib().hydhip_csv_parse_i64
+    _check(fn(text.ptr, text.nbytes, count, out.ptr, ctypes.byref(st), stream))
+    return out, st
+
+
+def csv_format(vals, row_len=None, stream=None, with_exact=False):
+    """Text the reference's write_array / write_array_float would write for
+    the values of `vals` (DeviceArray, int64 or float64), rows of `row_len`."""
+    n = vals.size
+    row_len = n if row_len is None else row_len
+    cap = max(1, lib().hydhip_csv_text_bound(n))
+    text = DeviceArray((cap,), np.uint8)
+    nb = _I64()
+    exact = _I64()
+    if vals.dtype == np.float64:
+        _check(lib().hydhip_csv_format_f64(vals.ptr, n, row_len, text.ptr, cap, ctypes.byref(nb),
+                                           ctypes.byref(exact), stream))
+    else:
+        _check(lib().hydhip_csv_format_i64(vals.ptr, n, row_len, text.ptr, cap, ctypes.byref(nb), stream))
+    out = copy_out(text.ptr, nb.value, np.uint8).tobytes() if nb.value else b""
+    text.free()
+    return (out, exact.value) if with_exact else out
+
+
+def host_csv_scan(strings, is_float, force_exact=False):
+    """CPU build of the device scanner on each string: (r, values, consumed, exact)."""
+    n = len(strings)
+    r = np.zeros(n, np.int32)
+    vals = np.zeros(n, np.float64 if is_float else np.int64)
+    consumed = np.zeros(n, np.int64)
+    exact = np.zeros(n, np.int32)
+    c = _I64()
+    e = _INT()
+    for k, s in enumerate(strings):
+        r[k] = lib().hydhip_host_csv_scan(s, len(s), 1 if is_float else 0, 1 if force_exact else 0,
+                                          vals.ctypes.data + 8 * k, ctypes.byref(c), ctypes.byref(e))
+        consumed[k] = c.value
+        exact[k] = e.value
+    return r, vals, consumed, exact
+
+
+def host_csv_format(vals, is_float, stride=40, force_exact=False):
+    """CPU build of the device formatter: (list of bytes, exact-path count)."""
+    v = np.ascontiguousarray(vals, np.float64 if is_float else np.int64)
+    n = v.size
+    out = np.zeros(max(1, n * stride), np.uint8)
+    lens = np.zeros(max(1, n), np.int32)
+    exact = lib().hydhip_host_csv_format(v.ctypes.data, n, 1 if is_float else 0, 1 if force_exact else 0,
+                                         out.ctypes.data, stride, lens.ctypes.data)
+    buf = out.tobytes()
+    return [buf[k * stride:k * stride + int(lens[k])] for k in range(n)], exact
 
 
 def default_stream():

@@ -232,6 +232,58 @@ int hydhip_coo_to_csr_i64(int64_t nrows, int64_t ncols, int64_t nnz,
                           const int64_t* vals, int64_t* row_ptr, int64_t* col_idx,
                           int64_t* out_vals, int64_t* nnz_out, void* stream);
 
+/* ---- CSV text <-> arrays (device buffers) ------------------------------ */
+/* The element reads of read_array / read_array_float
+ * (runtime/hydrangea_runtime.h:549-594): text[0, nbytes) in device memory
+ * holds numbers as fscanf "%lld" / "%lf" reads them (glibc rules, correctly
+ * rounded fp64), separated as the reference's loop skips them (',' ' ' '\n'
+ * '\r', then isspace).  The first `count` values go to out (device).  A text
+ * the reference would reject is reported in *status (host), not by exit:
+ * error_kind HYDHIP_CSV_PARSE_ERROR ("parse error at element N") or
+ * HYDHIP_CSV_TOO_FEW ("too few values"), at error_element.  Synchronises the
+ * stream (the status is read back). */
+#define HYDHIP_CSV_OK 0
+#define HYDHIP_CSV_PARSE_ERROR 1
+#define HYDHIP_CSV_TOO_FEW 2
+typedef struct {
+    int64_t error_element; /* first element the reference fails on, or -1 */
+    int32_t error_kind;    /* HYDHIP_CSV_OK / _PARSE_ERROR / _TOO_FEW */
+    int32_t reserved;
+    int64_t numbers_found; /* numbers read before the first failure */
+    int64_t exact_path;    /* fp64 numbers rounded by the big-integer path */
+} hydhip_csv_status_t;
+int hydhip_csv_parse_i64(const void* text, int64_t nbytes, int64_t count,
+                         int64_t* out, hydhip_csv_status_t* status, void* stream);
+int hydhip_csv_parse_f64(const void* text, int64_t nbytes, int64_t count,
+                         double* out, hydhip_csv_status_t* status, void* stream);
+/* The row writers of write_array / write_array_float
+ * (runtime/hyd_write_csv.c:5-59): fprintf "%lld" / "%.17g" text of `count`
+ * values, ',' between the elements of a row and '\n' after every row_len-th
+ * (row_len = columns of a 2-D array, else count), into text (device,
+ * `capacity` bytes; hydhip_csv_text_bound(count) always suffices).  *nbytes
+ * (host) receives the length; *exact_path (host, may be NULL) the fp64
+ * values rounded by the big-integer path.  Synchronises the stream. */
+int64_t hydhip_csv_text_bound(int64_t count);
+int hydhip_csv_format_i64(const int64_t* vals, int64_t count, int64_t row_len,
+                          void* text, int64_t capacity, int64_t* nbytes,
+                          void* stream);
+int hydhip_csv_format_f64(const double* vals, int64_t count, int64_t row_len,
+                          void* text, int64_t capacity, int64_t* nbytes,
+                          int64_t* exact_path, void* stream);
+/* CPU test hooks: the same conversions compiled for the host (no GPU), for
+ * checking the algorithms against the C library at volumes the GPU tests do
+ * not reach.  Not part of the product path.
+ * scan: fscanf-like on s[0, len): skips isspace, reads one number; returns 1
+ * (value, *consumed = characters used), 0 (no number), -1 (only space).
+ * format: text of each value at out + k * stride (no separator), lengths in
+ * lens; returns how many fp64 values took the big-integer path.
+ * flags & 1 sends every fp64 value through the big-integer path. */
+int hydhip_host_csv_scan(const char* s, int64_t len, int is_float, int flags,
+                         void* value, int64_t* consumed, int* exact);
+int64_t hydhip_host_csv_format(const void* vals, int64_t n, int is_float,
+                               int flags, char* out, int64_t stride,
+                               int32_t* lens);
+
 #ifdef __cplusplus
 }
 #endif

@@ -73,6 +73,17 @@ double* hyd_array_data_f64(hyd_array_t* arr);
  * of a 1-D int64 key array, returning the permutation.  Runs on the GPU. */
 hyd_array_t* hyd_sort_indices(hyd_array_t* keys);
 
+/* CSV files: runtime/hydrangea_runtime.h:549-594 (read_array_float,
+ * read_array) and runtime/hyd_write_csv.c:24-59 (write_array,
+ * write_array_float).  Same names, arguments, text and messages; a missing
+ * file or a text the reference rejects prints the reference's message and
+ * exits with status 1.  The text is parsed / formatted on the GPU; read
+ * results are device-resident (and copied to arr->data under host sync). */
+hyd_array_t* hyd_read_array_csv(const char* filename, hyd_tuple_t shape);
+hyd_array_t* hyd_read_float_array_csv(const char* filename, hyd_tuple_t shape);
+int64_t hyd_write_array_csv(hyd_array_t* arr, const char* filename);
+int64_t hyd_write_array_csv_float(hyd_array_t* arr, const char* filename);
+
 /* hyd_get_env_int: runtime/hydrangea_runtime.h:601-615 (unset or
  * unparsable variable: message on stderr, exit(1)). */
 int64_t hyd_get_env_int(const char* name);

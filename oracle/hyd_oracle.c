@@ -18,6 +18,7 @@
  * parallel reduction order of oracle_dot (OpenMP decides it in the
  * reference; it is checked against an exactly rounded sum within 1e-12).
  */
+#define _POSIX_C_SOURCE 200809L /* fmemopen, open_memstream */
 #include <math.h>
 #include <omp.h>
 #include <stdint.h>
@@ -567,4 +568,93 @@ int64_t oracle_coo_to_csr(int64_t nrows, int64_t ncols, int64_t nnz, const int64
     free(cc);
     free(cv);
     return m;
+}
+
+/* ---- CSV text (runtime/hydrangea_runtime.h:549-594, runtime/hyd_write_csv.c:5-59) ----
+ * The reference reads element i with fscanf "%lld" / "%lf" after (for i > 0)
+ * skipping ',', ' ', '\n', '\r' with fgetc + ungetc, and writes rows with
+ * fprintf "%lld" / "%.17g", ',' between elements and '\n' after each row (a
+ * 2-D array row by row, anything else as one row).  These restatements run the
+ * same loops over in-memory FILEs (fmemopen / open_memstream), so the number
+ * conversions are the C library's own, as in the reference.  Where the
+ * reference prints a message and exits, the read returns the element index
+ * and the kind: 1 = "parse error at element i", 2 = "too few values". */
+int oracle_csv_read(const char* text, int64_t nbytes, int64_t count, int is_float, void* out,
+                    int64_t* err_elem) {
+    static char empty[1];
+    FILE* f = fmemopen(nbytes ? (void*)text : (void*)empty, (size_t)nbytes, "r");
+    int kind = 0;
+    *err_elem = -1;
+    if (!f) return -1;
+    for (int64_t i = 0; i < count; i++) {
+        if (i > 0) {
+            int c;
+            while ((c = fgetc(f)) != EOF && (c == ',' || c == ' ' || c == '\n' || c == '\r'))
+                ;
+            if (c == EOF) {
+                kind = 2;
+                *err_elem = i;
+                break;
+            }
+            ungetc(c, f);
+        }
+        int r = is_float ? fscanf(f, "%lf", (double*)out + i)
+                         : fscanf(f, "%lld", (long long*)((int64_t*)out + i));
+        if (r != 1) {
+            kind = 1;
+            *err_elem = i;
+            break;
+        }
+    }
+    fclose(f);
+    return kind;
+}
+
+/* Writes rows x row_len values; returns a malloc'd buffer (oracle_free). */
+char* oracle_csv_write(const void* vals, int64_t rows, int64_t row_len, int is_float, int64_t* nbytes) {
+    char* buf = NULL;
+    size_t len = 0;
+    FILE* f = open_memstream(&buf, &len);
+    if (!f) return NULL;
+    for (int64_t i = 0; i < rows; i++) {
+        for (int64_t j = 0; j < row_len; j++) {
+            if (j) fputc(',', f);
+            if (is_float) fprintf(f, "%.17g", ((const double*)vals)[i * row_len + j]);
+            else fprintf(f, "%lld", (long long)((const int64_t*)vals)[i * row_len + j]);
+        }
+        fputc('\n', f);
+    }
+    fclose(f);
+    *nbytes = (int64_t)len;
+    return buf;
+}
+
+void oracle_free(void* p) { free(p); }
+
+/* One fscanf "%lld" / "%lf" per string (strings back to back in text, string
+ * k = [offs[k], offs[k+1])): r[k] = fscanf's result, consumed[k] = ftell
+ * afterwards, value in out[k]. */
+void oracle_scan_tokens(const char* text, const int64_t* offs, int64_t n, int is_float, int32_t* r,
+                        void* out, int64_t* consumed) {
+    static char empty[1];
+    for (int64_t k = 0; k < n; k++) {
+        const int64_t len = offs[k + 1] - offs[k];
+        FILE* f = fmemopen(len ? (void*)(text + offs[k]) : (void*)empty, (size_t)len, "r");
+        if (!f) {
+            r[k] = -2;
+            continue;
+        }
+        r[k] = is_float ? fscanf(f, "%lf", (double*)out + k) : fscanf(f, "%lld", (long long*)((int64_t*)out + k));
+        consumed[k] = (int64_t)ftell(f);
+        fclose(f);
+    }
+}
+
+/* fprintf "%.17g" / "%lld" text of each value: out + k * stride, length lens[k]. */
+void oracle_format_values(const void* vals, int64_t n, int is_float, char* out, int64_t stride, int32_t* lens) {
+    for (int64_t k = 0; k < n; k++) {
+        char* o = out + k * stride;
+        lens[k] = is_float ? snprintf(o, (size_t)stride, "%.17g", ((const double*)vals)[k])
+                           : snprintf(o, (size_t)stride, "%lld", (long long)((const int64_t*)vals)[k]);
+    }
 }

@@ -45,6 +45,11 @@ _SIGS = {
     "oracle_coo_sum_duplicates": (_I64, [_P, _P, _P, _I64, _I64, _P, _P, _P]),
     "oracle_csr_from_sorted_coo": (None, [_I64, _I64, _P, _P, _P, _I64, _P, _P, _P]),
     "oracle_coo_to_csr": (_I64, [_I64, _I64, _I64, _P, _P, _P, _P, _P, _P]),
+    "oracle_csv_read": (_INT, [ctypes.c_char_p, _I64, _I64, _INT, _P, _P]),
+    "oracle_csv_write": (_P, [_P, _I64, _I64, _INT, _P]),
+    "oracle_free": (None, [_P]),
+    "oracle_scan_tokens": (None, [ctypes.c_char_p, _P, _I64, _INT, _P, _P, _P]),
+    "oracle_format_values": (None, [_P, _I64, _INT, _P, _I64, _P]),
 }
 
 _lib = None
@@ -268,3 +273,58 @@ def coo_to_csr(nrows, ncols, rows, cols, vals):
     cv = np.empty(n, np.int64)
     m = lib().oracle_coo_to_csr(nrows, ncols, n, _p(rows), _p(cols), _p(vals), _p(rp), _p(ci), _p(cv))
     return rp, ci, cv, m
+
+
+# ---- CSV text (runtime/hydrangea_runtime.h:549-594, runtime/hyd_write_csv.c:5-59) ----
+CSV_OK, CSV_PARSE_ERROR, CSV_TOO_FEW = 0, 1, 2
+
+
+def csv_read(text, count, is_float):
+    """The reference's read loop over `text` (bytes): (values, kind, element).
+    kind 0 = all `count` values read, 1 = parse error at `element`, 2 = too few
+    values (the separator loop hit the end before `element`)."""
+    out = np.zeros(max(count, 1), np.float64 if is_float else np.int64)
+    err = np.zeros(1, np.int64)
+    kind = lib().oracle_csv_read(bytes(text), len(text), count, 1 if is_float else 0,
+                                 out.ctypes.data, err.ctypes.data)
+    if kind < 0:
+        raise RuntimeError("fmemopen failed")
+    return out[:count], kind, int(err[0])
+
+
+def csv_write(vals, rows, row_len, is_float):
+    """The reference's write loop: rows x row_len values -> bytes."""
+    v = _f64(vals) if is_float else _i64(vals)
+    n = np.zeros(1, np.int64)
+    p = lib().oracle_csv_write(_p(v), rows, row_len, 1 if is_float else 0, n.ctypes.data)
+    if not p:
+        raise RuntimeError("open_memstream failed")
+    try:
+        return ctypes.string_at(p, int(n[0]))
+    finally:
+        lib().oracle_free(p)
+
+
+def scan_tokens(strings, is_float):
+    """fscanf "%lf" / "%lld" on each string alone: (r, values, consumed)."""
+    offs = np.zeros(len(strings) + 1, np.int64)
+    offs[1:] = np.cumsum([len(s) for s in strings])
+    text = b"".join(strings)
+    n = len(strings)
+    r = np.zeros(n, np.int32)
+    vals = np.zeros(n, np.float64 if is_float else np.int64)
+    consumed = np.zeros(n, np.int64)
+    lib().oracle_scan_tokens(text, offs.ctypes.data, n, 1 if is_float else 0, r.ctypes.data,
+                             vals.ctypes.data, consumed.ctypes.data)
+    return r, vals, consumed
+
+
+def format_values(vals, is_float, stride=40):
+    """fprintf "%.17g" / "%lld" text of each value, as a list of bytes."""
+    v = _f64(vals) if is_float else _i64(vals)
+    n = v.size
+    out = np.zeros(n * stride, np.uint8)
+    lens = np.zeros(n, np.int32)
+    lib().oracle_format_values(_p(v), n, 1 if is_float else 0, _p(out), stride, _p(lens))
+    buf = out.tobytes()
+    return [buf[k * stride:k * stride + int(lens[k])] for k in range(n)]
