@@ -168,6 +168,74 @@ def bench_guarded(steps, warmup):
     return res
 
 
+def bench_csv(steps, warmup, cpu=True):
+    """CSV text <-> arrays in HBM, as read_array_float / write_array_float and
+    read_array / write_array read and write files (runtime/hydrangea_runtime.h:
+    549-594, runtime/hyd_write_csv.c:5-59): the map chain's 2^27-value fp64
+    input written as "%.17g" text and read back, and the histogram's 2^24
+    int64 input as "%lld" text.  Each call includes its status read-back.
+    Algorithmic HBM bytes: format reads 8 B per value and writes the text;
+    parse reads the text and writes 8 B per value.  Beside it, the
+    reference's loops (glibc fprintf / fscanf, one thread) on 2^20 values."""
+    import ctypes
+
+    import pyoracle
+
+    res = {}
+    L = hydhip.lib()
+    for name, n, is_float in (("f64_2^27", workloads.DOTMAP["n"], True), ("i64_2^24", workloads.HIST["n"], False)):
+        seed = 0x435356
+        x = hydhip.splitmix_f64(n, seed, 0, symmetric=True) if is_float else hydhip.splitmix_i64(n, seed)
+        cap = L.hydhip_csv_text_bound(n)
+        text = hydhip.DeviceArray((cap,), np.uint8)
+        y = hydhip.DeviceArray((n,), np.float64 if is_float else np.int64)
+        nb, ex = ctypes.c_int64(), ctypes.c_int64()
+        st = hydhip.CsvStatus()
+
+        def fmt(ev):
+            if ev:
+                ev[0].record()
+            if is_float:
+                hydhip._check(L.hydhip_csv_format_f64(x.ptr, n, n, text.ptr, cap, ctypes.byref(nb),
+                                                      ctypes.byref(ex), None))
+            else:
+                hydhip._check(L.hydhip_csv_format_i64(x.ptr, n, n, text.ptr, cap, ctypes.byref(nb), None))
+            if ev:
+                ev[1].record()
+
+        def parse(ev):
+            if ev:
+                ev[0].record()
+            fn = L.hydhip_csv_parse_f64 if is_float else L.hydhip_csv_parse_i64
+            hydhip._check(fn(text.ptr, nb.value, n, y.ptr, ctypes.byref(st), None))
+            if ev:
+                ev[1].record()
+
+        fms, _ = timed_loop(fmt, steps, warmup, events=True)
+        pms, _ = timed_loop(parse, steps, warmup, events=True)
+        ok = st.error_kind == 0 and st.numbers_found == n
+        t = nb.value
+        r = dict(values=n, text_bytes=t, parsed_all=bool(ok), exact_path=int(st.exact_path),
+                 format_ms=fms, parse_ms=pms, format_gvalues_s=n / fms / 1e6, parse_gvalues_s=n / pms / 1e6,
+                 format_hbm_gbs=(8 * n + t) / fms / 1e6, parse_hbm_gbs=(t + 8 * n) / pms / 1e6)
+        if cpu:
+            k = 1 << 20
+            xs = (pyoracle.splitmix_f64(k, seed, 0, symmetric=True) if is_float
+                  else pyoracle.splitmix_i64(k, seed))
+            t0 = time.perf_counter()
+            txt = pyoracle.csv_write(xs, 1, k, is_float)
+            t1 = time.perf_counter()
+            pyoracle.csv_read(txt, k, is_float)
+            t2 = time.perf_counter()
+            r["cpu_reference_loop"] = dict(sample="2^20 values, one thread (glibc)",
+                                           format_mvalues_s=k / (t1 - t0) / 1e6,
+                                           parse_mvalues_s=k / (t2 - t1) / 1e6)
+        res[name] = r
+        for a in (x, text, y):
+            a.free()
+    return res
+
+
 def bench_hist(steps, warmup):
     c = workloads.HIST
     d = hydhip.splitmix_i64(c["n"], c["seed"], bits=c["bits"])
@@ -406,6 +474,10 @@ def main():
     if not only or "batch" in only:
         log("batched histograms ...")
         kernels["batched_histogram"] = bench_batched(args.steps, args.warmup)
+
+    if not only or "csv" in only:
+        log("csv text ...")
+        kernels["csv"] = bench_csv(args.steps, args.warmup, cpu=not args.no_cpu)
 
     if not only or "serve" in only:
         log("served mix ...")

@@ -143,70 +143,122 @@ __device__ __forceinline__ int64_t token_len(const At& at, int64_t p) {
     return scan_i64(at, p, (int64_t*)nullptr);
 }
 
-// Walks the words that start in this thread's slice.  Emit(p_token) is called
-// for each number in order; returns the count, and in *err the count at the
-// first failure (-1: none).  Stops at the first failure.
-template <bool FLOAT, class At, class Emit>
-__device__ __forceinline__ int64_t walk_slice(const At& at, int64_t s0, int64_t n, int64_t* err,
-                                              const Emit& emit) {
-    int64_t cnt = 0;
-    *err = -1;
-    int prev = at(s0 - 1);
-    for (int k = 0; k < kSlice; ++k) {
-        const int64_t p = s0 + k;
-        if (p >= n) break;
-        const int c = at(p);
-        const bool start = !is_gap(c) && (prev < 0 || is_gap(prev));
-        prev = c;
-        if (!start) continue;
-        if (!gap_ok(at, p)) {
-            *err = cnt;
+constexpr int kMaxWords = kTile / 2 + 1;  // a word start follows a gap (or is the text's first byte)
+
+// Stages the tile, then lists the words that start in it: starts[0, words)
+// (tile offsets, in text order).  A byte-parallel pass: each thread marks the
+// starts among its kSlice bytes, a block scan places them.
+#define HYD_TILE_WORDS                                                                  \
+    __shared__ __attribute__((aligned(16))) unsigned char buf[kBuf];                    \
+    __shared__ uint16_t starts[kMaxWords];                                              \
+    __shared__ int64_t sh[kWaves];                                                      \
+    const int64_t t0 = (int64_t)blockIdx.x * kTile;                                     \
+    const int64_t base = t0 - kHaloLo;                                                  \
+    const int64_t lo = base < 0 ? 0 : base;            /* staged: text[lo, lim) */      \
+    const int64_t lim = n < base + kBuf ? n : base + kBuf;                              \
+    stage_tile<ALIGNED>(text, n, base, buf);                                            \
+    auto at = [&](int64_t p) -> int {                                                   \
+        if ((uint64_t)(p - lo) < (uint64_t)(lim - lo)) return buf[p - base];            \
+        return (p < 0 || p >= n) ? -1 : text[p];                                        \
+    };                                                                                  \
+    int words;                                                                          \
+    {                                                                                   \
+        const int64_t s0 = t0 + (int64_t)threadIdx.x * kSlice;                          \
+        uint32_t mask = 0;                                                              \
+        int prev = at(s0 - 1);                                                          \
+        _Pragma("unroll") for (int k = 0; k < kSlice; ++k) {                            \
+            const int c = at(s0 + k);                                                   \
+            if (c >= 0 && !is_gap(c) && (prev < 0 || is_gap(prev))) mask |= 1u << k;   \
+            prev = c;                                                                   \
+        }                                                                               \
+        int64_t total;                                                                  \
+        int off = (int)block_excl_scan(__builtin_popcount(mask), &total, sh);           \
+        while (mask) {                                                                  \
+            const int k = __builtin_ctz(mask);                                          \
+            mask &= mask - 1;                                                           \
+            starts[off++] = (uint16_t)(threadIdx.x * kSlice + k);                       \
+        }                                                                               \
+        words = (int)total;                                                             \
+        __syncthreads();                                                                \
+    }
+
+// Scans the word at p: how many numbers it holds (those before a failure),
+// and whether one failed (or the gap before the word is not one the
+// reference's loop skips).  first(q) scans the first number and returns its
+// length; the others are scanned for their lengths only.
+template <bool FLOAT, class At, class First>
+__device__ __forceinline__ int64_t scan_word(const At& at, int64_t p, bool* fail, const First& first) {
+    *fail = false;
+    if (!gap_ok(at, p)) {
+        *fail = true;
+        return 0;
+    }
+    int64_t cnt = 0, q = p;
+    for (;;) {
+        const int64_t len = cnt == 0 ? first(q) : token_len<FLOAT>(at, q);
+        if (len < 0) {
+            *fail = true;
             return cnt;
         }
-        int64_t q = p;
-        for (;;) {
-            const int64_t len = token_len<FLOAT>(at, q);
-            if (len < 0) {
-                *err = cnt;
-                return cnt;
-            }
-            emit(q, cnt);
-            ++cnt;
-            q += len;
-            const int c2 = at(q);
-            if (c2 < 0 || is_gap(c2)) break;
-        }
+        ++cnt;
+        q += len;
+        const int c = at(q);
+        if (c < 0 || is_gap(c)) return cnt;
     }
-    return cnt;
 }
-
-#define HYD_TILE_ACCESSOR                                                     \
-    __shared__ __attribute__((aligned(16))) unsigned char buf[kBuf];          \
-    const int64_t t0 = (int64_t)blockIdx.x * kTile;                           \
-    const int64_t base = t0 - kHaloLo;                                        \
-    stage_tile<ALIGNED>(text, n, base, buf);                                  \
-    auto at = [&](int64_t p) -> int {                                         \
-        if (p < 0 || p >= n) return -1;                                       \
-        const int64_t o = p - base;                                           \
-        if (o >= 0 && o < kBuf) return buf[o];                                \
-        return text[p];                                                       \
-    };                                                                        \
-    const int64_t s0 = t0 + (int64_t)threadIdx.x * kSlice;
 
 template <bool FLOAT, bool ALIGNED>
 __global__ __launch_bounds__(kPThreads) void csv_count_kernel(const unsigned char* __restrict__ text, int64_t n,
                                                               int64_t* __restrict__ tile_cnt,
                                                               int64_t* __restrict__ tile_err) {
-    HYD_TILE_ACCESSOR
-    __shared__ int64_t sh[kWaves];
-    int64_t err;
-    const int64_t cnt = walk_slice<FLOAT>(at, s0, n, &err, [](int64_t, int64_t) {});
-    int64_t total;
-    const int64_t off = block_excl_scan(cnt, &total, sh);
-    const int64_t e = block_min(err >= 0 ? off + err : INT64_MAX, sh);
+    HYD_TILE_WORDS
+    int64_t carry = 0, err = INT64_MAX;
+    for (int r0 = 0; r0 < words; r0 += kPThreads) {  // one word per thread per round
+        const int w = r0 + (int)threadIdx.x;
+        int64_t cnt = 0;
+        bool fail = false;
+        if (w < words)
+            cnt = scan_word<FLOAT>(at, t0 + starts[w], &fail, [&](int64_t q) { return token_len<FLOAT>(at, q); });
+        int64_t total;
+        const int64_t off = block_excl_scan(cnt, &total, sh);
+        if (fail) err = min(err, carry + off + cnt);
+        carry += total;
+    }
+    err = block_min(err, sh);
     if (threadIdx.x == 0) {
-        tile_cnt[blockIdx.x] = total;
-        tile_err[blockIdx.x] = e == INT64_MAX ? -1 : e;
+        tile_cnt[blockIdx.x] = carry;
+        tile_err[blockIdx.x] = err == INT64_MAX ? -1 : err;
+    }
+}
+
+// The value of the number at p: its bits (int64 value, or binary64 bits)
+// and whether the table rounding decided it; returns its length (-1: none).
+template <bool FLOAT, class At>
+__device__ __forceinline__ int64_t read_one(const At& at, int64_t p, uint64_t* bits, bool* decided) {
+    if (FLOAT) {
+        const FloatTok t = scan_f64_fast(at, p);
+        if (t.len >= 0) *decided = tok_to_bits(t, bits);
+        return t.len;
+    }
+    int64_t v;
+    const int64_t len = scan_i64(at, p, &v);
+    *bits = (uint64_t)v;
+    *decided = true;
+    return len;
+}
+
+// Stores a number read by read_one as element g (an undecided binary64 as
+// its text position, marked for the exact kernel).
+__device__ __forceinline__ void store_one(int64_t g, int64_t p, uint64_t bits, bool decided, int64_t count,
+                                          int64_t* __restrict__ out, uint32_t* __restrict__ exact_bits,
+                                          unsigned long long* __restrict__ exact_cnt) {
+    if (g >= count) return;
+    if (decided) {
+        out[g] = (int64_t)bits;
+    } else {
+        out[g] = p;  // the exact kernel rereads the text there
+        atomicOr(&exact_bits[g >> 5], 1u << (g & 31));
+        atomicAdd(exact_cnt, 1ULL);
     }
 }
 
@@ -216,32 +268,35 @@ __global__ __launch_bounds__(kPThreads) void csv_value_kernel(const unsigned cha
                                                               int64_t count, int64_t* __restrict__ out,
                                                               uint32_t* __restrict__ exact_bits,
                                                               unsigned long long* __restrict__ exact_cnt) {
-    HYD_TILE_ACCESSOR
-    __shared__ int64_t sh[kWaves];
-    int64_t err;
-    const int64_t cnt = walk_slice<FLOAT>(at, s0, n, &err, [](int64_t, int64_t) {});
-    int64_t total;
-    const int64_t g0 = tile_base[blockIdx.x] + block_excl_scan(cnt, &total, sh);
-    if (g0 >= count || cnt == 0) return;
-    walk_slice<FLOAT>(at, s0, n, &err, [&](int64_t p, int64_t j) {
-        const int64_t g = g0 + j;
-        if (g >= count) return;
-        if (FLOAT) {
-            const FloatTok t = scan_f64_fast(at, p);
-            uint64_t bits;
-            if (tok_to_bits(t, &bits)) {
-                out[g] = (int64_t)bits;
-            } else {
-                out[g] = p;  // the exact kernel rereads the text there
-                atomicOr(&exact_bits[g >> 5], 1u << (g & 31));
-                atomicAdd(exact_cnt, 1ULL);
-            }
-        } else {
-            int64_t v;
-            scan_i64(at, p, &v);
-            out[g] = v;
+    const int64_t g0 = tile_base[blockIdx.x];
+    if (g0 >= count) return;  // block-uniform: no element of this tile is kept
+    HYD_TILE_WORDS
+    int64_t carry = g0;
+    for (int r0 = 0; r0 < words; r0 += kPThreads) {
+        const int w = r0 + (int)threadIdx.x;
+        int64_t cnt = 0, p = 0, len1 = 0;
+        uint64_t bits1 = 0;
+        bool dec1 = true, fail = false;
+        if (w < words) {
+            p = t0 + starts[w];
+            // the word's first number is read in the counting scan itself
+            cnt = scan_word<FLOAT>(at, p, &fail, [&](int64_t q) { return len1 = read_one<FLOAT>(at, q, &bits1, &dec1); });
         }
-    });
+        int64_t total;
+        const int64_t g = carry + block_excl_scan(cnt, &total, sh);
+        if (cnt > 0) {
+            store_one(g, p, bits1, dec1, count, out, exact_bits, exact_cnt);
+            int64_t q = p + len1;
+            for (int64_t j = 1; j < cnt; ++j) {  // numbers that abut ("1-2"): rare
+                uint64_t b;
+                bool d;
+                const int64_t l = read_one<FLOAT>(at, q, &b, &d);
+                store_one(g + j, q, b, d, count, out, exact_bits, exact_cnt);
+                q += l;
+            }
+        }
+        carry += total;
+    }
 }
 
 // Exact rounding of the marked decimals (few; runs on a small grid).
