@@ -116,16 +116,23 @@ __device__ __forceinline__ void stage_tile(const unsigned char* __restrict__ tex
     __syncthreads();
 }
 
-// Backward check of the gap that ends at p - 1 (p: a word's first char).
+// What the LDS-only accessor returns for a text byte that is not staged: no
+// character class matches it, so a scan stops there, and the caller redoes
+// the word with the full accessor.
+constexpr int kSpill = 0x100;
+
+// Backward check of the gap that ends at p - 1 (p: a word's first char):
+// 1 = a gap the reference's loop skips, 0 = not, 2 = ran out of staged text.
 template <class At>
-__device__ __forceinline__ bool gap_ok(const At& at, int64_t p) {
+__device__ __forceinline__ int gap_state(const At& at, int64_t p) {
     bool comma_seen = false;  // a ',' lies between here and the word
     for (int64_t i = p - 1;; --i) {
         const int c = at(i);
-        if (c < 0) return !comma_seen;  // start of text: only isspace allowed
-        if (!is_gap(c)) return true;    // end of the previous word
+        if (c == kSpill) return 2;
+        if (c < 0) return comma_seen ? 0 : 1;  // start of text: only isspace allowed
+        if (!is_gap(c)) return 1;              // end of the previous word
         if (c == ',') comma_seen = true;
-        else if (comma_seen && !is_sep(c)) return false;  // '\t' '\v' '\f' before a ','
+        else if (comma_seen && !is_sep(c)) return 0;  // '\t' '\v' '\f' before a ','
     }
 }
 
@@ -156,19 +163,30 @@ constexpr int kMaxWords = kTile / 2 + 1;  // a word start follows a gap (or is t
     const int64_t base = t0 - kHaloLo;                                                  \
     const int64_t lo = base < 0 ? 0 : base;            /* staged: text[lo, lim) */      \
     const int64_t lim = n < base + kBuf ? n : base + kBuf;                              \
+    const int64_t stage_end = lim < n ? lim : INT64_MAX; /* unstaged text from here */  \
     stage_tile<ALIGNED>(text, n, base, buf);                                            \
-    auto at = [&](int64_t p) -> int {                                                   \
+    auto at = [&](int64_t p) -> int {  /* any position */                              \
         if ((uint64_t)(p - lo) < (uint64_t)(lim - lo)) return buf[p - base];            \
         return (p < 0 || p >= n) ? -1 : text[p];                                        \
+    };                                                                                  \
+    auto atl = [&](int64_t p) -> int {  /* staged bytes only, without a branch */      \
+        const bool in = (uint64_t)(p - lo) < (uint64_t)(lim - lo);                      \
+        const int c = buf[in ? p - base : 0];                                           \
+        return in ? c : ((p < 0 || p >= n) ? -1 : kSpill);                              \
     };                                                                                  \
     int words;                                                                          \
     {                                                                                   \
         const int64_t s0 = t0 + (int64_t)threadIdx.x * kSlice;                          \
+        const int64_t rest = n - s0;                                                    \
+        const int valid = rest <= 0 ? 0 : (rest < kSlice ? (int)rest : kSlice);         \
+        const uint4 qa = *reinterpret_cast<const uint4*>(buf + (s0 - base));            \
+        const uint4 qb = *reinterpret_cast<const uint4*>(buf + (s0 - base) + 16);       \
+        const uint32_t wv[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};        \
         uint32_t mask = 0;                                                              \
         int prev = at(s0 - 1);                                                          \
         _Pragma("unroll") for (int k = 0; k < kSlice; ++k) {                            \
-            const int c = at(s0 + k);                                                   \
-            if (c >= 0 && !is_gap(c) && (prev < 0 || is_gap(prev))) mask |= 1u << k;   \
+            const int c = (int)((wv[k >> 2] >> (8 * (k & 3))) & 0xFF);                  \
+            if (k < valid && !is_gap(c) && (prev < 0 || is_gap(prev))) mask |= 1u << k; \
             prev = c;                                                                   \
         }                                                                               \
         int64_t total;                                                                  \
@@ -185,12 +203,18 @@ constexpr int kMaxWords = kTile / 2 + 1;  // a word start follows a gap (or is t
 // Scans the word at p: how many numbers it holds (those before a failure),
 // and whether one failed (or the gap before the word is not one the
 // reference's loop skips).  first(q) scans the first number and returns its
-// length; the others are scanned for their lengths only.
+// length; the others are scanned for their lengths only.  *redo is set when
+// the scan may have met an unstaged byte (the LDS-only accessor): a number
+// scan reads at most 8 bytes past where it stops.
 template <bool FLOAT, class At, class First>
-__device__ __forceinline__ int64_t scan_word(const At& at, int64_t p, bool* fail, const First& first) {
+__device__ __forceinline__ int64_t scan_word(const At& at, int64_t p, int64_t stage_end, bool* fail, bool* redo,
+                                             const First& first) {
     *fail = false;
-    if (!gap_ok(at, p)) {
-        *fail = true;
+    *redo = false;
+    const int g = gap_state(at, p);
+    if (g != 1) {
+        *redo = g == 2;
+        *fail = g == 0;
         return 0;
     }
     int64_t cnt = 0, q = p;
@@ -198,12 +222,16 @@ __device__ __forceinline__ int64_t scan_word(const At& at, int64_t p, bool* fail
         const int64_t len = cnt == 0 ? first(q) : token_len<FLOAT>(at, q);
         if (len < 0) {
             *fail = true;
+            *redo = q + 8 >= stage_end;
             return cnt;
         }
         ++cnt;
         q += len;
         const int c = at(q);
-        if (c < 0 || is_gap(c)) return cnt;
+        if (c < 0 || is_gap(c) || c == kSpill) {
+            *redo = q + 8 >= stage_end;
+            return cnt;
+        }
     }
 }
 
@@ -217,8 +245,15 @@ __global__ __launch_bounds__(kPThreads) void csv_count_kernel(const unsigned cha
         const int w = r0 + (int)threadIdx.x;
         int64_t cnt = 0;
         bool fail = false;
-        if (w < words)
-            cnt = scan_word<FLOAT>(at, t0 + starts[w], &fail, [&](int64_t q) { return token_len<FLOAT>(at, q); });
+        if (w < words) {
+            const int64_t p = t0 + starts[w];
+            bool redo;
+            cnt = scan_word<FLOAT>(atl, p, stage_end, &fail, &redo,
+                                   [&](int64_t q) { return token_len<FLOAT>(atl, q); });
+            if (redo)  // a word longer than the staged halo, or a gap reaching before it
+                cnt = scan_word<FLOAT>(at, p, INT64_MAX, &fail, &redo,
+                                       [&](int64_t q) { return token_len<FLOAT>(at, q); });
+        }
         int64_t total;
         const int64_t off = block_excl_scan(cnt, &total, sh);
         if (fail) err = min(err, carry + off + cnt);
@@ -265,11 +300,15 @@ __device__ __forceinline__ void store_one(int64_t g, int64_t p, uint64_t bits, b
 template <bool FLOAT, bool ALIGNED>
 __global__ __launch_bounds__(kPThreads) void csv_value_kernel(const unsigned char* __restrict__ text, int64_t n,
                                                               const int64_t* __restrict__ tile_base,
+                                                              const int64_t* __restrict__ tile_err,
                                                               int64_t count, int64_t* __restrict__ out,
                                                               uint32_t* __restrict__ exact_bits,
-                                                              unsigned long long* __restrict__ exact_cnt) {
+                                                              unsigned long long* __restrict__ exact_cnt,
+                                                              unsigned long long* __restrict__ err_enc) {
     const int64_t g0 = tile_base[blockIdx.x];
     if (g0 >= count) return;  // block-uniform: no element of this tile is kept
+    if (threadIdx.x == 0 && tile_err[blockIdx.x] >= 0)
+        atomicMax(err_enc, (unsigned long long)(INT64_MAX - (g0 + tile_err[blockIdx.x])));
     HYD_TILE_WORDS
     int64_t carry = g0;
     for (int r0 = 0; r0 < words; r0 += kPThreads) {
@@ -280,7 +319,12 @@ __global__ __launch_bounds__(kPThreads) void csv_value_kernel(const unsigned cha
         if (w < words) {
             p = t0 + starts[w];
             // the word's first number is read in the counting scan itself
-            cnt = scan_word<FLOAT>(at, p, &fail, [&](int64_t q) { return len1 = read_one<FLOAT>(at, q, &bits1, &dec1); });
+            bool redo;
+            cnt = scan_word<FLOAT>(atl, p, stage_end, &fail, &redo,
+                                   [&](int64_t q) { return len1 = read_one<FLOAT>(atl, q, &bits1, &dec1); });
+            if (redo)
+                cnt = scan_word<FLOAT>(at, p, INT64_MAX, &fail, &redo,
+                                       [&](int64_t q) { return len1 = read_one<FLOAT>(at, q, &bits1, &dec1); });
         }
         int64_t total;
         const int64_t g = carry + block_excl_scan(cnt, &total, sh);
@@ -326,22 +370,18 @@ __global__ __launch_bounds__(64) void csv_exact_kernel(const unsigned char* __re
 
 // status[0] = first failing element (-1: none below count), status[1] = kind,
 // status[2] = numbers found (capped by the first failure), status[3] = exact.
+// err_enc = INT64_MAX - (first failing element over all tiles), 0: none.
 __global__ __launch_bounds__(kPThreads) void csv_status_kernel(const unsigned char* __restrict__ text, int64_t n,
                                                                int64_t count, int64_t tiles,
                                                                const int64_t* __restrict__ tile_cnt,
-                                                               const int64_t* __restrict__ tile_err,
                                                                const int64_t* __restrict__ tile_base,
                                                                const unsigned long long* __restrict__ exact_cnt,
+                                                               const unsigned long long* __restrict__ err_enc,
                                                                int64_t* __restrict__ status) {
     __shared__ int64_t sh[kWaves];
-    int64_t e = INT64_MAX;
-    for (int64_t t = threadIdx.x; t < tiles; t += kPThreads)
-        if (tile_err[t] >= 0) {
-            const int64_t g = tile_base[t] + tile_err[t];
-            e = g < e ? g : e;
-        }
-    e = block_min(e, sh);
-    const int64_t found = tiles ? tile_base[tiles - 1] + tile_cnt[tiles - 1] : 0;
+    int64_t e = *err_enc ? INT64_MAX - (int64_t)*err_enc : INT64_MAX;
+    int64_t found = tiles ? tile_base[tiles - 1] + tile_cnt[tiles - 1] : 0;
+    if (e < found) found = e;  // numbers after a failure are not the reference's elements
     int64_t kind = 0;
     if (e != INT64_MAX) {
         kind = HYDHIP_CSV_PARSE_ERROR;
@@ -552,6 +592,7 @@ int csv_parse(const unsigned char* text, int64_t nbytes, int64_t count, bool is_
     int64_t* scan_ws = tile_base + tiles;
     int64_t* status = scan_ws + scan_workspace_words(tiles);       // 4 words
     unsigned long long* exact_cnt = reinterpret_cast<unsigned long long*>(status + 4);
+    unsigned long long* err_enc = reinterpret_cast<unsigned long long*>(status + 5);
     uint32_t* exact_bits = reinterpret_cast<uint32_t*>(status + 8);
     HYD_TRY(fill_zero(status, (size_t)(8 + bw) * 8, s));
     const bool aligned = (reinterpret_cast<uintptr_t>(text) & 15) == 0;
@@ -561,8 +602,8 @@ int csv_parse(const unsigned char* text, int64_t nbytes, int64_t count, bool is_
         csv_count_kernel<F, A><<<g, kPThreads, 0, s>>>(text, nbytes, tile_cnt, tile_err);               \
         HYD_CHECK_LAUNCH();                                                                             \
         HYD_TRY(scan_i64(tile_cnt, tiles, 0, tile_base, false, false, s, scan_ws));                    \
-        csv_value_kernel<F, A><<<g, kPThreads, 0, s>>>(text, nbytes, tile_base, count, out, exact_bits, \
-                                                       exact_cnt);                                      \
+        csv_value_kernel<F, A><<<g, kPThreads, 0, s>>>(text, nbytes, tile_base, tile_err, count, out,   \
+                                                       exact_bits, exact_cnt, err_enc);                 \
         HYD_CHECK_LAUNCH();                                                                             \
     } while (0)
     if (is_float) {
@@ -575,8 +616,8 @@ int csv_parse(const unsigned char* text, int64_t nbytes, int64_t count, bool is_
         else HYD_PARSE_LAUNCH(false, false);
     }
 #undef HYD_PARSE_LAUNCH
-    csv_status_kernel<<<1, kPThreads, 0, s>>>(text, nbytes, count, tiles, tile_cnt, tile_err, tile_base,
-                                              exact_cnt, status);
+    csv_status_kernel<<<1, kPThreads, 0, s>>>(text, nbytes, count, tiles, tile_cnt, tile_base, exact_cnt, err_enc,
+                                              status);
     HYD_CHECK_LAUNCH();
     int64_t h[4];
     HYD_TRY_HIP(hipMemcpyAsync(h, status, sizeof h, hipMemcpyDeviceToHost, s));

@@ -272,17 +272,17 @@ struct FastSink {
     int64_t qadj = 0;
     bool truncated = false;
     HYD_HD void digit(int d, bool after_point, bool hex) {
+        // leading zeros only move the point; the first kMax significant
+        // digits build w; later ones only mark truncation (and, before the
+        // point, scale w)
         const int unit = hex ? 4 : 1;
-        if (nsig == 0 && d == 0) {
-            if (after_point) qadj -= unit;
-        } else if (nsig < (hex ? kMaxHex : kMaxDec)) {
-            w = hex ? ((w << 4) | (uint64_t)d) : (w * 10 + (uint64_t)d);
-            ++nsig;
-            if (after_point) qadj -= unit;
-        } else {
-            if (d) truncated = true;
-            if (!after_point) qadj += unit;
-        }
+        const bool lead = nsig == 0 && d == 0;
+        const bool keep = !lead && nsig < (hex ? kMaxHex : kMaxDec);
+        const uint64_t wk = hex ? ((w << 4) | (uint64_t)d) : (w * 10 + (uint64_t)d);
+        w = keep ? wk : w;
+        nsig += keep ? 1 : 0;
+        truncated = truncated || (!lead && !keep && d != 0);
+        qadj += after_point ? (lead || keep ? -unit : 0) : (!lead && !keep ? unit : 0);
     }
 };
 
@@ -326,45 +326,47 @@ HYD_HD FloatTok scan_f64(const At& at, int64_t p, Sink& sink) {
         r.len = i - p;
         return r;
     }
-    bool hex = false, got_digit = false, got_dot = false, got_e = false, mant_digit = false;
+    bool hex = false, mant_digit = false;
     if (c == '0') {
         c = at(++i);
         if (lower(c) == 'x') {
             hex = true;
             c = at(++i);
         } else {
-            got_digit = true;
             mant_digit = true;
             sink.digit(0, false, false);
         }
     }
-    int64_t exp_pos = -2, expv = 0;
+    // the buffer glibc's loop collects, in its three possible parts:
+    // mantissa digits, '.' and digits, then (after a mantissa digit) the
+    // exponent marker, one sign and decimal digits
+    int64_t expv = 0;
     bool eneg = false, exp_digit = false;
-    for (;;) {
-        const bool dig = is_digit(c);
-        if (dig || (hex && !got_e && is_hex_alpha(c))) {
-            if (!got_e) {
-                const int d = dig ? c - '0' : lower(c) - 'a' + 10;
-                sink.digit(d, got_dot, hex);
-                mant_digit = true;
-            } else {
-                expv = expv > kExpCap / 10 ? kExpCap : expv * 10 + (c - '0');
-                if (expv > kExpCap) expv = kExpCap;
-                exp_digit = true;
-            }
-            got_digit = true;
-        } else if (got_e && i == exp_pos + 1 && (c == '+' || c == '-')) {
-            eneg = c == '-';
-        } else if (got_digit && !got_e && lower(c) == (hex ? 'p' : 'e')) {
-            got_e = true;
-            got_dot = true;
-            exp_pos = i;
-        } else if (!got_dot && c == '.') {
-            got_dot = true;
-        } else {
-            break;
+    for (bool after_point = false;; after_point = true) {
+        for (;;) {
+            int d;
+            if (is_digit(c)) d = c - '0';
+            else if (hex && is_hex_alpha(c)) d = lower(c) - 'a' + 10;
+            else break;
+            sink.digit(d, after_point, hex);
+            mant_digit = true;
+            c = at(++i);
         }
+        if (after_point || c != '.') break;
         c = at(++i);
+    }
+    if (mant_digit && lower(c) == (hex ? 'p' : 'e')) {
+        c = at(++i);
+        if (c == '+' || c == '-') {
+            eneg = c == '-';
+            c = at(++i);
+        }
+        while (is_digit(c)) {
+            expv = expv > kExpCap / 10 ? kExpCap : expv * 10 + (c - '0');
+            if (expv > kExpCap) expv = kExpCap;
+            exp_digit = true;
+            c = at(++i);
+        }
     }
     const int64_t size = i - p;
     if (size == sign_len) return r;
