@@ -235,6 +235,43 @@ __device__ __forceinline__ int64_t scan_word(const At& at, int64_t p, int64_t st
     }
 }
 
+// Words that start in each tile (no number scan): the element index of
+// every word when each holds one number, the common case.
+template <bool ALIGNED>
+__global__ __launch_bounds__(kPThreads) void csv_words_kernel(const unsigned char* __restrict__ text, int64_t n,
+                                                              int64_t* __restrict__ tile_cnt) {
+    __shared__ int64_t sh[kWaves];
+    const int64_t s0 = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kSlice;
+    uint32_t wv[8];
+    if (ALIGNED && s0 + kSlice <= n) {
+        const uint4 qa = *reinterpret_cast<const uint4*>(text + s0);
+        const uint4 qb = *reinterpret_cast<const uint4*>(text + s0 + 16);
+        wv[0] = qa.x; wv[1] = qa.y; wv[2] = qa.z; wv[3] = qa.w;
+        wv[4] = qb.x; wv[5] = qb.y; wv[6] = qb.z; wv[7] = qb.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            uint32_t v = 0;
+            for (int b = 0; b < 4; ++b)
+                if (s0 + 4 * k + b < n) v |= (uint32_t)text[s0 + 4 * k + b] << (8 * b);
+            wv[k] = v;
+        }
+    }
+    const int64_t rest = n - s0;
+    const int valid = rest <= 0 ? 0 : (rest < kSlice ? (int)rest : kSlice);
+    int prev = (s0 > 0 && s0 - 1 < n) ? text[s0 - 1] : -1;
+    int cnt = 0;
+#pragma unroll
+    for (int k = 0; k < kSlice; ++k) {
+        const int c = (int)((wv[k >> 2] >> (8 * (k & 3))) & 0xFF);
+        cnt += (k < valid && !is_gap(c) && (prev < 0 || is_gap(prev))) ? 1 : 0;
+        prev = c;
+    }
+    int64_t total;
+    block_excl_scan(cnt, &total, sh);
+    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = total;
+}
+
 template <bool FLOAT, bool ALIGNED>
 __global__ __launch_bounds__(kPThreads) void csv_count_kernel(const unsigned char* __restrict__ text, int64_t n,
                                                               int64_t* __restrict__ tile_cnt,
@@ -297,17 +334,23 @@ __device__ __forceinline__ void store_one(int64_t g, int64_t p, uint64_t bits, b
     }
 }
 
-template <bool FLOAT, bool ALIGNED>
+// Values.  ONE_PER_WORD: tile_base counts words (csv_words_kernel), which
+// equals the numbers' indices when every word holds exactly one number after
+// a gap the reference skips; a word where that fails sets *irregular and the
+// caller redoes the text with counted numbers (ONE_PER_WORD = false, tile_base
+// from csv_count_kernel).
+template <bool FLOAT, bool ALIGNED, bool ONE_PER_WORD>
 __global__ __launch_bounds__(kPThreads) void csv_value_kernel(const unsigned char* __restrict__ text, int64_t n,
                                                               const int64_t* __restrict__ tile_base,
                                                               const int64_t* __restrict__ tile_err,
                                                               int64_t count, int64_t* __restrict__ out,
                                                               uint32_t* __restrict__ exact_bits,
                                                               unsigned long long* __restrict__ exact_cnt,
-                                                              unsigned long long* __restrict__ err_enc) {
+                                                              unsigned long long* __restrict__ err_enc,
+                                                              unsigned long long* __restrict__ irregular) {
     const int64_t g0 = tile_base[blockIdx.x];
     if (g0 >= count) return;  // block-uniform: no element of this tile is kept
-    if (threadIdx.x == 0 && tile_err[blockIdx.x] >= 0)
+    if (!ONE_PER_WORD && threadIdx.x == 0 && tile_err[blockIdx.x] >= 0)
         atomicMax(err_enc, (unsigned long long)(INT64_MAX - (g0 + tile_err[blockIdx.x])));
     HYD_TILE_WORDS
     int64_t carry = g0;
@@ -316,7 +359,8 @@ __global__ __launch_bounds__(kPThreads) void csv_value_kernel(const unsigned cha
         int64_t cnt = 0, p = 0, len1 = 0;
         uint64_t bits1 = 0;
         bool dec1 = true, fail = false;
-        if (w < words) {
+        const bool mine = w < words && (!ONE_PER_WORD || g0 + w < count);
+        if (mine) {
             p = t0 + starts[w];
             // the word's first number is read in the counting scan itself
             bool redo;
@@ -325,6 +369,13 @@ __global__ __launch_bounds__(kPThreads) void csv_value_kernel(const unsigned cha
             if (redo)
                 cnt = scan_word<FLOAT>(at, p, INT64_MAX, &fail, &redo,
                                        [&](int64_t q) { return len1 = read_one<FLOAT>(at, q, &bits1, &dec1); });
+        }
+        if (ONE_PER_WORD) {  // word w is element g0 + w
+            if (mine) {
+                if (cnt != 1 || fail) atomicOr(irregular, 1ULL);
+                else store_one(g0 + w, p, bits1, dec1, count, out, exact_bits, exact_cnt);
+            }
+            continue;
         }
         int64_t total;
         const int64_t g = carry + block_excl_scan(cnt, &total, sh);
@@ -594,34 +645,47 @@ int csv_parse(const unsigned char* text, int64_t nbytes, int64_t count, bool is_
     unsigned long long* exact_cnt = reinterpret_cast<unsigned long long*>(status + 4);
     unsigned long long* err_enc = reinterpret_cast<unsigned long long*>(status + 5);
     uint32_t* exact_bits = reinterpret_cast<uint32_t*>(status + 8);
-    HYD_TRY(fill_zero(status, (size_t)(8 + bw) * 8, s));
+    unsigned long long* irregular = reinterpret_cast<unsigned long long*>(status + 6);
     const bool aligned = (reinterpret_cast<uintptr_t>(text) & 15) == 0;
     const unsigned g = (unsigned)tiles;
-#define HYD_PARSE_LAUNCH(F, A)                                                                          \
-    do {                                                                                                \
-        csv_count_kernel<F, A><<<g, kPThreads, 0, s>>>(text, nbytes, tile_cnt, tile_err);               \
-        HYD_CHECK_LAUNCH();                                                                             \
-        HYD_TRY(scan_i64(tile_cnt, tiles, 0, tile_base, false, false, s, scan_ws));                    \
-        csv_value_kernel<F, A><<<g, kPThreads, 0, s>>>(text, nbytes, tile_base, tile_err, count, out,   \
-                                                       exact_bits, exact_cnt, err_enc);                 \
-        HYD_CHECK_LAUNCH();                                                                             \
+    int64_t h[7];
+    // pass 1 assumes one number per word; pass 2 (only when a word broke
+    // that) counts the numbers first
+    for (int pass = 0; pass < 2; ++pass) {
+        HYD_TRY(fill_zero(status, (size_t)(8 + bw) * 8, s));
+#define HYD_PARSE_LAUNCH(F, A)                                                                            \
+    do {                                                                                                  \
+        if (pass == 0) csv_words_kernel<A><<<g, kPThreads, 0, s>>>(text, nbytes, tile_cnt);               \
+        else csv_count_kernel<F, A><<<g, kPThreads, 0, s>>>(text, nbytes, tile_cnt, tile_err);            \
+        HYD_CHECK_LAUNCH();                                                                               \
+        HYD_TRY(scan_i64(tile_cnt, tiles, 0, tile_base, false, false, s, scan_ws));                      \
+        if (pass == 0)                                                                                    \
+            csv_value_kernel<F, A, true><<<g, kPThreads, 0, s>>>(text, nbytes, tile_base, tile_err, count, \
+                                                                 out, exact_bits, exact_cnt, err_enc,     \
+                                                                 irregular);                             \
+        else                                                                                              \
+            csv_value_kernel<F, A, false><<<g, kPThreads, 0, s>>>(text, nbytes, tile_base, tile_err, count, \
+                                                                  out, exact_bits, exact_cnt, err_enc,    \
+                                                                  irregular);                            \
+        HYD_CHECK_LAUNCH();                                                                               \
     } while (0)
-    if (is_float) {
-        if (aligned) HYD_PARSE_LAUNCH(true, true);
-        else HYD_PARSE_LAUNCH(true, false);
-        csv_exact_kernel<<<64, 64, 0, s>>>(text, nbytes, count, exact_bits, exact_cnt, out);
-        HYD_CHECK_LAUNCH();
-    } else {
-        if (aligned) HYD_PARSE_LAUNCH(false, true);
-        else HYD_PARSE_LAUNCH(false, false);
-    }
+        if (is_float) {
+            if (aligned) HYD_PARSE_LAUNCH(true, true);
+            else HYD_PARSE_LAUNCH(true, false);
+            csv_exact_kernel<<<64, 64, 0, s>>>(text, nbytes, count, exact_bits, exact_cnt, out);
+            HYD_CHECK_LAUNCH();
+        } else {
+            if (aligned) HYD_PARSE_LAUNCH(false, true);
+            else HYD_PARSE_LAUNCH(false, false);
+        }
 #undef HYD_PARSE_LAUNCH
-    csv_status_kernel<<<1, kPThreads, 0, s>>>(text, nbytes, count, tiles, tile_cnt, tile_base, exact_cnt, err_enc,
-                                              status);
-    HYD_CHECK_LAUNCH();
-    int64_t h[4];
-    HYD_TRY_HIP(hipMemcpyAsync(h, status, sizeof h, hipMemcpyDeviceToHost, s));
-    HYD_TRY_HIP(hipStreamSynchronize(s));
+        csv_status_kernel<<<1, kPThreads, 0, s>>>(text, nbytes, count, tiles, tile_cnt, tile_base, exact_cnt,
+                                                  err_enc, status);
+        HYD_CHECK_LAUNCH();
+        HYD_TRY_HIP(hipMemcpyAsync(h, status, sizeof h, hipMemcpyDeviceToHost, s));
+        HYD_TRY_HIP(hipStreamSynchronize(s));
+        if (!h[6]) break;
+    }
     st->error_element = h[0];
     st->error_kind = (int32_t)h[1];
     st->numbers_found = h[2];
