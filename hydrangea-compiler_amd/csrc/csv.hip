@@ -353,6 +353,26 @@ __global__ __launch_bounds__(kPThreads) void csv_value_kernel(const unsigned cha
     if (!ONE_PER_WORD && threadIdx.x == 0 && tile_err[blockIdx.x] >= 0)
         atomicMax(err_enc, (unsigned long long)(INT64_MAX - (g0 + tile_err[blockIdx.x])));
     HYD_TILE_WORDS
+    // 8 staged bytes from q (q - base + 16 <= kBuf)
+    auto rd8 = [&](int64_t q) -> uint64_t {
+        const int o = (int)(q - base);
+        const int a = o & ~7, sh = (o & 7) * 8;
+        const uint64_t lo8 = *reinterpret_cast<const uint64_t*>(buf + a);
+        const uint64_t hi8 = *reinterpret_cast<const uint64_t*>(buf + a + 8);
+        return sh ? (lo8 >> sh) | (hi8 << (64 - sh)) : lo8;
+    };
+    // the word's first number, read in the counting scan itself: the 8-byte
+    // path when its bytes are staged, else the character scanner
+    auto first_lds = [&](int64_t q, uint64_t* bits, bool* dec) -> int64_t {
+        if (FLOAT && q + kSwarSpan + 8 <= lim) {
+            FloatTok t;
+            if (scan_dec_swar(rd8, atl, q, &t)) {
+                *dec = tok_to_bits(t, bits);
+                return t.len;
+            }
+        }
+        return read_one<FLOAT>(atl, q, bits, dec);
+    };
     int64_t carry = g0;
     for (int r0 = 0; r0 < words; r0 += kPThreads) {
         const int w = r0 + (int)threadIdx.x;
@@ -362,10 +382,9 @@ __global__ __launch_bounds__(kPThreads) void csv_value_kernel(const unsigned cha
         const bool mine = w < words && (!ONE_PER_WORD || g0 + w < count);
         if (mine) {
             p = t0 + starts[w];
-            // the word's first number is read in the counting scan itself
             bool redo;
             cnt = scan_word<FLOAT>(atl, p, stage_end, &fail, &redo,
-                                   [&](int64_t q) { return len1 = read_one<FLOAT>(atl, q, &bits1, &dec1); });
+                                   [&](int64_t q) { return len1 = first_lds(q, &bits1, &dec1); });
             if (redo)
                 cnt = scan_word<FLOAT>(at, p, INT64_MAX, &fail, &redo,
                                        [&](int64_t q) { return len1 = read_one<FLOAT>(at, q, &bits1, &dec1); });
@@ -983,7 +1002,21 @@ extern "C" int hydhip_host_csv_scan(const char* s, int64_t len, int is_float, in
         *consumed = p + l;
         return 1;
     }
-    const FloatTok t = scan_f64_fast(at, p);
+    FloatTok t;
+    bool swar = false;
+    if (flags & 2) {  // the 8-byte path, over the text followed by gap bytes as a file would have
+        std::string padded(s + p, (size_t)(len - p));
+        padded.append(kSwarSpan + 16, ',');
+        auto atp = [&](int64_t q) -> int { return (q < 0 || q >= (int64_t)padded.size()) ? -1 : (unsigned char)padded[q]; };
+        auto rd8 = [&](int64_t q) -> uint64_t {
+            uint64_t v;
+            memcpy(&v, padded.data() + q, 8);
+            return v;
+        };
+        swar = scan_dec_swar(rd8, atp, 0, &t);
+        if (swar) *exact = 2;
+    }
+    if (!swar) t = scan_f64_fast(at, p);
     if (t.len < 0) return 0;
     uint64_t bits;
     const bool decided = tok_to_bits(t, &bits);
@@ -991,7 +1024,7 @@ extern "C" int hydhip_host_csv_scan(const char* s, int64_t len, int is_float, in
         Big* big = new Big[3];
         bits = dec_exact_bits(at, p, bits, big[0], big[1], big[2]);
         delete[] big;
-        *exact = 1;
+        *exact |= 1;
     }
     memcpy(value, &bits, 8);
     *consumed = p + t.len;

@@ -393,6 +393,132 @@ HYD_HD FloatTok scan_f64_fast(const At& at, int64_t p) {
     return r;
 }
 
+// ---- the common decimal shape, 8 characters at a time -------------------------
+// [sign] digits [. digits] [e|E [sign] digits], found with byte masks on
+// 64-bit words (byte 0 = the first character) instead of one character per
+// step.  Same result as scan_f64 + FastSink; anything else (hex, inf, nan, a
+// lone '.', runs reaching past kSwarSpan bytes) is left to scan_f64.
+
+constexpr int kSwarSpan = 48;  // bytes from the number's start the reader may touch
+
+// bit 7 of each byte -> bit i for byte i (distinct shifted copies: no carries)
+HYD_HD uint32_t byte_mask(uint64_t f80) {
+    return (uint32_t)(((f80 & 0x8080808080808080ULL) * 0x0002040810204081ULL) >> 56);
+}
+// bit 7 set in exactly the zero bytes of x
+HYD_HD uint64_t zero_bytes(uint64_t x) {
+    return ~(((x & 0x7F7F7F7F7F7F7F7FULL) + 0x7F7F7F7F7F7F7F7FULL) | x) & 0x8080808080808080ULL;
+}
+HYD_HD uint32_t digit_mask(uint64_t x) {
+    const uint64_t hi = (x & 0xF0F0F0F0F0F0F0F0ULL) ^ 0x3030303030303030ULL;  // 0 iff high nibble 3
+    const uint64_t lo = ((x & 0x0F0F0F0F0F0F0F0FULL) + 0x0606060606060606ULL) & 0xF0F0F0F0F0F0F0F0ULL;  // 0 iff low <= 9
+    return byte_mask(zero_bytes(hi | lo));
+}
+HYD_HD uint32_t eq_mask(uint64_t x, uint32_t c) { return byte_mask(zero_bytes(x ^ (0x0101010101010101ULL * c))); }
+HYD_HD int lead_digits(uint64_t x) {  // digit bytes at the start of x (0..8)
+    const uint32_t nd = ~digit_mask(x) & 0xFF;
+    return nd ? __builtin_ctz(nd) : 8;
+}
+HYD_HD uint32_t pow10_small(int k) {  // 10^k, 0 <= k <= 8, without a table
+    uint32_t p = (k & 1) ? 10u : 1u;
+    p *= (k & 2) ? 100u : 1u;
+    p *= (k & 4) ? 10000u : 1u;
+    p *= (k & 8) ? 100000000u : 1u;
+    return p;
+}
+// value of the first k (1..8) digit bytes of x: padded with leading '0's to
+// eight digits, then pairs, quads and the whole combined by three multiplies
+HYD_HD uint32_t digits_value(uint64_t x, int k) {
+    if (k < 8) x = (x << (8 * (8 - k))) | (0x3030303030303030ULL >> (8 * k));
+    x -= 0x3030303030303030ULL;
+    x = x * 10 + (x >> 8);
+    x = (((x & 0x000000FF000000FFULL) * (100 + (1000000ULL << 32))) +
+         (((x >> 16) & 0x000000FF000000FFULL) * (1 + (10000ULL << 32)))) >> 32;
+    return (uint32_t)x;
+}
+
+// FastSink for a run of k (<= 8) digit bytes at the start of x.
+struct RunSink {
+    uint64_t w = 0;
+    int nsig = 0;
+    int64_t qadj = 0;
+    bool truncated = false;
+    HYD_HD void run(uint64_t x, int k, bool after_point) {
+        if (nsig == 0) {  // leading zeros only move the point
+            const uint32_t nz = ~eq_mask(x, '0') & 0xFF;
+            int z = nz ? __builtin_ctz(nz) : 8;
+            z = z < k ? z : k;
+            if (after_point) qadj -= z;
+            x = z < 8 ? (x >> (8 * z)) : 0;
+            k -= z;
+            if (k == 0) return;
+        }
+        int take = FastSink::kMaxDec - nsig;
+        take = take < k ? take : k;
+        if (take > 0) {
+            w = w * pow10_small(take) + digits_value(x, take);
+            nsig += take;
+            if (after_point) qadj -= take;
+        }
+        const int rest = k - take;
+        if (rest > 0) {
+            if (!after_point) qadj += rest;
+            const uint32_t dropped = ((1u << k) - 1) & ~((1u << take) - 1);
+            if (~eq_mask(x, '0') & dropped) truncated = true;
+        }
+    }
+};
+
+// rd8(q): the 8 bytes at q..q+7 (valid for q < p + kSwarSpan); at(q): one byte.
+template <class Rd, class At>
+HYD_HD bool scan_dec_swar(const Rd& rd8, const At& at, int64_t p, FloatTok* t) {
+    int64_t i = p;
+    int c = at(i);
+    bool neg = false;
+    if (c == '+' || c == '-') {
+        neg = c == '-';
+        c = at(++i);
+    }
+    if (c == '0' && lower(at(i + 1)) == 'x') return false;  // hexadecimal
+    RunSink s;
+    int64_t mant = 0;
+    for (int part = 0; part < 2; ++part) {  // before and after the point
+        for (;;) {
+            if (i - p > kSwarSpan - 8) return false;
+            const uint64_t x = rd8(i);
+            const int k = lead_digits(x);
+            if (k) s.run(x, k, part == 1);
+            mant += k;
+            i += k;
+            if (k < 8) break;
+        }
+        if (part == 1 || at(i) != '.') break;
+        ++i;
+    }
+    if (mant == 0) return false;  // "." , "-.", letters
+    int64_t e = 0;
+    if (lower(at(i)) == 'e') {
+        int64_t j = i + 1;
+        const int cs = at(j);
+        const bool eneg = cs == '-';
+        if (cs == '+' || cs == '-') ++j;
+        if (j - p > kSwarSpan - 8) return false;
+        const uint64_t x = rd8(j);
+        const int k = lead_digits(x);
+        if (k == 8) return false;  // eight or more exponent digits
+        if (k) e = digits_value(x, k);
+        if (eneg) e = -e;
+        i = j + k;  // no digits: the marker (and sign) are taken but ignored
+    }
+    t->len = i - p;
+    t->kind = kDec;
+    t->neg = neg;
+    t->w = s.w;
+    t->q = e + s.qadj;
+    t->truncated = s.truncated;
+    return true;
+}
+
 // Bits of a scanned token when the table path decides them; false when the
 // exact path must (decimal tokens only).
 HYD_HD bool tok_to_bits(const FloatTok& t, uint64_t* out) {

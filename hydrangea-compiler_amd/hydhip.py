@@ -695,7 +695,7 @@ def csv_format(vals, row_len=None, stream=None, with_exact=False):
     return (out, exact.value) if with_exact else out
 
 
-def host_csv_scan(strings, is_float, force_exact=False):
+def host_csv_scan(strings, is_float, force_exact=False, swar=False):
     """CPU build of the device scanner on each string: (r, values, consumed, exact)."""
     n = len(strings)
     r = np.zeros(n, np.int32)
@@ -705,7 +705,8 @@ def host_csv_scan(strings, is_float, force_exact=False):
     c = _I64()
     e = _INT()
     for k, s in enumerate(strings):
-        r[k] = lib().hydhip_host_csv_scan(s, len(s), 1 if is_float else 0, 1 if force_exact else 0,
+        r[k] = lib().hydhip_host_csv_scan(s, len(s), 1 if is_float else 0,
+                                          (1 if force_exact else 0) | (2 if swar else 0),
                                           vals.ctypes.data + 8 * k, ctypes.byref(c), ctypes.byref(e))
         consumed[k] = c.value
         exact[k] = e.value

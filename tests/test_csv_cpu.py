@@ -153,8 +153,8 @@ def test_oracle_parses_like_python(orc):
 
 
 # ---- the device conversions, built for the host, against the oracle ----------------
-def check_scan(host, orc, strs, is_float, force_exact=False):
-    r, v, c, _ = host.host_csv_scan(strs, is_float, force_exact)
+def check_scan(host, orc, strs, is_float, force_exact=False, swar=False):
+    r, v, c, _ = host.host_csv_scan(strs, is_float, force_exact, swar)
     r2, v2, c2 = orc.scan_tokens(strs, is_float)
     assert (r == r2).all()
     ok = r == 1
@@ -181,12 +181,13 @@ def test_device_formatter_matches_glibc(host, orc, force_exact):
     assert host.host_csv_format(i, False)[0] == orc.format_values(i, False)
 
 
-@pytest.mark.parametrize("force_exact", [False, True])
-def test_device_scanner_matches_glibc(host, orc, force_exact):
+# swar: the 8-characters-at-a-time path the value kernel tries first
+@pytest.mark.parametrize("force_exact,swar", [(False, False), (True, False), (False, True), (True, True)])
+def test_device_scanner_matches_glibc(host, orc, force_exact, swar):
     rng = np.random.default_rng(22)
     R = random.Random(22)
-    check_scan(host, orc, orc.format_values(random_doubles(rng, 50_000), True), True, force_exact)
-    check_scan(host, orc, near_midpoints(rng, 6000), True, force_exact)
+    check_scan(host, orc, orc.format_values(random_doubles(rng, 50_000), True), True, force_exact, swar)
+    check_scan(host, orc, near_midpoints(rng, 6000), True, force_exact, swar)
     strs = []
     for _ in range(30_000):
         digs = "".join(R.choice("0123456789") for _ in range(R.randint(1, 30)))
@@ -204,7 +205,7 @@ def test_device_scanner_matches_glibc(host, orc, force_exact):
         if R.random() < 0.6:
             s += R.choice("pP") + R.choice(["", "+", "-"]) + str(R.randint(0, 1200))
         strs.append((R.choice(["", "-"]) + s).encode())
-    check_scan(host, orc, strs, True, force_exact)
+    check_scan(host, orc, strs, True, force_exact, swar)
 
 
 def test_device_scanner_matches_glibc_on_malformed_text(host, orc):
@@ -214,6 +215,7 @@ def test_device_scanner_matches_glibc_on_malformed_text(host, orc):
     strs += [b"1e,", b"1e+,", b"0x", b"0x.", b"0x.p1", b"0xp3", b"0x1p", b"infinit,", b"infx", b"nan(1)", b".",
              b"-.", b"-", b"+,", b"1..", b"1ee5", b"--1", b" ", b""]
     check_scan(host, orc, strs, True)
+    check_scan(host, orc, strs, True, swar=True)
     ints = ["".join(R.choice("0123456789+- x.,\t") for _ in range(R.randint(1, 25))).encode()
             for _ in range(30_000)]
     ints += [b"99999999999999999999", b"-99999999999999999999", b"9223372036854775808",
