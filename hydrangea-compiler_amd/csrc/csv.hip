@@ -559,10 +559,6 @@ __device__ __forceinline__ int elem_len(const uint64_t* vals, const Dig* dig, in
     return i64_len((int64_t)vals[i]);
 }
 
-// Row position of element i: ',' after it, or '\n' at the end of a row.
-__device__ __forceinline__ char sep_after(int64_t i, int64_t row_len) {
-    return (i % row_len) == row_len - 1 ? '\n' : ',';
-}
 
 template <bool FLOAT>
 __global__ __launch_bounds__(kFThreads) void fmt_len_kernel(const uint64_t* __restrict__ vals,
@@ -601,6 +597,7 @@ __global__ __launch_bounds__(kFThreads) void fmt_write_kernel(const uint64_t* __
     const int lead = (int)((reinterpret_cast<uintptr_t>(text) + (uintptr_t)gbase) & 15);
     auto put = [&](int64_t pos, char ch) { stage[lead + pos] = (unsigned char)ch; };
     int64_t pos = off;
+    int64_t col = i0 < n ? i0 % row_len : 0;  // row position: ',' after an element, '\n' after a row's last
 #pragma unroll
     for (int k = 0; k < kFPer; ++k) {
         const int64_t i = i0 + k;
@@ -612,7 +609,8 @@ __global__ __launch_bounds__(kFThreads) void fmt_write_kernel(const uint64_t* __
             } else {
                 i64_write((int64_t)vals[i], put, pos);
             }
-            put(pos + lens[k] - 1, sep_after(i, row_len));
+            put(pos + lens[k] - 1, col == row_len - 1 ? '\n' : ',');
+            col = col == row_len - 1 ? 0 : col + 1;
             pos += lens[k];
         }
     }

@@ -816,24 +816,45 @@ HYD_HD int i64_write(int64_t v, const Put& put, int64_t pos) {
     return len;
 }
 
-// Finite nonzero value: N (17 digits) and X as from f64_to_dec17.
+// 8 ASCII digits of v < 10^8, the most significant in the lowest byte: the two
+// 4-digit halves, then their 2-digit halves, then single digits, each split
+// by a multiply-shift division in parallel 32- / 16-bit lanes (exact for the
+// ranges involved; checked for every v < 10^8).
+HYD_HD uint64_t ascii8(uint32_t v) {
+    uint64_t x = (uint64_t)(v / 10000) | ((uint64_t)(v % 10000) << 32);
+    const uint64_t y = ((x * 10486) >> 20) & 0x0000007F0000007FULL;  // x / 100 per lane (x < 10^4)
+    x = y | ((x - y * 100) << 16);
+    const uint64_t z = ((x * 103) >> 10) & 0x000F000F000F000FULL;    // x / 10 per lane (x < 100)
+    x = z | ((x - z * 10) << 8);
+    return x | 0x3030303030303030ULL;
+}
+
+// A finite nonzero value's 17 digits N (from f64_to_dec17) and exponent X:
+// digit 0, then digits 1..8 and 9..16 as ASCII words; nd = digits left once
+// the trailing zeros are dropped.
 struct G17 {
-    uint64_t m;  // N without its trailing zeros
-    int nd;      // digits of m
+    uint64_t a0, a1;  // ASCII digits 1..8 and 9..16 (digit 1 in the low byte)
+    int d0;           // leading digit, 1..9
+    int nd;
     int X;
     bool neg;
 };
+
+HYD_HD int trailing_zero_digits(uint64_t a) {  // '0' bytes at the top end of an ASCII word
+    const uint32_t other = ~eq_mask(a, '0') & 0xFF;
+    return other ? __builtin_clz(other) - 24 : 8;
+}
 
 HYD_HD G17 g17_make(bool neg, uint64_t N, int X) {
     G17 g;
     g.neg = neg;
     g.X = X;
-    int tz = 0;
-    while (N % 10 == 0) {
-        N /= 10;
-        ++tz;
-    }
-    g.m = N;
+    g.d0 = (int)(N / kTen16);
+    const uint64_t r = N - (uint64_t)g.d0 * kTen16;
+    const uint32_t hi = (uint32_t)(r / 100000000ULL), lo = (uint32_t)(r - (uint64_t)hi * 100000000ULL);
+    g.a0 = ascii8(hi);
+    g.a1 = ascii8(lo);
+    const int tz = lo ? trailing_zero_digits(g.a1) : (hi ? 8 + trailing_zero_digits(g.a0) : 16);
     g.nd = 17 - tz;
     return g;
 }
@@ -850,55 +871,50 @@ HYD_HD int g17_len(const G17& g) {
     return s + 2 + (-g.X - 1) + g.nd;
 }
 
+HYD_HD char g17_digit(const G17& g, int j) {  // j is a constant after unrolling
+    if (j == 0) return (char)('0' + g.d0);
+    return (char)((j <= 8 ? g.a0 >> (8 * (j - 1)) : g.a1 >> (8 * (j - 9))) & 0xFF);
+}
+
 template <class Put>
 HYD_HD int g17_write(const G17& g, const Put& put, int64_t pos) {
     const int len = g17_len(g);
-    int64_t q = pos + len - 1;  // written backwards
-    uint64_t m = g.m;
-    if (g17_exp_form(g.X)) {
-        int ax = g.X < 0 ? -g.X : g.X;
-        const int ed = ax >= 100 ? 3 : 2;
-        for (int i = 0; i < ed; ++i) {
-            put(q--, (char)('0' + ax % 10));
-            ax /= 10;
-        }
-        put(q--, g.X < 0 ? '-' : '+');
-        put(q--, 'e');
-        for (int i = g.nd - 1; i >= 1; --i) {
-            put(q--, (char)('0' + m % 10));
-            m /= 10;
-        }
-        if (g.nd > 1) put(q--, '.');
-        put(q--, (char)('0' + m % 10));
+    const int s = g.neg ? 1 : 0;
+    if (g.neg) put(pos, '-');
+    const bool ef = g17_exp_form(g.X);
+    // digit j goes to pos + s + j + gap(j); digits written: nd, or the whole
+    // integer part of the %f form when it is longer (its tail digits are 0s)
+    int ndw, lead, dot_after;  // dot after digit dot_after (-1: none); lead: chars before digit 0
+    if (ef) {
+        ndw = g.nd;
+        lead = 0;
+        dot_after = g.nd > 1 ? 0 : -1;
     } else if (g.X >= 0) {
-        const int ip = g.X + 1;  // integer digits
-        if (g.nd > ip) {
-            for (int i = g.nd - 1; i >= ip; --i) {
-                put(q--, (char)('0' + m % 10));
-                m /= 10;
-            }
-            put(q--, '.');
-            for (int i = ip - 1; i >= 0; --i) {
-                put(q--, (char)('0' + m % 10));
-                m /= 10;
-            }
-        } else {
-            for (int i = ip - 1; i >= g.nd; --i) put(q--, '0');
-            for (int i = g.nd - 1; i >= 0; --i) {
-                put(q--, (char)('0' + m % 10));
-                m /= 10;
-            }
-        }
+        ndw = g.nd > g.X + 1 ? g.nd : g.X + 1;
+        lead = 0;
+        dot_after = g.nd > g.X + 1 ? g.X : -1;
     } else {
-        for (int i = g.nd - 1; i >= 0; --i) {
-            put(q--, (char)('0' + m % 10));
-            m /= 10;
-        }
-        for (int i = 0; i < -g.X - 1; ++i) put(q--, '0');
-        put(q--, '.');
-        put(q--, '0');
+        ndw = g.nd;
+        lead = 1 - g.X;  // "0." and -X-1 zeros
+        put(pos + s, '0');
+        put(pos + s + 1, '.');
+        for (int i = 0; i < -g.X - 1; ++i) put(pos + s + 2 + i, '0');
+        dot_after = -1;
     }
-    if (g.neg) put(q--, '-');
+    const int64_t d = pos + s + lead;
+#pragma unroll
+    for (int j = 0; j < 17; ++j)
+        if (j < ndw) put(d + j + (dot_after >= 0 && j > dot_after ? 1 : 0), g17_digit(g, j));
+    if (dot_after >= 0) put(d + dot_after + 1, '.');
+    if (ef) {
+        int64_t q = d + ndw + (dot_after >= 0 ? 1 : 0);
+        put(q++, 'e');
+        put(q++, g.X < 0 ? '-' : '+');
+        const int ax = g.X < 0 ? -g.X : g.X;
+        if (ax >= 100) put(q++, (char)('0' + ax / 100));
+        put(q++, (char)('0' + (ax / 10) % 10));
+        put(q, (char)('0' + ax % 10));
+    }
     return len;
 }
 
