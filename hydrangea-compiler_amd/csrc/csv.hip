@@ -502,92 +502,124 @@ struct Dig {
     int32_t flags;  // 1 = special (zero, inf, nan), 2 = negative
 };
 
-__global__ __launch_bounds__(256) void fmt_digits_kernel(const uint64_t* __restrict__ vals, int64_t n,
-                                                         Dig* __restrict__ dig, uint32_t* __restrict__ exact_bits,
-                                                         unsigned long long* __restrict__ exact_cnt) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t b = vals[i];
-        Dig d;
-        if (special_len(b)) {
-            d.N = b;
-            d.X = 0;
-            d.flags = 1;
-        } else {
-            uint64_t N;
-            int X;
-            if (!f64_to_dec17(b & ~kSignBit, &N, &X)) {
-                atomicOr(&exact_bits[i >> 5], 1u << (i & 31));
-                atomicAdd(exact_cnt, 1ULL);
-            }
-            d.N = N;
-            d.X = X;
-            d.flags = (b >> 63) ? 2 : 0;
-        }
-        dig[i] = d;
+// The %.17g digits of value b by the table path; *decided = false when the
+// exact kernel must settle them (N, X then hold its starting point).  FORCE
+// (a test mode) leaves every finite nonzero value to the exact kernel.
+template <bool FORCE = false>
+__device__ __forceinline__ Dig table_digits(uint64_t b, bool* decided) {
+    Dig d;
+    *decided = true;
+    if (special_len(b)) {
+        d.N = b;
+        d.X = 0;
+        d.flags = 1;
+        return d;
     }
+    uint64_t N;
+    int X;
+    *decided = f64_to_dec17<FORCE>(b & ~kSignBit, &N, &X);
+    d.N = N;
+    d.X = X;
+    d.flags = (b >> 63) ? 2 : 0;
+    return d;
 }
 
-__global__ __launch_bounds__(64) void fmt_exact_kernel(const uint64_t* __restrict__ vals, int64_t n,
-                                                       Dig* __restrict__ dig, const uint32_t* __restrict__ exact_bits,
-                                                       const unsigned long long* __restrict__ exact_cnt) {
-    if (*exact_cnt == 0) return;
-    const int64_t words = (n + 31) >> 5;
-    for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words;
-         w += (int64_t)gridDim.x * blockDim.x) {
-        uint32_t b = exact_bits[w];
-        while (b) {
-            const int bit = __builtin_ctz(b);
-            b &= b - 1;
-            const int64_t i = w * 32 + bit;
-            Big D, L, R;
-            uint64_t N;
-            int X;
-            dec17_exact(vals[i] & ~kSignBit, dig[i].N, dig[i].X, &N, &X, D, L, R);
-            dig[i].N = N;
-            dig[i].X = X;
-        }
-    }
+__device__ __forceinline__ int dig_len(const Dig& d) {
+    return (d.flags & 1) ? special_len(d.N) : g17_len(g17_make(d.flags & 2, d.N, d.X));
 }
 
-template <bool FLOAT>
-__device__ __forceinline__ int elem_len(const uint64_t* vals, const Dig* dig, int64_t i) {
-    if (FLOAT) {
-        const Dig d = dig[i];
-        if (d.flags & 1) return special_len(d.N);
-        return g17_len(g17_make(d.flags & 2, d.N, d.X));
-    }
-    return i64_len((int64_t)vals[i]);
-}
-
-
-template <bool FLOAT>
-__global__ __launch_bounds__(kFThreads) void fmt_len_kernel(const uint64_t* __restrict__ vals,
-                                                            const Dig* __restrict__ dig, int64_t n,
-                                                            int64_t* __restrict__ block_len) {
+// Text length of every 1024-value block (with separators).  fp64 values the
+// table path cannot round with certainty are marked (their provisional length
+// is corrected by fmt_exact_kernel).
+template <bool FLOAT, bool FORCE>
+__global__ __launch_bounds__(kFThreads) void fmt_len_kernel(const uint64_t* __restrict__ vals, int64_t n,
+                                                            int64_t* __restrict__ block_len,
+                                                            uint32_t* __restrict__ exact_bits,
+                                                            unsigned long long* __restrict__ exact_cnt) {
     __shared__ int64_t sh[kWaves];
     const int64_t i0 = (int64_t)blockIdx.x * kFBlock + threadIdx.x * kFPer;
     int64_t len = 0;
 #pragma unroll
-    for (int k = 0; k < kFPer; ++k)
-        if (i0 + k < n) len += elem_len<FLOAT>(vals, dig, i0 + k) + 1;
+    for (int k = 0; k < kFPer; ++k) {
+        const int64_t i = i0 + k;
+        if (i >= n) continue;
+        if (FLOAT) {
+            bool decided;
+            const Dig d = table_digits<FORCE>(vals[i], &decided);
+            len += dig_len(d) + 1;
+            if (!decided) {
+                atomicOr(&exact_bits[i >> 5], 1u << (i & 31));
+                atomicAdd(exact_cnt, 1ULL);
+            }
+        } else {
+            len += i64_len((int64_t)vals[i]) + 1;
+        }
+    }
     int64_t total;
     block_excl_scan(len, &total, sh);
     if (threadIdx.x == 0) block_len[blockIdx.x] = total;
 }
 
+// Settles the marked values exactly: their digits go to exact_dig[i] (a
+// sparse table: only marked entries are written or read) and their block's
+// length is corrected.
+__global__ __launch_bounds__(64) void fmt_exact_kernel(const uint64_t* __restrict__ vals, int64_t n,
+                                                       Dig* __restrict__ exact_dig,
+                                                       const uint32_t* __restrict__ exact_bits,
+                                                       const unsigned long long* __restrict__ exact_cnt,
+                                                       int64_t* __restrict__ block_len) {
+    if (*exact_cnt == 0) return;
+    const int64_t words = (n + 31) >> 5;
+    for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words;
+         w += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t bm = exact_bits[w];
+        while (bm) {
+            const int bit = __builtin_ctz(bm);
+            bm &= bm - 1;
+            const int64_t i = w * 32 + bit;
+            bool decided;
+            Dig d = table_digits<true>(vals[i], &decided);  // the starting point the length kernel counted
+            const int before = dig_len(d);
+            Big D, L, R;
+            uint64_t N;
+            int X;
+            dec17_exact(vals[i] & ~kSignBit, d.N, d.X, &N, &X, D, L, R);
+            d.N = N;
+            d.X = X;
+            exact_dig[i] = d;
+            atomicAdd(reinterpret_cast<unsigned long long*>(&block_len[i / kFBlock]),
+                      (unsigned long long)(int64_t)(dig_len(d) - before));
+        }
+    }
+}
+
 template <bool FLOAT>
-__global__ __launch_bounds__(kFThreads) void fmt_write_kernel(const uint64_t* __restrict__ vals,
-                                                              const Dig* __restrict__ dig, int64_t n, int64_t row_len,
-                                                              const int64_t* __restrict__ block_off,
-                                                              unsigned char* __restrict__ text) {
+__global__ __launch_bounds__(kFThreads) void fmt_write_kernel(const uint64_t* __restrict__ vals, int64_t n,
+                                                              int64_t row_len, const int64_t* __restrict__ block_off,
+                                                              unsigned char* __restrict__ text,
+                                                              const Dig* __restrict__ exact_dig,
+                                                              const uint32_t* __restrict__ exact_bits,
+                                                              const unsigned long long* __restrict__ exact_cnt) {
     __shared__ int64_t sh[kWaves];
     __shared__ __attribute__((aligned(16))) unsigned char stage[kStage];
     const int64_t i0 = (int64_t)blockIdx.x * kFBlock + threadIdx.x * kFPer;
+    const bool any_exact = FLOAT && *exact_cnt != 0;
+    Dig dg[kFPer];
     int lens[kFPer];
     int64_t len = 0;
 #pragma unroll
     for (int k = 0; k < kFPer; ++k) {
-        lens[k] = i0 + k < n ? elem_len<FLOAT>(vals, dig, i0 + k) + 1 : 0;
+        const int64_t i = i0 + k;
+        lens[k] = 0;
+        if (i >= n) continue;
+        if (FLOAT) {
+            bool decided;
+            dg[k] = table_digits(vals[i], &decided);
+            if (any_exact && ((exact_bits[i >> 5] >> (i & 31)) & 1)) dg[k] = exact_dig[i];
+            lens[k] = dig_len(dg[k]) + 1;
+        } else {
+            lens[k] = i64_len((int64_t)vals[i]) + 1;
+        }
         len += lens[k];
     }
     int64_t total;
@@ -603,9 +635,8 @@ __global__ __launch_bounds__(kFThreads) void fmt_write_kernel(const uint64_t* __
         const int64_t i = i0 + k;
         if (i < n) {
             if (FLOAT) {
-                const Dig d = dig[i];
-                if (d.flags & 1) special_write(d.N, put, pos);
-                else g17_write(g17_make(d.flags & 2, d.N, d.X), put, pos);
+                if (dg[k].flags & 1) special_write(dg[k].N, put, pos);
+                else g17_write(g17_make(dg[k].flags & 2, dg[k].N, dg[k].X), put, pos);
             } else {
                 i64_write((int64_t)vals[i], put, pos);
             }
@@ -632,6 +663,10 @@ __global__ __launch_bounds__(kFThreads) void fmt_write_kernel(const uint64_t* __
 }
 
 int64_t bitmap_words64(int64_t count) { return ((count + 31) / 32 + 1) / 2 + 1; }
+
+// Test mode (hydhip_csv_test_force_exact): every fp64 value is formatted by
+// the exact kernel.
+int g_force_exact = 0;
 
 }  // namespace
 
@@ -731,16 +766,14 @@ int csv_format(const uint64_t* vals, int64_t count, int64_t row_len, bool is_flo
     uint32_t* exact_bits = reinterpret_cast<uint32_t*>(misc + 8);
     Dig* dig = reinterpret_cast<Dig*>(misc + 8 + bw);
     const unsigned g = (unsigned)blocks;
+    HYD_TRY(fill_zero(misc, (size_t)(8 + bw) * 8, s));
     if (is_float) {
-        HYD_TRY(fill_zero(misc, (size_t)(8 + bw) * 8, s));
-        const int64_t dg = std::min<int64_t>(ceil_div(count, 256), (int64_t)device_info().num_cus * 64);
-        fmt_digits_kernel<<<(unsigned)dg, 256, 0, s>>>(vals, count, dig, exact_bits, exact_cnt);
+        if (g_force_exact) fmt_len_kernel<true, true><<<g, kFThreads, 0, s>>>(vals, count, block_len, exact_bits, exact_cnt);
+        else fmt_len_kernel<true, false><<<g, kFThreads, 0, s>>>(vals, count, block_len, exact_bits, exact_cnt);
         HYD_CHECK_LAUNCH();
-        fmt_exact_kernel<<<64, 64, 0, s>>>(vals, count, dig, exact_bits, exact_cnt);
-        HYD_CHECK_LAUNCH();
-        fmt_len_kernel<true><<<g, kFThreads, 0, s>>>(vals, dig, count, block_len);
+        fmt_exact_kernel<<<64, 64, 0, s>>>(vals, count, dig, exact_bits, exact_cnt, block_len);
     } else {
-        fmt_len_kernel<false><<<g, kFThreads, 0, s>>>(vals, dig, count, block_len);
+        fmt_len_kernel<false, false><<<g, kFThreads, 0, s>>>(vals, count, block_len, exact_bits, exact_cnt);
     }
     HYD_CHECK_LAUNCH();
     HYD_TRY(scan_i64(block_len, blocks, 0, block_off, false, false, s, scan_ws));
@@ -754,8 +787,11 @@ int csv_format(const uint64_t* vals, int64_t count, int64_t row_len, bool is_flo
     if (total > capacity)
         return set_error(HYD_E_INVALID, "csv: text needs %lld bytes, capacity %lld", (long long)total,
                          (long long)capacity);
-    if (is_float) fmt_write_kernel<true><<<g, kFThreads, 0, s>>>(vals, dig, count, row_len, block_off, text);
-    else fmt_write_kernel<false><<<g, kFThreads, 0, s>>>(vals, dig, count, row_len, block_off, text);
+    if (is_float)
+        fmt_write_kernel<true><<<g, kFThreads, 0, s>>>(vals, count, row_len, block_off, text, dig, exact_bits, exact_cnt);
+    else
+        fmt_write_kernel<false><<<g, kFThreads, 0, s>>>(vals, count, row_len, block_off, text, dig, exact_bits,
+                                                        exact_cnt);
     HYD_CHECK_LAUNCH();
     *nbytes = total;
     return HYD_OK;
@@ -981,6 +1017,10 @@ extern "C" int64_t hyd_write_array_csv(hyd_array_t* arr, const char* filename) {
 extern "C" int64_t hyd_write_array_csv_float(hyd_array_t* arr, const char* filename) {
     return write_csv_file(arr, filename, true);
 }
+
+// Test hook: 1 = format every fp64 value through the exact kernel (the path
+// the table leaves undecided values to, which real values almost never take).
+extern "C" void hydhip_csv_test_force_exact(int on) { g_force_exact = on ? 1 : 0; }
 
 // ---- CPU test hooks: the same conversions, compiled for the host ------------
 // fscanf-like: skips isspace, then reads one number from s[0, len).  Returns 1

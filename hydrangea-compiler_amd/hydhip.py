@@ -174,6 +174,7 @@ _SIGS = {
     "hydhip_host_csv_scan": (_INT, [ctypes.c_char_p, _I64, _INT, _INT, _P, ctypes.POINTER(_I64),
                                     ctypes.POINTER(_INT)]),
     "hydhip_host_csv_format": (_I64, [_P, _I64, _INT, _INT, _P, _I64, _P]),
+    "hydhip_csv_test_force_exact": (None, [_INT]),
     # serving
     "hyd_server_create": (_P, [_I64, _INT, _INT]),
     "hyd_server_destroy": (None, [_P]),

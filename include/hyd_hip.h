@@ -278,6 +278,10 @@ int hydhip_csv_format_f64(const double* vals, int64_t count, int64_t row_len,
  * format: text of each value at out + k * stride (no separator), lengths in
  * lens; returns how many fp64 values took the big-integer path.
  * flags & 1 sends every fp64 value through the big-integer path. */
+/* Test hook: on = 1 formats every fp64 value through the exact (big-integer)
+ * kernel, which real values almost never need, so that its GPU path can be
+ * checked; on = 0 restores the normal path.  Not for production use. */
+void hydhip_csv_test_force_exact(int on);
 int hydhip_host_csv_scan(const char* s, int64_t len, int is_float, int flags,
                          void* value, int64_t* consumed, int* exact);
 int64_t hydhip_host_csv_format(const void* vals, int64_t n, int is_float,

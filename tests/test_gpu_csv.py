@@ -176,6 +176,23 @@ def test_format_matches_reference_writer(hyd, orc, n, rows):
     assert hyd.csv_format(di, n // rows) == orc.csv_write(i, rows, n // rows, False)
 
 
+def test_format_exact_kernel_on_gpu(hyd, orc):
+    # every value through the exact kernel (sparse digit table, block length
+    # correction): same bytes as the reference writer
+    rng = np.random.default_rng(77)
+    v = random_doubles(rng, 300_000)
+    v[:6] = [0.0, -0.0, np.inf, 5e-324, 2.0**-25, 1e23]
+    d = hyd.DeviceArray.from_numpy(v)
+    hyd.lib().hydhip_csv_test_force_exact(1)
+    try:
+        got, exact = hyd.csv_format(d, 1000, with_exact=True)
+    finally:
+        hyd.lib().hydhip_csv_test_force_exact(0)
+    assert exact == np.count_nonzero(np.isfinite(v) & (v != 0))
+    assert got == orc.csv_write(v, 300, 1000, True)
+    assert hyd.csv_format(d, 1000) == got
+
+
 def test_format_known_answer(hyd):
     # test/Language/Hydrangea/EndToEndSpec.hs:241-259
     d = hyd.DeviceArray.from_numpy(np.arange(6, dtype=np.float64) + 0.5)
