@@ -168,6 +168,35 @@ def bench_guarded(steps, warmup):
     return res
 
 
+def csv_file_round_trip(x, n):
+    """The drop-in file functions end to end on a device-resident array of n
+    fp64 values: hyd_write_array_csv_float (format on the GPU, text back over
+    PCIe through pinned buffers, fwrite) and hyd_read_float_array_csv (fread
+    into pinned buffers, copy to HBM, parse, copy the values to arr->data as
+    the reference's callers expect).  The file lives in /dev/shm when the host
+    has it (page cache only, no disk)."""
+    import tempfile
+
+    L = hydhip.lib()
+    d = "/dev/shm" if os.path.isdir("/dev/shm") else tempfile.gettempdir()
+    path = os.path.join(d, "hyd_bench_%d.csv" % os.getpid()).encode()
+    arr = L.hyd_array_alloc_device(hydhip.hyd_tuple(n), 8)
+    hydhip._check(L.hydhip_memcpy_d2d(L.hyd_array_device_data(arr), x.ptr, 8 * n, None))
+    hydhip.device_sync()
+    t0 = time.perf_counter()
+    L.hyd_write_array_csv_float(arr, path)
+    t1 = time.perf_counter()
+    back = L.hyd_read_float_array_csv(path, hydhip.hyd_tuple(n))
+    t2 = time.perf_counter()
+    size = os.path.getsize(path)
+    same = bool((hydhip.read_array(back, np.float64).view(np.uint64) == x.numpy().view(np.uint64)).all())
+    os.remove(path)
+    L.hyd_array_free(arr)
+    L.hyd_array_free(back)
+    return dict(where=d, file_bytes=size, write_s=t1 - t0, read_s=t2 - t1, write_gbs=size / (t1 - t0) / 1e9,
+                read_gbs=size / (t2 - t1) / 1e9, values_back_identical=same)
+
+
 def bench_csv(steps, warmup, cpu=True):
     """CSV text <-> arrays in HBM, as read_array_float / write_array_float and
     read_array / write_array read and write files (runtime/hydrangea_runtime.h:
@@ -218,6 +247,8 @@ def bench_csv(steps, warmup, cpu=True):
         r = dict(values=n, text_bytes=t, parsed_all=bool(ok), exact_path=int(st.exact_path),
                  format_ms=fms, parse_ms=pms, format_gvalues_s=n / fms / 1e6, parse_gvalues_s=n / pms / 1e6,
                  format_hbm_gbs=(8 * n + t) / fms / 1e6, parse_hbm_gbs=(t + 8 * n) / pms / 1e6)
+        if is_float:
+            r["file"] = csv_file_round_trip(x, n)
         if cpu:
             k = 1 << 20
             xs = (pyoracle.splitmix_f64(k, seed, 0, symmetric=True) if is_float

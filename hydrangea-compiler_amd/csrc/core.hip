@@ -176,8 +176,11 @@ static void release_staging() {
     g_stage.clear();
 }
 
+void release_csv_buffers();  // csv.hip
+
 static void release_workspaces() {
     release_staging();
+    release_csv_buffers();
     std::lock_guard<std::mutex> lk(g_ws_mu);
     for (auto& kv : g_ws)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);

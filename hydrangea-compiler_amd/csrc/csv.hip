@@ -32,9 +32,16 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <errno.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "../../include/hyd_hip_runtime.h"
 #include "csv_num.h"
@@ -797,47 +804,116 @@ int csv_format(const uint64_t* vals, int64_t count, int64_t row_len, bool is_flo
     return HYD_OK;
 }
 
-// ---- file I/O (pinned double buffers) -------------------------------------
+// ---- file I/O (pinned double buffers, parallel pread) ----------------------
+// A file's bytes move through two pinned halves: while one half is on PCIe,
+// the host fills (read: several threads' pread of disjoint pieces, since one
+// thread copying from the page cache is far slower than the link) or drains
+// (write: pwrite) the other.
 namespace {
 
-constexpr size_t kIoChunk = (size_t)64 << 20;
+constexpr size_t kIoHalf = (size_t)256 << 20;
 
-struct D2HBuffers {
+struct PinnedPair {
     std::mutex mu;
-    void* host[2] = {nullptr, nullptr};
+    unsigned char* host[2] = {nullptr, nullptr};
     hipEvent_t ev[2] = {nullptr, nullptr};
+    bool pending[2] = {false, false};
 };
-D2HBuffers g_d2h;
+PinnedPair g_io;
+
+int io_buffers() {
+    for (int k = 0; k < 2; ++k) {
+        if (!g_io.host[k]) HYD_TRY_HIP(hipHostMalloc((void**)&g_io.host[k], kIoHalf, hipHostMallocDefault));
+        if (!g_io.ev[k]) HYD_TRY_HIP(hipEventCreateWithFlags(&g_io.ev[k], hipEventDisableTiming));
+    }
+    return HYD_OK;
+}
+
+int io_threads() {
+    const unsigned hw = std::thread::hardware_concurrency();
+    return hw == 0 ? 4 : (hw > 16 ? 16 : (int)hw);
+}
+
+// buf[0, len) <-> file bytes [off, off + len): reads by io_threads()
+// threads; writes by one (the kernel serialises writers of one file on its
+// inode lock, so more threads only add contention).
+bool parallel_io(int fd, unsigned char* buf, size_t len, off_t off, bool write) {
+    const int nt = (write || len < ((size_t)4 << 20)) ? 1 : io_threads();
+    std::atomic<bool> ok{true};
+    auto piece = [&](int t) {
+        const size_t a = len * t / nt, e = len * (t + 1) / nt;
+        size_t done = a;
+        while (done < e && ok.load()) {
+            const ssize_t r = write ? pwrite(fd, buf + done, e - done, off + (off_t)done)
+                                    : pread(fd, buf + done, e - done, off + (off_t)done);
+            if (r > 0) done += (size_t)r;
+            else if (r < 0 && errno == EINTR) continue;
+            else ok = false;  // error, or the file ended early
+        }
+    };
+    if (nt == 1) {
+        piece(0);
+    } else {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t) th.emplace_back(piece, t);
+        for (auto& t : th) t.join();
+    }
+    return ok.load();
+}
 
 [[noreturn]] void die_hip(const char* what) {
     fprintf(stderr, "%s: %s\n", what, hydhip_last_error());
     exit(1);
 }
 
-// Streams dev[0, bytes) to f through two pinned buffers (the copy of chunk
-// i + 1 overlaps the fwrite of chunk i).
-int d2h_write(FILE* f, const unsigned char* dev, int64_t bytes, hipStream_t s) {
-    std::lock_guard<std::mutex> lk(g_d2h.mu);
-    for (int k = 0; k < 2; ++k) {
-        if (!g_d2h.host[k]) HYD_TRY_HIP(hipHostMalloc(&g_d2h.host[k], kIoChunk, hipHostMallocDefault));
-        if (!g_d2h.ev[k]) HYD_TRY_HIP(hipEventCreateWithFlags(&g_d2h.ev[k], hipEventDisableTiming));
+// file bytes [0, nbytes) -> dev; the copies are queued on s.
+int file_to_device(int fd, unsigned char* dev, int64_t nbytes, hipStream_t s, bool* short_read) {
+    std::lock_guard<std::mutex> lk(g_io.mu);
+    HYD_TRY(io_buffers());
+    *short_read = false;
+    int64_t c = 0;
+    for (int64_t off = 0; off < nbytes; off += (int64_t)kIoHalf, ++c) {
+        const int k = (int)(c & 1);
+        const size_t len = (size_t)std::min<int64_t>((int64_t)kIoHalf, nbytes - off);
+        if (g_io.pending[k]) HYD_TRY_HIP(hipEventSynchronize(g_io.ev[k]));
+        g_io.pending[k] = false;
+        if (!parallel_io(fd, g_io.host[k], len, (off_t)off, false)) {
+            *short_read = true;
+            break;
+        }
+        HYD_TRY_HIP(hipMemcpyAsync(dev + off, g_io.host[k], len, hipMemcpyHostToDevice, s));
+        HYD_TRY_HIP(hipEventRecord(g_io.ev[k], s));
+        g_io.pending[k] = true;
     }
-    const int64_t chunks = ceil_div(bytes, (int64_t)kIoChunk);
+    for (int k = 0; k < 2; ++k)
+        if (g_io.pending[k]) {
+            HYD_TRY_HIP(hipEventSynchronize(g_io.ev[k]));
+            g_io.pending[k] = false;
+        }
+    return HYD_OK;
+}
+
+// dev[0, bytes) -> file bytes [0, bytes): the copy of half i + 1 overlaps the
+// pwrite of half i.
+int device_to_file(int fd, const unsigned char* dev, int64_t bytes, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_io.mu);
+    HYD_TRY(io_buffers());
+    const int64_t halves = ceil_div(bytes, (int64_t)kIoHalf);
     auto issue = [&](int64_t c) -> int {
-        const int64_t off = c * (int64_t)kIoChunk;
-        const int64_t len = std::min<int64_t>((int64_t)kIoChunk, bytes - off);
-        HYD_TRY_HIP(hipMemcpyAsync(g_d2h.host[c & 1], dev + off, (size_t)len, hipMemcpyDeviceToHost, s));
-        HYD_TRY_HIP(hipEventRecord(g_d2h.ev[c & 1], s));
+        const int64_t off = c * (int64_t)kIoHalf;
+        const int64_t len = std::min<int64_t>((int64_t)kIoHalf, bytes - off);
+        HYD_TRY_HIP(hipMemcpyAsync(g_io.host[c & 1], dev + off, (size_t)len, hipMemcpyDeviceToHost, s));
+        HYD_TRY_HIP(hipEventRecord(g_io.ev[c & 1], s));
         return HYD_OK;
     };
-    if (chunks > 0) HYD_TRY(issue(0));
-    for (int64_t c = 0; c < chunks; ++c) {
-        if (c + 1 < chunks) HYD_TRY(issue(c + 1));
-        HYD_TRY_HIP(hipEventSynchronize(g_d2h.ev[c & 1]));
-        const int64_t off = c * (int64_t)kIoChunk;
-        const int64_t len = std::min<int64_t>((int64_t)kIoChunk, bytes - off);
-        if (fwrite(g_d2h.host[c & 1], 1, (size_t)len, f) != (size_t)len)
-            return set_error(HYD_E_INVALID, "csv: short write");
+    if (halves > 0) HYD_TRY(issue(0));
+    for (int64_t c = 0; c < halves; ++c) {
+        if (c + 1 < halves) HYD_TRY(issue(c + 1));  // its buffer's pwrite (half c - 1) has finished
+        HYD_TRY_HIP(hipEventSynchronize(g_io.ev[c & 1]));
+        const int64_t off = c * (int64_t)kIoHalf;
+        const int64_t len = std::min<int64_t>((int64_t)kIoHalf, bytes - off);
+        if (!parallel_io(fd, g_io.host[c & 1], (size_t)len, (off_t)off, true))
+            return set_error(HYD_E_INVALID, "csv: write failed: %s", strerror(errno));
     }
     return HYD_OK;
 }
@@ -863,6 +939,30 @@ int text_buffer(size_t bytes, unsigned char** out) {
     return HYD_OK;
 }
 
+}  // namespace
+
+// Frees the CSV file buffers (pinned halves, HBM text buffer); from the
+// library's cleanup.
+void release_csv_buffers() {
+    {
+        std::lock_guard<std::mutex> lk(g_io.mu);
+        for (int k = 0; k < 2; ++k) {
+            if (g_io.pending[k] && g_io.ev[k]) (void)hipEventSynchronize(g_io.ev[k]);
+            if (g_io.ev[k]) (void)hipEventDestroy(g_io.ev[k]);
+            if (g_io.host[k]) (void)hipHostFree(g_io.host[k]);
+            g_io.ev[k] = nullptr;
+            g_io.host[k] = nullptr;
+            g_io.pending[k] = false;
+        }
+    }
+    std::lock_guard<std::mutex> lk(g_text.mu);
+    if (g_text.dev) (void)hipFree(g_text.dev);
+    g_text.dev = nullptr;
+    g_text.bytes = 0;
+}
+
+namespace {
+
 hyd_array_t* read_csv_file(const char* filename, hyd_tuple_t shape, bool is_float, const char* who) {
     if (ensure_init() != HYD_OK) die_hip(who);
     hyd_array_t* arr = hyd_array_alloc_device(shape, sizeof(int64_t));
@@ -876,25 +976,19 @@ hyd_array_t* read_csv_file(const char* filename, hyd_tuple_t shape, bool is_floa
     std::lock_guard<std::mutex> lk(g_text.mu);
     unsigned char* dev = nullptr;
     int64_t nbytes = 0;
-    off_t end = -1;
-    if (fseeko(f, 0, SEEK_END) == 0) end = ftello(f);
-    if (end >= 0 && fseeko(f, 0, SEEK_SET) == 0) {
-        // a regular file: read straight into pinned chunks on their way to HBM
-        nbytes = (int64_t)end;
+    struct stat sb;
+    const int fd = fileno(f);
+    if (fstat(fd, &sb) == 0 && S_ISREG(sb.st_mode)) {
+        nbytes = (int64_t)sb.st_size;
         if (text_buffer((size_t)nbytes + 16, &dev) != HYD_OK) die_hip(who);
-        for (int64_t off = 0; off < nbytes; off += (int64_t)kIoChunk) {
-            const size_t len = (size_t)std::min<int64_t>((int64_t)kIoChunk, nbytes - off);
-            size_t got = 0;
-            if (stage_h2d(s, dev + off, len, [&](unsigned char* b) { got = fread(b, 1, len, f); }) != HYD_OK)
-                die_hip(who);
-            if (got != len) {
-                fprintf(stderr, "%s: cannot read '%s'\n", who, filename);
-                exit(1);
-            }
+        bool short_read = false;
+        if (file_to_device(fd, dev, nbytes, s, &short_read) != HYD_OK) die_hip(who);
+        if (short_read) {
+            fprintf(stderr, "%s: cannot read '%s'\n", who, filename);
+            exit(1);
         }
     } else {
         // a pipe or similar: read it all, then copy
-        clearerr(f);
         std::string all;
         char tmp[1 << 16];
         size_t got;
@@ -945,7 +1039,7 @@ int64_t write_csv_file(hyd_array_t* arr, const char* filename, bool is_float) {
         int64_t nbytes = 0;
         if (text_buffer((size_t)cap, &dev) != HYD_OK ||
             csv_format(vals, size, row_len, is_float, dev, cap, &nbytes, nullptr, s) != HYD_OK ||
-            d2h_write(f, dev, nbytes, s) != HYD_OK)
+            device_to_file(fileno(f), dev, nbytes, s) != HYD_OK)
             die_hip("write_array");
     }
     fclose(f);
