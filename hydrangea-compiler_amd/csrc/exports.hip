@@ -8,6 +8,7 @@
 // the result back so arr->data is valid on return.
 #include <stdarg.h>
 #include <stdio.h>
+#include <time.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -207,6 +208,39 @@ hyd_array_t* hyd_sort_indices(hyd_array_t* keys) {
     }
     finish(out);
     return out;
+}
+
+// The benchmark scaffold the C backend emits in --bench mode
+// (src/Language/Hydrangea/CodegenC.hs:515-580): `warmup` untimed calls, then
+// `iters` calls each timed with CLOCK_MONOTONIC, then
+// "benchmark[<name>]: min=<ms> ms  mean=<ms> ms" on stderr.  Here the body is a
+// host callback around hyd_export_* calls, and each timed call also waits for
+// the GPU (the library stream and the device) so that kernels left in HBM
+// without host sync are inside the measurement.
+int64_t hyd_hip_bench(const char* name, void (*body)(void*), void* arg, int64_t warmup, int64_t iters,
+                      double* min_ms, double* mean_ms) {
+    if (!body || iters <= 0 || warmup < 0) {
+        fprintf(stderr, "hyd_hip_bench: bad arguments\n");
+        exit(1);
+    }
+    for (int64_t w = 0; w < warmup; ++w) body(arg);
+    (void)hipDeviceSynchronize();
+    double mn = 0.0, sum = 0.0;
+    for (int64_t i = 0; i < iters; ++i) {
+        struct timespec t0, t1;
+        clock_gettime(CLOCK_MONOTONIC, &t0);
+        body(arg);
+        (void)hipDeviceSynchronize();
+        clock_gettime(CLOCK_MONOTONIC, &t1);
+        const double ns = (double)(t1.tv_sec - t0.tv_sec) * 1.0e9 + (double)(t1.tv_nsec - t0.tv_nsec);
+        if (i == 0 || ns < mn) mn = ns;
+        sum += ns;
+    }
+    fprintf(stderr, "benchmark[%s]: min=%.3f ms  mean=%.3f ms\n", name ? name : "", mn / 1.0e6,
+            sum / (double)iters / 1.0e6);
+    if (min_ms) *min_ms = mn / 1.0e6;
+    if (mean_ms) *mean_ms = sum / (double)iters / 1.0e6;
+    return 0;
 }
 
 }  // extern "C"

@@ -57,6 +57,14 @@ hyd_record_col_idx_ai_ncols_i_nnz_i_nrows_i_row_ptr_ai_vals_ai_t
 hyd_export_coo_to_csr(int64_t nrows, int64_t ncols, int64_t nnz, hyd_array_t* rows,
                       hyd_array_t* cols, hyd_array_t* vals);
 
+/* The --bench scaffold of the C backend (src/Language/Hydrangea/CodegenC.hs:
+ * 515-580): warmup untimed calls of body(arg), then iters timed calls (each
+ * including a wait for the GPU), then "benchmark[<name>]: min=%.3f ms
+ * mean=%.3f ms" on stderr as the emitted main prints it.  min/mean (ms) are
+ * also returned through the pointers when not NULL. */
+int64_t hyd_hip_bench(const char* name, void (*body)(void*), void* arg, int64_t warmup,
+                      int64_t iters, double* min_ms, double* mean_ms);
+
 #ifdef __cplusplus
 }
 #endif

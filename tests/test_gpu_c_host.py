@@ -22,8 +22,21 @@ HOST_C = r"""
 #include "hydrangea_runtime.h"
 #include "hyd_exports.h"
 
+static void laplacian_once(void* arg) {
+    hyd_array_t* r = hyd_export_laplacian((hyd_array_t*)arg);
+    hyd_array_free(r);
+}
+
 int main(int argc, char** argv) {
     if (argc < 4) return 2;
+    if (argv[3][0] == 'b') {  /* the --bench scaffold around the Laplacian */
+        hyd_array_t* input = hyd_read_float_array_csv(argv[1], hyd_tuple_make(2, (int64_t)64, (int64_t)64));
+        double mn = -1.0, mean = -1.0;
+        hyd_hip_bench("laplacian", laplacian_once, input, 2, 5, &mn, &mean);
+        printf("%s\n", (mn > 0.0 && mean >= mn) ? "ok" : "bad");
+        hyd_array_free(input);
+        return 0;
+    }
     if (argv[3][0] == 'i') {  /* read_array [n] and write it back */
         int64_t n = hyd_get_env_int("CSV_N");
         hyd_array_t* a = hyd_read_array_csv(argv[1], hyd_tuple_make(1, n));
@@ -103,3 +116,14 @@ def test_missing_file_exits_from_c(host_exe, tmp_path):
     p = run(host_exe, [missing, str(tmp_path / "o.csv"), "f"], {"STENCIL_H": "2", "STENCIL_W": "2"})
     assert p.returncode == 1
     assert "read_array_float: cannot open '%s'" % missing in p.stderr
+
+
+def test_bench_scaffold_from_c(orc, host_exe, tmp_path):
+    # CodegenC.hs:515-580: "benchmark[<name>]: min=%.3f ms  mean=%.3f ms" on stderr
+    import re
+    inp = tmp_path / "input.csv"
+    inp.write_bytes(orc.csv_write(np.arange(64 * 64, dtype=np.float64), 64, 64, True))
+    p = run(host_exe, [str(inp), str(tmp_path / "o.csv"), "b"])
+    assert p.returncode == 0, p.stderr
+    assert p.stdout.strip() == "ok"
+    assert re.search(r"^benchmark\[laplacian\]: min=\d+\.\d{3} ms  mean=\d+\.\d{3} ms$", p.stderr, re.M), p.stderr
