@@ -243,6 +243,118 @@ int64_t hyd_get_env_int(const char* name) {
     return result;
 }
 
+const char* hyd_get_env_string(const char* name) {
+    char* val = getenv(name);
+    if (!val) {
+        fprintf(stderr, "get_env_string: environment variable '%s' not set\n", name);
+        exit(1);
+    }
+    return val;
+}
+
+/* runtime/hyd_write_csv.c:61-72: the helpers the emitted C calls for
+ * get_env_int / get_env_string (Lowering.hs:1921-1930).  atoll, as there. */
+int64_t getenv_int(const char* name) {
+    char* val = getenv(name);
+    if (!val) {
+        fprintf(stderr, "environment variable %s not set\n", name);
+        exit(1);
+    }
+    return atoll(val);
+}
+
+const char* getenv_string(const char* name) {
+    char* val = getenv(name);
+    if (!val) {
+        fprintf(stderr, "environment variable %s not set\n", name);
+        exit(1);
+    }
+    return val;
+}
+
+/* ---- slices, element access, printing (runtime/hydrangea_runtime.h:277-290,
+ * 458-548): the host helpers emitted programs call.  Readers of arr->data
+ * first bring a device-resident array's bytes back. */
+hyd_tuple_t hyd_slice_shape(hyd_tuple_t src, hyd_tuple_t specs) {
+    hyd_tuple_t out;
+    memset(&out, 0, sizeof out);
+    out.ndims = src.ndims;
+    for (int i = 0; i < src.ndims && i < HYD_MAX_DIMS; i++) {
+        const int64_t start = 2 * i < HYD_MAX_DIMS ? specs.elems[2 * i] : -1;
+        const int64_t len = 2 * i + 1 < HYD_MAX_DIMS ? specs.elems[2 * i + 1] : -1;
+        out.elems[i] = (start == -1 && len == -1) ? src.elems[i] : len;  /* (-1, -1): the whole axis */
+    }
+    return out;
+}
+
+static void host_view(hyd_array_t* arr) {
+    if (hyd_array_sync_host(arr) != HYD_OK) die("%s", hydhip_last_error());
+}
+
+double hyd_array_get_float(hyd_array_t* arr, int64_t idx) {
+    host_view(arr);
+    double v;
+    memcpy(&v, arr->data + (size_t)idx * arr->elem_size, sizeof v);
+    return v;
+}
+
+void hyd_array_set_float(hyd_array_t* arr, int64_t idx, double v) {
+    host_view(arr);
+    memcpy(arr->data + (size_t)idx * arr->elem_size, &v, sizeof v);
+    if (hyd_array_is_device_resident(arr)) hyd_array_mark_host_dirty(arr);  /* the host copy is newer now */
+}
+
+void hyd_print_int_inline(int64_t v) { printf("%lld", (long long)v); }
+void hyd_print_int(int64_t v) { printf("%lld\n", (long long)v); }
+void hyd_print_double_inline(double v) { printf("%.17g", v); }
+void hyd_print_double(double v) { printf("%.17g\n", v); }
+
+static void print_shape_suffix(const hyd_array_t* arr) {
+    printf("] (shape: [");
+    for (int i = 0; i < arr->shape.ndims; i++) printf(i ? ", %lld" : "%lld", (long long)arr->shape.elems[i]);
+    printf("])");
+}
+
+void hyd_print_array_inline(hyd_array_t* arr) {
+    host_view(arr);
+    const int64_t n = hyd_shape_size(arr->shape);
+    printf("[");
+    for (int64_t i = 0; i < n; i++) {
+        int64_t v;
+        memcpy(&v, arr->data + (size_t)i * arr->elem_size, sizeof v);
+        printf(i ? ", %lld" : "%lld", (long long)v);
+    }
+    print_shape_suffix(arr);
+}
+
+void hyd_print_array(hyd_array_t* arr) {
+    hyd_print_array_inline(arr);
+    printf("\n");
+}
+
+void hyd_print_float_array_inline(hyd_array_t* arr) {
+    host_view(arr);
+    const int64_t n = hyd_shape_size(arr->shape);
+    printf("[");
+    for (int64_t i = 0; i < n; i++) {
+        double v;
+        memcpy(&v, arr->data + (size_t)i * arr->elem_size, sizeof v);
+        printf(i ? ", %.17g" : "%.17g", v);
+    }
+    print_shape_suffix(arr);
+}
+
+void hyd_print_float_array(hyd_array_t* arr) {
+    hyd_print_float_array_inline(arr);
+    printf("\n");
+}
+
+void hyd_print_tuple(hyd_tuple_t t) {
+    printf("(");
+    for (int i = 0; i < t.ndims; i++) printf(i ? ", %lld" : "%lld", (long long)t.elems[i]);
+    printf(")\n");
+}
+
 /* ---- residency ---------------------------------------------------------- */
 
 hyd_array_t* hyd_array_alloc_device(hyd_tuple_t shape, size_t elem_size) {

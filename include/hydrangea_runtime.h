@@ -87,6 +87,39 @@ int64_t hyd_write_array_csv_float(hyd_array_t* arr, const char* filename);
 /* hyd_get_env_int: runtime/hydrangea_runtime.h:601-615 (unset or
  * unparsable variable: message on stderr, exit(1)). */
 int64_t hyd_get_env_int(const char* name);
+/* hyd_get_env_string: runtime/hydrangea_runtime.h:617-624. */
+const char* hyd_get_env_string(const char* name);
+/* getenv_int / getenv_string: runtime/hyd_write_csv.c:61-72 (called by the
+ * emitted C for get_env_int / get_env_string, Lowering.hs:1921-1930; atoll;
+ * unset variable: message on stderr, exit(1)). */
+int64_t getenv_int(const char* name);
+const char* getenv_string(const char* name);
+
+/* Slices, element access and printing: runtime/hydrangea_runtime.h:277-290,
+ * 458-548 (same output text).  Readers of arr->data copy a device-resident
+ * array's bytes back first. */
+hyd_tuple_t hyd_slice_shape(hyd_tuple_t src_shape, hyd_tuple_t slice_specs);
+double hyd_array_get_float(hyd_array_t* arr, int64_t idx);
+void hyd_array_set_float(hyd_array_t* arr, int64_t idx, double v);
+void hyd_print_int_inline(int64_t v);
+void hyd_print_int(int64_t v);
+void hyd_print_double_inline(double v);
+void hyd_print_double(double v);
+void hyd_print_array_inline(hyd_array_t* arr);
+void hyd_print_array(hyd_array_t* arr);
+void hyd_print_float_array_inline(hyd_array_t* arr);
+void hyd_print_float_array(hyd_array_t* arr);
+void hyd_print_tuple(hyd_tuple_t t);
+#ifndef __cplusplus
+/* Pair printing (C11 _Generic), as the reference header defines it. */
+#define _hyd_print_val(x) _Generic((x), \
+    double: printf("%.17g", (double)(x)), \
+    float: printf("%.17g", (double)(x)), \
+    default: printf("%lld", (long long)(x)))
+#define hyd_print_pair(a, b) do { \
+    printf("("); _hyd_print_val(a); printf(", "); _hyd_print_val(b); printf(")\n"); \
+} while (0)
+#endif
 
 #ifdef __cplusplus
 }
