@@ -23,6 +23,7 @@ def declared_functions():
         text = open(os.path.join(INCLUDE, f)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         text = re.sub(r"//[^\n]*", "", text)
+        text = re.sub(r"^\s*#define(?:[^\n]*\\\n)*[^\n]*", "", text, flags=re.M)  # macros, with continuations
         for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(([^;{}]*)\)\s*;", text):
             name = m.group(1)
             if name in ("if", "while", "for", "sizeof", "return"):
