@@ -41,7 +41,10 @@
 namespace hyd {
 
 // Autotuner interface (tune.hip): 0 = LDS tiles, 1 = global atomics, -1 untuned.
-int tuned_batch_mode(int64_t n, int64_t bins);
+// A snapshot of the whole table (one lock per batch, not per instance).
+void tuned_batch_snapshot(TuneSnapshot* out);
+int tune_class_bins(int64_t bins);
+int tune_class_n(int64_t n);
 
 namespace {
 
@@ -314,11 +317,26 @@ int batched_histogram_ptrs(const int64_t* const* src, const int64_t* len, const 
     std::vector<Tile> flat;
     std::vector<Group> groups;
     std::vector<int64_t> zero;  // (dst pointer, bins) pairs
+    TuneSnapshot tune;
+    if (policy != 1 && policy != 2) tuned_batch_snapshot(&tune);
+    // instances repeat bin counts: keep each count's divisor constants
+    int64_t mod_bins[4] = {0, 0, 0, 0};
+    ModConst mod_cache[4];
+    int mod_next = 0;
     for (int64_t k = 0; k < count; ++k) {
         if (len[k] < 0 || bins[k] < 1 || bins[k] > ((int64_t)1 << 31) || !dst[k] || (len[k] > 0 && !src[k]))
             return set_error(HYD_E_INVALID, "batched_histogram: bad descriptor %lld", (long long)k);
         const int64_t n = len[k], nb = bins[k];
-        const ModConst mod = make_mod_const(nb);
+        int slot = -1;
+        for (int i = 0; i < 4; ++i)
+            if (mod_bins[i] == nb) slot = i;
+        if (slot < 0) {
+            slot = mod_next;
+            mod_next = (mod_next + 1) & 3;
+            mod_bins[slot] = nb;
+            mod_cache[slot] = make_mod_const(nb);
+        }
+        const ModConst mod = mod_cache[slot];
         unsigned long long* d = reinterpret_cast<unsigned long long*>(dst[k]);
         int mode;
         if (policy == 1)
@@ -326,7 +344,7 @@ int batched_histogram_ptrs(const int64_t* const* src, const int64_t* len, const 
         else if (policy == 2)
             mode = 1;
         else {
-            mode = tuned_batch_mode(n, nb);
+            mode = tune.mode[tune_class_bins(nb)][tune_class_n(n)];
             if (mode < 0) mode = rule_mode(n, nb);
         }
         const int64_t first = (int64_t)flat.size();

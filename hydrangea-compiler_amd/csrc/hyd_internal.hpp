@@ -55,6 +55,12 @@ int stage_h2d(hipStream_t s, void* dev, size_t bytes, const std::function<void(u
 // Zero-fill kernel launch (graph-capture safe, unlike hipMemsetAsync).
 int fill_zero(void* dptr, size_t bytes, hipStream_t stream);
 
+// The batched-scatter autotuner's table as a dense array, mode[ceil log2
+// bins][floor log2 n]: 0 = LDS tiles, 1 = global atomics, -1 = untuned.
+struct TuneSnapshot {
+    int8_t mode[64][64];
+};
+
 }  // namespace hyd
 
 #define HYD_TRY_HIP(expr)                                                          \

@@ -75,11 +75,18 @@ int load_locked(const std::string& path) {
 
 }  // namespace
 
-int tuned_batch_mode(int64_t n, int64_t bins) {
+
+int tune_class_bins(int64_t bins) { return ilog2_ceil(bins); }
+int tune_class_n(int64_t n) { return ilog2_floor(std::max<int64_t>(n, 1)); }
+
+// The table as a dense array (-1: untuned), under one lock.
+void tuned_batch_snapshot(TuneSnapshot* out) {
+    memset(out->mode, -1, sizeof out->mode);
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_loaded) load_locked(default_path());
-    auto it = g_table.find({ilog2_ceil(bins), ilog2_floor(std::max<int64_t>(n, 1))});
-    return it == g_table.end() ? -1 : it->second;
+    for (auto& kv : g_table)
+        if (kv.first.first >= 0 && kv.first.first < 64 && kv.first.second >= 0 && kv.first.second < 64)
+            out->mode[kv.first.first][kv.first.second] = (int8_t)kv.second;
 }
 
 }  // namespace hyd
@@ -117,6 +124,8 @@ int hydhip_tune_batched(const char* path, void* stream) {
     int rc = tune_impl(path, stream, &measured);
     return rc == HYD_OK ? measured : -rc;
 }
+
+constexpr float kGlobalMargin = 0.9f;  // global must be 10 % faster than LDS
 
 static int tune_impl(const char* path, void* stream, int* measured) {
     HYD_TRY(ensure_init());
@@ -162,7 +171,10 @@ static int tune_impl(const char* path, void* stream, int* measured) {
                     if (rep > 0 && ms < best[mode]) best[mode] = ms;
                 }
             }
-            found[{ilog2_ceil(nb), nl}] = best[0] <= best[1] ? 0 : 1;
+            // the global form only where it wins clearly: in a mixed batch its
+            // atomics share L2 and HBM with the streaming LDS tiles, which an
+            // isolated measurement does not see
+            found[{ilog2_ceil(nb), nl}] = best[1] < kGlobalMargin * best[0] ? 1 : 0;
         }
     }
     (void)hipEventDestroy(e0);
