@@ -159,6 +159,17 @@ __device__ __forceinline__ int64_t token_len(const At& at, int64_t p) {
 
 constexpr int kMaxWords = kTile / 2 + 1;  // a word start follows a gap (or is the text's first byte)
 
+// Word starts among 32 bytes (qa, qb: bytes 0-15, 16-31): bit k set when byte
+// k is not a gap and byte k - 1 is one (before_gap: the byte before byte 0 is
+// a gap or the text's start).  Bytes from `valid` on are past the text.
+__device__ __forceinline__ uint32_t word_starts32(uint4 qa, uint4 qb, bool before_gap, int valid) {
+    const uint32_t gap = gap_mask8(((uint64_t)qa.y << 32) | qa.x) | (gap_mask8(((uint64_t)qa.w << 32) | qa.z) << 8) |
+                         (gap_mask8(((uint64_t)qb.y << 32) | qb.x) << 16) |
+                         (gap_mask8(((uint64_t)qb.w << 32) | qb.z) << 24);
+    const uint32_t in = valid >= 32 ? 0xFFFFFFFFu : ((1u << valid) - 1);
+    return ~gap & ((gap << 1) | (before_gap ? 1u : 0u)) & in;
+}
+
 // Stages the tile, then lists the words that start in it: starts[0, words)
 // (tile offsets, in text order).  A byte-parallel pass: each thread marks the
 // starts among its kSlice bytes, a block scan places them.
@@ -188,14 +199,8 @@ constexpr int kMaxWords = kTile / 2 + 1;  // a word start follows a gap (or is t
         const int valid = rest <= 0 ? 0 : (rest < kSlice ? (int)rest : kSlice);         \
         const uint4 qa = *reinterpret_cast<const uint4*>(buf + (s0 - base));            \
         const uint4 qb = *reinterpret_cast<const uint4*>(buf + (s0 - base) + 16);       \
-        const uint32_t wv[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};        \
-        uint32_t mask = 0;                                                              \
-        int prev = at(s0 - 1);                                                          \
-        _Pragma("unroll") for (int k = 0; k < kSlice; ++k) {                            \
-            const int c = (int)((wv[k >> 2] >> (8 * (k & 3))) & 0xFF);                  \
-            if (k < valid && !is_gap(c) && (prev < 0 || is_gap(prev))) mask |= 1u << k; \
-            prev = c;                                                                   \
-        }                                                                               \
+        const int prev = at(s0 - 1);                                                    \
+        uint32_t mask = word_starts32(qa, qb, prev < 0 || is_gap(prev), valid);         \
         int64_t total;                                                                  \
         int off = (int)block_excl_scan(__builtin_popcount(mask), &total, sh);           \
         while (mask) {                                                                  \
@@ -266,14 +271,9 @@ __global__ __launch_bounds__(kPThreads) void csv_words_kernel(const unsigned cha
     }
     const int64_t rest = n - s0;
     const int valid = rest <= 0 ? 0 : (rest < kSlice ? (int)rest : kSlice);
-    int prev = (s0 > 0 && s0 - 1 < n) ? text[s0 - 1] : -1;
-    int cnt = 0;
-#pragma unroll
-    for (int k = 0; k < kSlice; ++k) {
-        const int c = (int)((wv[k >> 2] >> (8 * (k & 3))) & 0xFF);
-        cnt += (k < valid && !is_gap(c) && (prev < 0 || is_gap(prev))) ? 1 : 0;
-        prev = c;
-    }
+    const int prev = (s0 > 0 && s0 - 1 < n) ? text[s0 - 1] : -1;
+    const uint4 qa = make_uint4(wv[0], wv[1], wv[2], wv[3]), qb = make_uint4(wv[4], wv[5], wv[6], wv[7]);
+    const int cnt = __builtin_popcount(word_starts32(qa, qb, prev < 0 || is_gap(prev), valid));
     int64_t total;
     block_excl_scan(cnt, &total, sh);
     if (threadIdx.x == 0) tile_cnt[blockIdx.x] = total;
@@ -1110,6 +1110,17 @@ extern "C" int64_t hyd_write_array_csv(hyd_array_t* arr, const char* filename) {
 
 extern "C" int64_t hyd_write_array_csv_float(hyd_array_t* arr, const char* filename) {
     return write_csv_file(arr, filename, true);
+}
+
+// CPU test hook: per-byte gap flags of bytes[0, n) by the 8-byte classifier the
+// word kernels use (n a multiple of 8).
+extern "C" void hydhip_host_csv_gap_flags(const void* bytes, int64_t n, unsigned char* out) {
+    for (int64_t i = 0; i + 8 <= n; i += 8) {
+        uint64_t x;
+        memcpy(&x, static_cast<const unsigned char*>(bytes) + i, 8);
+        const uint32_t m = gap_mask8(x);
+        for (int k = 0; k < 8; ++k) out[i + k] = (m >> k) & 1;
+    }
 }
 
 // Test hook: 1 = format every fp64 value through the exact kernel (the path

@@ -415,6 +415,17 @@ HYD_HD uint32_t digit_mask(uint64_t x) {
     return byte_mask(zero_bytes(hi | lo));
 }
 HYD_HD uint32_t eq_mask(uint64_t x, uint32_t c) { return byte_mask(zero_bytes(x ^ (0x0101010101010101ULL * c))); }
+// bit i set when byte i of x is a gap character (',' or a C-locale isspace:
+// ' ', 9..13).  The 9..13 test adds to the low 7 bits of each byte, which
+// cannot carry into the next byte.
+HYD_HD uint32_t gap_mask8(uint64_t x) {
+    const uint64_t ones = 0x0101010101010101ULL;
+    const uint64_t lo7 = x & (ones * 0x7F);
+    const uint64_t ge9 = lo7 + ones * (0x80 - 9);    // bit 7: low bits >= 9
+    const uint64_t ge14 = lo7 + ones * (0x80 - 14);  // bit 7: low bits >= 14
+    const uint64_t ctl = ge9 & ~ge14 & ~x;          // and the byte itself < 0x80
+    return byte_mask(zero_bytes(x ^ (ones * ',')) | zero_bytes(x ^ (ones * ' ')) | ctl);
+}
 HYD_HD int lead_digits(uint64_t x) {  // digit bytes at the start of x (0..8)
     const uint32_t nd = ~digit_mask(x) & 0xFF;
     return nd ? __builtin_ctz(nd) : 8;

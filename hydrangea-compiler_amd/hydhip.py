@@ -175,6 +175,7 @@ _SIGS = {
                                     ctypes.POINTER(_INT)]),
     "hydhip_host_csv_format": (_I64, [_P, _I64, _INT, _INT, _P, _I64, _P]),
     "hydhip_csv_test_force_exact": (None, [_INT]),
+    "hydhip_host_csv_gap_flags": (None, [_P, _I64, _P]),
     "hyd_hip_bench": (_I64, [ctypes.c_char_p, _P, _P, _I64, _I64, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     # serving
     "hyd_server_create": (_P, [_I64, _INT, _INT]),

@@ -282,6 +282,9 @@ int hydhip_csv_format_f64(const double* vals, int64_t count, int64_t row_len,
  * kernel, which real values almost never need, so that its GPU path can be
  * checked; on = 0 restores the normal path.  Not for production use. */
 void hydhip_csv_test_force_exact(int on);
+/* CPU test hook: out[i] = 1 when bytes[i] is a gap character (',' or
+ * isspace) by the 8-byte classifier of the parse kernels; n % 8 == 0. */
+void hydhip_host_csv_gap_flags(const void* bytes, int64_t n, unsigned char* out);
 int hydhip_host_csv_scan(const char* s, int64_t len, int is_float, int flags,
                          void* value, int64_t* consumed, int* exact);
 int64_t hydhip_host_csv_format(const void* vals, int64_t n, int is_float,

@@ -221,3 +221,20 @@ def test_device_scanner_matches_glibc_on_malformed_text(host, orc):
     ints += [b"99999999999999999999", b"-99999999999999999999", b"9223372036854775808",
              b"-9223372036854775808", b"0x10", b"+5", b"-", b"00012", b"-0"]
     check_scan(host, orc, ints, False)
+
+
+def test_gap_classifier_all_bytes(host):
+    # the 8-byte gap classifier of the word kernels against the definition
+    # (',' or C-locale isspace), every byte value at every position among
+    # random neighbours
+    import ctypes
+    rng = np.random.default_rng(31)
+    buf = rng.integers(0, 256, 8 * 256 * 8 * 16, dtype=np.uint8)
+    for v in range(256):
+        for pos in range(8):
+            for rep in range(16):
+                buf[((v * 8 + pos) * 16 + rep) * 8 + pos] = v
+    out = np.zeros_like(buf)
+    host.lib().hydhip_host_csv_gap_flags(buf.ctypes.data, buf.size, out.ctypes.data)
+    want = np.isin(buf, [ord(","), ord(" "), 9, 10, 11, 12, 13]).astype(np.uint8)
+    assert (out == want).all()
