@@ -10,7 +10,10 @@ The roofline object is the map-chain kernel's (the step's dominant kernel),
 timed with HIP events on the stream it runs on.  cpu_baseline is the CPU
 oracle (a C restatement of the emitted OpenMP C) on the same workload, on
 all host cores.  The other fused kernels are measured in the same run and
-reported under "kernels".
+reported under "kernels"; unless --no-cpu, each one also carries the
+oracle's time on the host for the same inputs (timing.h style: warmup, then
+timed runs, min and mean; bounded samples where the CPU would take minutes)
+and whether this run's GPU result matches the oracle's ("matches_oracle").
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -68,7 +71,28 @@ def timed_loop(fn, steps, warmup, events=None):
     return (t1 - t0) * 1e3 / steps, ev
 
 
-def bench_dotmap(steps, warmup):
+def cpu_time(fn, warmup=3, iters=10):
+    """The oracle on the host (all cores), timed like the reference's timing.h:
+    `warmup` untimed runs, then `iters` timed ones.  Returns (timing, result)."""
+    r = None
+    for _ in range(warmup):
+        r = fn()
+    ts = []
+    for _ in range(iters):
+        t0 = time.perf_counter()
+        r = fn()
+        ts.append(time.perf_counter() - t0)
+    return dict(min_ms=min(ts) * 1e3, mean_ms=float(np.mean(ts)) * 1e3, warmup=warmup, iters=iters), r
+
+
+def bits_equal(a, b):
+    a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+    if a.dtype == np.float64:
+        return bool(a.shape == b.shape and (a.view(np.uint64) == b.view(np.uint64)).all())
+    return bool(np.array_equal(a, b))
+
+
+def bench_dotmap(steps, warmup, cpu=True):
     c = workloads.DOTMAP
     n = c["n"]
     xs = hydhip.splitmix_f64(n, c["seed"], 0, symmetric=True)
@@ -106,6 +130,16 @@ def bench_dotmap(steps, warmup):
         dot=dict(ms=dot_ms, bytes=dot_bytes, gbs=dot_bytes / dot_ms / 1e6),
         result=float(dres.numpy()[0]),
     )
+    if cpu:
+        # the same run's results against the oracle: map chain bit for bit,
+        # dot within 1e-12 of a double-double sum
+        import pyoracle
+
+        hx = pyoracle.splitmix_f64(n, c["seed"], 0, symmetric=True)
+        hy = pyoracle.splitmix_f64(n, c["seed"], n, symmetric=True)
+        ref = pyoracle.dot_dd(hx, hy)
+        res["dot"]["matches_oracle"] = abs(res["result"] - ref) <= 1e-12 * max(1.0, abs(ref))
+        res["map_chain"]["matches_oracle"] = bits_equal(out.numpy(), pyoracle.map_chain(hx, hy))
     for a in (xs, ys, out):
         a.free()
     return res
@@ -133,7 +167,7 @@ def bench_copy(steps, warmup):
     return dict(ms=ms, bytes=16 * n, gbs=16 * n / ms / 1e6)
 
 
-def bench_guarded(steps, warmup):
+def bench_guarded(steps, warmup, cpu=True):
     """Guarded scatters on the reference's voxel benchmark shapes: int64 (+)
     rasterization (5M points) and the order-exact fp64 (+.) trilinear splat
     (1M points = 8M contributions), both into 64^3 voxels.  Algorithmic bytes:
@@ -163,6 +197,14 @@ def bench_guarded(steps, warmup):
         b = 24 * idx.size + 16 * nvox
         res[name] = dict(elements=int(idx.size), kept=int(guard.sum()), voxels=nvox, ms=kms,
                          bytes=b, gbs=b / kms / 1e6, melem_s=idx.size / kms / 1e3)
+        if cpu:
+            import pyoracle
+
+            zh = np.zeros(nvox, dtype)
+            t, ref = cpu_time(lambda: pyoracle.scatter_guarded_add(idx, vals.astype(dtype), guard, zh),
+                              warmup=1, iters=3)
+            res[name]["matches_oracle"] = bits_equal(dst.numpy(), ref)
+            res[name]["cpu_oracle"] = dict(t, threads=1, note="the serial loop the reference emits")
         for a in (di, dv, dg, dst, zeros):
             a.free()
     return res
@@ -267,7 +309,7 @@ def bench_csv(steps, warmup, cpu=True):
     return res
 
 
-def bench_hist(steps, warmup):
+def bench_hist(steps, warmup, cpu=True):
     c = workloads.HIST
     d = hydhip.splitmix_i64(c["n"], c["seed"], bits=c["bits"])
     out = hydhip.DeviceArray((c["bins"],), np.int64)
@@ -284,11 +326,22 @@ def bench_hist(steps, warmup):
         kms = float(np.mean(ev))
         b = 8 * c["n"] + 8 * c["bins"]
         res[name] = dict(ms_per_call=ms, kernel_ms=kms, bytes=b, gbs=b / kms / 1e6)
+        if cpu:
+            got = out.numpy()
+            res[name]["got"] = got
+    if cpu:
+        import pyoracle
+
+        host = pyoracle.splitmix_i64(c["n"], c["seed"], bits=c["bits"])
+        t, ref = cpu_time(lambda: pyoracle.weighted_histogram(host, c["bins"]))
+        for name in ("auto", "lds", "global"):
+            res[name]["matches_oracle"] = bits_equal(res[name].pop("got"), ref)
+        res["cpu_oracle"] = dict(t, threads=pyoracle.num_threads(), gbs=8 * c["n"] / t["mean_ms"] / 1e6)
     d.free()
     return res
 
 
-def bench_jacobi(steps, warmup):
+def bench_jacobi(steps, warmup, cpu=True):
     c = workloads.JACOBI
     h, w, iters = c["h"], c["w"], c["iters"]
     u = hydhip.splitmix_f64(h * w, c["seed"])
@@ -308,12 +361,24 @@ def bench_jacobi(steps, warmup):
     step_bytes = 2 * 8 * h * w
     res = dict(iters=iters, graph_ms=gms, ms_per_iterate=ms, per_step_us=gms * 1e3 / iters,
                bytes_per_step=step_bytes, gbs=step_bytes * iters / gms / 1e6)
+    if cpu:
+        # a bounded sample for the CPU: 20 of the 200 steps, the GPU's 20 steps
+        # checked bit for bit against the oracle's
+        import pyoracle
+
+        k = 20
+        host = pyoracle.splitmix_f64(h * w, c["seed"]).reshape(h, w)
+        t, ref = cpu_time(lambda: pyoracle.iterate_stencil2d(host, k, alpha=c["alpha"]), warmup=1, iters=3)
+        got, _, _ = hydhip.iterate_stencil2d(u, k, alpha=c["alpha"], buf0=b0, buf1=b1, use_graph=True)
+        res["matches_oracle_20_steps"] = bits_equal(got.numpy(), ref)
+        res["cpu_oracle"] = dict(t, sample="20 steps", threads=pyoracle.num_threads(),
+                                 per_step_ms=t["mean_ms"] / k)
     for a in (u, b0, b1):
         a.free()
     return res
 
 
-def bench_batched(steps, warmup):
+def bench_batched(steps, warmup, cpu=True):
     c = workloads.BATCH
     lens, bins = workloads.batch_instances()
     off, hoff = hydhip.batch_offsets(lens, bins)
@@ -351,6 +416,15 @@ def bench_batched(steps, warmup):
     ms, _ = timed_loop(percall, 1, 1)
     res["per_call"] = dict(ms_per_batch=ms, req_s=lens.size / ms * 1e3)
     res["speedup_vs_per_call"] = res["per_call"]["ms_per_batch"] / res["tuned"]["ms_per_batch"]
+    if cpu:
+        import pyoracle
+
+        host = pyoracle.splitmix_i64(total, c["seed"], bits=c["bits"])
+        hist_dev = hist.numpy()
+        hydhip.batched_histogram(d, off, lens, bins, hoff, hist=hist, policy=hydhip.BATCH_TUNED)
+        t, (ref, _) = cpu_time(lambda: pyoracle.batched_histograms(host, off, lens, bins), warmup=1, iters=3)
+        res["matches_oracle"] = bits_equal(hist.numpy(), ref) and bits_equal(hist_dev, ref)
+        res["cpu_oracle"] = dict(t, threads=pyoracle.num_threads(), gbs=res["bytes"] / t["mean_ms"] / 1e6)
     d.free()
     hist.free()
     return res
@@ -486,7 +560,7 @@ def main():
     only = set(filter(None, args.only.split(",")))
 
     log("dot + map chain ...")
-    dm = bench_dotmap(args.steps, args.warmup)
+    dm = bench_dotmap(args.steps, args.warmup, cpu=not args.no_cpu)
     kernels["dot"] = dm["dot"]
     kernels["map_chain"] = dm["map_chain"]
     if not only or "copy" in only:
@@ -494,17 +568,17 @@ def main():
         kernels["copy_reference"] = bench_copy(args.steps, args.warmup)
     if not only or "hist" in only:
         log("histogram ...")
-        kernels["weighted_histogram"] = bench_hist(args.steps, args.warmup)
+        kernels["weighted_histogram"] = bench_hist(args.steps, args.warmup, cpu=not args.no_cpu)
     if not only or "guarded" in only:
         log("guarded scatters ...")
-        kernels["scatter_guarded"] = bench_guarded(args.steps, args.warmup)
+        kernels["scatter_guarded"] = bench_guarded(args.steps, args.warmup, cpu=not args.no_cpu)
     if not only or "jacobi" in only:
         log("jacobi ...")
-        kernels["jacobi_heat"] = bench_jacobi(args.steps, args.warmup)
+        kernels["jacobi_heat"] = bench_jacobi(args.steps, args.warmup, cpu=not args.no_cpu)
 
     if not only or "batch" in only:
         log("batched histograms ...")
-        kernels["batched_histogram"] = bench_batched(args.steps, args.warmup)
+        kernels["batched_histogram"] = bench_batched(args.steps, args.warmup, cpu=not args.no_cpu)
 
     if not only or "csv" in only:
         log("csv text ...")
