@@ -21,6 +21,7 @@
 // inside its tile with wave64 ballots (a lane's rank among equal digits is
 // the popcount of the equal-digit lanes below it) plus per-wave LDS counts.
 #include <algorithm>
+#include <climits>
 
 #include "hyd_internal.hpp"
 
@@ -202,8 +203,8 @@ __global__ __launch_bounds__(kSortBlock) void radix_count_kernel(const int64_t* 
             (int64_t)hist[0][d] + hist[1][d] + hist[2][d] + hist[3][d];
 }
 
-// Stable scatter of one tile.  idx_in == nullptr means the identity (first
-// pass of sort_indices).
+// Stable scatter of one tile, keys with their payloads.  idx_in == nullptr
+// means the identity (the first pass of an index sort).
 __global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
     const int64_t* __restrict__ keys_in, const int64_t* __restrict__ idx_in, int64_t n, int shift,
     int64_t ntiles, const int64_t* __restrict__ offsets, int64_t* __restrict__ keys_out,
@@ -268,18 +269,22 @@ inline unsigned grid_for(int64_t n, int block = 256, int64_t cap = 8192) {
 
 }  // namespace
 
-// Workspace layout for sort_pairs: keys x2, idx x1 (perm out is the other),
-// digit counts 256*ntiles, scan tile sums, 1 word for the max key.
+// Workspace layout for sort_pairs_i64: keys x2, payload x1 (the output is
+// the other), digit counts 256*ntiles, scan tile sums, 1 word for the max key.
 int64_t sort_workspace_words(int64_t n) {
     int64_t ntiles = ceil_div(std::max<int64_t>(n, 1), kSortTile);
     return 2 * n + n + 256 * ntiles + scan_workspace_words(256 * ntiles) + 8;
 }
 
-// Stable sort of keys[0..n) -> perm (indices).  `ws` holds
-// sort_workspace_words(n) words.  Passes follow the reference: the number of
-// bytes of max(0, max key), at least one (reads the max back to the host).
-int sort_indices_i64(const int64_t* keys, int64_t n, int64_t* perm, hipStream_t s, int64_t* ws,
-                     int* passes_out) {
+// Stable LSD radix sort of keys[0..n), one byte per pass, carrying a 64-bit
+// payload per key (payload == nullptr: the key's index, which makes the
+// output the sorting permutation).  The pass count is the number of bytes of
+// max(0, max key), at least one, as in the reference; max_key >= 0 states a
+// bound on the keys instead, which skips the max pass and its read back to
+// the host.  Sorted payloads go to payload_out, sorted keys to keys_out when
+// it is non-null.  `ws` holds sort_workspace_words(n) words.
+int sort_pairs_i64(const int64_t* keys, const int64_t* payload, int64_t n, int64_t max_key,
+                   int64_t* keys_out, int64_t* payload_out, hipStream_t s, int64_t* ws, int* passes_out) {
     if (n == 0) return HYD_OK;
     int64_t ntiles = ceil_div(n, kSortTile);
     int64_t* kbuf0 = ws;
@@ -287,39 +292,44 @@ int sort_indices_i64(const int64_t* keys, int64_t n, int64_t* perm, hipStream_t 
     int64_t* ibuf = ws + 2 * n;
     int64_t* counts = ws + 3 * n;
     int64_t* scan_ws = counts + 256 * ntiles;
-    unsigned long long* maxw =
-        reinterpret_cast<unsigned long long*>(scan_ws + scan_workspace_words(256 * ntiles));
-
-    HYD_TRY(fill_zero(maxw, 8, s));
-    max_key_kernel<<<grid_for(n, kBlock, 1024), kBlock, 0, s>>>(keys, n, maxw);
-    HYD_CHECK_LAUNCH();
-    unsigned long long maxk = 0;
-    HYD_TRY_HIP(hipMemcpyAsync(&maxk, maxw, 8, hipMemcpyDeviceToHost, s));
-    HYD_TRY_HIP(hipStreamSynchronize(s));
+    unsigned long long maxk = (unsigned long long)std::max<int64_t>(max_key, 0);
+    if (max_key < 0) {
+        unsigned long long* maxw =
+            reinterpret_cast<unsigned long long*>(scan_ws + scan_workspace_words(256 * ntiles));
+        HYD_TRY(fill_zero(maxw, 8, s));
+        max_key_kernel<<<grid_for(n, kBlock, 1024), kBlock, 0, s>>>(keys, n, maxw);
+        HYD_CHECK_LAUNCH();
+        HYD_TRY_HIP(hipMemcpyAsync(&maxk, maxw, 8, hipMemcpyDeviceToHost, s));
+        HYD_TRY_HIP(hipStreamSynchronize(s));
+    }
     int passes = 0;
     for (unsigned long long m = maxk; m; m >>= 8) ++passes;
     if (passes == 0) passes = 1;
     if (passes_out) *passes_out = passes;
 
-    // Ping-pong so that the last pass writes its indices into `perm`.
+    // Ping-pong so that the last pass writes into payload_out (and keys_out).
     const int64_t* kin = keys;
-    const int64_t* iin = nullptr;
+    const int64_t* iin = payload;
     for (int p = 0; p < passes; ++p) {
-        bool last = p == passes - 1;
-        int64_t* kout = (p & 1) ? kbuf1 : kbuf0;
-        // indices alternate so the final pass lands in perm
-        int64_t* iout = ((passes - 1 - p) & 1) ? ibuf : perm;
+        const bool last = p == passes - 1;
+        int64_t* kout = (last && keys_out) ? keys_out : ((p & 1) ? kbuf1 : kbuf0);
+        int64_t* iout = ((passes - 1 - p) & 1) ? ibuf : payload_out;
         radix_count_kernel<<<(unsigned)ntiles, kSortBlock, 0, s>>>(kin, n, p * 8, ntiles, counts);
         HYD_CHECK_LAUNCH();
         HYD_TRY(scan_i64(counts, 256 * ntiles, 0, counts, false, false, s, scan_ws));
         radix_scatter_kernel<<<(unsigned)ntiles, kSortBlock, 0, s>>>(kin, iin, n, p * 8, ntiles,
                                                                       counts, kout, iout);
         HYD_CHECK_LAUNCH();
-        (void)last;
         kin = kout;
         iin = iout;
     }
     return HYD_OK;
+}
+
+// Stable sort of keys[0..n) -> perm (indices).
+int sort_indices_i64(const int64_t* keys, int64_t n, int64_t* perm, hipStream_t s, int64_t* ws,
+                     int* passes_out) {
+    return sort_pairs_i64(keys, nullptr, n, -1, nullptr, perm, s, ws, passes_out);
 }
 
 namespace {
@@ -488,34 +498,79 @@ __global__ void scatter_keys_kernel(const int64_t* __restrict__ idx, const int64
     }
 }
 
-// keep[i] = 1 when element i contributes (its key is a bin < ndst).
-__global__ void keep_flags_kernel(const int64_t* __restrict__ keys, int64_t n, int64_t ndst,
-                                  int64_t* __restrict__ keep) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        keep[i] = keys[i] < ndst ? 1 : 0;
-}
+// ---- order-preserving compaction of the contributing elements ------------
+// Element i contributes when its guard holds (guard NULL: always) and its
+// route idx[i] is a bin in [0, ndst).  Two passes over tiles of 16 rounds of
+// 256 elements, element e = tile * 4096 + round * 256 + thread:
+//  compact_count_kernel  one 64-bit ballot per (round, wave) and the tile's
+//                        count;
+//  compact_write_kernel  after a scan of the tile counts, writes the
+//                        contributing keys and value bits in element order:
+//                        rank = earlier rounds + earlier waves of this round
+//                        + set bits of the ballot below the lane.
+// The second pass reads the ballots instead of the guard, and idx and vals
+// only where an element contributes.
+constexpr int kCompactRounds = 16;
+constexpr int64_t kCompactTile = 256 * kCompactRounds;
 
-// Order-preserving compaction of the contributing elements: their keys and
-// their positions in the original sequence.
-__global__ void compact_kernel(const int64_t* __restrict__ keys, const int64_t* __restrict__ incl,
-                               int64_t n, int64_t ndst, int64_t* __restrict__ ckeys,
-                               int64_t* __restrict__ cpos) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        if (keys[i] >= ndst) continue;
-        const int64_t j = incl[i] - 1;
-        ckeys[j] = keys[i];
-        cpos[j] = i;
+__global__ __launch_bounds__(256) void compact_count_kernel(const int64_t* __restrict__ idx,
+                                                            const int64_t* __restrict__ guard, int64_t n,
+                                                            int64_t ndst, uint64_t* __restrict__ masks,
+                                                            int64_t* __restrict__ counts) {
+    __shared__ int wave_total[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t base = (int64_t)blockIdx.x * kCompactTile + threadIdx.x;
+    bool keep[kCompactRounds];
+#pragma unroll
+    for (int r = 0; r < kCompactRounds; ++r) {
+        const int64_t e = base + r * 256;
+        keep[r] = false;
+        if (e < n) {
+            const int64_t bin = idx[e];
+            keep[r] = (!guard || guard[e] != 0) && bin >= 0 && bin < ndst;
+        }
     }
+    int total = 0;
+#pragma unroll
+    for (int r = 0; r < kCompactRounds; ++r) {
+        const uint64_t m = __ballot(keep[r]);
+        if (lane == 0) masks[(int64_t)blockIdx.x * (kCompactRounds * 4) + r * 4 + wave] = m;
+        total += __popcll(m);
+    }
+    if (lane == 0) wave_total[wave] = total;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        counts[blockIdx.x] = (int64_t)wave_total[0] + wave_total[1] + wave_total[2] + wave_total[3];
 }
 
-// out[j] = src[pos[perm[j]]] (the values, bit-copied, in sorted order).
-__global__ void gather2_kernel(const int64_t* __restrict__ perm, const int64_t* __restrict__ pos, int64_t m,
-                               const int64_t* __restrict__ src, int64_t* __restrict__ out) {
-    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m;
-         j += (int64_t)gridDim.x * blockDim.x)
-        out[j] = src[pos[perm[j]]];
+__global__ __launch_bounds__(256) void compact_write_kernel(const int64_t* __restrict__ idx,
+                                                            const int64_t* __restrict__ vals, int64_t n,
+                                                            const uint64_t* __restrict__ masks,
+                                                            const int64_t* __restrict__ incl_counts,
+                                                            int64_t* __restrict__ ckeys,
+                                                            int64_t* __restrict__ cvals) {
+    __shared__ uint64_t tile_masks[kCompactRounds * 4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x < kCompactRounds * 4)
+        tile_masks[threadIdx.x] = masks[(int64_t)blockIdx.x * (kCompactRounds * 4) + threadIdx.x];
+    __syncthreads();
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const int64_t base = (int64_t)blockIdx.x * kCompactTile + threadIdx.x;
+    int64_t out = blockIdx.x ? incl_counts[blockIdx.x - 1] : 0;
+#pragma unroll 4
+    for (int r = 0; r < kCompactRounds; ++r) {
+        const uint64_t* rm = tile_masks + r * 4;
+        const uint64_t m = rm[wave];
+        if ((m >> lane) & 1) {
+            int64_t pos = out + __popcll(m & below);
+            for (int w = 0; w < wave; ++w) pos += __popcll(rm[w]);
+            const int64_t e = base + r * 256;  // < n: set bits only come from e < n
+            ckeys[pos] = idx[e];
+            cvals[pos] = vals[e];
+        }
+        out += __popcll(rm[0]) + __popcll(rm[1]) + __popcll(rm[2]) + __popcll(rm[3]);
+    }
+    (void)n;
 }
 
 // Runs of equal bins in the sorted keys (bins < ndst only): head[j] = 1 at
@@ -546,12 +601,9 @@ __global__ void run_bounds_kernel(const int64_t* __restrict__ skeys, const int64
 
 // Folds the runs: old = dst[b]; new = val + old for the run's values in order
 // (the combine write of Lowering.hs:3553-3557, operands in the reference's
-// order), the values already gathered into sorted order.  Short runs: one
-// thread each (fold_short_runs_kernel).  A long run (more than kLongRun
-// values - a grid whose points hit few voxels) gets a whole wave
-// (fold_long_runs_kernel): 64 coalesced loads per step with the next step's
-// loads in flight, and the 64 adds in order on the broadcast values.  The sum
-// stays sequential either way, so the bits are the serial loop's.
+// order), the values already in sorted order.  Short runs: one thread each
+// (fold_short_runs_kernel).  A long run (more than kLongRun values - a grid
+// whose points hit few voxels) gets a whole wave (fold_long_runs_kernel).
 constexpr int64_t kLongRun = 32;
 
 __global__ void fold_short_runs_kernel(const int64_t* __restrict__ skeys, const double* __restrict__ svals,
@@ -579,7 +631,40 @@ __device__ __forceinline__ double lane_value(double v, int l) {
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
+// Exponent of the lowest set bit of a finite binary64 magnitude u (|x| =
+// M * 2^k with M odd), INT_MAX for zero.  Subnormals hold mant * 2^-1074,
+// normals (2^52 | mant) * 2^(E-1075); bit 63 stands in for "no bit" so that
+// the count is defined for zero.  Straight-line code, no branches.
+__device__ __forceinline__ int low_bit_exponent(uint64_t u) {
+    const int E = (int)(u >> 52);
+    const uint64_t M = (u & 0xFFFFFFFFFFFFFull) | ((uint64_t)(E != 0) << 52) | (1ull << 63);
+    const int k = (int)__builtin_ctzll(M) + (E > 1 ? E : 1) - 1075;
+    return u != 0 ? k : INT_MAX;
+}
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) x = x + __shfl_xor(x, off, 64);
+    return x;
+}
+
 // One wave per run index; waves whose run is short leave at once.
+//
+// Exact shortcut: let every operand (the old value and the run's values) be
+// finite, 2^k the largest power of two dividing all of them and S the sum of
+// their magnitudes.  If S < 2^(53+k), every partial sum in ANY order is an
+// integer multiple of 2^k below 2^(53+k) in magnitude, hence representable:
+// no addition rounds, and the serial fold and a tree sum both give the exact
+// sum.  Zero results agree too: under round-to-nearest an exact zero sum is
+// -0 only when both operands are -0, so either order yields -0 iff every
+// operand is -0 (the lane sums start from -0, the identity that keeps it).
+// S is summed in floating point in some order; with all terms multiples of
+// 2^k the computed S reaches 2^(53+k) iff the true S does (partial sums are
+// exact below that bound, rounding is monotone and the bound is
+// representable), so the test is exact.  Dyadic data (weights n/64, integer
+// counts) takes the shortcut; anything else folds serially: the 64 lanes
+// load a step of 64 values together, the next step's loads in flight, and
+// the adds run in order on the broadcast values.
 __global__ __launch_bounds__(256) void fold_long_runs_kernel(const int64_t* __restrict__ skeys,
                                                              const double* __restrict__ svals,
                                                              const int64_t* __restrict__ start,
@@ -588,20 +673,68 @@ __global__ __launch_bounds__(256) void fold_long_runs_kernel(const int64_t* __re
                                                              double* __restrict__ dst) {
     const int64_t R = *nruns;
     const int lane = threadIdx.x & 63;
-    const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
-    for (int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64; r < R; r += waves) {
-        const int64_t a = start[r], e = end[r];  // uniform across the wave
+    // wave-uniform indices: scalar loads of the bounds, scalar loop control
+    const int wid = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    const int waves = (int)(gridDim.x * (blockDim.x >> 6));
+    for (int64_t r = wid; r < R; r += waves) {
+        const int64_t a = start[r], e = end[r];
         if (e - a <= kLongRun) continue;
         const int64_t b = skeys[a];
-        double acc = dst[b];
-        double v = a + lane < e ? svals[a + lane] : 0.0;
+        const double old = dst[b];
+        int kmin = INT_MAX;
+        bool finite = true;
+        double sabs = -0.0, part = -0.0;
+        auto take = [&](double x) {
+            const uint64_t u = __builtin_bit_cast(uint64_t, x) & 0x7FFFFFFFFFFFFFFFull;
+            finite &= (u >> 52) != 0x7FF;
+            kmin = min(kmin, low_bit_exponent(u));
+            sabs = __builtin_bit_cast(double, u) + sabs;
+            part = x + part;
+        };
+        if (lane == 0) take(old);
+        // 16 loads in flight per lane; the tail's loads are clamped into the run
+        int64_t t = a + lane;
+        for (; t + 64 * 15 < e; t += 64 * 16) {
+            double x[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) x[j] = svals[t + 64 * j];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) take(x[j]);
+        }
+        {
+            double x[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) x[j] = svals[min(t + 64 * j, e - 1)];
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if (t + 64 * j < e) take(x[j]);
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) kmin = min(kmin, __shfl_xor(kmin, off, 64));
+        const bool all_finite = __all(finite);
+        sabs = wave_sum(sabs);
+        part = wave_sum(part);
+        bool exact = all_finite;
+        if (exact && kmin != INT_MAX)  // kmin == INT_MAX: every operand is a zero
+            exact = kmin <= 970 &&
+                    sabs < __builtin_bit_cast(double, (uint64_t)(53 + kmin + 1023) << 52);
+        // lane 0 decides for the wave (its S may round differently; any order is valid)
+        if (__builtin_amdgcn_readfirstlane((int)exact)) {
+            if (lane == 0) dst[b] = part;
+            continue;
+        }
+        // Serial fold.  Loads are unconditional (clamped to the run's last
+        // value) so that the wait before a step's adds covers only that step.
+        double acc = old;
+        double v = svals[min(a + lane, e - 1)];
         for (int64_t t0 = a; t0 < e; t0 += 64) {
-            const int64_t tn = t0 + 64;
-            const double vn = tn + lane < e ? svals[tn + lane] : 0.0;  // next step in flight
-            const int cnt = e - t0 < 64 ? (int)(e - t0) : 64;
-            for (int i = 0; i < cnt; ++i) {
-                const double x = lane_value(v, i);
-                acc = x + acc;
+            const double vn = svals[min(t0 + 64 + lane, e - 1)];  // next step in flight
+            if (e - t0 >= 64) {
+#pragma unroll
+                for (int i = 0; i < 64; ++i) acc = lane_value(v, i) + acc;
+            } else {
+                const int cnt = (int)(e - t0);
+                for (int i = 0; i < cnt; ++i) acc = lane_value(v, i) + acc;
             }
             v = vn;
         }
@@ -700,37 +833,37 @@ int csr_from_sorted_coo_i64(int64_t nrows, int64_t nnz, const int64_t* rows, con
 
 int64_t csr_workspace_words(int64_t nrows) { return 3 * nrows + 2 + scan_workspace_words(nrows + 1) + 8; }
 
-// Order-exact scatter (+.) of vals (n, element order) with bin keys (ndst =
-// dropped) into dst.  ws: scatter_ordered_workspace_words(n).
-int scatter_ordered_add_f64(const int64_t* keys, const double* vals, int64_t n, int64_t ndst, double* dst,
-                            hipStream_t s, int64_t* ws) {
+// Order-exact scatter (+.) of vals (n, element order) routed by idx into
+// dst[0, ndst): element i contributes when guard[i] != 0 (guard NULL: every
+// element) and 0 <= idx[i] < ndst.  ws: scatter_ordered_workspace_words(n).
+int scatter_ordered_add_f64(const int64_t* idx, const int64_t* guard, const double* vals, int64_t n,
+                            int64_t ndst, double* dst, hipStream_t s, int64_t* ws) {
     if (n == 0) return HYD_OK;
-    int64_t* keep = ws;            // n, later the run heads
-    int64_t* incl = ws + n;        // n
+    const int64_t ntiles = ceil_div(n, kCompactTile);
+    int64_t* head = ws;            // m: run heads
+    int64_t* incl = ws + n;        // m: their inclusive scan
     int64_t* ckeys = ws + 2 * n;   // m: the contributing keys, element order
-    int64_t* cpos = ws + 3 * n;    // m: their positions
-    int64_t* perm = ws + 4 * n;    // m
-    int64_t* skeys = ws + 5 * n;   // m
+    int64_t* cvals = ws + 3 * n;   // m: their values' bits
+    int64_t* skeys = ws + 5 * n;   // m: sorted keys
     int64_t* svals = ws + 6 * n;   // m: the values' bits in sorted order
     int64_t* start = ws + 7 * n;   // runs
     int64_t* end = ws + 8 * n;     // runs
-    int64_t* scratch = ws + 9 * n; // sort, then scans
+    uint64_t* masks = reinterpret_cast<uint64_t*>(ws + 9 * n);  // 64 per tile
+    int64_t* tcounts = ws + 9 * n + 64 * ntiles;                // per tile, then scanned
+    int64_t* scratch = tcounts + ntiles;                         // sort, then scans
     // only the contributing elements are sorted (the guard often drops most)
-    keep_flags_kernel<<<grid_for(n), 256, 0, s>>>(keys, n, ndst, keep);
+    compact_count_kernel<<<(unsigned)ntiles, 256, 0, s>>>(idx, guard, n, ndst, masks, tcounts);
     HYD_CHECK_LAUNCH();
-    HYD_TRY(scan_i64(keep, n, 0, incl, true, false, s, scratch));
-    compact_kernel<<<grid_for(n), 256, 0, s>>>(keys, incl, n, ndst, ckeys, cpos);
+    HYD_TRY(scan_i64(tcounts, ntiles, 0, tcounts, true, false, s, scratch));
+    compact_write_kernel<<<(unsigned)ntiles, 256, 0, s>>>(idx, reinterpret_cast<const int64_t*>(vals), n, masks,
+                                                         tcounts, ckeys, cvals);
     HYD_CHECK_LAUNCH();
     int64_t m = 0;
-    HYD_TRY_HIP(hipMemcpyAsync(&m, incl + (n - 1), 8, hipMemcpyDeviceToHost, s));
+    HYD_TRY_HIP(hipMemcpyAsync(&m, tcounts + (ntiles - 1), 8, hipMemcpyDeviceToHost, s));
     HYD_TRY_HIP(hipStreamSynchronize(s));
     if (m == 0) return HYD_OK;
-    HYD_TRY(sort_indices_i64(ckeys, m, perm, s, scratch, nullptr));
-    gather_kernel<<<grid_for(m), 256, 0, s>>>(perm, m, ckeys, m, skeys);
-    HYD_CHECK_LAUNCH();
-    gather2_kernel<<<grid_for(m), 256, 0, s>>>(perm, cpos, m, reinterpret_cast<const int64_t*>(vals), svals);
-    HYD_CHECK_LAUNCH();
-    int64_t* head = keep;
+    // contributing keys lie in [0, ndst): that bound sets the pass count
+    HYD_TRY(sort_pairs_i64(ckeys, cvals, m, ndst - 1, skeys, svals, s, scratch, nullptr));
     run_heads_kernel<<<grid_for(m), 256, 0, s>>>(skeys, m, ndst, head);
     HYD_CHECK_LAUNCH();
     HYD_TRY(scan_i64(head, m, 0, incl, true, false, s, scratch));
@@ -745,7 +878,8 @@ int scatter_ordered_add_f64(const int64_t* keys, const double* vals, int64_t n, 
 }
 
 int64_t scatter_ordered_workspace_words(int64_t n) {
-    return 9 * n + std::max(sort_workspace_words(n), scan_workspace_words(n)) + 8;
+    const int64_t ntiles = ceil_div(std::max<int64_t>(n, 1), kCompactTile);
+    return 9 * n + 65 * ntiles + std::max(sort_workspace_words(n), scan_workspace_words(n)) + 8;
 }
 
 int pack_keys_i64(const int64_t* rows, const int64_t* cols, int64_t n, int64_t ncols, int64_t* keys,
@@ -909,13 +1043,9 @@ extern "C" int hydhip_scatter_guarded_add_f64(const int64_t* idx, const double* 
         return set_error(HYD_E_INVALID, "hydhip_scatter_guarded_add_f64: bad arguments");
     if (n == 0) return HYD_OK;
     hipStream_t s = resolve_stream(stream);
-    int64_t* ws = static_cast<int64_t*>(
-        stream_workspace(s, (size_t)(n + scatter_ordered_workspace_words(n)) * 8));
+    int64_t* ws = static_cast<int64_t*>(stream_workspace(s, (size_t)scatter_ordered_workspace_words(n) * 8));
     if (!ws) return set_error(HYD_E_OOM, "hydhip_scatter_guarded_add_f64: workspace allocation failed");
-    int64_t* keys = ws;
-    scatter_keys_kernel<<<grid_for(n), 256, 0, s>>>(idx, guard, n, ndst, keys);
-    HYD_CHECK_LAUNCH();
-    return scatter_ordered_add_f64(keys, vals, n, ndst, dst, s, ws + n);
+    return scatter_ordered_add_f64(idx, guard, vals, n, ndst, dst, s, ws);
 }
 
 // Scatter chain (EScatterChain, Lowering.hs:1622-1858) with (+.) on fp64: the
@@ -952,5 +1082,5 @@ extern "C" int hydhip_scatter_chain_add_f64(int64_t nphases, const int64_t* cons
         HYD_TRY_HIP(hipMemcpyAsync(cvals + off, vals[k], (size_t)m * 8, hipMemcpyDeviceToDevice, s));
         off += m;
     }
-    return scatter_ordered_add_f64(keys, cvals, total, ndst, dst, s, ws + 2 * total);
+    return scatter_ordered_add_f64(keys, nullptr, cvals, total, ndst, dst, s, ws + 2 * total);
 }

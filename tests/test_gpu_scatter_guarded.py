@@ -102,6 +102,59 @@ def test_guarded_f64_serial_order(hyd, orc, n, ndst):
     assert np.array_equal(bits(dst.numpy()), bits(ref))
 
 
+def long_run_cases():
+    """Per-bin (old value, run) pairs for the long-run fold: each run is longer
+    than one thread's share, so a wave folds it, and the cases sit on both
+    sides of the exact-sum shortcut (every operand a multiple of 2^k and the
+    magnitudes summing below 2^(53+k))."""
+    rng = np.random.default_rng(21)
+    one = np.ones
+    cases = [
+        (0.25, rng.integers(-2**20, 2**20, 5000) / 64.0),                 # dyadic: exact
+        (0.0, np.concatenate([[2.0**53], one(100)])),                      # S >= 2^53: 2^53 + 1 + ... ties to even
+        (2.0**53 - 64, one(63)),                                           # S = 2^53 - 1: exact
+        (2.0**53 - 64, one(64)),                                           # S = 2^53: serial, exact anyway
+        (2.0**53 - 64, np.concatenate([one(63), [0.5]])),                  # k = -1: the last add ties to even
+        (-0.0, np.full(40, -0.0)),                                         # -0 kept
+        (0.0, np.full(40, -0.0)),                                          # +0 + -0 = +0
+        (-0.0, np.concatenate([np.full(39, -0.0), [0.0]])),
+        (1.0, np.concatenate([one(20), [np.nan], one(20)])),
+        (1.0, np.concatenate([one(20), [np.inf], one(20)])),
+        (0.0, np.concatenate([[np.inf], one(20), [-np.inf], one(20)])),
+        (0.0, rng.integers(0, 2**40, 100) * 2.0**-1074),                   # subnormal multiples
+        (0.0, np.full(40, 2.0**971)),                                      # k = 971: serial
+        (0.0, np.full(40, 1e308)),                                         # overflows to inf
+        (3.0, rng.standard_normal(3000)),                                  # no shortcut
+        (0.0, np.concatenate([[1e16], one(33), [-1e16]])),                 # absorbed ones
+    ]
+    return cases
+
+
+def test_guarded_f64_long_runs(hyd, orc):
+    # The wave-per-run fold, both sides of its exact-sum shortcut, against the
+    # serial loop bit for bit; bins interleaved in element order.
+    cases = long_run_cases()
+    ndst = len(cases) + 3
+    rng = np.random.default_rng(5)
+    labels = np.concatenate([np.full(len(v), b) for b, (_, v) in enumerate(cases)])
+    rng.shuffle(labels)
+    vals = np.empty(labels.size)
+    for b, (_, v) in enumerate(cases):
+        vals[labels == b] = v               # each bin keeps its run's order
+    defaults = np.concatenate([[o for o, _ in cases], [7.0, -7.0, 0.5]])
+    guard = np.ones(labels.size, np.int64)
+    print("guarded f64 long runs n=%d bins=%d" % (labels.size, ndst), flush=True)
+    dst = dev(hyd, defaults)
+    hyd.scatter_guarded_add(dev(hyd, labels.astype(np.int64)), dev(hyd, vals), dev(hyd, guard), dst)
+    ref = orc.scatter_guarded_add(labels.astype(np.int64), vals, guard, defaults)
+    got = dst.numpy()
+    same = (bits(got) == bits(ref)) | (np.isnan(got) & np.isnan(ref))
+    assert same.all(), [(b, got[b], ref[b]) for b in np.flatnonzero(~same)]
+    # the cases do separate the two orders: the serial loop loses the ones
+    assert ref[1] == 2.0**53 and ref[4] == 2.0**53 and ref[15] == 0.0
+    assert np.signbit(ref[5]) and not np.signbit(ref[6]) and not np.signbit(ref[7])
+
+
 def test_chain_i64(hyd, orc):
     rng = np.random.default_rng(8)
     ndst = 3000
