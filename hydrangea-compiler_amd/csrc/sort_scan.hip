@@ -181,39 +181,49 @@ __global__ __launch_bounds__(kBlock) void max_key_kernel(const int64_t* __restri
         __hip_atomic_fetch_max(out, (unsigned long long)m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ int digit_of(int64_t key, int shift) { return (int)((key >> shift) & 0xFF); }
+// Digits of 8 to 11 bits: a pass covers `BITS` bits of the key at `shift`.
+constexpr int kMaxDigitBits = 11;
+
+template <int BITS>
+__device__ __forceinline__ int digit_of(int64_t key, int shift) {
+    return (int)(((uint64_t)key >> shift) & ((1u << BITS) - 1));
+}
 
 // counts[d * ntiles + tile] = number of keys of `tile` whose digit is d.
+template <int BITS>
 __global__ __launch_bounds__(kSortBlock) void radix_count_kernel(const int64_t* __restrict__ keys,
                                                                  int64_t n, int shift, int64_t ntiles,
                                                                  int64_t* __restrict__ counts) {
-    __shared__ unsigned int hist[4][256];
-    for (int i = threadIdx.x; i < 4 * 256; i += kSortBlock) (&hist[0][0])[i] = 0;
+    constexpr int D = 1 << BITS;
+    __shared__ unsigned int hist[4][D];
+    for (int i = threadIdx.x; i < 4 * D; i += kSortBlock) (&hist[0][0])[i] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kSortTile;
     unsigned int* mine = hist[threadIdx.x >> 6];
 #pragma unroll 4
     for (int r = 0; r < kSortRounds; ++r) {
         int64_t e = base + (int64_t)r * kSortBlock + threadIdx.x;
-        if (e < n) atomicAdd(&mine[digit_of(keys[e], shift)], 1u);
+        if (e < n) atomicAdd(&mine[digit_of<BITS>(keys[e], shift)], 1u);
     }
     __syncthreads();
-    for (int d = threadIdx.x; d < 256; d += kSortBlock)
+    for (int d = threadIdx.x; d < D; d += kSortBlock)
         counts[(int64_t)d * ntiles + blockIdx.x] =
             (int64_t)hist[0][d] + hist[1][d] + hist[2][d] + hist[3][d];
 }
 
 // Stable scatter of one tile, keys with their payloads.  idx_in == nullptr
 // means the identity (the first pass of an index sort).
+template <int BITS>
 __global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
     const int64_t* __restrict__ keys_in, const int64_t* __restrict__ idx_in, int64_t n, int shift,
     int64_t ntiles, const int64_t* __restrict__ offsets, int64_t* __restrict__ keys_out,
     int64_t* __restrict__ idx_out) {
-    __shared__ int64_t run[256];            // next output slot per digit
-    __shared__ unsigned int wave_cnt[4][256];
+    constexpr int D = 1 << BITS;
+    __shared__ int64_t run[D];            // next output slot per digit
+    __shared__ unsigned int wave_cnt[4][D];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int d = threadIdx.x; d < 256; d += kSortBlock) run[d] = offsets[(int64_t)d * ntiles + blockIdx.x];
-    for (int i = threadIdx.x; i < 4 * 256; i += kSortBlock) (&wave_cnt[0][0])[i] = 0;
+    for (int d = threadIdx.x; d < D; d += kSortBlock) run[d] = offsets[(int64_t)d * ntiles + blockIdx.x];
+    for (int i = threadIdx.x; i < 4 * D; i += kSortBlock) (&wave_cnt[0][0])[i] = 0;
     __syncthreads();
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const int64_t base = (int64_t)blockIdx.x * kSortTile;
@@ -222,11 +232,11 @@ __global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
         const bool valid = e < n;
         int64_t key = valid ? keys_in[e] : 0;
         int64_t idx = valid ? (idx_in ? idx_in[e] : e) : 0;
-        int d = valid ? digit_of(key, shift) : 0;
+        int d = valid ? digit_of<BITS>(key, shift) : 0;
         // lanes of this wave holding the same digit (and a valid element)
         uint64_t peers = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < BITS; ++b) {
             uint64_t m = __ballot((d >> b) & 1);
             peers &= ((d >> b) & 1) ? m : ~m;
         }
@@ -234,17 +244,23 @@ __global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
         bool leader = valid && rank_in_wave == 0;
         if (leader) wave_cnt[wave][d] = (unsigned)__popcll(peers);
         __syncthreads();
+        // the first wave holding digit d (no earlier wave counted it) folds
+        // this round's counts of d into run[d] and clears them: only digits
+        // present in the round are touched, not all 2^BITS
+        bool folds = false;
         if (valid) {
-            int64_t before = run[d];
+            const int64_t r0 = run[d];
+            int64_t before = r0;
             for (int w = 0; w < wave; ++w) before += wave_cnt[w][d];
-            int64_t pos = before + rank_in_wave;
+            folds = leader && before == r0;
+            const int64_t pos = before + rank_in_wave;
             keys_out[pos] = key;
             idx_out[pos] = idx;
         }
         __syncthreads();
-        for (int dd = threadIdx.x; dd < 256; dd += kSortBlock) {
-            run[dd] += (int64_t)wave_cnt[0][dd] + wave_cnt[1][dd] + wave_cnt[2][dd] + wave_cnt[3][dd];
-            wave_cnt[0][dd] = wave_cnt[1][dd] = wave_cnt[2][dd] = wave_cnt[3][dd] = 0;
+        if (folds) {
+            run[d] += (int64_t)wave_cnt[0][d] + wave_cnt[1][d] + wave_cnt[2][d] + wave_cnt[3][d];
+            wave_cnt[0][d] = wave_cnt[1][d] = wave_cnt[2][d] = wave_cnt[3][d] = 0;
         }
         __syncthreads();
     }
@@ -270,19 +286,46 @@ inline unsigned grid_for(int64_t n, int block = 256, int64_t cap = 8192) {
 }  // namespace
 
 // Workspace layout for sort_pairs_i64: keys x2, payload x1 (the output is
-// the other), digit counts 256*ntiles, scan tile sums, 1 word for the max key.
+// the other), digit counts 2^11 * ntiles, scan tile sums, 1 word for the max key.
 int64_t sort_workspace_words(int64_t n) {
     int64_t ntiles = ceil_div(std::max<int64_t>(n, 1), kSortTile);
-    return 2 * n + n + 256 * ntiles + scan_workspace_words(256 * ntiles) + 8;
+    const int64_t ncounts = ((int64_t)1 << kMaxDigitBits) * ntiles;
+    return 2 * n + n + ncounts + scan_workspace_words(ncounts) + 8;
 }
 
-// Stable LSD radix sort of keys[0..n), one byte per pass, carrying a 64-bit
-// payload per key (payload == nullptr: the key's index, which makes the
-// output the sorting permutation).  The pass count is the number of bytes of
-// max(0, max key), at least one, as in the reference; max_key >= 0 states a
-// bound on the keys instead, which skips the max pass and its read back to
-// the host.  Sorted payloads go to payload_out, sorted keys to keys_out when
-// it is non-null.  `ws` holds sort_workspace_words(n) words.
+// Digit widths for sorting the low `bits` bits: the fewest passes of at most
+// 11 bits, the widths as even as possible and at least 8 (a pass may cover
+// bits above `bits`, which are zero for keys that fit).
+static int digit_widths(int bits, int* widths) {
+    bits = std::max(bits, 8);
+    const int passes = (bits + kMaxDigitBits - 1) / kMaxDigitBits;
+    int per = bits / passes, extra = bits % passes;
+    for (int p = 0; p < passes; ++p) widths[p] = std::max(8, per + (p < extra ? 1 : 0));
+    return passes;
+}
+
+template <int BITS>
+static int radix_pass(const int64_t* kin, const int64_t* iin, int64_t n, int shift, int64_t ntiles,
+                      int64_t* counts, int64_t* scan_ws, int64_t* kout, int64_t* iout, hipStream_t s) {
+    radix_count_kernel<BITS><<<(unsigned)ntiles, kSortBlock, 0, s>>>(kin, n, shift, ntiles, counts);
+    HYD_CHECK_LAUNCH();
+    HYD_TRY(scan_i64(counts, ((int64_t)1 << BITS) * ntiles, 0, counts, false, false, s, scan_ws));
+    radix_scatter_kernel<BITS><<<(unsigned)ntiles, kSortBlock, 0, s>>>(kin, iin, n, shift, ntiles, counts,
+                                                                       kout, iout);
+    HYD_CHECK_LAUNCH();
+    return HYD_OK;
+}
+
+// Stable LSD radix sort of keys[0..n) carrying a 64-bit payload per key
+// (payload == nullptr: the key's index, which makes the output the sorting
+// permutation).  Without a bound (max_key < 0) the sort orders the low
+// 8 * B bits, B = the number of bytes of max(0, max key) and at least one,
+// as the reference's byte passes do (negative keys included); the max is
+// read back to the host once.  max_key >= 0 promises keys in [0, max_key]
+// and sorts just its bits.  Each pass takes a digit of 8 to 11 bits.
+// Sorted payloads go to payload_out, sorted keys to keys_out when it is
+// non-null; *passes_out gets the reference's byte-pass count.  `ws` holds
+// sort_workspace_words(n) words.
 int sort_pairs_i64(const int64_t* keys, const int64_t* payload, int64_t n, int64_t max_key,
                    int64_t* keys_out, int64_t* payload_out, hipStream_t s, int64_t* ws, int* passes_out) {
     if (n == 0) return HYD_OK;
@@ -291,35 +334,44 @@ int sort_pairs_i64(const int64_t* keys, const int64_t* payload, int64_t n, int64
     int64_t* kbuf1 = ws + n;
     int64_t* ibuf = ws + 2 * n;
     int64_t* counts = ws + 3 * n;
-    int64_t* scan_ws = counts + 256 * ntiles;
-    unsigned long long maxk = (unsigned long long)std::max<int64_t>(max_key, 0);
+    int64_t* scan_ws = counts + ((int64_t)1 << kMaxDigitBits) * ntiles;
+    int bits;
     if (max_key < 0) {
-        unsigned long long* maxw =
-            reinterpret_cast<unsigned long long*>(scan_ws + scan_workspace_words(256 * ntiles));
+        unsigned long long* maxw = reinterpret_cast<unsigned long long*>(
+            scan_ws + scan_workspace_words(((int64_t)1 << kMaxDigitBits) * ntiles));
         HYD_TRY(fill_zero(maxw, 8, s));
         max_key_kernel<<<grid_for(n, kBlock, 1024), kBlock, 0, s>>>(keys, n, maxw);
         HYD_CHECK_LAUNCH();
+        unsigned long long maxk = 0;
         HYD_TRY_HIP(hipMemcpyAsync(&maxk, maxw, 8, hipMemcpyDeviceToHost, s));
         HYD_TRY_HIP(hipStreamSynchronize(s));
+        int bytes = 0;
+        for (unsigned long long m = maxk; m; m >>= 8) ++bytes;
+        bytes = std::max(bytes, 1);
+        if (passes_out) *passes_out = bytes;
+        bits = 8 * bytes;
+    } else {
+        bits = 64 - __builtin_clzll((unsigned long long)max_key | 1);
+        if (passes_out) *passes_out = (bits + 7) / 8;
     }
-    int passes = 0;
-    for (unsigned long long m = maxk; m; m >>= 8) ++passes;
-    if (passes == 0) passes = 1;
-    if (passes_out) *passes_out = passes;
+    int widths[8];
+    const int passes = digit_widths(bits, widths);
 
     // Ping-pong so that the last pass writes into payload_out (and keys_out).
     const int64_t* kin = keys;
     const int64_t* iin = payload;
+    int shift = 0;
     for (int p = 0; p < passes; ++p) {
         const bool last = p == passes - 1;
         int64_t* kout = (last && keys_out) ? keys_out : ((p & 1) ? kbuf1 : kbuf0);
         int64_t* iout = ((passes - 1 - p) & 1) ? ibuf : payload_out;
-        radix_count_kernel<<<(unsigned)ntiles, kSortBlock, 0, s>>>(kin, n, p * 8, ntiles, counts);
-        HYD_CHECK_LAUNCH();
-        HYD_TRY(scan_i64(counts, 256 * ntiles, 0, counts, false, false, s, scan_ws));
-        radix_scatter_kernel<<<(unsigned)ntiles, kSortBlock, 0, s>>>(kin, iin, n, p * 8, ntiles,
-                                                                      counts, kout, iout);
-        HYD_CHECK_LAUNCH();
+        switch (widths[p]) {
+            case 8: HYD_TRY(radix_pass<8>(kin, iin, n, shift, ntiles, counts, scan_ws, kout, iout, s)); break;
+            case 9: HYD_TRY(radix_pass<9>(kin, iin, n, shift, ntiles, counts, scan_ws, kout, iout, s)); break;
+            case 10: HYD_TRY(radix_pass<10>(kin, iin, n, shift, ntiles, counts, scan_ws, kout, iout, s)); break;
+            default: HYD_TRY(radix_pass<11>(kin, iin, n, shift, ntiles, counts, scan_ws, kout, iout, s)); break;
+        }
+        shift += widths[p];
         kin = kout;
         iin = iout;
     }

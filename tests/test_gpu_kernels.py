@@ -306,7 +306,7 @@ def test_scan_known(hyd, case):
 
 
 @pytest.mark.parametrize("n", [1, 5, 4095, 4096, 4097, 1_000_003])
-@pytest.mark.parametrize("bits", [3, 8, 9, 20, 40, 63])
+@pytest.mark.parametrize("bits", [3, 8, 9, 20, 32, 40, 63])
 def test_sort_indices(hyd, orc, n, bits):
     keys = orc.splitmix_i64(n, 601 + n + bits, bits=bits)
     got = hyd.sort_indices(dev(hyd, keys)).numpy()
@@ -319,6 +319,10 @@ def test_sort_known_and_negative_keys(hyd, orc):
     assert hyd.sort_indices(dev(hyd, np.array(c["inputs"]["keys"], np.int64))).numpy().tolist() == c["expected"]
     # negative keys: the reference's pass count comes from max(0, max key)
     keys = orc.splitmix_i64(10_000, 606, bits=20) - (1 << 19)
+    assert bits_equal(hyd.sort_indices(dev(hyd, keys)).numpy(), orc.sort_indices(keys)[0])
+    # five bytes of max key: the low 40 bits are sorted (in wider digits), the
+    # negative keys' upper bits ignored as in the reference's byte passes
+    keys = orc.splitmix_i64(50_000, 607, bits=34) - (1 << 33)
     assert bits_equal(hyd.sort_indices(dev(hyd, keys)).numpy(), orc.sort_indices(keys)[0])
 
 
