@@ -189,83 +189,106 @@ __device__ __forceinline__ double from_upper_lane(double v) { return dpp_shift(v
 
 // Temporal blocking: `steps` (<= kTK) stencil steps in one launch.  A
 // workgroup owns a kTR x kTC output tile and loads it with a kTK-cell halo:
-// a 64-row x 128-column box, lane l holding columns 2l and 2l+1 of the box,
-// each of the 4 waves 16 rows, all in registers.  (A 128-row box of 8 waves
-// recomputes fewer halo cells - 1.31x instead of 1.52x per output cell - but
-// measured slower on MI355X: 47.8 instead of 40 us per 8192^2 step.)  Each step recomputes the
-// whole box: north/south come from the wave's own registers (the rows just
-// outside its strip are exchanged through LDS), west/east from the pair
-// itself or the neighbouring lane (DPP); clamping applies at the grid edges
-// (EDGE workgroups only - interior boxes touch no grid edge).  The box edges
-// go stale by one cell per step, but the region still exact after step s is
-// the box shrunk by s cells (not shrunk at grid edges), so after kTK steps the
-// output tile holds exactly the values of kTK single-step launches: the same
-// operations on the same neighbour values, bit for bit.  HBM sees one read of
-// the haloed box and one write of the tile per `steps` steps.
+// a 64-row x 256-column box, lane l holding columns 4l .. 4l+3 of the box,
+// each of the 4 waves 16 rows, all in registers.  Four columns per lane need
+// one cross-lane move per cell and step (the two outer columns' neighbours),
+// against two with two columns per lane, and the wider box recomputes 1.42x
+// instead of 1.52x per output cell: the step is VALU-issue bound (7 fp64
+// operations per cell fixed by the body).  Measured on MI355X per 8192^2
+// step: 35.8 us, against 38.4 us with two columns per lane; taller boxes
+// (5, 6 or 8 waves; 20 or 24 rows per wave) and sweeps alternating direction
+// (no register copies, but 256+ VGPRs) ran slower, for lack of occupancy.
+// Each step recomputes the whole box: north/south come from
+// the wave's own registers (the rows just outside its strip are exchanged
+// through LDS), west/east from the lane's own columns or the neighbouring
+// lane (DPP); clamping applies at the grid edges (EDGE workgroups only -
+// interior boxes touch no grid edge).  The box edges go stale by one cell per
+// step, but the region still exact after step s is the box shrunk by s cells
+// (not shrunk at grid edges), so after kTK steps the output tile holds
+// exactly the values of kTK single-step launches: the same operations on the
+// same neighbour values, bit for bit.  HBM sees one read of the haloed box
+// and one write of the tile per `steps` steps.
 constexpr int kTK = 8;                  // fused steps
 constexpr int kTRows = 16;              // rows per wave
 constexpr int kTWaves = 4;
+constexpr int kTCols = 4;               // columns per lane
 constexpr int kTBoxR = kTRows * kTWaves;  // 64
-constexpr int kTBoxC = 128;               // 2 columns per lane
+constexpr int kTBoxC = 64 * kTCols;       // 256
 constexpr int kTR = kTBoxR - 2 * kTK;     // 48 output rows
-constexpr int kTC = kTBoxC - 2 * kTK;     // 112 output columns
+constexpr int kTC = kTBoxC - 2 * kTK;     // 240 output columns
 constexpr int kTBlock = 64 * kTWaves;
+// a lane's columns then lie all inside the output tile or all outside it
+static_assert(kTK % kTCols == 0 && kTC % kTCols == 0, "halo and tile widths must be whole lanes");
+
+struct LaneRow {  // one lane's columns of one row
+    double c[kTCols];
+};
 
 template <class Body, bool EDGE>
 __device__ __forceinline__ void temporal_box(const double* __restrict__ in, int64_t h, int64_t w,
                                              int steps, Body body, double* __restrict__ out,
-                                             int64_t r0, int64_t c0, v2d (*halo)[kTWaves][2][64]) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int64_t c = c0 - kTK + 2 * lane;                    // this lane's first column (even)
+                                             int64_t r0, int64_t c0, LaneRow (*halo)[kTWaves][2][64]) {
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // wave-uniform
+    const int64_t c = c0 - kTK + kTCols * lane;               // this lane's first column
     const int64_t rbase = r0 - kTK + (int64_t)wv * kTRows;  // this wave's first row
-    v2d v[kTRows];
+    LaneRow v[kTRows];
 #pragma unroll
     for (int k = 0; k < kTRows; ++k) {
         const int64_t r = rbase + k;
-        if (EDGE) {
-            const bool row_in = r >= 0 && r < h;
-            v[k].x = (row_in && c >= 0 && c < w) ? in[r * w + c] : 0.0;
-            v[k].y = (row_in && c + 1 >= 0 && c + 1 < w) ? in[r * w + c + 1] : 0.0;
-        } else {
-            v[k].x = in[r * w + c];
-            v[k].y = in[r * w + c + 1];
+#pragma unroll
+        for (int j = 0; j < kTCols; ++j) {
+            if (EDGE)
+                v[k].c[j] = (r >= 0 && r < h && c + j >= 0 && c + j < w) ? in[r * w + c + j] : 0.0;
+            else
+                v[k].c[j] = in[r * w + c + j];
         }
     }
-    // grid-edge flags (EDGE only): x-cell at column c, y-cell at column c + 1
-    const bool x_west_edge = c == 0, x_east_edge = c == w - 1, y_east_edge = c + 1 == w - 1;
     for (int s = 0; s < steps; ++s) {
         const int p = s & 1;
         halo[p][wv][0][lane] = v[0];
         halo[p][wv][1][lane] = v[kTRows - 1];
         __syncthreads();  // parity-double-buffered: one barrier per step
-        v2d prev = wv > 0 ? halo[p][wv - 1][1][lane] : v[0];  // row above the strip
-        const v2d dn = wv < kTWaves - 1 ? halo[p][wv + 1][0][lane] : v[kTRows - 1];
+        LaneRow prev = wv > 0 ? halo[p][wv - 1][1][lane] : v[0];  // row above the strip
+        const LaneRow dn = wv < kTWaves - 1 ? halo[p][wv + 1][0][lane] : v[kTRows - 1];
 #pragma unroll
         for (int k = 0; k < kTRows; ++k) {
-            const v2d cur = v[k];
-            v2d north = prev, south = (k + 1 < kTRows) ? v[k + 1] : dn;
-            double west_x = from_lower_lane(cur.y), east_x = cur.y;
-            double west_y = cur.x, east_y = from_upper_lane(cur.x);
+            const LaneRow cur = v[k];
+            const LaneRow south = k + 1 < kTRows ? v[k + 1] : dn;
+            const double from_west = from_lower_lane(cur.c[kTCols - 1]);
+            const double from_east = from_upper_lane(cur.c[0]);
+            bool top = false, bottom = false;
             if (EDGE) {
                 const int64_t r = rbase + k;
-                if (r == 0) north = cur;
-                if (r == h - 1) south = cur;
-                if (x_west_edge) west_x = cur.x;
-                if (x_east_edge) east_x = cur.x;
-                if (y_east_edge) east_y = cur.y;
+                top = r == 0;
+                bottom = r == h - 1;
+            }
+#pragma unroll
+            for (int j = 0; j < kTCols; ++j) {
+                double west = j == 0 ? from_west : cur.c[j - 1];
+                double east = j == kTCols - 1 ? from_east : cur.c[j + 1];
+                double nn = prev.c[j], ss = south.c[j];
+                if (EDGE) {
+                    if (top) nn = cur.c[j];
+                    if (bottom) ss = cur.c[j];
+                    if (c + j == 0) west = cur.c[j];
+                    if (c + j == w - 1) east = cur.c[j];
+                }
+                v[k].c[j] = body(nn, ss, west, east, cur.c[j]);
             }
             prev = cur;
-            v[k].x = body(north.x, south.x, west_x, east_x, cur.x);
-            v[k].y = body(north.y, south.y, west_y, east_y, cur.y);
         }
     }
+    // the lane's columns are all inside the output tile or all outside it
+    // (c0 - c and kTC are multiples of kTCols)
     const bool col_out = c >= c0 && c < c0 + kTC;
 #pragma unroll
     for (int k = 0; k < kTRows; ++k) {
         const int64_t r = rbase + k;
         if (col_out && r >= r0 && r < r0 + kTR && r < h) {
-            if (c < w) out[r * w + c] = v[k].x;
-            if (c + 1 < w) out[r * w + c + 1] = v[k].y;
+#pragma unroll
+            for (int j = 0; j < kTCols; ++j)
+                if (!EDGE || c + j < w) out[r * w + c + j] = v[k].c[j];
         }
     }
 }
@@ -276,7 +299,7 @@ __global__ __launch_bounds__(kTBlock) void stencil2d_temporal_kernel(const doubl
                                                                      Body body,
                                                                      double* __restrict__ out) {
     // [step parity][wave][first/last row of the strip][lane]
-    __shared__ v2d halo[2][kTWaves][2][64];
+    __shared__ LaneRow halo[2][kTWaves][2][64];
     const int64_t r0 = (int64_t)blockIdx.y * kTR, c0 = (int64_t)blockIdx.x * kTC;
     // interior: every box cell is inside the grid and off its edge rows/columns
     const bool interior = r0 - kTK >= 1 && r0 - kTK + kTBoxR <= h - 1 && c0 - kTK >= 1 &&
