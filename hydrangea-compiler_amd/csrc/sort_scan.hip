@@ -167,11 +167,19 @@ constexpr int kSortTile = kSortBlock * kSortRounds;  // 4096 keys per tile
 __global__ __launch_bounds__(kBlock) void max_key_kernel(const int64_t* __restrict__ keys, int64_t n,
                                                          unsigned long long* __restrict__ out) {
     // max(0, max key) as unsigned: the pass count only needs non-negative
-    // maxima (a negative maximum means one pass, like max_key = 0).
+    // maxima (a negative maximum means one pass, like max_key = 0).  Eight
+    // loads in flight per thread per trip.
     int64_t m = 0;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kBlock)
-        m = keys[i] > m ? keys[i] : m;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    for (; i + 7 * stride < n; i += 8 * stride) {
+        int64_t x[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) x[k] = __builtin_nontemporal_load(&keys[i + k * stride]);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) m = x[k] > m ? x[k] : m;
+    }
+    for (; i < n; i += stride) m = keys[i] > m ? keys[i] : m;
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
         int64_t o = __shfl_xor(m, off, 64);
@@ -181,8 +189,8 @@ __global__ __launch_bounds__(kBlock) void max_key_kernel(const int64_t* __restri
         __hip_atomic_fetch_max(out, (unsigned long long)m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Digits of 8 to 11 bits: a pass covers `BITS` bits of the key at `shift`.
-constexpr int kMaxDigitBits = 11;
+// Digits of 8 to 10 bits: a pass covers `BITS` bits of the key at `shift`.
+constexpr int kMaxDigitBits = 10;
 
 template <int BITS>
 __device__ __forceinline__ int digit_of(int64_t key, int shift) {
@@ -200,11 +208,15 @@ __global__ __launch_bounds__(kSortBlock) void radix_count_kernel(const int64_t* 
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kSortTile;
     unsigned int* mine = hist[threadIdx.x >> 6];
-#pragma unroll 4
+    int64_t k[kSortRounds];  // all of the thread's loads in flight at once
+#pragma unroll
     for (int r = 0; r < kSortRounds; ++r) {
-        int64_t e = base + (int64_t)r * kSortBlock + threadIdx.x;
-        if (e < n) atomicAdd(&mine[digit_of<BITS>(keys[e], shift)], 1u);
+        const int64_t e = base + (int64_t)r * kSortBlock + threadIdx.x;
+        k[r] = e < n ? keys[e] : 0;
     }
+#pragma unroll
+    for (int r = 0; r < kSortRounds; ++r)
+        if (base + (int64_t)r * kSortBlock + threadIdx.x < n) atomicAdd(&mine[digit_of<BITS>(k[r], shift)], 1u);
     __syncthreads();
     for (int d = threadIdx.x; d < D; d += kSortBlock)
         counts[(int64_t)d * ntiles + blockIdx.x] =
@@ -213,57 +225,130 @@ __global__ __launch_bounds__(kSortBlock) void radix_count_kernel(const int64_t* 
 
 // Stable scatter of one tile, keys with their payloads.  idx_in == nullptr
 // means the identity (the first pass of an index sort).
+//
+// The tile's 4096 elements are split into four contiguous quarters, one per
+// wave (element e = wave * 1024 + round * 64 + lane).  Each wave ranks its
+// quarter alone, with no workgroup barrier: per round, the lanes holding the
+// same digit find each other with wave64 ballots, read the wave's running
+// count of that digit from LDS and the first of them adds the group's size
+// (a wave's LDS operations take effect in program order).  One barrier later
+// the slot of an element is the tile's start of its digit (an exclusive scan
+// of the tile's digit counts, which the scanned count table holds as
+// differences) + the digit's counts in earlier waves + its rank in its wave.
+// The keys are then placed in LDS at their slots and written out in slot
+// order - consecutive threads write consecutive addresses of a digit's run -
+// and the payloads after them the same way.
 template <int BITS>
 __global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
     const int64_t* __restrict__ keys_in, const int64_t* __restrict__ idx_in, int64_t n, int shift,
     int64_t ntiles, const int64_t* __restrict__ offsets, int64_t* __restrict__ keys_out,
     int64_t* __restrict__ idx_out) {
     constexpr int D = 1 << BITS;
-    __shared__ int64_t run[D];            // next output slot per digit
-    __shared__ unsigned int wave_cnt[4][D];
+    constexpr int PER = D / kSortBlock;  // digits per thread in the tile scan
+    constexpr int kWaves = kSortBlock / 64;
+    constexpr int kQuarter = kSortTile / kWaves;  // elements per wave
+    static_assert(PER >= 1 && D % kSortBlock == 0 && kQuarter == 64 * kSortRounds, "tile shape");
+    __shared__ int64_t stage[kSortTile];        // keys, then payloads, in slot order
+    __shared__ uint16_t slot_digit[kSortTile];  // the digit of each slot
+    __shared__ int64_t gbase[D];                // global position of slot 0's digit run
+    __shared__ uint16_t wave_run[kWaves][D];    // per wave: its count of each digit, then its start
+    __shared__ unsigned wave_sum[kWaves];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int d = threadIdx.x; d < D; d += kSortBlock) run[d] = offsets[(int64_t)d * ntiles + blockIdx.x];
-    for (int i = threadIdx.x; i < 4 * D; i += kSortBlock) (&wave_cnt[0][0])[i] = 0;
-    __syncthreads();
-    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    const int64_t base = (int64_t)blockIdx.x * kSortTile;
+    const int64_t tile = blockIdx.x;
+    const int64_t base = tile * kSortTile;
+    const int tile_n = (int)(n - base < kSortTile ? n - base : kSortTile);
+
+    for (int i = threadIdx.x; i < kWaves * D; i += kSortBlock) (&wave_run[0][0])[i] = 0;
+    const int first = wave * kQuarter + lane;  // this lane's element of round 0
+    int64_t key[kSortRounds], idx[kSortRounds];
+#pragma unroll
     for (int r = 0; r < kSortRounds; ++r) {
-        const int64_t e = base + (int64_t)r * kSortBlock + threadIdx.x;
-        const bool valid = e < n;
-        int64_t key = valid ? keys_in[e] : 0;
-        int64_t idx = valid ? (idx_in ? idx_in[e] : e) : 0;
-        int d = valid ? digit_of<BITS>(key, shift) : 0;
-        // lanes of this wave holding the same digit (and a valid element)
+        const int e = first + r * 64;
+        key[r] = e < tile_n ? keys_in[base + e] : 0;
+        idx[r] = e < tile_n ? (idx_in ? idx_in[base + e] : base + e) : 0;
+    }
+    __syncthreads();
+
+    // rank within the wave's quarter
+    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    uint16_t* mine = wave_run[wave];
+    int rank[kSortRounds];
+#pragma unroll
+    for (int r = 0; r < kSortRounds; ++r) {
+        const bool valid = first + r * 64 < tile_n;
+        const int d = valid ? digit_of<BITS>(key[r], shift) : 0;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < BITS; ++b) {
-            uint64_t m = __ballot((d >> b) & 1);
+            const uint64_t m = __ballot((d >> b) & 1);
             peers &= ((d >> b) & 1) ? m : ~m;
         }
-        int rank_in_wave = __popcll(peers & lt_mask);
-        bool leader = valid && rank_in_wave == 0;
-        if (leader) wave_cnt[wave][d] = (unsigned)__popcll(peers);
-        __syncthreads();
-        // the first wave holding digit d (no earlier wave counted it) folds
-        // this round's counts of d into run[d] and clears them: only digits
-        // present in the round are touched, not all 2^BITS
-        bool folds = false;
-        if (valid) {
-            const int64_t r0 = run[d];
-            int64_t before = r0;
-            for (int w = 0; w < wave; ++w) before += wave_cnt[w][d];
-            folds = leader && before == r0;
-            const int64_t pos = before + rank_in_wave;
-            keys_out[pos] = key;
-            idx_out[pos] = idx;
-        }
-        __syncthreads();
-        if (folds) {
-            run[d] += (int64_t)wave_cnt[0][d] + wave_cnt[1][d] + wave_cnt[2][d] + wave_cnt[3][d];
-            wave_cnt[0][d] = wave_cnt[1][d] = wave_cnt[2][d] = wave_cnt[3][d] = 0;
-        }
-        __syncthreads();
+        const int below = __popcll(peers & lt_mask);
+        int at = 0;
+        if (valid) at = mine[d];
+        rank[r] = at + below;
+        if (valid && below == 0) mine[d] = (uint16_t)(at + __popcll(peers));
     }
+
+    // the tile's digit counts (differences of the digit-major scanned table),
+    // their exclusive scan, and each wave's start within each digit's run
+    unsigned cnt[PER], sum = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int64_t at = (int64_t)(threadIdx.x * PER + q) * ntiles + tile;
+        const int64_t next = at + 1 < (int64_t)D * ntiles ? offsets[at + 1] : n;
+        cnt[q] = (unsigned)(next - offsets[at]);
+        sum += cnt[q];
+    }
+    unsigned incl = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned o = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += o;
+    }
+    if (lane == 63) wave_sum[wave] = incl;
+    __syncthreads();  // wave sums and every wave's digit counts are in
+    unsigned excl = incl - sum;
+    for (int w = 0; w < wave; ++w) excl += wave_sum[w];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int d = threadIdx.x * PER + q;
+        gbase[d] = offsets[(int64_t)d * ntiles + tile] - excl;
+        unsigned o = excl;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const unsigned c = wave_run[w][d];
+            wave_run[w][d] = (uint16_t)o;
+            o += c;
+        }
+        excl += cnt[q];
+    }
+    __syncthreads();
+
+    // keys in slot order
+    int slot[kSortRounds];
+#pragma unroll
+    for (int r = 0; r < kSortRounds; ++r) {
+        slot[r] = 0;
+        if (first + r * 64 < tile_n) {
+            slot[r] = mine[digit_of<BITS>(key[r], shift)] + rank[r];
+            stage[slot[r]] = key[r];
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < tile_n; i += kSortBlock) {
+        const int64_t k = stage[i];
+        const int d = digit_of<BITS>(k, shift);
+        slot_digit[i] = (uint16_t)d;
+        keys_out[gbase[d] + i] = k;
+    }
+    __syncthreads();
+    // payloads in slot order
+#pragma unroll
+    for (int r = 0; r < kSortRounds; ++r)
+        if (first + r * 64 < tile_n) stage[slot[r]] = idx[r];
+    __syncthreads();
+    for (int i = threadIdx.x; i < tile_n; i += kSortBlock) idx_out[gbase[slot_digit[i]] + i] = stage[i];
 }
 
 __global__ void copy_i64_kernel(const int64_t* __restrict__ src, int64_t n, int64_t* __restrict__ dst) {
@@ -286,7 +371,7 @@ inline unsigned grid_for(int64_t n, int block = 256, int64_t cap = 8192) {
 }  // namespace
 
 // Workspace layout for sort_pairs_i64: keys x2, payload x1 (the output is
-// the other), digit counts 2^11 * ntiles, scan tile sums, 1 word for the max key.
+// the other), digit counts 2^10 * ntiles, scan tile sums, 1 word for the max key.
 int64_t sort_workspace_words(int64_t n) {
     int64_t ntiles = ceil_div(std::max<int64_t>(n, 1), kSortTile);
     const int64_t ncounts = ((int64_t)1 << kMaxDigitBits) * ntiles;
@@ -294,7 +379,7 @@ int64_t sort_workspace_words(int64_t n) {
 }
 
 // Digit widths for sorting the low `bits` bits: the fewest passes of at most
-// 11 bits, the widths as even as possible and at least 8 (a pass may cover
+// 10 bits, the widths as even as possible and at least 8 (a pass may cover
 // bits above `bits`, which are zero for keys that fit).
 static int digit_widths(int bits, int* widths) {
     bits = std::max(bits, 8);
@@ -322,7 +407,7 @@ static int radix_pass(const int64_t* kin, const int64_t* iin, int64_t n, int shi
 // 8 * B bits, B = the number of bytes of max(0, max key) and at least one,
 // as the reference's byte passes do (negative keys included); the max is
 // read back to the host once.  max_key >= 0 promises keys in [0, max_key]
-// and sorts just its bits.  Each pass takes a digit of 8 to 11 bits.
+// and sorts just its bits.  Each pass takes a digit of 8 to 10 bits.
 // Sorted payloads go to payload_out, sorted keys to keys_out when it is
 // non-null; *passes_out gets the reference's byte-pass count.  `ws` holds
 // sort_workspace_words(n) words.
@@ -368,8 +453,7 @@ int sort_pairs_i64(const int64_t* keys, const int64_t* payload, int64_t n, int64
         switch (widths[p]) {
             case 8: HYD_TRY(radix_pass<8>(kin, iin, n, shift, ntiles, counts, scan_ws, kout, iout, s)); break;
             case 9: HYD_TRY(radix_pass<9>(kin, iin, n, shift, ntiles, counts, scan_ws, kout, iout, s)); break;
-            case 10: HYD_TRY(radix_pass<10>(kin, iin, n, shift, ntiles, counts, scan_ws, kout, iout, s)); break;
-            default: HYD_TRY(radix_pass<11>(kin, iin, n, shift, ntiles, counts, scan_ws, kout, iout, s)); break;
+            default: HYD_TRY(radix_pass<10>(kin, iin, n, shift, ntiles, counts, scan_ws, kout, iout, s)); break;
         }
         shift += widths[p];
         kin = kout;
