@@ -511,6 +511,74 @@ __global__ void pack_keys_kernel(const int64_t* __restrict__ rows, const int64_t
         keys[i] = (int64_t)((uint64_t)rows[i] * (uint64_t)ncols + (uint64_t)cols[i]);
 }
 
+// pack_keys_kernel plus a range check: info[0] = the largest key among the
+// elements with 0 <= row < nrows and 0 <= col < ncols, info[1] != 0 when
+// any element is outside that box (info zeroed by the caller).  Inside it
+// (with nrows * ncols < 2^62) the packing is one-to-one, so equal keys mean
+// equal (row, col) pairs.
+__global__ __launch_bounds__(256) void pack_keys_check_kernel(const int64_t* __restrict__ rows,
+                                                              const int64_t* __restrict__ cols, int64_t n,
+                                                              int64_t nrows, int64_t ncols,
+                                                              int64_t* __restrict__ keys,
+                                                              unsigned long long* __restrict__ info) {
+    uint64_t mx = 0;
+    bool bad = false;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = rows[i], c = cols[i];
+        const uint64_t k = (uint64_t)r * (uint64_t)ncols + (uint64_t)c;
+        keys[i] = (int64_t)k;
+        const bool in = r >= 0 && r < nrows && c >= 0 && c < ncols;
+        bad |= !in;
+        if (in) mx = k > mx ? k : mx;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const uint64_t o = __shfl_xor(mx, off, 64);
+        mx = o > mx ? o : mx;
+    }
+    const bool any_bad = __any(bad);
+    if ((threadIdx.x & 63) == 0) {
+        __hip_atomic_fetch_max(&info[0], (unsigned long long)mx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (any_bad) __hip_atomic_fetch_max(&info[1], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Keyed duplicate merge over keys sorted with their values: a run of equal
+// keys is a run of equal (row, col) pairs.  head[i] = 1 at a run's first
+// element.
+__global__ void key_heads_kernel(const int64_t* __restrict__ skeys, int64_t n, int64_t* __restrict__ head) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        head[i] = (i == 0 || skeys[i] != skeys[i - 1]) ? 1 : 0;
+}
+
+// As coo_run_ends_kernel, with row and col decoded from the key (a shift and
+// a mask when ncols is a power of two: col_shift >= 0).
+__global__ void key_run_ends_kernel(const int64_t* __restrict__ skeys, const int64_t* __restrict__ head,
+                                    const int64_t* __restrict__ slot_incl, const int64_t* __restrict__ psum,
+                                    int64_t n, int64_t ncols, int col_shift, int64_t* __restrict__ out_rows,
+                                    int64_t* __restrict__ out_cols, int64_t* __restrict__ end_p) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t slot = slot_incl[i] - 1;
+        if (head[i]) {
+            const uint64_t k = (uint64_t)skeys[i];
+            uint64_t r, c;
+            if (col_shift >= 0) {
+                r = k >> col_shift;
+                c = k & (((uint64_t)1 << col_shift) - 1);
+            } else {
+                r = k / (uint64_t)ncols;
+                c = k - r * (uint64_t)ncols;
+            }
+            out_rows[slot] = (int64_t)r;
+            out_cols[slot] = (int64_t)c;
+        }
+        if (i == n - 1 || head[i + 1]) end_p[slot] = psum[i];
+    }
+}
+
 // head[i] = 1 when element i starts a new (row, col) run of the (optionally
 // permuted) sequence.
 __global__ void coo_heads_kernel(const int64_t* __restrict__ perm, const int64_t* __restrict__ rows,
@@ -932,6 +1000,35 @@ int coo_sum_duplicates_i64(const int64_t* perm, const int64_t* rows, const int64
 
 int64_t coo_workspace_words(int64_t n) { return 5 * n + scan_workspace_words(n) + 8; }
 
+// coo_sum_duplicates over (key, value) pairs already sorted by key, the keys
+// one-to-one with (row, col) (row * ncols + col inside the box).  Same
+// outputs and workspace as coo_sum_duplicates_i64.
+static int coo_sum_duplicates_keyed_i64(const int64_t* skeys, const int64_t* svals, int64_t n, int64_t ncols,
+                                        int64_t len, int64_t* out_rows, int64_t* out_cols, int64_t* out_vals,
+                                        int64_t* d_nnz_out, hipStream_t s, int64_t* ws) {
+    HYD_TRY(fill_zero(out_rows, (size_t)len * 8, s));
+    HYD_TRY(fill_zero(out_cols, (size_t)len * 8, s));
+    HYD_TRY(fill_zero(out_vals, (size_t)len * 8, s));
+    if (n == 0) return fill_zero(d_nnz_out, 8, s);
+    int64_t* head = ws;
+    int64_t* incl = ws + n;
+    int64_t* psum = ws + 3 * n;
+    int64_t* end_p = ws + 4 * n;
+    int64_t* scan_ws = ws + 5 * n;
+    key_heads_kernel<<<grid_for(n), 256, 0, s>>>(skeys, n, head);
+    HYD_CHECK_LAUNCH();
+    HYD_TRY(scan_i64(head, n, 0, incl, true, false, s, scan_ws));
+    HYD_TRY(scan_i64(svals, n, 0, psum, true, false, s, scan_ws));
+    const int col_shift = (ncols > 0 && (ncols & (ncols - 1)) == 0) ? __builtin_ctzll((uint64_t)ncols) : -1;
+    key_run_ends_kernel<<<grid_for(n), 256, 0, s>>>(skeys, head, incl, psum, n, ncols, col_shift, out_rows,
+                                                    out_cols, end_p);
+    HYD_CHECK_LAUNCH();
+    coo_run_sums_kernel<<<grid_for(n), 256, 0, s>>>(end_p, incl + (n - 1), n, out_vals);
+    HYD_CHECK_LAUNCH();
+    HYD_TRY_HIP(hipMemcpyAsync(d_nnz_out, incl + (n - 1), 8, hipMemcpyDeviceToDevice, s));
+    return HYD_OK;
+}
+
 // csr_from_sorted_coo with nnz known on the host.  Row counts without atomics
 // when the rows are sorted (the usual case: the COO pipeline sorts by
 // row-major key), one atomic per element otherwise; a device flag picks the
@@ -1138,28 +1235,56 @@ extern "C" int hydhip_coo_to_csr_i64(int64_t nrows, int64_t ncols, int64_t nnz, 
 }
 
 namespace hyd {
-// Words of workspace coo_to_csr_ws needs: keys, perm, canonical
-// rows/cols/vals, and the largest of the sort / merge / CSR scratch.
+// Words of workspace coo_to_csr_ws needs: keys, perm (or sorted keys),
+// sorted values, canonical rows/cols/vals, and the largest of the sort /
+// merge / CSR scratch, then 2 words of range-check results.
 int64_t coo_to_csr_workspace_words(int64_t nnz, int64_t nrows) {
     int64_t scratch = std::max(sort_workspace_words(nnz),
                                std::max(coo_workspace_words(nnz), csr_workspace_words(nrows)));
-    return 5 * nnz + scratch + 8;
+    return 6 * nnz + scratch + 8;
 }
 
 int coo_to_csr_ws(int64_t nrows, int64_t ncols, int64_t nnz, const int64_t* rows, const int64_t* cols,
                   const int64_t* vals, int64_t* row_ptr, int64_t* col_idx, int64_t* out_vals,
                   int64_t* nnz_out, hipStream_t s, int64_t* ws) {
-    int64_t scratch = coo_to_csr_workspace_words(nnz, nrows) - 5 * nnz - 8;
+    int64_t scratch = coo_to_csr_workspace_words(nnz, nrows) - 6 * nnz - 8;
     int64_t* keys = ws;
-    int64_t* perm = ws + nnz;
+    int64_t* perm = ws + nnz;  // or the sorted keys
     int64_t* crow = ws + 2 * nnz;
     int64_t* ccol = ws + 3 * nnz;
     int64_t* cval = ws + 4 * nnz;
-    int64_t* scr = ws + 5 * nnz;
+    int64_t* svals = ws + 5 * nnz;
+    int64_t* scr = ws + 6 * nnz;
     int64_t* d_cnt = scr + scratch;
-    HYD_TRY(pack_keys_i64(rows, cols, nnz, ncols, keys, s));
-    HYD_TRY(sort_indices_i64(keys, nnz, perm, s, scr, nullptr));
-    HYD_TRY(coo_sum_duplicates_i64(perm, rows, cols, vals, nnz, nnz, crow, ccol, cval, d_cnt, s, scr));
+    unsigned long long* info = reinterpret_cast<unsigned long long*>(d_cnt + 1);  // 2 words
+    // Inside the box [0, nrows) x [0, ncols) (and nrows * ncols < 2^62) the
+    // packed key is one-to-one with (row, col): sort (key, value) pairs and
+    // merge runs of equal keys, with no gathers through a permutation.  Any
+    // element outside it takes the reference's sequence: sort_indices, then
+    // the merge comparing the (row, col) pairs themselves.
+    const bool box_fits = nrows > 0 && ncols > 0 && (unsigned __int128)nrows * (unsigned __int128)ncols <
+                                                         ((unsigned __int128)1 << 62);
+    bool keyed = false;
+    unsigned long long range[2] = {0, 1};
+    if (nnz > 0 && box_fits) {
+        HYD_TRY(fill_zero(info, 16, s));
+        pack_keys_check_kernel<<<grid_for(nnz, 256, 1024), 256, 0, s>>>(rows, cols, nnz, nrows, ncols, keys, info);
+        HYD_CHECK_LAUNCH();
+        HYD_TRY_HIP(hipMemcpyAsync(range, info, 16, hipMemcpyDeviceToHost, s));
+        HYD_TRY_HIP(hipStreamSynchronize(s));
+        keyed = range[1] == 0;
+    } else {
+        HYD_TRY(pack_keys_i64(rows, cols, nnz, ncols, keys, s));
+    }
+    if (keyed) {
+        // the keys are non-negative and at most range[0]: sorting their bits
+        // orders them as the reference's byte passes do
+        HYD_TRY(sort_pairs_i64(keys, vals, nnz, (int64_t)range[0], perm, svals, s, scr, nullptr));
+        HYD_TRY(coo_sum_duplicates_keyed_i64(perm, svals, nnz, ncols, nnz, crow, ccol, cval, d_cnt, s, scr));
+    } else {
+        HYD_TRY(sort_indices_i64(keys, nnz, perm, s, scr, nullptr));
+        HYD_TRY(coo_sum_duplicates_i64(perm, rows, cols, vals, nnz, nnz, crow, ccol, cval, d_cnt, s, scr));
+    }
     int64_t cnt = 0;
     HYD_TRY_HIP(hipMemcpyAsync(&cnt, d_cnt, 8, hipMemcpyDeviceToHost, s));
     HYD_TRY_HIP(hipStreamSynchronize(s));

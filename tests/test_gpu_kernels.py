@@ -380,3 +380,22 @@ def test_coo_to_csr_random(hyd, orc, nnz, nrows, ncols, dupbits):
     # invariants: row_ptr monotone, row_ptr[nrows] == merged count
     r = rp.numpy()
     assert np.all(np.diff(r) >= 0) and r[-1] == m
+
+
+@pytest.mark.parametrize("nrows,ncols,colrange", [(1000, 10, (-5, 25)), (4, 1 << 61, (0, 1 << 61))])
+def test_coo_to_csr_outside_the_box(hyd, orc, nrows, ncols, colrange):
+    # Columns outside [0, ncols) pack onto other rows' keys ((1, 5) and (0, 15)
+    # both give 15 when ncols = 10), and nrows * ncols >= 2^62 wraps the keys:
+    # both take the pair-comparing merge, as the reference's loop does.
+    nnz = 20_000
+    rows = orc.splitmix_i64(nnz, 811, bits=40) % nrows
+    lo, hi = colrange
+    cols = lo + (orc.splitmix_i64(nnz, 812, bits=62) % (hi - lo))
+    cols[:4] = [5, 15, 15, 5] if ncols == 10 else cols[:4]
+    rows[:4] = [1, 0, 0, 1] if ncols == 10 else rows[:4]
+    vals = orc.splitmix_i64(nnz, 813, bits=63) - (1 << 62)
+    print("coo outside box nrows=%d ncols=%d" % (nrows, ncols), flush=True)
+    rp, ci, cv, m = hyd.coo_to_csr(nrows, ncols, dev(hyd, rows), dev(hyd, cols), dev(hyd, vals))
+    rrp, rci, rcv, rm = orc.coo_to_csr(nrows, ncols, rows, cols, vals)
+    assert m == rm
+    assert bits_equal(rp.numpy(), rrp) and bits_equal(ci.numpy(), rci) and bits_equal(cv.numpy(), rcv)
