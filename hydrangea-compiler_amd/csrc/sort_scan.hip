@@ -351,18 +351,6 @@ __global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
     for (int i = threadIdx.x; i < tile_n; i += kSortBlock) idx_out[gbase[slot_digit[i]] + i] = stage[i];
 }
 
-__global__ void copy_i64_kernel(const int64_t* __restrict__ src, int64_t n, int64_t* __restrict__ dst) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
-}
-
-__global__ void iota_kernel(int64_t n, int64_t* __restrict__ dst) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        dst[i] = i;
-}
-
 inline unsigned grid_for(int64_t n, int block = 256, int64_t cap = 8192) {
     int64_t g = ceil_div(n, block);
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, cap));
