@@ -158,7 +158,8 @@ __device__ __forceinline__ void load_pairs(const double* __restrict__ xs, const 
 }
 
 // One workgroup per CU with 6 pairs per array in flight (measured best on
-// MI355X: 6.4 TB/s).  The exp table is copied into LDS: with the lookups as
+// MI355X: 6.4 TB/s; software-pipelining over two register sets, or two
+// workgroups per CU, measured slower).  The exp table is copied into LDS: with the lookups as
 // global loads they shared the L1 path with the stream, and the kernel then
 // needed 8 workgroups per CU to reach 5.4 TB/s.
 template <bool VEC, int D>
