@@ -28,6 +28,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "hydrangea-compiler_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
+# the CPU oracle's OpenMP threads: all of the host's cores, bound close (read
+# when the oracle library loads; OMP_NUM_THREADS, when set, gives the count)
+os.environ.setdefault("OMP_PROC_BIND", "close")
 
 import hydhip  # noqa: E402
 import workloads  # noqa: E402
