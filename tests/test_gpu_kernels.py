@@ -368,7 +368,8 @@ def test_coo_known(hyd):
 
 @pytest.mark.parametrize("nnz,nrows,ncols,dupbits", [(1, 1, 1, 0), (1000, 50, 40, 4),
                                                      (200_000, 1 << 20, 1 << 20, 0),
-                                                     (300_000, 1000, 3, 0)])
+                                                     (300_000, 1000, 3, 0),
+                                                     (100_000, 1_000_003, 999_983, 0)])
 def test_coo_to_csr_random(hyd, orc, nnz, nrows, ncols, dupbits):
     rows = orc.splitmix_i64(nnz, 801, bits=40) % nrows
     cols = orc.splitmix_i64(nnz, 802, bits=40) % ncols
