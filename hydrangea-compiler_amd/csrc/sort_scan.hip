@@ -105,7 +105,8 @@ __global__ __launch_bounds__(kBlock) void scan_sums_kernel(int64_t* __restrict__
     }
 }
 
-// Per tile: out[e] = init + (exclusive or inclusive) prefix of the sequence.
+// Per tile: out[e] = init + (exclusive or inclusive) prefix of the sequence
+// (tile_off NULL: a single tile, offset 0).
 __global__ __launch_bounds__(kBlock) void scan_apply(const int64_t* __restrict__ in, int64_t n,
                                                      bool reverse, bool inclusive, int64_t init,
                                                      const int64_t* __restrict__ tile_off,
@@ -120,7 +121,7 @@ __global__ __launch_bounds__(kBlock) void scan_apply(const int64_t* __restrict__
         local += v[k];
     }
     int64_t total;
-    int64_t acc = block_excl_scan(local, &total) + tile_off[blockIdx.x] + init;
+    int64_t acc = block_excl_scan(local, &total) + (tile_off ? tile_off[blockIdx.x] : 0) + init;
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         int64_t e = base + k;
@@ -146,6 +147,11 @@ int scan_i64(const int64_t* in, int64_t n, int64_t init, int64_t* out, bool incl
     if (n == 0) return HYD_OK;
     int64_t tiles = ceil_div(n, kTile);
     if (tiles > 0x7fffffff) return set_error(HYD_E_INVALID, "scan: too many elements");
+    if (tiles == 1) {  // one launch instead of three
+        scan_apply<<<1, kBlock, 0, s>>>(in, n, reverse, inclusive, init, nullptr, out);
+        HYD_CHECK_LAUNCH();
+        return HYD_OK;
+    }
     scan_tile_sums<<<(unsigned)tiles, kBlock, 0, s>>>(in, n, reverse, tile_ws);
     HYD_CHECK_LAUNCH();
     scan_sums_kernel<<<1, kBlock, 0, s>>>(tile_ws, tiles);
