@@ -223,14 +223,23 @@ int64_t hyd_hip_bench(const char* name, void (*body)(void*), void* arg, int64_t 
         fprintf(stderr, "hyd_hip_bench: bad arguments\n");
         exit(1);
     }
+    // a kernel that failed asynchronously surfaces at the wait: the drop-in's
+    // error behaviour (message, exit status 1), not a timing of broken work
+    auto wait_gpu = [&]() {
+        const hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) {
+            fprintf(stderr, "hydrangea: hyd_hip_bench(%s): %s\n", name ? name : "", hipGetErrorString(e));
+            exit(1);
+        }
+    };
     for (int64_t w = 0; w < warmup; ++w) body(arg);
-    (void)hipDeviceSynchronize();
+    wait_gpu();
     double mn = 0.0, sum = 0.0;
     for (int64_t i = 0; i < iters; ++i) {
         struct timespec t0, t1;
         clock_gettime(CLOCK_MONOTONIC, &t0);
         body(arg);
-        (void)hipDeviceSynchronize();
+        wait_gpu();
         clock_gettime(CLOCK_MONOTONIC, &t1);
         const double ns = (double)(t1.tv_sec - t0.tv_sec) * 1.0e9 + (double)(t1.tv_nsec - t0.tv_nsec);
         if (i == 0 || ns < mn) mn = ns;
