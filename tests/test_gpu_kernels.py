@@ -236,7 +236,13 @@ def test_stencil_known_answers(hyd):
     assert bits_equal(hyd.stencil2d(dev(hyd, c), hyd.STENCIL_HEAT, 0.1).numpy(), c)
 
 
-@pytest.mark.parametrize("shape", SHAPES + [(49, 97), (96, 96), (200, 47)])
+# the temporal kernel's output tile is 48 rows x 240 columns (a 64 x 256 box,
+# 4 columns per lane): widths at tile edges, and 489 where the second tile
+# column is the first interior one (its box ends at column 488 = w - 1)
+TEMPORAL_EDGES = [(97, 240), (49, 241), (130, 481), (300, 489), (113, 720)]
+
+
+@pytest.mark.parametrize("shape", SHAPES + [(49, 97), (96, 96), (200, 47)] + TEMPORAL_EDGES)
 @pytest.mark.parametrize("body", [0, 1, 2, 3])
 def test_iterate_temporal_blocks(hyd, orc, shape, body):
     # 11 steps = one fused pass of 8 and one of 3, on tile-unaligned shapes
