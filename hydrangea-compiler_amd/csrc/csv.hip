@@ -535,14 +535,36 @@ __device__ __forceinline__ int dig_len(const Dig& d) {
     return (d.flags & 1) ? special_len(d.N) : g17_len(g17_make(d.flags & 2, d.N, d.X));
 }
 
+// A finite nonzero value's table digits in one word for the write kernel:
+// N - 10^16 (< 9 * 10^16 < 2^57) in the low 57 bits, X - dec17_x0(bits)
+// (0, 1 or 2) above them.  Zero, inf and nan are told by the value's bits.
+__device__ __forceinline__ uint64_t pack_digits(uint64_t b, const Dig& d) {
+    return (d.flags & 1) ? 0 : (d.N - kTen16) | ((uint64_t)(d.X - dec17_x0(b & ~kSignBit)) << 57);
+}
+__device__ __forceinline__ Dig unpack_digits(uint64_t b, uint64_t packed) {
+    Dig d;
+    if (special_len(b)) {
+        d.N = b;
+        d.X = 0;
+        d.flags = 1;
+    } else {
+        d.N = (packed & ((1ULL << 57) - 1)) + kTen16;
+        d.X = dec17_x0(b & ~kSignBit) + (int)(packed >> 57);
+        d.flags = (b >> 63) ? 2 : 0;
+    }
+    return d;
+}
+
 // Text length of every 1024-value block (with separators).  fp64 values the
 // table path cannot round with certainty are marked (their provisional length
-// is corrected by fmt_exact_kernel).
+// is corrected by fmt_exact_kernel).  The fp64 digits go to `packed` (one
+// word per value) so that the write kernel does not compute them again.
 template <bool FLOAT, bool FORCE>
 __global__ __launch_bounds__(kFThreads) void fmt_len_kernel(const uint64_t* __restrict__ vals, int64_t n,
                                                             int64_t* __restrict__ block_len,
                                                             uint32_t* __restrict__ exact_bits,
-                                                            unsigned long long* __restrict__ exact_cnt) {
+                                                            unsigned long long* __restrict__ exact_cnt,
+                                                            uint64_t* __restrict__ packed) {
     __shared__ int64_t sh[kWaves];
     const int64_t i0 = (int64_t)blockIdx.x * kFBlock + threadIdx.x * kFPer;
     int64_t len = 0;
@@ -552,7 +574,9 @@ __global__ __launch_bounds__(kFThreads) void fmt_len_kernel(const uint64_t* __re
         if (i >= n) continue;
         if (FLOAT) {
             bool decided;
-            const Dig d = table_digits<FORCE>(vals[i], &decided);
+            const uint64_t b = vals[i];
+            const Dig d = table_digits<FORCE>(b, &decided);
+            packed[i] = pack_digits(b, d);
             len += dig_len(d) + 1;
             if (!decided) {
                 atomicOr(&exact_bits[i >> 5], 1u << (i & 31));
@@ -606,7 +630,8 @@ __global__ __launch_bounds__(kFThreads) void fmt_write_kernel(const uint64_t* __
                                                               unsigned char* __restrict__ text,
                                                               const Dig* __restrict__ exact_dig,
                                                               const uint32_t* __restrict__ exact_bits,
-                                                              const unsigned long long* __restrict__ exact_cnt) {
+                                                              const unsigned long long* __restrict__ exact_cnt,
+                                                              const uint64_t* __restrict__ packed) {
     __shared__ int64_t sh[kWaves];
     __shared__ __attribute__((aligned(16))) unsigned char stage[kStage];
     const int64_t i0 = (int64_t)blockIdx.x * kFBlock + threadIdx.x * kFPer;
@@ -620,8 +645,7 @@ __global__ __launch_bounds__(kFThreads) void fmt_write_kernel(const uint64_t* __
         lens[k] = 0;
         if (i >= n) continue;
         if (FLOAT) {
-            bool decided;
-            dg[k] = table_digits(vals[i], &decided);
+            dg[k] = unpack_digits(vals[i], packed[i]);
             if (any_exact && ((exact_bits[i >> 5] >> (i & 31)) & 1)) dg[k] = exact_dig[i];
             lens[k] = dig_len(dg[k]) + 1;
         } else {
@@ -761,7 +785,7 @@ int csv_format(const uint64_t* vals, int64_t count, int64_t row_len, bool is_flo
     const int64_t blocks = ceil_div(count, kFBlock);
     if (blocks > 0x7fffffff) return set_error(HYD_E_INVALID, "csv: too many values");
     const int64_t bw = bitmap_words64(count);
-    const int64_t dig_words = is_float ? 2 * count : 0;
+    const int64_t dig_words = is_float ? 3 * count : 0;  // sparse exact digits, packed table digits
     const int64_t words = 2 * blocks + scan_workspace_words(blocks) + 8 + bw + dig_words;
     int64_t* ws = static_cast<int64_t*>(stream_workspace(s, (size_t)words * 8));
     if (!ws) return set_error(HYD_E_OOM, "csv: workspace of %lld words", (long long)words);
@@ -772,15 +796,19 @@ int csv_format(const uint64_t* vals, int64_t count, int64_t row_len, bool is_flo
     unsigned long long* exact_cnt = reinterpret_cast<unsigned long long*>(misc);
     uint32_t* exact_bits = reinterpret_cast<uint32_t*>(misc + 8);
     Dig* dig = reinterpret_cast<Dig*>(misc + 8 + bw);
+    uint64_t* packed = reinterpret_cast<uint64_t*>(misc + 8 + bw + 2 * count);
     const unsigned g = (unsigned)blocks;
     HYD_TRY(fill_zero(misc, (size_t)(8 + bw) * 8, s));
     if (is_float) {
-        if (g_force_exact) fmt_len_kernel<true, true><<<g, kFThreads, 0, s>>>(vals, count, block_len, exact_bits, exact_cnt);
-        else fmt_len_kernel<true, false><<<g, kFThreads, 0, s>>>(vals, count, block_len, exact_bits, exact_cnt);
+        if (g_force_exact)
+            fmt_len_kernel<true, true><<<g, kFThreads, 0, s>>>(vals, count, block_len, exact_bits, exact_cnt, packed);
+        else
+            fmt_len_kernel<true, false><<<g, kFThreads, 0, s>>>(vals, count, block_len, exact_bits, exact_cnt, packed);
         HYD_CHECK_LAUNCH();
         fmt_exact_kernel<<<64, 64, 0, s>>>(vals, count, dig, exact_bits, exact_cnt, block_len);
     } else {
-        fmt_len_kernel<false, false><<<g, kFThreads, 0, s>>>(vals, count, block_len, exact_bits, exact_cnt);
+        fmt_len_kernel<false, false><<<g, kFThreads, 0, s>>>(vals, count, block_len, exact_bits, exact_cnt,
+                                                             nullptr);
     }
     HYD_CHECK_LAUNCH();
     HYD_TRY(scan_i64(block_len, blocks, 0, block_off, false, false, s, scan_ws));
@@ -795,10 +823,11 @@ int csv_format(const uint64_t* vals, int64_t count, int64_t row_len, bool is_flo
         return set_error(HYD_E_INVALID, "csv: text needs %lld bytes, capacity %lld", (long long)total,
                          (long long)capacity);
     if (is_float)
-        fmt_write_kernel<true><<<g, kFThreads, 0, s>>>(vals, count, row_len, block_off, text, dig, exact_bits, exact_cnt);
+        fmt_write_kernel<true><<<g, kFThreads, 0, s>>>(vals, count, row_len, block_off, text, dig, exact_bits,
+                                                       exact_cnt, packed);
     else
         fmt_write_kernel<false><<<g, kFThreads, 0, s>>>(vals, count, row_len, block_off, text, dig, exact_bits,
-                                                        exact_cnt);
+                                                        exact_cnt, nullptr);
     HYD_CHECK_LAUNCH();
     *nbytes = total;
     return HYD_OK;

@@ -732,6 +732,18 @@ HYD_HD uint64_t dec_exact_bits(const At& at, int64_t p, uint64_t cand, Big& D, B
 // rounding: then N holds the table's floor of |v| * 10^(16 - X) (exact, or one
 // above a value less than half a unit below it) and X the scale used.
 // FORCE (test builds only) hands every value to the exact check.
+// The estimate x0 = floor(e2 * log10 2) of a finite nonzero |v| = m * 2^e
+// (e2 = floor(log2 |v|)), from which f64_to_dec17 starts: its X is x0, x0 + 1
+// (the estimate one low) or x0 + 2 (17 nines rounded up to 10^17).
+HYD_HD int dec17_x0(uint64_t bits) {
+    const uint64_t frac = bits & ((1ULL << 52) - 1);
+    const int be = (int)((bits >> 52) & 0x7FF);
+    const uint64_t m = be ? (frac | (1ULL << 52)) : frac;
+    const int e = be ? be - 1075 : -1074;
+    const int e2 = e + (63 - clz64(m));
+    return (e2 * 78913) >> 18;
+}
+
 template <bool FORCE = false>
 HYD_HD bool f64_to_dec17(uint64_t bits, uint64_t* N, int* X) {
     const uint64_t frac = bits & ((1ULL << 52) - 1);
@@ -739,8 +751,7 @@ HYD_HD bool f64_to_dec17(uint64_t bits, uint64_t* N, int* X) {
     const uint64_t m = be ? (frac | (1ULL << 52)) : frac;
     const int e = be ? be - 1075 : -1074;
     const int lz = clz64(m);
-    const int e2 = e + (63 - lz);
-    int x0 = (e2 * 78913) >> 18;  // floor(e2 * log10(2)): x0 <= X <= x0 + 1
+    int x0 = dec17_x0(bits);  // floor(e2 * log10(2)): x0 <= X <= x0 + 1 before rounding
     const uint64_t mn = m << lz;
     for (int pass = 0;; ++pass) {
         const int s = 16 - x0;
