@@ -28,9 +28,10 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "hydrangea-compiler_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
-# the CPU oracle's OpenMP threads: all of the host's cores, bound close (read
-# when the oracle library loads; OMP_NUM_THREADS, when set, gives the count)
-os.environ.setdefault("OMP_PROC_BIND", "close")
+# The CPU oracle's OpenMP threads are left unbound: with OMP_PROC_BIND set,
+# libgomp pins the process's main thread to one core once the oracle runs,
+# and every thread started after that (the serving workers, the parallel file
+# reads) inherits the one-core mask.
 
 import hydhip  # noqa: E402
 import workloads  # noqa: E402
