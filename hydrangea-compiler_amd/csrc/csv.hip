@@ -658,8 +658,9 @@ __global__ __launch_bounds__(kFThreads) void fmt_write_kernel(const uint64_t* __
     const int64_t gbase = block_off[blockIdx.x];
     // LDS byte j holds text[gbase - lead + j], with text + gbase - lead 16-byte aligned
     const int lead = (int)((reinterpret_cast<uintptr_t>(text) + (uintptr_t)gbase) & 15);
-    auto put = [&](int64_t pos, char ch) { stage[lead + pos] = (unsigned char)ch; };
-    int64_t pos = off;
+    // positions are 32-bit offsets into the block's text (< kStage bytes)
+    auto put = [&](int pos, char ch) { stage[lead + pos] = (unsigned char)ch; };
+    int pos = (int)off;
     int64_t col = i0 < n ? i0 % row_len : 0;  // row position: ',' after an element, '\n' after a row's last
 #pragma unroll
     for (int k = 0; k < kFPer; ++k) {

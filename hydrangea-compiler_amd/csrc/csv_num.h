@@ -824,20 +824,6 @@ HYD_HD int i64_len(int64_t v) {
     return (v < 0) + (a ? dec_digits(a) : 1);
 }
 
-template <class Put>
-HYD_HD int i64_write(int64_t v, const Put& put, int64_t pos) {
-    const uint64_t a0 = v < 0 ? 0 - (uint64_t)v : (uint64_t)v;
-    const int len = i64_len(v);
-    if (v < 0) put(pos, '-');
-    uint64_t a = a0;
-    int64_t q = pos + len - 1;
-    do {
-        put(q--, (char)('0' + a % 10));
-        a /= 10;
-    } while (a);
-    return len;
-}
-
 // 8 ASCII digits of v < 10^8, the most significant in the lowest byte: the two
 // 4-digit halves, then their 2-digit halves, then single digits, each split
 // by a multiply-shift division in parallel 32- / 16-bit lanes (exact for the
@@ -850,6 +836,36 @@ HYD_HD uint64_t ascii8(uint32_t v) {
     x = z | ((x - z * 10) << 8);
     return x | 0x3030303030303030ULL;
 }
+
+// Writes %lld of v at pos (Pos: the caller's position type - 32-bit offsets
+// into a workgroup's staging buffer on the device).  The 19-digit decimal
+// string of |v| is 3 leading digits and two 8-digit ASCII words (ascii8); the
+// last len of its digits are written, each at a constant offset from one base.
+template <class Put, class Pos>
+HYD_HD int i64_write(int64_t v, const Put& put, Pos pos) {
+    const uint64_t a = v < 0 ? 0 - (uint64_t)v : (uint64_t)v;
+    const int nd = a ? dec_digits(a) : 1;
+    const int len = (v < 0) + nd;
+    if (v < 0) put(pos, '-');
+    const uint32_t top = (uint32_t)(a / 10000000000000000ULL);  // < 923
+    const uint64_t rem = a - (uint64_t)top * 10000000000000000ULL;
+    const uint32_t mid = (uint32_t)(rem / 100000000ULL), low = (uint32_t)(rem - (uint64_t)mid * 100000000ULL);
+    const uint64_t am = ascii8(mid), al = ascii8(low);
+    const Pos base = pos + (Pos)(len - 19);  // digit k of the 19 goes to base + k
+#pragma unroll
+    for (int k = 0; k < 19; ++k) {
+        if (k < 19 - nd) continue;
+        char ch;
+        if (k == 0) ch = (char)('0' + top / 100);
+        else if (k == 1) ch = (char)('0' + (top / 10) % 10);
+        else if (k == 2) ch = (char)('0' + top % 10);
+        else if (k < 11) ch = (char)((am >> (8 * (k - 3))) & 0xFF);
+        else ch = (char)((al >> (8 * (k - 11))) & 0xFF);
+        put(base + (Pos)k, ch);
+    }
+    return len;
+}
+
 
 // A finite nonzero value's 17 digits N (from f64_to_dec17) and exponent X:
 // digit 0, then digits 1..8 and 9..16 as ASCII words; nd = digits left once
@@ -898,8 +914,8 @@ HYD_HD char g17_digit(const G17& g, int j) {  // j is a constant after unrolling
     return (char)((j <= 8 ? g.a0 >> (8 * (j - 1)) : g.a1 >> (8 * (j - 9))) & 0xFF);
 }
 
-template <class Put>
-HYD_HD int g17_write(const G17& g, const Put& put, int64_t pos) {
+template <class Put, class Pos>
+HYD_HD int g17_write(const G17& g, const Put& put, Pos pos) {
     const int len = g17_len(g);
     const int s = g.neg ? 1 : 0;
     if (g.neg) put(pos, '-');
@@ -923,13 +939,13 @@ HYD_HD int g17_write(const G17& g, const Put& put, int64_t pos) {
         for (int i = 0; i < -g.X - 1; ++i) put(pos + s + 2 + i, '0');
         dot_after = -1;
     }
-    const int64_t d = pos + s + lead;
+    const Pos d = pos + (Pos)(s + lead);
 #pragma unroll
     for (int j = 0; j < 17; ++j)
         if (j < ndw) put(d + j + (dot_after >= 0 && j > dot_after ? 1 : 0), g17_digit(g, j));
     if (dot_after >= 0) put(d + dot_after + 1, '.');
     if (ef) {
-        int64_t q = d + ndw + (dot_after >= 0 ? 1 : 0);
+        Pos q = d + (Pos)(ndw + (dot_after >= 0 ? 1 : 0));
         put(q++, 'e');
         put(q++, g.X < 0 ? '-' : '+');
         const int ax = g.X < 0 ? -g.X : g.X;
@@ -949,8 +965,8 @@ HYD_HD int special_len(uint64_t bits) {
     return 0;
 }
 
-template <class Put>
-HYD_HD void special_write(uint64_t bits, const Put& put, int64_t pos) {
+template <class Put, class Pos>
+HYD_HD void special_write(uint64_t bits, const Put& put, Pos pos) {
     const bool neg = bits >> 63;
     const uint64_t mag = bits & ~kSignBit;
     if (neg) put(pos++, '-');

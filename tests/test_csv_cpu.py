@@ -176,8 +176,13 @@ def test_device_formatter_matches_glibc(host, orc, force_exact):
     assert got == orc.format_values(v, True)
     if force_exact:
         assert exact == np.count_nonzero(np.isfinite(v) & (v != 0))
+    # every digit count: 10^k - 1, 10^k, 10^k + 1 and their negatives (the
+    # writer splits |v| into 3 + 8 + 8 digits), small values, the extremes
+    pw = [10**k + d for k in range(19) for d in (-1, 0, 1)]
+    edges = np.array([x for p in pw for x in (p, -p) if -2**63 <= x < 2**63], np.int64)
     i = np.concatenate([rng.integers(-2**63, 2**63 - 1, 20_000, dtype=np.int64),
-                        np.array([0, -1, 2**63 - 1, -2**63], np.int64)])
+                        rng.integers(-1000, 1000, 2000, dtype=np.int64), edges,
+                        np.array([0, -1, 2**63 - 1, -2**63, -2**63 + 1], np.int64)])
     assert host.host_csv_format(i, False)[0] == orc.format_values(i, False)
 
 
