@@ -17,6 +17,8 @@ namespace hyd {
 namespace {
 
 constexpr uint64_t kMagic = 0x4859444849505354ULL;
+// host <-> HBM copies of at least this size go through the pinned halves
+constexpr size_t kStagedBytes = (size_t)8 << 20;
 
 // The storage block handed out through hyd_array_t::storage.  The reference
 // layout {data, refcount} comes first, so code that only knows the reference
@@ -374,11 +376,15 @@ void* hyd_array_device_data(hyd_array_t* arr) {
     if (ensure_dev(st) != HYD_OK) die("%s", hydhip_last_error());
     if (!st->device_resident) {
         // host copy authoritative: upload (ordered before the kernels that
-        // follow on the export stream)
+        // follow on the export stream; large arrays through the pinned halves)
         hipStream_t s = export_stream();
-        if (hipMemcpyAsync(st->dev, st->base.data, st->bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess)
+        if (st->bytes >= kStagedBytes) {
+            if (staged_h2d(st->dev, st->base.data, st->bytes, s) != HYD_OK)
+                die("upload of %zu bytes failed: %s", st->bytes, hydhip_last_error());
+        } else if (hipMemcpyAsync(st->dev, st->base.data, st->bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
+                   hipStreamSynchronize(s) != hipSuccess) {
             die("upload of %zu bytes failed", st->bytes);
+        }
     }
     return (unsigned char*)st->dev + (arr->data - st->base.data);
 }
@@ -387,8 +393,12 @@ int hyd_array_sync_host(hyd_array_t* arr) {
     HipStorage* st = storage_of(arr);
     if (!st->device_resident || st->host_valid || st->bytes == 0) return HYD_OK;
     hipStream_t s = export_stream();
-    HYD_TRY_HIP(hipMemcpyAsync(st->base.data, st->dev, st->bytes, hipMemcpyDeviceToHost, s));
-    HYD_TRY_HIP(hipStreamSynchronize(s));
+    if (st->bytes >= kStagedBytes) {
+        HYD_TRY(staged_d2h(st->base.data, st->dev, st->bytes, s));
+    } else {
+        HYD_TRY_HIP(hipMemcpyAsync(st->base.data, st->dev, st->bytes, hipMemcpyDeviceToHost, s));
+        HYD_TRY_HIP(hipStreamSynchronize(s));
+    }
     st->host_valid = true;
     return HYD_OK;
 }

@@ -834,7 +834,7 @@ int csv_format(const uint64_t* vals, int64_t count, int64_t row_len, bool is_flo
     return HYD_OK;
 }
 
-// ---- file I/O (pinned double buffers, parallel pread) ----------------------
+// ---- file I/O and large host copies (pinned double buffers) ----------------
 // A file's bytes move through two pinned halves: while one half is on PCIe,
 // the host fills (read: several threads' pread of disjoint pieces, since one
 // thread copying from the page cache is far slower than the link) or drains
@@ -889,6 +889,23 @@ bool parallel_io(int fd, unsigned char* buf, size_t len, off_t off, bool write) 
         for (auto& t : th) t.join();
     }
     return ok.load();
+}
+
+// dst[0, len) = src[0, len) by io_threads() threads (one below 4 MiB): a
+// single thread copying pageable memory into the pinned halves is slower
+// than the link.
+void parallel_copy(unsigned char* dst, const unsigned char* src, size_t len) {
+    const int nt = len < ((size_t)4 << 20) ? 1 : io_threads();
+    if (nt == 1) {
+        memcpy(dst, src, len);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t) {
+        const size_t a = len * t / nt, e = len * (t + 1) / nt;
+        th.emplace_back([=]() { memcpy(dst + a, src + a, e - a); });
+    }
+    for (auto& t : th) t.join();
 }
 
 [[noreturn]] void die_hip(const char* what) {
@@ -947,6 +964,61 @@ int device_to_file(int fd, const unsigned char* dev, int64_t bytes, hipStream_t 
     }
     return HYD_OK;
 }
+
+}  // namespace
+
+// Host memory <-> HBM through the pinned halves: the host copy of one half
+// (parallel memcpy) overlaps the DMA of the other.  Both return when the
+// data has arrived.  For large pageable buffers, where hipMemcpy would stage
+// through the runtime's own small buffers one copy at a time.
+int staged_h2d(void* dev, const void* host, size_t bytes, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_io.mu);
+    HYD_TRY(io_buffers());
+    unsigned char* d = static_cast<unsigned char*>(dev);
+    const unsigned char* h = static_cast<const unsigned char*>(host);
+    int64_t c = 0;
+    for (size_t off = 0; off < bytes; off += kIoHalf, ++c) {
+        const int k = (int)(c & 1);
+        const size_t len = std::min(kIoHalf, bytes - off);
+        if (g_io.pending[k]) HYD_TRY_HIP(hipEventSynchronize(g_io.ev[k]));
+        g_io.pending[k] = false;
+        parallel_copy(g_io.host[k], h + off, len);
+        HYD_TRY_HIP(hipMemcpyAsync(d + off, g_io.host[k], len, hipMemcpyHostToDevice, s));
+        HYD_TRY_HIP(hipEventRecord(g_io.ev[k], s));
+        g_io.pending[k] = true;
+    }
+    for (int k = 0; k < 2; ++k)
+        if (g_io.pending[k]) {
+            HYD_TRY_HIP(hipEventSynchronize(g_io.ev[k]));
+            g_io.pending[k] = false;
+        }
+    return HYD_OK;
+}
+
+int staged_d2h(void* host, const void* dev, size_t bytes, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_io.mu);
+    HYD_TRY(io_buffers());
+    unsigned char* h = static_cast<unsigned char*>(host);
+    const unsigned char* d = static_cast<const unsigned char*>(dev);
+    const int64_t halves = ceil_div((int64_t)bytes, (int64_t)kIoHalf);
+    auto issue = [&](int64_t c) -> int {
+        const size_t off = (size_t)c * kIoHalf;
+        const size_t len = std::min(kIoHalf, bytes - off);
+        HYD_TRY_HIP(hipMemcpyAsync(g_io.host[c & 1], d + off, len, hipMemcpyDeviceToHost, s));
+        HYD_TRY_HIP(hipEventRecord(g_io.ev[c & 1], s));
+        return HYD_OK;
+    };
+    if (halves > 0) HYD_TRY(issue(0));
+    for (int64_t c = 0; c < halves; ++c) {
+        if (c + 1 < halves) HYD_TRY(issue(c + 1));  // its buffer's host copy (half c - 1) has finished
+        HYD_TRY_HIP(hipEventSynchronize(g_io.ev[c & 1]));
+        const size_t off = (size_t)c * kIoHalf;
+        parallel_copy(h + off, g_io.host[c & 1], std::min(kIoHalf, bytes - off));
+    }
+    return HYD_OK;
+}
+
+namespace {
 
 // Device copy of a file's bytes (a buffer kept for the next call).
 struct TextBuffer {

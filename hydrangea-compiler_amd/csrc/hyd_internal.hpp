@@ -55,6 +55,12 @@ int stage_h2d(hipStream_t s, void* dev, size_t bytes, const std::function<void(u
 // Zero-fill kernel launch (graph-capture safe, unlike hipMemsetAsync).
 int fill_zero(void* dptr, size_t bytes, hipStream_t stream);
 
+// Large pageable host buffer <-> HBM through two pinned 256 MiB halves, the
+// host copy of one half (several threads) overlapping the DMA of the other
+// (csv.hip).  Both return when the data has arrived.
+int staged_h2d(void* dev, const void* host, size_t bytes, hipStream_t s);
+int staged_d2h(void* host, const void* dev, size_t bytes, hipStream_t s);
+
 // The batched-scatter autotuner's table as a dense array, mode[ceil log2
 // bins][floor log2 n]: 0 = LDS tiles, 1 = global atomics, -1 = untuned.
 struct TuneSnapshot {

@@ -30,6 +30,25 @@ def test_export_weighted_histogram(hyd, orc, lib):
     lib.hyd_array_free(arr)
 
 
+@pytest.mark.parametrize("n", [(1 << 20) + 3, (1 << 26) + 12_345])
+def test_export_large_host_arrays(hyd, orc, lib, n):
+    # inputs of 8 MiB and more are uploaded, and results downloaded, through
+    # the pinned 256 MiB halves (the second size spans three of them, the
+    # last one partial): the drop-in results stay bit-exact
+    print("export large host arrays n=%d" % n, flush=True)
+    xs = orc.splitmix_f64(n, 5, symmetric=True)
+    ys = orc.splitmix_f64(n, 5, counter0=n, symmetric=True)
+    ax, ay = hyd.host_array(xs), hyd.host_array(ys)
+    m = lib.hyd_export_map_chain(ax, ay)
+    got = hyd.read_array(m, np.float64)
+    assert np.array_equal(got.view(np.uint64), orc.map_chain(xs, ys).view(np.uint64))
+    d = lib.hyd_export_dot(ax, ay)
+    ref = orc.dot(xs, ys)
+    assert abs(d - ref) <= 1e-12 * abs(ref)
+    for x in (m, ax, ay):
+        lib.hyd_array_free(x)
+
+
 def test_export_input_rewritten_between_calls(hyd, orc, lib):
     # host copy of an input is authoritative: rewriting arr->data is seen
     data = np.arange(1000, dtype=np.int64)
