@@ -229,6 +229,26 @@ def test_round_trip_full_size(hyd, orc):
         a.free()
 
 
+def test_drop_in_file_write_large(hyd, orc, tmp_path):
+    # the text goes out through the pinned halves, one pwrite per half (this
+    # text spans two 256 MiB halves, the second partial), also to a device
+    # file such as /dev/null
+    n = 14_000_003
+    print("drop-in file write n=%d" % n, flush=True)
+    v = orc.splitmix_f64(n, 88, symmetric=True)
+    arr = hyd.host_array(v)
+    path = str(tmp_path / "big.csv").encode()
+    assert hyd.lib().hyd_write_array_csv_float(arr, path) == 0
+    dv = hyd.DeviceArray.from_numpy(v)
+    expect = hyd.csv_format(dv)
+    with open(path, "rb") as f:
+        got = f.read()
+    assert len(got) == len(expect) > 256 << 20 and got == expect
+    assert hyd.lib().hyd_write_array_csv_float(arr, b"/dev/null") == 0
+    hyd.lib().hyd_array_free(arr)
+    dv.free()
+
+
 def test_drop_in_files(hyd, orc, tmp_path):
     rng = np.random.default_rng(8)
     v = random_doubles(rng, 12_000).reshape(30, 400)
