@@ -14,6 +14,8 @@ reported under "kernels"; unless --no-cpu, each one also carries the
 oracle's time on the host for the same inputs (timing.h style: warmup, then
 timed runs, min and mean; bounded samples where the CPU would take minutes)
 and whether this run's GPU result matches the oracle's ("matches_oracle").
+"dropin_host_arrays" is the same step through the drop-in exports with the
+arrays on the host, PCIe transfers included (a reported figure, not value).
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -169,6 +171,41 @@ def bench_copy(steps, warmup):
     a.free()
     b.free()
     return dict(ms=ms, bytes=16 * n, gbs=16 * n / ms / 1e6)
+
+
+def bench_dropin_host(reps=4):
+    """The dot + map chain step through the drop-in exports with host-resident
+    inputs (hyd_export_dot, hyd_export_map_chain; host sync on): each call
+    uploads its inputs and the map chain's result comes back, so the rate
+    includes 5 GB over PCIe per step.  A reported figure, never `value`."""
+    c = workloads.DOTMAP
+    n = c["n"]
+    lib = hydhip.lib()
+    prev = lib.hyd_hip_get_host_sync()
+    lib.hyd_hip_set_host_sync(1)
+    dx = hydhip.splitmix_f64(n, c["seed"], 0, symmetric=True)
+    dy = hydhip.splitmix_f64(n, c["seed"], n, symmetric=True)
+    xs, ys = dx.numpy(), dy.numpy()
+    dx.free()
+    dy.free()
+    ax, ay = hydhip.host_array(xs), hydhip.host_array(ys)
+    del xs, ys
+    times = []
+    for r in range(reps + 1):
+        t0 = time.perf_counter()
+        lib.hyd_export_dot(ax, ay)
+        m = lib.hyd_export_map_chain(ax, ay)
+        dt = time.perf_counter() - t0
+        lib.hyd_array_free(m)
+        if r:  # the first call sets up the pinned halves
+            times.append(dt)
+    for a in (ax, ay):
+        lib.hyd_array_free(a)
+    lib.hyd_hip_set_host_sync(prev)
+    ms = 1e3 * min(times)
+    step_bytes = 40 * n
+    return dict(ms_per_step=ms, algorithmic_gbs=step_bytes / ms / 1e6, pcie_bytes_per_step=40 * n,
+                note="host inputs uploaded on every call (2 x 2 GiB), result downloaded (1 GiB)")
 
 
 def bench_guarded(steps, warmup, cpu=True):
@@ -570,6 +607,9 @@ def main():
     if not only or "copy" in only:
         log("copy reference ...")
         kernels["copy_reference"] = bench_copy(args.steps, args.warmup)
+    if not only or "dropin" in only:
+        log("drop-in with host arrays ...")
+        kernels["dropin_host_arrays"] = bench_dropin_host()
     if not only or "hist" in only:
         log("histogram ...")
         kernels["weighted_histogram"] = bench_hist(args.steps, args.warmup, cpu=not args.no_cpu)
