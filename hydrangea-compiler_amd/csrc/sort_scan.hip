@@ -1049,9 +1049,13 @@ int csr_from_sorted_coo_i64(int64_t nrows, int64_t nnz, const int64_t* rows, con
     // exclusive scan over nrows+1 entries (the extra zero count makes
     // row_ptr[nrows] the total)
     HYD_TRY(scan_i64(counts, nrows + 1, 0, row_ptr, false, false, s, scan_ws));
-    if (len > 0) {
+    // a column or value array that already is the output (written there by
+    // the duplicate merge, zero tail included) needs no copy
+    if (len > 0 && cols != col_idx) {
         copy_prefix_kernel<<<grid_for(len), 256, 0, s>>>(cols, nnz, len, col_idx);
         HYD_CHECK_LAUNCH();
+    }
+    if (len > 0 && vals != out_vals) {
         copy_prefix_kernel<<<grid_for(len), 256, 0, s>>>(vals, nnz, len, out_vals);
         HYD_CHECK_LAUNCH();
     }
@@ -1272,8 +1276,12 @@ int coo_to_csr_ws(int64_t nrows, int64_t ncols, int64_t nnz, const int64_t* rows
     }
     if (keyed) {
         // the keys are non-negative and at most range[0]: sorting their bits
-        // orders them as the reference's byte passes do
+        // orders them as the reference's byte passes do.  The inputs are all
+        // read by now, so the merge writes the columns and values straight
+        // into the outputs (zero tails included).
         HYD_TRY(sort_pairs_i64(keys, vals, nnz, (int64_t)range[0], perm, svals, s, scr, nullptr));
+        ccol = col_idx;
+        cval = out_vals;
         HYD_TRY(coo_sum_duplicates_keyed_i64(perm, svals, nnz, ncols, nnz, crow, ccol, cval, d_cnt, s, scr));
     } else {
         HYD_TRY(sort_indices_i64(keys, nnz, perm, s, scr, nullptr));
