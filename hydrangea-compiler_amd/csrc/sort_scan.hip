@@ -67,14 +67,31 @@ __device__ __forceinline__ int64_t seq_index(int64_t e, int64_t n, bool reverse)
     return reverse ? n - 1 - e : e;
 }
 
-__global__ __launch_bounds__(kBlock) void scan_tile_sums(const int64_t* __restrict__ in, int64_t n,
-                                                         bool reverse, int64_t* __restrict__ sums) {
+// Elements of a scanned sequence.  ArraySrc: a[e] (forward) or a[n-1-e]
+// (reverse).  HeadsSrc: 1 where a run of equal keys starts in a sorted key
+// array, else 0 - the run-head flags, computed on the fly instead of written
+// out and read back.
+struct ArraySrc {
+    const int64_t* a;
+    int64_t n;
+    bool reverse;
+    __device__ __forceinline__ int64_t operator()(int64_t e) const { return a[seq_index(e, n, reverse)]; }
+};
+struct HeadsSrc {
+    const int64_t* keys;
+    __device__ __forceinline__ int64_t operator()(int64_t e) const {
+        return (e == 0 || keys[e] != keys[e - 1]) ? 1 : 0;
+    }
+};
+
+template <class Src>
+__global__ __launch_bounds__(kBlock) void scan_tile_sums(Src src, int64_t n, int64_t* __restrict__ sums) {
     const int64_t base = (int64_t)blockIdx.x * kTile;
     int64_t v = 0;
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         int64_t e = base + (int64_t)k * kBlock + threadIdx.x;
-        if (e < n) v += in[seq_index(e, n, reverse)];
+        if (e < n) v += src(e);
     }
     int64_t total;
     (void)block_excl_scan(v, &total);
@@ -106,36 +123,73 @@ __global__ __launch_bounds__(kBlock) void scan_sums_kernel(int64_t* __restrict__
 }
 
 // Per tile: out[e] = init + (exclusive or inclusive) prefix of the sequence
-// (tile_off NULL: a single tile, offset 0).
-__global__ __launch_bounds__(kBlock) void scan_apply(const int64_t* __restrict__ in, int64_t n,
-                                                     bool reverse, bool inclusive, int64_t init,
-                                                     const int64_t* __restrict__ tile_off,
+// (tile_off NULL: a single tile, offset 0); out is indexed like the source
+// (reversed for a reverse scan).  The tile goes through LDS both ways:
+// coalesced loads and stores (lane = consecutive element), while each thread
+// scans kItems consecutive elements (row t of the tile, padded to an odd
+// stride of 64-bit words against bank conflicts).
+constexpr int kRow = kItems + 1;
+
+template <class Src>
+__global__ __launch_bounds__(kBlock) void scan_apply(Src src, int64_t n, bool reverse, bool inclusive,
+                                                     int64_t init, const int64_t* __restrict__ tile_off,
                                                      int64_t* __restrict__ out) {
-    const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kItems;
+    __shared__ int64_t tile[kBlock * kRow];
+    const int64_t base = (int64_t)blockIdx.x * kTile;
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {  // element j = k * kBlock + t of the tile
+        const int j = k * kBlock + t;
+        const int64_t e = base + j;
+        tile[(j / kItems) * kRow + j % kItems] = e < n ? src(e) : 0;
+    }
+    __syncthreads();
     int64_t v[kItems];
     int64_t local = 0;
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
-        int64_t e = base + k;
-        v[k] = e < n ? in[seq_index(e, n, reverse)] : 0;
+        v[k] = tile[t * kRow + k];
         local += v[k];
     }
     int64_t total;
     int64_t acc = block_excl_scan(local, &total) + (tile_off ? tile_off[blockIdx.x] : 0) + init;
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
-        int64_t e = base + k;
-        if (e < n) {
-            int64_t idx = seq_index(e, n, reverse);
-            if (inclusive) {
-                acc += v[k];
-                out[idx] = acc;
-            } else {
-                out[idx] = acc;
-                acc += v[k];
-            }
+        if (inclusive) {
+            acc += v[k];
+            tile[t * kRow + k] = acc;
+        } else {
+            tile[t * kRow + k] = acc;
+            acc += v[k];
         }
     }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const int j = k * kBlock + t;
+        const int64_t e = base + j;
+        if (e < n) out[seq_index(e, n, reverse)] = tile[(j / kItems) * kRow + j % kItems];
+    }
+}
+
+template <class Src>
+int scan_src(Src src, int64_t n, int64_t init, int64_t* out, bool inclusive, bool reverse, hipStream_t s,
+             int64_t* tile_ws) {
+    if (n == 0) return HYD_OK;
+    int64_t tiles = ceil_div(n, kTile);
+    if (tiles > 0x7fffffff) return set_error(HYD_E_INVALID, "scan: too many elements");
+    if (tiles == 1) {  // one launch instead of three
+        scan_apply<Src><<<1, kBlock, 0, s>>>(src, n, reverse, inclusive, init, nullptr, out);
+        HYD_CHECK_LAUNCH();
+        return HYD_OK;
+    }
+    scan_tile_sums<Src><<<(unsigned)tiles, kBlock, 0, s>>>(src, n, tile_ws);
+    HYD_CHECK_LAUNCH();
+    scan_sums_kernel<<<1, kBlock, 0, s>>>(tile_ws, tiles);
+    HYD_CHECK_LAUNCH();
+    scan_apply<Src><<<(unsigned)tiles, kBlock, 0, s>>>(src, n, reverse, inclusive, init, tile_ws, out);
+    HYD_CHECK_LAUNCH();
+    return HYD_OK;
 }
 
 }  // namespace
@@ -144,21 +198,14 @@ __global__ __launch_bounds__(kBlock) void scan_apply(const int64_t* __restrict__
 // sort and the COO kernels).
 int scan_i64(const int64_t* in, int64_t n, int64_t init, int64_t* out, bool inclusive, bool reverse,
              hipStream_t s, int64_t* tile_ws) {
-    if (n == 0) return HYD_OK;
-    int64_t tiles = ceil_div(n, kTile);
-    if (tiles > 0x7fffffff) return set_error(HYD_E_INVALID, "scan: too many elements");
-    if (tiles == 1) {  // one launch instead of three
-        scan_apply<<<1, kBlock, 0, s>>>(in, n, reverse, inclusive, init, nullptr, out);
-        HYD_CHECK_LAUNCH();
-        return HYD_OK;
-    }
-    scan_tile_sums<<<(unsigned)tiles, kBlock, 0, s>>>(in, n, reverse, tile_ws);
-    HYD_CHECK_LAUNCH();
-    scan_sums_kernel<<<1, kBlock, 0, s>>>(tile_ws, tiles);
-    HYD_CHECK_LAUNCH();
-    scan_apply<<<(unsigned)tiles, kBlock, 0, s>>>(in, n, reverse, inclusive, init, tile_ws, out);
-    HYD_CHECK_LAUNCH();
-    return HYD_OK;
+    return scan_src(ArraySrc{in, n, reverse}, n, init, out, inclusive, reverse, s, tile_ws);
+}
+
+// out[i] = number of runs of equal keys that start at or before i in the
+// sorted keys[0, n) (an inclusive scan of the run-head flags, which are
+// never stored).  ws: scan_workspace_words(n).
+int scan_heads_i64(const int64_t* keys, int64_t n, int64_t* out, hipStream_t s, int64_t* tile_ws) {
+    return scan_src(HeadsSrc{keys}, n, 0, out, true, false, s, tile_ws);
 }
 
 int64_t scan_workspace_words(int64_t n) { return ceil_div(n, kTile) + 1; }
@@ -539,25 +586,18 @@ __global__ __launch_bounds__(256) void pack_keys_check_kernel(const int64_t* __r
 }
 
 // Keyed duplicate merge over keys sorted with their values: a run of equal
-// keys is a run of equal (row, col) pairs.  head[i] = 1 at a run's first
-// element.
-__global__ void key_heads_kernel(const int64_t* __restrict__ skeys, int64_t n, int64_t* __restrict__ head) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        head[i] = (i == 0 || skeys[i] != skeys[i - 1]) ? 1 : 0;
-}
-
-// As coo_run_ends_kernel, with row and col decoded from the key (a shift and
+// keys is a run of equal (row, col) pairs.  As coo_run_ends_kernel, with the
+// run heads read off the keys and row and col decoded from them (a shift and
 // a mask when ncols is a power of two: col_shift >= 0).
-__global__ void key_run_ends_kernel(const int64_t* __restrict__ skeys, const int64_t* __restrict__ head,
-                                    const int64_t* __restrict__ slot_incl, const int64_t* __restrict__ psum,
-                                    int64_t n, int64_t ncols, int col_shift, int64_t* __restrict__ out_rows,
-                                    int64_t* __restrict__ out_cols, int64_t* __restrict__ end_p) {
+__global__ void key_run_ends_kernel(const int64_t* __restrict__ skeys, const int64_t* __restrict__ slot_incl,
+                                    const int64_t* __restrict__ psum, int64_t n, int64_t ncols, int col_shift,
+                                    int64_t* __restrict__ out_rows, int64_t* __restrict__ out_cols,
+                                    int64_t* __restrict__ end_p) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t slot = slot_incl[i] - 1;
-        if (head[i]) {
-            const uint64_t k = (uint64_t)skeys[i];
+        const uint64_t k = (uint64_t)skeys[i];
+        if (i == 0 || (uint64_t)skeys[i - 1] != k) {
             uint64_t r, c;
             if (col_shift >= 0) {
                 r = k >> col_shift;
@@ -569,7 +609,7 @@ __global__ void key_run_ends_kernel(const int64_t* __restrict__ skeys, const int
             out_rows[slot] = (int64_t)r;
             out_cols[slot] = (int64_t)c;
         }
-        if (i == n - 1 || head[i + 1]) end_p[slot] = psum[i];
+        if (i == n - 1 || (uint64_t)skeys[i + 1] != k) end_p[slot] = psum[i];
     }
 }
 
@@ -769,17 +809,6 @@ __global__ __launch_bounds__(256) void compact_write_kernel(const int64_t* __res
         out += __popcll(rm[0]) + __popcll(rm[1]) + __popcll(rm[2]) + __popcll(rm[3]);
     }
     (void)n;
-}
-
-// Runs of equal bins in the sorted keys (bins < ndst only): head[j] = 1 at
-// the first element of a run.
-__global__ void run_heads_kernel(const int64_t* __restrict__ skeys, int64_t n, int64_t ndst,
-                                 int64_t* __restrict__ head) {
-    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n;
-         j += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t b = skeys[j];
-        head[j] = (b < ndst && (j == 0 || skeys[j - 1] != b)) ? 1 : 0;
-    }
 }
 
 // With r = (inclusive count of heads up to j) - 1: the first element of run r
@@ -1004,18 +1033,15 @@ static int coo_sum_duplicates_keyed_i64(const int64_t* skeys, const int64_t* sva
     HYD_TRY(fill_zero(out_cols, (size_t)len * 8, s));
     HYD_TRY(fill_zero(out_vals, (size_t)len * 8, s));
     if (n == 0) return fill_zero(d_nnz_out, 8, s);
-    int64_t* head = ws;
     int64_t* incl = ws + n;
     int64_t* psum = ws + 3 * n;
     int64_t* end_p = ws + 4 * n;
     int64_t* scan_ws = ws + 5 * n;
-    key_heads_kernel<<<grid_for(n), 256, 0, s>>>(skeys, n, head);
-    HYD_CHECK_LAUNCH();
-    HYD_TRY(scan_i64(head, n, 0, incl, true, false, s, scan_ws));
+    HYD_TRY(scan_heads_i64(skeys, n, incl, s, scan_ws));  // slot + 1 of each element
     HYD_TRY(scan_i64(svals, n, 0, psum, true, false, s, scan_ws));
     const int col_shift = (ncols > 0 && (ncols & (ncols - 1)) == 0) ? __builtin_ctzll((uint64_t)ncols) : -1;
-    key_run_ends_kernel<<<grid_for(n), 256, 0, s>>>(skeys, head, incl, psum, n, ncols, col_shift, out_rows,
-                                                    out_cols, end_p);
+    key_run_ends_kernel<<<grid_for(n), 256, 0, s>>>(skeys, incl, psum, n, ncols, col_shift, out_rows, out_cols,
+                                                    end_p);
     HYD_CHECK_LAUNCH();
     coo_run_sums_kernel<<<grid_for(n), 256, 0, s>>>(end_p, incl + (n - 1), n, out_vals);
     HYD_CHECK_LAUNCH();
@@ -1071,8 +1097,7 @@ int scatter_ordered_add_f64(const int64_t* idx, const int64_t* guard, const doub
                             int64_t ndst, double* dst, hipStream_t s, int64_t* ws) {
     if (n == 0) return HYD_OK;
     const int64_t ntiles = ceil_div(n, kCompactTile);
-    int64_t* head = ws;            // m: run heads
-    int64_t* incl = ws + n;        // m: their inclusive scan
+    int64_t* incl = ws + n;        // m: inclusive count of run heads
     int64_t* ckeys = ws + 2 * n;   // m: the contributing keys, element order
     int64_t* cvals = ws + 3 * n;   // m: their values' bits
     int64_t* skeys = ws + 5 * n;   // m: sorted keys
@@ -1095,9 +1120,7 @@ int scatter_ordered_add_f64(const int64_t* idx, const int64_t* guard, const doub
     if (m == 0) return HYD_OK;
     // contributing keys lie in [0, ndst): that bound sets the pass count
     HYD_TRY(sort_pairs_i64(ckeys, cvals, m, ndst - 1, skeys, svals, s, scratch, nullptr));
-    run_heads_kernel<<<grid_for(m), 256, 0, s>>>(skeys, m, ndst, head);
-    HYD_CHECK_LAUNCH();
-    HYD_TRY(scan_i64(head, m, 0, incl, true, false, s, scratch));
+    HYD_TRY(scan_heads_i64(skeys, m, incl, s, scratch));  // run index + 1 of each element
     run_bounds_kernel<<<grid_for(m), 256, 0, s>>>(skeys, incl, m, ndst, start, end);
     HYD_CHECK_LAUNCH();
     const double* sv = reinterpret_cast<const double*>(svals);
