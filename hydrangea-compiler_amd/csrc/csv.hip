@@ -555,6 +555,34 @@ __device__ __forceinline__ Dig unpack_digits(uint64_t b, uint64_t packed) {
     return d;
 }
 
+// A thread's kFPer consecutive words a[i0, i0 + kFPer) (i0 a multiple of
+// kFPer): two 16-byte accesses when all lie in range and the array is 16-byte
+// aligned, so that a wave's loads and stores cover whole lines.
+__device__ __forceinline__ void load_group(const uint64_t* a, int64_t i0, int64_t n, uint64_t* v) {
+    if (i0 + kFPer <= n && (reinterpret_cast<uintptr_t>(a) & 15) == 0) {
+        const ulonglong2 p0 = *reinterpret_cast<const ulonglong2*>(a + i0);
+        const ulonglong2 p1 = *reinterpret_cast<const ulonglong2*>(a + i0 + 2);
+        v[0] = p0.x;
+        v[1] = p0.y;
+        v[2] = p1.x;
+        v[3] = p1.y;
+    } else {
+#pragma unroll
+        for (int k = 0; k < kFPer; ++k) v[k] = i0 + k < n ? a[i0 + k] : 0;
+    }
+}
+__device__ __forceinline__ void store_group(uint64_t* a, int64_t i0, int64_t n, const uint64_t* v) {
+    if (i0 + kFPer <= n && (reinterpret_cast<uintptr_t>(a) & 15) == 0) {
+        *reinterpret_cast<ulonglong2*>(a + i0) = make_ulonglong2(v[0], v[1]);
+        *reinterpret_cast<ulonglong2*>(a + i0 + 2) = make_ulonglong2(v[2], v[3]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < kFPer; ++k)
+            if (i0 + k < n) a[i0 + k] = v[k];
+    }
+}
+static_assert(kFPer == 4, "load_group moves 4 words");
+
 // Text length of every 1024-value block (with separators).  fp64 values the
 // table path cannot round with certainty are marked (their provisional length
 // is corrected by fmt_exact_kernel).  The fp64 digits go to `packed` (one
@@ -568,24 +596,27 @@ __global__ __launch_bounds__(kFThreads) void fmt_len_kernel(const uint64_t* __re
     __shared__ int64_t sh[kWaves];
     const int64_t i0 = (int64_t)blockIdx.x * kFBlock + threadIdx.x * kFPer;
     int64_t len = 0;
+    uint64_t b[kFPer], pk[kFPer];
+    load_group(vals, i0, n, b);
 #pragma unroll
     for (int k = 0; k < kFPer; ++k) {
         const int64_t i = i0 + k;
+        pk[k] = 0;
         if (i >= n) continue;
         if (FLOAT) {
             bool decided;
-            const uint64_t b = vals[i];
-            const Dig d = table_digits<FORCE>(b, &decided);
-            packed[i] = pack_digits(b, d);
+            const Dig d = table_digits<FORCE>(b[k], &decided);
+            pk[k] = pack_digits(b[k], d);
             len += dig_len(d) + 1;
             if (!decided) {
                 atomicOr(&exact_bits[i >> 5], 1u << (i & 31));
                 atomicAdd(exact_cnt, 1ULL);
             }
         } else {
-            len += i64_len((int64_t)vals[i]) + 1;
+            len += i64_len((int64_t)b[k]) + 1;
         }
     }
+    if (FLOAT) store_group(packed, i0, n, pk);
     int64_t total;
     block_excl_scan(len, &total, sh);
     if (threadIdx.x == 0) block_len[blockIdx.x] = total;
@@ -639,17 +670,20 @@ __global__ __launch_bounds__(kFThreads) void fmt_write_kernel(const uint64_t* __
     Dig dg[kFPer];
     int lens[kFPer];
     int64_t len = 0;
+    uint64_t b[kFPer], pk[kFPer];
+    load_group(vals, i0, n, b);
+    if (FLOAT) load_group(packed, i0, n, pk);
 #pragma unroll
     for (int k = 0; k < kFPer; ++k) {
         const int64_t i = i0 + k;
         lens[k] = 0;
         if (i >= n) continue;
         if (FLOAT) {
-            dg[k] = unpack_digits(vals[i], packed[i]);
+            dg[k] = unpack_digits(b[k], pk[k]);
             if (any_exact && ((exact_bits[i >> 5] >> (i & 31)) & 1)) dg[k] = exact_dig[i];
             lens[k] = dig_len(dg[k]) + 1;
         } else {
-            lens[k] = i64_len((int64_t)vals[i]) + 1;
+            lens[k] = i64_len((int64_t)b[k]) + 1;
         }
         len += lens[k];
     }
@@ -670,7 +704,7 @@ __global__ __launch_bounds__(kFThreads) void fmt_write_kernel(const uint64_t* __
                 if (dg[k].flags & 1) special_write(dg[k].N, put, pos);
                 else g17_write(g17_make(dg[k].flags & 2, dg[k].N, dg[k].X), put, pos);
             } else {
-                i64_write((int64_t)vals[i], put, pos);
+                i64_write((int64_t)b[k], put, pos);
             }
             put(pos + lens[k] - 1, col == row_len - 1 ? '\n' : ',');
             col = col == row_len - 1 ? 0 : col + 1;
