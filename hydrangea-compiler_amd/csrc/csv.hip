@@ -993,8 +993,13 @@ int device_to_file(int fd, const unsigned char* dev, int64_t bytes, hipStream_t 
         HYD_TRY_HIP(hipEventSynchronize(g_io.ev[c & 1]));
         const int64_t off = c * (int64_t)kIoHalf;
         const int64_t len = std::min<int64_t>((int64_t)kIoHalf, bytes - off);
-        if (!parallel_io(fd, g_io.host[c & 1], (size_t)len, (off_t)off, true))
-            return set_error(HYD_E_INVALID, "csv: write failed: %s", strerror(errno));
+        if (!parallel_io(fd, g_io.host[c & 1], (size_t)len, (off_t)off, true)) {
+            const int err = errno;
+            // the next half's copy may be in flight: no later caller may get
+            // a half that a copy still fills
+            for (int k = 0; k < 2; ++k) (void)hipEventSynchronize(g_io.ev[k]);
+            return set_error(HYD_E_INVALID, "csv: write failed: %s", strerror(err));
+        }
     }
     return HYD_OK;
 }
