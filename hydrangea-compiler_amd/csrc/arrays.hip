@@ -102,6 +102,10 @@ int host_sync_enabled() { return g_host_sync; }
 
 using namespace hyd;
 
+namespace hyd {
+void mark_device_written(hyd_array_t* arr);
+}
+
 extern "C" {
 
 hyd_tuple_t hyd_tuple_make(int n, ...) {
@@ -411,6 +415,11 @@ int hyd_array_mark_host_dirty(hyd_array_t* arr) {
 }
 
 int hyd_array_is_device_resident(hyd_array_t* arr) { return storage_of(arr)->device_resident ? 1 : 0; }
+
+int hyd_array_mark_device_written(hyd_array_t* arr) {
+    hyd::mark_device_written(arr);
+    return HYD_OK;
+}
 
 void hyd_hip_set_host_sync(int on) { g_host_sync = on ? 1 : 0; }
 int hyd_hip_get_host_sync(void) { return g_host_sync; }

@@ -35,6 +35,10 @@ int hyd_array_sync_host(hyd_array_t* arr);
 /* The caller rewrote arr->data of a device-resident array: the next kernel
  * re-uploads it. */
 int hyd_array_mark_host_dirty(hyd_array_t* arr);
+/* A kernel called through hyd_hip.h wrote the HBM copy (the pointer from
+ * hyd_array_device_data): it becomes the authoritative one, and
+ * hyd_array_sync_host copies it back. */
+int hyd_array_mark_device_written(hyd_array_t* arr);
 /* 1 if the HBM copy is the authoritative one. */
 int hyd_array_is_device_resident(hyd_array_t* arr);
 /* Host sync of hyd_export_* results: 1 (default) or 0 (leave in HBM). */
