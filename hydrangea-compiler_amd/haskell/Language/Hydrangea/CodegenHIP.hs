@@ -17,6 +17,10 @@
 --   'ParallelScatterAtomicAddInt' / 'ParallelScatterPrivatizedIntAdd'
 --   (int64, 'CAdd') or the Float variants (fp64, 'CAddF'), CFG.hs:184-191.
 --   It becomes @_discarded = hydg_scatter_add_{i64,f64}(dst, idx, vals, n)@.
+-- * the fused weighted histogram, the same scatter loop (int64) whose body
+--   loads @x@ at the iterator, routes it by @x % nbins@ ('CMod') and adds
+--   @x * x@ ('CMul').  It becomes
+--   @_discarded = hydg_weighted_histogram_i64(dst, data, nbins, n)@.
 -- * dot: a reduction loop ('lsRed' with 'RAdd') whose body loads @x@ and @y@
 --   at the iterator and adds @x *. y@ to the accumulator.  It becomes
 --   @acc = hydg_dot_f64(xs, ys, n, acc)@.
@@ -89,7 +93,7 @@ offload (SLoop spec body) = do
   [bound] <- Just (lsBounds spec)
   guard (all (== IConst 0) (lsOrigins spec))
   n <- boundAtom bound
-  scatterAdd spec i n body <|> dot spec i n body
+  scatterAdd spec i n body <|> weightedHistogram spec i n body <|> dot spec i n body
 offload _ = Nothing
 
 boundAtom :: IndexExpr -> Maybe Atom
@@ -128,6 +132,31 @@ scatterAdd spec i n body = do
       ParallelScatterAtomicAddFloat -> Just (CAddF, "hydg_scatter_add_f64")
       ParallelScatterPrivatizedFloatAdd -> Just (CAddF, "hydg_scatter_add_f64")
       _ -> Nothing
+
+weightedHistogram :: LoopSpec -> CVar -> Atom -> [Stmt] -> Maybe (HipSite, [Stmt])
+weightedHistogram spec i n body = do
+  Parallel ps <- Just (lsExec spec)
+  guard (psStrategy ps `elem` [ParallelScatterAtomicAddInt, ParallelScatterPrivatizedIntAdd])
+  [s1, s2, s3, SAssign old (RArrayLoad (AVar dst) (AVar r)), SAssign new (RBinOp CAdd a b), SArrayWrite (AVar dst') (AVar r') (AVar new')] <-
+    Just body
+  guard (dst' == dst && r' == r && new' == new)
+  w <- case (a, b) of
+    (AVar p, AVar q) | p == old -> Just q
+                     | q == old -> Just p
+    _ -> Nothing
+  -- the load of x at i, the route r = x % nbins and the weight w = x * x,
+  -- the last two in either order
+  (x, dataArr) <- case s1 of
+    SAssign v (RArrayLoad (AVar arr) (AVar j)) | j == i -> Just (v, arr)
+    _ -> Nothing
+  let route (SAssign v (RBinOp CMod (AVar x') nb)) | v == r && x' == x = Just nb
+      route _ = Nothing
+      weight (SAssign v (RBinOp CMul (AVar p) (AVar q))) = v == w && p == x && q == x
+      weight _ = False
+  nbins <- (route s2 <* guard (weight s3)) <|> (route s3 <* guard (weight s2))
+  guard (dst /= dataArr && r /= w)
+  let helper = "hydg_weighted_histogram_i64"
+  pure (HipSite helper [dst, dataArr], [SAssign "_discarded" (RCall helper [AVar dst, AVar dataArr, nbins, n])])
 
 dot :: LoopSpec -> CVar -> Atom -> [Stmt] -> Maybe (HipSite, [Stmt])
 dot spec i n body = do

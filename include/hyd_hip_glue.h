@@ -73,6 +73,35 @@ static inline int64_t hydg_scatter_add_f64(hyd_array_t* dst, hyd_array_t* idx, h
     return 0;
 }
 
+/* for i in [0, n): x = data[i]; dst[x % nbins] += x * x on int64 (mod 2^64):
+ * the fused weighted histogram `scatter (+) hist (map (x % nbins) data)
+ * (map (x * x) data)`, CodegenC.hs:1119-1288.  The kernel counts into a
+ * zeroed device histogram of nbins bins, which is added into dst on the
+ * host; a route outside [0, nbins) or past dst is dropped where the C loop
+ * would write out of bounds. */
+static inline int64_t hydg_weighted_histogram_i64(hyd_array_t* dst, hyd_array_t* data, int64_t nbins, int64_t n) {
+    if (n <= 0 || nbins <= 0) return 0;
+    hydg_check_len("hydg_weighted_histogram_i64", data, n);
+    int64_t* hist = NULL;
+    int64_t* host = (int64_t*)malloc((size_t)nbins * sizeof(int64_t));
+    if (!host) {
+        fprintf(stderr, "hydg_weighted_histogram_i64: out of memory\n");
+        exit(1);
+    }
+    if (hydhip_malloc((void**)&hist, nbins * (int64_t)sizeof(int64_t)) != 0 ||
+        hydhip_weighted_histogram_i64((const int64_t*)hyd_array_device_data(data), n, nbins, hist, 0, NULL) != 0 ||
+        hydhip_memcpy_d2h(host, hist, nbins * (int64_t)sizeof(int64_t), NULL) != 0 ||
+        hydhip_stream_sync(NULL) != 0 || hydhip_free(hist) != 0)
+        hydg_die("hydg_weighted_histogram_i64");
+    if (hyd_array_sync_host(dst) != 0) hydg_die("hydg_weighted_histogram_i64");  /* an HBM copy may be newer */
+    const int64_t ndst = hyd_shape_size(dst->shape);
+    uint64_t* d = (uint64_t*)dst->data;
+    for (int64_t b = 0; b < nbins && b < ndst; ++b) d[b] += (uint64_t)host[b];
+    free(host);
+    hyd_array_mark_host_dirty(dst);
+    return 0;
+}
+
 /* acc + sum over i in [0, n) of xs[i] * ys[i]: the reduction loop of
  * `reduce (+.) init (zipwith (*.) xs ys)` (CodegenC.hs:1306-1413), with the
  * accumulator's value on entry as init; the fixed tree of hydhip_dot_f64. */

@@ -27,7 +27,7 @@ def test_emitted_helpers_are_defined():
     hs = _read(os.path.join(HS, "Language", "Hydrangea", "CodegenHIP.hs"))
     emitted = set(re.findall(r'"(hydg_\w+)"', hs))
     defined = set(re.findall(r"static inline \w+\*? (hydg_\w+)\(", _read(GLUE)))
-    assert emitted == {"hydg_scatter_add_i64", "hydg_scatter_add_f64", "hydg_dot_f64"}
+    assert emitted == {"hydg_scatter_add_i64", "hydg_scatter_add_f64", "hydg_weighted_histogram_i64", "hydg_dot_f64"}
     assert emitted <= defined
 
 

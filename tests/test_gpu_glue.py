@@ -2,6 +2,7 @@
 called the way the C that CodegenC prints after the HIP offload pass calls
 them: arrays from the reference's hyd_array_alloc_bytes, host bytes written
 by the program, `_discarded = hydg_scatter_add_*(dst, idx, vals, n)` and
+`_discarded = hydg_weighted_histogram_i64(hist, data, 64, n)`,
 `acc = hydg_dot_f64(xs, ys, n, acc)`, then arr->data read on the host.
 Checked against the oracle: scatters bit for bit, the dot within 1e-12."""
 import os
@@ -56,6 +57,10 @@ int main(int argc, char** argv) {
        library; the next call must see the write */
     ((int64_t*)di->data)[0] += 1;
     _discarded = hydg_scatter_add_i64(di, idx, vi, n);
+    hyd_array_t* data = load(PATH("data"), n);
+    hyd_array_t* dh = load(PATH("dh"), ndst);
+    _discarded = hydg_weighted_histogram_i64(dh, data, 64, n);
+    save(PATH("dh_out"), dh, ndst);
     double acc = 0.25;
     acc = hydg_dot_f64(xs, ys, n, acc);
     (void)_discarded;
@@ -88,7 +93,9 @@ def test_glue_helpers_from_c(orc, glue_exe, tmp_path, n, ndst, seed):
     di = rng.integers(-1000, 1000, ndst, dtype=np.int64)
     df = rng.standard_normal(ndst)
     xs, ys = rng.standard_normal(n), rng.standard_normal(n)
-    for name, a in dict(idx=idx, vi=vi, vf=vf, di=di, df=df, xs=xs, ys=ys).items():
+    data = rng.integers(0, 1 << 40, n, dtype=np.int64)
+    dh = rng.integers(-1000, 1000, ndst, dtype=np.int64)
+    for name, a in dict(idx=idx, vi=vi, vf=vf, di=di, df=df, xs=xs, ys=ys, data=data, dh=dh).items():
         a.tofile(tmp_path / name)
     r = subprocess.run([glue_exe, str(tmp_path), str(n), str(ndst)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
@@ -100,6 +107,11 @@ def test_glue_helpers_from_c(orc, glue_exe, tmp_path, n, ndst, seed):
     got_f = np.fromfile(tmp_path / "df_out", dtype=np.float64)
     assert np.array_equal(got_i, want_i)
     assert np.array_equal(got_f.view(np.uint64), want_f.view(np.uint64))
+    hist = orc.weighted_histogram(data, 64)  # dst[x % 64] += x * x from zero
+    want_h = dh.copy()
+    k = min(64, ndst)
+    want_h[:k] = (want_h[:k].view(np.uint64) + np.asarray(hist[:k], np.int64).view(np.uint64)).view(np.int64)
+    assert np.array_equal(np.fromfile(tmp_path / "dh_out", dtype=np.int64), want_h)
     got_dot = float.fromhex(r.stdout.split()[0])
     want_dot = orc.dot_serial(xs, ys, 0.25)
     assert abs(got_dot - want_dot) <= 1e-12 * max(1.0, float(np.sum(np.abs(xs * ys))))
