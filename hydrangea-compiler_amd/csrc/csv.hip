@@ -106,6 +106,18 @@ __device__ __forceinline__ int64_t block_min(int64_t v, int64_t* sh) {
     return m;
 }
 
+// Sum of v over the workgroup, valid in thread 0 (one barrier; |sum| < 2^31).
+__device__ __forceinline__ int block_sum_i32(int v, int* sh) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int t = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) t += sh[w];
+    return t;
+}
+
 // Stages text[base, base + kBuf) into LDS (bytes outside [0, n) are left
 // unused: the accessor checks the range first).
 template <bool ALIGNED>
@@ -252,7 +264,7 @@ __device__ __forceinline__ int64_t scan_word(const At& at, int64_t p, int64_t st
 template <bool ALIGNED>
 __global__ __launch_bounds__(kPThreads) void csv_words_kernel(const unsigned char* __restrict__ text, int64_t n,
                                                               int64_t* __restrict__ tile_cnt) {
-    __shared__ int64_t sh[kWaves];
+    __shared__ int sh[kWaves];
     const int64_t s0 = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kSlice;
     uint32_t wv[8];
     if (ALIGNED && s0 + kSlice <= n) {
@@ -274,8 +286,7 @@ __global__ __launch_bounds__(kPThreads) void csv_words_kernel(const unsigned cha
     const int prev = (s0 > 0 && s0 - 1 < n) ? text[s0 - 1] : -1;
     const uint4 qa = make_uint4(wv[0], wv[1], wv[2], wv[3]), qb = make_uint4(wv[4], wv[5], wv[6], wv[7]);
     const int cnt = __builtin_popcount(word_starts32(qa, qb, prev < 0 || is_gap(prev), valid));
-    int64_t total;
-    block_excl_scan(cnt, &total, sh);
+    const int total = block_sum_i32(cnt, sh);  // at most kTile
     if (threadIdx.x == 0) tile_cnt[blockIdx.x] = total;
 }
 
