@@ -604,7 +604,8 @@ __global__ __launch_bounds__(kFThreads) void fmt_len_kernel(const uint64_t* __re
                                                             uint32_t* __restrict__ exact_bits,
                                                             unsigned long long* __restrict__ exact_cnt,
                                                             uint64_t* __restrict__ packed) {
-    __shared__ int64_t sh[kWaves];
+    static_assert(kFThreads == kPThreads, "block_sum_i32 sums kPThreads threads");
+    __shared__ int sh[kWaves];
     const int64_t i0 = (int64_t)blockIdx.x * kFBlock + threadIdx.x * kFPer;
     int64_t len = 0;
     uint64_t b[kFPer], pk[kFPer];
@@ -628,8 +629,7 @@ __global__ __launch_bounds__(kFThreads) void fmt_len_kernel(const uint64_t* __re
         }
     }
     if (FLOAT) store_group(packed, i0, n, pk);
-    int64_t total;
-    block_excl_scan(len, &total, sh);
+    const int total = block_sum_i32((int)len, sh);  // at most kFBlock * kMaxText bytes
     if (threadIdx.x == 0) block_len[blockIdx.x] = total;
 }
 
