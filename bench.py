@@ -471,7 +471,57 @@ def bench_batched(steps, warmup, cpu=True):
     return res
 
 
-def bench_serve(count=1500, budget=8 << 30, workers=8, load=0.7):
+def serve_sample(reqs, per_kind=3):
+    """Indices of the served requests checked against the oracle: the largest
+    request of each class and the first `per_kind` of each class."""
+    size = dict(stencil=lambda r: r["side"] ** 2 * r["iters"], hist=lambda r: r["n"], dot=lambda r: r["n"],
+                coo=lambda r: r["nnz"])
+    picked = set()
+    for kind, key in size.items():
+        idx = [k for k, r in enumerate(reqs) if r["kind"] == kind]
+        if not idx:
+            continue
+        picked.add(max(idx, key=lambda k: key(reqs[k])))
+        picked.update(idx[:per_kind])
+    return picked
+
+
+def serve_check(reqs, kept, pools):
+    """The kept served results against the oracle (integers and stencils bit
+    for bit, dots within 1e-12 relative)."""
+    import pyoracle
+
+    host = {}
+
+    def pool(name):
+        if name not in host:
+            host[name] = getattr(pools, name).numpy()
+        return host[name]
+
+    ok = {}
+    for k, got in sorted(kept.items()):
+        r = reqs[k]
+        if r["kind"] == "stencil":
+            u = pool("grid")[: r["side"] ** 2].reshape(r["side"], r["side"])
+            good = bits_equal(got, pyoracle.iterate_stencil2d(u, r["iters"], 0, 0.1).ravel())
+        elif r["kind"] == "hist":
+            good = bits_equal(got, pyoracle.weighted_histogram(pool("data")[: r["n"]], r["bins"]))
+        elif r["kind"] == "dot":
+            ref = pyoracle.dot(pool("xs")[: r["n"]], pool("ys")[: r["n"]])
+            good = bool(abs(got[0] - ref) <= 1e-12 * abs(ref))
+        else:
+            n = r["nnz"]
+            rp, ci, cv, m = pyoracle.coo_to_csr(r["nrows"], r["ncols"], pool("rows")[:n], pool("cols")[:n],
+                                                pool("vals")[:n])
+            good = bool(got[3] == m and bits_equal(got[0], rp) and bits_equal(got[1], ci) and bits_equal(got[2], cv))
+        ok[k] = good
+    return dict(all=all(ok.values()), checked=len(ok), failed=[k for k, v in ok.items() if not v],
+                sample="largest request of each class and the first 3 of each class, from an untimed "
+                       "all-at-once pass of the same mix",
+                requests={str(k): reqs[k] for k in sorted(ok)})
+
+
+def bench_serve(count=1500, budget=8 << 30, workers=8, load=0.7, cpu=True):
     """Served mixed stream vs plain per-call launches on the same requests.
 
     plain: one synchronous hydhip_run_request_plain per request (hipMalloc'd
@@ -543,6 +593,24 @@ def bench_serve(count=1500, budget=8 << 30, workers=8, load=0.7):
             drain(srv, outstanding, lat, None, 1000.0)
         poisson_s = time.perf_counter() - t0
         st = srv.stats()
+        check = None
+        if cpu:
+            # untimed: the whole mix once more, all at once, with a sample's
+            # results kept and compared with the oracle afterwards
+            sample = serve_sample(reqs)
+            ids = [srv.submit(r) for r in hreqs]
+            assert srv.wait_all(600_000)
+            kept = {}
+            for k, rid in enumerate(ids):
+                res = srv.wait(rid)
+                if res.status != hydhip.HYD_OK:
+                    raise hydhip.HydError(res.status, "served request failed")
+                if k in sample:
+                    kept[k] = servemix.fetch(reqs[k], res)
+                srv.release(rid)
+            while srv.next_done(0.0) is not None:
+                pass
+            check = serve_check(reqs, kept, pools)
     finally:
         srv.close()
         pools.free()
@@ -560,6 +628,8 @@ def bench_serve(count=1500, budget=8 << 30, workers=8, load=0.7):
                             p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99))),
         speedup_vs_plain=plain_s / sat_s,
         totals=dict(completed=st["completed"], failed=st["failed"], rejected=st["rejected"]),
+        matches_oracle=None if check is None else check["all"],
+        oracle_sample=check,
     )
 
 
@@ -630,7 +700,7 @@ def main():
 
     if not only or "serve" in only:
         log("served mix ...")
-        kernels["served_mix"] = bench_serve()
+        kernels["served_mix"] = bench_serve(cpu=not args.no_cpu)
 
     cpu = None
     if not args.no_cpu:

@@ -912,18 +912,27 @@ int io_threads() {
 // buf[0, len) <-> file bytes [off, off + len): reads by io_threads()
 // threads; writes by one (the kernel serialises writers of one file on its
 // inode lock, so more threads only add contention).
-bool parallel_io(int fd, unsigned char* buf, size_t len, off_t off, bool write) {
+// On failure *err is the errno of the failing call, or EIO when a write
+// stored nothing (0 returned, errno untouched) or a read met the file's end.
+bool parallel_io(int fd, unsigned char* buf, size_t len, off_t off, bool write, int* err) {
     const int nt = (write || len < ((size_t)4 << 20)) ? 1 : io_threads();
     std::atomic<bool> ok{true};
+    std::atomic<int> first_err{0};
     auto piece = [&](int t) {
         const size_t a = len * t / nt, e = len * (t + 1) / nt;
         size_t done = a;
         while (done < e && ok.load()) {
             const ssize_t r = write ? pwrite(fd, buf + done, e - done, off + (off_t)done)
                                     : pread(fd, buf + done, e - done, off + (off_t)done);
-            if (r > 0) done += (size_t)r;
-            else if (r < 0 && errno == EINTR) continue;
-            else ok = false;  // error, or the file ended early
+            if (r > 0) {
+                done += (size_t)r;
+            } else if (r < 0 && errno == EINTR) {
+                continue;
+            } else {  // error, or the file ended early
+                int expected = 0;
+                first_err.compare_exchange_strong(expected, r < 0 ? errno : EIO);
+                ok = false;
+            }
         }
     };
     if (nt == 1) {
@@ -933,8 +942,23 @@ bool parallel_io(int fd, unsigned char* buf, size_t len, off_t off, bool write) 
         for (int t = 0; t < nt; ++t) th.emplace_back(piece, t);
         for (auto& t : th) t.join();
     }
+    if (!ok.load() && err) *err = first_err.load();
     return ok.load();
 }
+
+// Armed for a call that queues copies into the pinned halves on s: on an
+// early (error) return it waits for every copy the call queued, so that no
+// later caller gets a half that a copy still reads or fills.
+struct DrainOnError {
+    hipStream_t s;
+    bool ok = false;
+    explicit DrainOnError(hipStream_t st) : s(st) {}
+    ~DrainOnError() {
+        if (ok) return;
+        (void)hipStreamSynchronize(s);
+        g_io.pending[0] = g_io.pending[1] = false;
+    }
+};
 
 // dst[0, len) = src[0, len) by io_threads() threads (one below 4 MiB): a
 // single thread copying pageable memory into the pinned halves is slower
@@ -962,6 +986,7 @@ void parallel_copy(unsigned char* dst, const unsigned char* src, size_t len) {
 int file_to_device(int fd, unsigned char* dev, int64_t nbytes, hipStream_t s, bool* short_read) {
     std::lock_guard<std::mutex> lk(g_io.mu);
     HYD_TRY(io_buffers());
+    DrainOnError drain(s);
     *short_read = false;
     int64_t c = 0;
     for (int64_t off = 0; off < nbytes; off += (int64_t)kIoHalf, ++c) {
@@ -969,7 +994,7 @@ int file_to_device(int fd, unsigned char* dev, int64_t nbytes, hipStream_t s, bo
         const size_t len = (size_t)std::min<int64_t>((int64_t)kIoHalf, nbytes - off);
         if (g_io.pending[k]) HYD_TRY_HIP(hipEventSynchronize(g_io.ev[k]));
         g_io.pending[k] = false;
-        if (!parallel_io(fd, g_io.host[k], len, (off_t)off, false)) {
+        if (!parallel_io(fd, g_io.host[k], len, (off_t)off, false, nullptr)) {
             *short_read = true;
             break;
         }
@@ -982,6 +1007,7 @@ int file_to_device(int fd, unsigned char* dev, int64_t nbytes, hipStream_t s, bo
             HYD_TRY_HIP(hipEventSynchronize(g_io.ev[k]));
             g_io.pending[k] = false;
         }
+    drain.ok = true;
     return HYD_OK;
 }
 
@@ -990,6 +1016,7 @@ int file_to_device(int fd, unsigned char* dev, int64_t nbytes, hipStream_t s, bo
 int device_to_file(int fd, const unsigned char* dev, int64_t bytes, hipStream_t s) {
     std::lock_guard<std::mutex> lk(g_io.mu);
     HYD_TRY(io_buffers());
+    DrainOnError drain(s);
     const int64_t halves = ceil_div(bytes, (int64_t)kIoHalf);
     auto issue = [&](int64_t c) -> int {
         const int64_t off = c * (int64_t)kIoHalf;
@@ -1004,14 +1031,11 @@ int device_to_file(int fd, const unsigned char* dev, int64_t bytes, hipStream_t 
         HYD_TRY_HIP(hipEventSynchronize(g_io.ev[c & 1]));
         const int64_t off = c * (int64_t)kIoHalf;
         const int64_t len = std::min<int64_t>((int64_t)kIoHalf, bytes - off);
-        if (!parallel_io(fd, g_io.host[c & 1], (size_t)len, (off_t)off, true)) {
-            const int err = errno;
-            // the next half's copy may be in flight: no later caller may get
-            // a half that a copy still fills
-            for (int k = 0; k < 2; ++k) (void)hipEventSynchronize(g_io.ev[k]);
+        int err = 0;
+        if (!parallel_io(fd, g_io.host[c & 1], (size_t)len, (off_t)off, true, &err))
             return set_error(HYD_E_INVALID, "csv: write failed: %s", strerror(err));
-        }
     }
+    drain.ok = true;
     return HYD_OK;
 }
 
@@ -1024,6 +1048,7 @@ int device_to_file(int fd, const unsigned char* dev, int64_t bytes, hipStream_t 
 int staged_h2d(void* dev, const void* host, size_t bytes, hipStream_t s) {
     std::lock_guard<std::mutex> lk(g_io.mu);
     HYD_TRY(io_buffers());
+    DrainOnError drain(s);
     unsigned char* d = static_cast<unsigned char*>(dev);
     const unsigned char* h = static_cast<const unsigned char*>(host);
     int64_t c = 0;
@@ -1042,12 +1067,14 @@ int staged_h2d(void* dev, const void* host, size_t bytes, hipStream_t s) {
             HYD_TRY_HIP(hipEventSynchronize(g_io.ev[k]));
             g_io.pending[k] = false;
         }
+    drain.ok = true;
     return HYD_OK;
 }
 
 int staged_d2h(void* host, const void* dev, size_t bytes, hipStream_t s) {
     std::lock_guard<std::mutex> lk(g_io.mu);
     HYD_TRY(io_buffers());
+    DrainOnError drain(s);
     unsigned char* h = static_cast<unsigned char*>(host);
     const unsigned char* d = static_cast<const unsigned char*>(dev);
     const int64_t halves = ceil_div((int64_t)bytes, (int64_t)kIoHalf);
@@ -1065,6 +1092,7 @@ int staged_d2h(void* host, const void* dev, size_t bytes, hipStream_t s) {
         const size_t off = (size_t)c * kIoHalf;
         parallel_copy(h + off, g_io.host[c & 1], std::min(kIoHalf, bytes - off));
     }
+    drain.ok = true;
     return HYD_OK;
 }
 

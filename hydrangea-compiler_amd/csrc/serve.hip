@@ -150,6 +150,82 @@ int validate(const hyd_request_t& r) {
     return HYD_E_INVALID;
 }
 
+// ---- the plain per-call path's buffers ----------------------------------
+// Byte sizes of the device buffers one plain call holds, in output order
+// (the stencil's second entry is the ping-pong partner, 0 when one step or
+// none needs no partner).  Returns how many there are.
+int plain_buffer_sizes(const hyd_request_t& r, int64_t sizes[3]) {
+    switch (r.kind) {
+        case HYD_REQ_STENCIL:
+            sizes[0] = r.h * r.w * 8;
+            sizes[1] = r.iters > 1 ? r.h * r.w * 8 : 0;
+            return 2;
+        case HYD_REQ_HIST: sizes[0] = r.bins * 8; return 1;
+        case HYD_REQ_DOT: sizes[0] = 8; return 1;
+        default:
+            sizes[0] = (r.nrows + 1) * 8;
+            sizes[1] = r.n * 8;
+            sizes[2] = r.n * 8;
+            return 3;
+    }
+}
+
+using AllocFn = int (*)(void**, int64_t);
+using FreeFn = void (*)(void*);
+
+// Allocates every nonzero size in order; on a failure frees what it had
+// allocated, leaves every pointer null and returns the failure's status.
+int plain_alloc_all(const int64_t* sizes, int n, void** ptrs, AllocFn alloc, FreeFn release) {
+    for (int k = 0; k < n; ++k) ptrs[k] = nullptr;
+    for (int k = 0; k < n; ++k) {
+        if (sizes[k] <= 0) continue;
+        const int rc = alloc(&ptrs[k], sizes[k]);
+        if (rc != HYD_OK) {
+            for (int j = 0; j < k; ++j) {
+                release(ptrs[j]);
+                ptrs[j] = nullptr;
+            }
+            ptrs[k] = nullptr;
+            return rc;
+        }
+    }
+    return HYD_OK;
+}
+
+int device_alloc(void** p, int64_t bytes) {
+    if (hipMalloc(p, (size_t)std::max<int64_t>(bytes, 8)) != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return set_error(HYD_E_OOM, "hipMalloc(%lld) failed", (long long)bytes);
+    }
+    return HYD_OK;
+}
+
+void device_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+// The host-only allocator of hydhip_test_plain_alloc.
+int g_test_fail_at = -1, g_test_calls = 0;
+int64_t g_test_live = 0;
+
+int test_alloc(void** p, int64_t bytes) {
+    if (g_test_calls++ == g_test_fail_at) {
+        *p = nullptr;
+        return set_error(HYD_E_OOM, "injected allocation failure (%lld bytes)", (long long)bytes);
+    }
+    *p = malloc((size_t)std::max<int64_t>(bytes, 8));
+    if (!*p) return set_error(HYD_E_OOM, "malloc(%lld) failed", (long long)bytes);
+    ++g_test_live;
+    return HYD_OK;
+}
+
+void test_free(void* p) {
+    if (!p) return;
+    free(p);
+    --g_test_live;
+}
+
 }  // namespace
 
 struct hyd_server {
@@ -589,46 +665,68 @@ int hydhip_run_request_plain(const hyd_request_t* req, hyd_result_t* out) {
     HYD_TRY(ensure_init());
     const hyd_request_t& r = *req;
     hipStream_t s = resolve_stream(nullptr);
+    int64_t sizes[3];
+    void* buf[3] = {nullptr, nullptr, nullptr};
+    const int nbuf = plain_buffer_sizes(r, sizes);
+    HYD_TRY(plain_alloc_all(sizes, nbuf, buf, device_alloc, device_free));
     int rc = HYD_OK;
-    auto dmalloc = [&](void** p, int64_t bytes) -> int {
-        if (hipMalloc(p, (size_t)std::max<int64_t>(bytes, 8)) != hipSuccess) {
-            (void)hipGetLastError();
-            return set_error(HYD_E_OOM, "hipMalloc(%lld) failed", (long long)bytes);
-        }
-        return HYD_OK;
-    };
+    int64_t count = 0;
+    double* result = nullptr;
     if (r.kind == HYD_REQ_STENCIL) {
-        void *a = nullptr, *b = nullptr;
-        HYD_TRY(dmalloc(&a, r.h * r.w * 8));
-        if (r.iters > 1) HYD_TRY(dmalloc(&b, r.h * r.w * 8));
-        double* result = nullptr;
         rc = hydhip_stencil2d_iterate_f64(static_cast<const double*>(r.in0), r.h, r.w, HYD_STENCIL_HEAT,
-                                          0.1, r.iters, static_cast<double*>(a), static_cast<double*>(b), 0,
-                                          &result, s);
-        HYD_TRY_HIP(hipStreamSynchronize(s));
-        if (result == b) std::swap(a, b);
-        if (b) (void)hipFree(b);
-        out->out0 = a;
+                                          0.1, r.iters, static_cast<double*>(buf[0]),
+                                          static_cast<double*>(buf[1]), 0, &result, s);
     } else if (r.kind == HYD_REQ_HIST) {
-        HYD_TRY(dmalloc(&out->out0, r.bins * 8));
         rc = hydhip_weighted_histogram_i64(static_cast<const int64_t*>(r.in0), r.n, r.bins,
-                                           static_cast<int64_t*>(out->out0), HYD_SCATTER_AUTO, s);
+                                           static_cast<int64_t*>(buf[0]), HYD_SCATTER_AUTO, s);
     } else if (r.kind == HYD_REQ_DOT) {
-        HYD_TRY(dmalloc(&out->out0, 8));
         rc = hydhip_dot_f64(static_cast<const double*>(r.in0), static_cast<const double*>(r.in1), r.n, 0.0,
-                            static_cast<double*>(out->out0), s);
+                            static_cast<double*>(buf[0]), s);
     } else {
-        HYD_TRY(dmalloc(&out->out0, (r.nrows + 1) * 8));
-        HYD_TRY(dmalloc(&out->out1, r.n * 8));
-        HYD_TRY(dmalloc(&out->out2, r.n * 8));
         rc = hydhip_coo_to_csr_i64(r.nrows, r.ncols, r.n, static_cast<const int64_t*>(r.in0),
                                    static_cast<const int64_t*>(r.in1), static_cast<const int64_t*>(r.in2),
-                                   static_cast<int64_t*>(out->out0), static_cast<int64_t*>(out->out1),
-                                   static_cast<int64_t*>(out->out2), &out->count, s);
+                                   static_cast<int64_t*>(buf[0]), static_cast<int64_t*>(buf[1]),
+                                   static_cast<int64_t*>(buf[2]), &count, s);
     }
-    HYD_TRY_HIP(hipStreamSynchronize(s));
-    out->status = rc;
+    // wait for the work even after a failed call: kernels queued before the
+    // failure may still be reading or writing the buffers about to be freed
+    const hipError_t se = hipStreamSynchronize(s);
+    if (rc == HYD_OK && se != hipSuccess)
+        rc = set_error(HYD_E_HIP, "hipStreamSynchronize failed: %s", hipGetErrorString(se));
+    if (rc != HYD_OK) {
+        for (int k = 0; k < nbuf; ++k) device_free(buf[k]);
+        return rc;
+    }
+    if (r.kind == HYD_REQ_STENCIL && result == static_cast<double*>(buf[1])) std::swap(buf[0], buf[1]);
+    if (r.kind == HYD_REQ_STENCIL) {
+        device_free(buf[1]);  // the partner buffer of the ping-pong
+        buf[1] = nullptr;
+    }
+    out->out0 = buf[0];
+    out->out1 = buf[1];
+    out->out2 = buf[2];
+    out->count = count;
+    out->status = HYD_OK;
     out->done = 1;
+    return HYD_OK;
+}
+
+// Host-only test of the plain path's allocation sequence: a counting host
+// allocator whose call number `fail_at` (0-based; negative: none) fails.
+// Returns the sequence's status; *live_after is the number of allocations
+// still held afterwards (the buffers on success, 0 after a failure).
+int hydhip_test_plain_alloc(const hyd_request_t* req, int fail_at, int64_t* live_after) {
+    if (!req || !live_after) return set_error(HYD_E_INVALID, "hydhip_test_plain_alloc: null argument");
+    if (validate(*req) != HYD_OK) return set_error(HYD_E_INVALID, "hydhip_test_plain_alloc: malformed request");
+    int64_t sizes[3];
+    void* buf[3] = {nullptr, nullptr, nullptr};
+    const int nbuf = plain_buffer_sizes(*req, sizes);
+    g_test_fail_at = fail_at;
+    g_test_calls = 0;
+    g_test_live = 0;
+    const int rc = plain_alloc_all(sizes, nbuf, buf, test_alloc, test_free);
+    *live_after = g_test_live;
+    for (int k = 0; k < nbuf; ++k) test_free(buf[k]);
     return rc;
 }
 

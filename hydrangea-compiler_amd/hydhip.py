@@ -191,6 +191,7 @@ _SIGS = {
     "hyd_request_bytes": (_I64, [ctypes.POINTER(HydRequest)]),
     "hydhip_run_request_plain": (_INT, [ctypes.POINTER(HydRequest), ctypes.POINTER(HydResult)]),
     "hydhip_result_free_plain": (_INT, [ctypes.POINTER(HydResult)]),
+    "hydhip_test_plain_alloc": (_INT, [ctypes.POINTER(HydRequest), _INT, ctypes.POINTER(_I64)]),
     # array ABI
     "hyd_shape_size": (_I64, [HydTuple]),
     "hyd_array_alloc_bytes": (_PA, [HydTuple, ctypes.c_size_t]),

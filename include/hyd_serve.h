@@ -90,9 +90,14 @@ int64_t hyd_request_bytes(const hyd_request_t* req);
 
 /* The per-call path the server is compared with: hipMalloc the outputs, run
  * the request's kernels with plain launches, synchronise.  Free with
- * hydhip_result_free_plain. */
+ * hydhip_result_free_plain.  On any failure nothing stays allocated and
+ * *out is left zeroed. */
 int hydhip_run_request_plain(const hyd_request_t* req, hyd_result_t* out);
 int hydhip_result_free_plain(hyd_result_t* res);
+/* Host-only test hook: runs the plain path's allocation sequence for `req`
+ * with a counting host allocator whose call number `fail_at` fails (0-based,
+ * negative: none); *live_after = allocations still held when it returns. */
+int hydhip_test_plain_alloc(const hyd_request_t* req, int fail_at, int64_t* live_after);
 
 #ifdef __cplusplus
 }

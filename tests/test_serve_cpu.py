@@ -148,3 +148,23 @@ def test_no_batch_flag(fake):
     assert s.wait_all(30_000)
     assert len({s.wait(r).batch_id for r in ids}) == 5
     assert s.stats()["batched_jobs"] == 0
+
+
+@pytest.mark.parametrize("req,nbufs", [(lambda: stencil(64, 5), 2), (lambda: stencil(64, 1), 1),
+                                       (lambda: hist(100), 1), (lambda: dot(100), 1),
+                                       (lambda: coo(1000, 100), 3)])
+def test_plain_path_frees_on_allocation_failure(req, nbufs):
+    # each allocation of the plain per-call path fails in turn: whatever was
+    # allocated before it is freed and the failure is reported
+    import ctypes
+
+    r = req()
+    live = ctypes.c_int64()
+    L = hydhip.lib()
+    assert L.hydhip_test_plain_alloc(ctypes.byref(r), -1, ctypes.byref(live)) == hydhip.HYD_OK
+    assert live.value == nbufs
+    for k in range(nbufs):
+        rc = L.hydhip_test_plain_alloc(ctypes.byref(r), k, ctypes.byref(live))
+        assert rc == hydhip.HYD_E_OOM, k
+        assert live.value == 0, k
+        assert b"injected" in L.hydhip_last_error()
