@@ -597,19 +597,30 @@ def bench_serve(count=1500, budget=8 << 30, workers=8, load=0.7, cpu=True):
         if cpu:
             # untimed: the whole mix once more, all at once, with a sample's
             # results kept and compared with the oracle afterwards
+            # (results are released as they arrive: outputs held until
+            # release count against the HBM budget)
             sample = serve_sample(reqs)
-            ids = [srv.submit(r) for r in hreqs]
-            assert srv.wait_all(600_000)
+            index = {}
             kept = {}
-            for k, rid in enumerate(ids):
-                res = srv.wait(rid)
-                if res.status != hydhip.HYD_OK:
-                    raise hydhip.HydError(res.status, "served request failed")
-                if k in sample:
-                    kept[k] = servemix.fetch(reqs[k], res)
-                srv.release(rid)
-            while srv.next_done(0.0) is not None:
-                pass
+
+            def keep(block_ms):
+                got = srv.next_done(block_ms)
+                while got is not None:
+                    rid, res = got
+                    if res.status != hydhip.HYD_OK:
+                        raise hydhip.HydError(res.status, "served request failed")
+                    k = index.pop(rid)
+                    if k in sample:
+                        kept[k] = servemix.fetch(reqs[k], res)
+                    srv.release(rid)
+                    got = srv.next_done(0.0)
+
+            for k, r in enumerate(hreqs):
+                index[srv.submit(r)] = k
+                keep(0.0)
+            while index:
+                keep(1000.0)
+            log("served mix: checking %d sampled results against the oracle" % len(kept))
             check = serve_check(reqs, kept, pools)
     finally:
         srv.close()
