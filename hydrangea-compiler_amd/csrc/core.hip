@@ -49,8 +49,14 @@ int ensure_init() {
     snprintf(g_info.name, sizeof g_info.name, "%s", prop.gcnArchName);
     // Default priority only: streams of mixed priorities are not used.
     HYD_TRY_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    HYD_TRY(noalloc_scratch_setup(dev));
     g_inited = true;
     return HYD_OK;
+}
+
+bool library_initialized() {
+    std::lock_guard<std::mutex> lk(g_init_mu);
+    return g_inited;
 }
 
 hipStream_t resolve_stream(void* stream) {

@@ -19,6 +19,14 @@ void clear_error();
 // created).  Returns HYD_OK or an error code.
 int ensure_init();
 
+// Whether ensure_init() has succeeded (no side effects).
+bool library_initialized();
+
+// Looks up, once per device, the device address of the no-allocation
+// histogram's file-scope scratch array (scatter.hip); `device` must be the
+// current device.  Called by ensure_init() for the library's device.
+int noalloc_scratch_setup(int device);
+
 // NULL -> the library's stream.
 hipStream_t resolve_stream(void* stream);
 

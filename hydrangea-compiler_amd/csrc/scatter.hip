@@ -20,6 +20,7 @@
 #include <stdio.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "hyd_internal.hpp"
 
@@ -251,9 +252,18 @@ __global__ __launch_bounds__(BLOCK) void scatter_global_kernel(Src src, int64_t 
 constexpr int kBlock = 512;
 constexpr size_t kLdsTarget = 64 * 1024;  // per workgroup: leaves room for 2 per CU
 
+// The LDS form's merge scratch: by default the stream's zeroed scratch
+// block; `words` long at `ptr` when a caller supplies its own (cleared by
+// the call before use when `clear` is set).
+struct MergeScratch {
+    unsigned long long* ptr = nullptr;
+    int64_t words = 0;
+    bool clear = false;
+};
+
 template <class Src>
 int launch_scatter(const Src& src, bool vec, int64_t n, int64_t nbins, int64_t* dst_i64,
-                   int strategy, bool zero_first, hipStream_t s) {
+                   int strategy, bool zero_first, hipStream_t s, MergeScratch own = MergeScratch()) {
     const DeviceInfo& di = device_info();
     unsigned long long* dst = reinterpret_cast<unsigned long long*>(dst_i64);
     // at most 128 KiB of the 160 KiB LDS per workgroup (power-of-two bin ranges)
@@ -267,6 +277,8 @@ int launch_scatter(const Src& src, bool vec, int64_t n, int64_t nbins, int64_t* 
         // for large bin counts with many elements, else global atomics.
         int64_t stride = nbins | 1;
         bool fits = (size_t)stride * 8 <= lds_cap;
+        // a supplied merge scratch must hold the one spread copy of larger bins
+        if (own.ptr && nbins > own.words) fits = false;
         if (fits && n >= 16 * nbins)
             strategy = HYD_SCATTER_LDS;
         else if (!fits && n >= 8 * nbins)
@@ -288,11 +300,23 @@ int launch_scatter(const Src& src, bool vec, int64_t n, int64_t nbins, int64_t* 
         int64_t blocks = std::min<int64_t>(want_blocks, (int64_t)di.num_cus);
         // spread copies in the zeroed scratch: 4096 words' worth, <= 32
         int ncopy = (int)std::max<int64_t>(1, std::min<int64_t>(32, 4096 / nbins));
-        unsigned char* zs = static_cast<unsigned char*>(stream_zero_scratch(s));
-        if (!zs) return set_error(HYD_E_OOM, "scatter: scratch allocation failed");
-        if ((size_t)ncopy * nbins * 8 > kZeroScratchBytes)
-            return set_error(HYD_E_INVALID, "scatter: %lld bins exceed the merge scratch", (long long)nbins);
-        auto* spread = reinterpret_cast<unsigned long long*>(zs);
+        unsigned long long* spread = nullptr;
+        if (own.ptr) {
+            ncopy = (int)std::max<int64_t>(1, std::min<int64_t>(ncopy, own.words / nbins));
+            if ((int64_t)ncopy * nbins > own.words)
+                return set_error(HYD_E_INVALID,
+                                 "scatter: %lld bins exceed the supplied merge scratch of %lld words",
+                                 (long long)nbins, (long long)own.words);
+            spread = own.ptr;
+            if (own.clear) HYD_TRY(fill_zero(spread, (size_t)ncopy * nbins * 8, s));
+        } else {
+            unsigned char* zs = static_cast<unsigned char*>(stream_zero_scratch(s));
+            if (!zs) return set_error(HYD_E_OOM, "scatter: scratch allocation failed");
+            if ((size_t)ncopy * nbins * 8 > kZeroScratchBytes)
+                return set_error(HYD_E_INVALID, "scatter: %lld bins exceed the merge scratch",
+                                 (long long)nbins);
+            spread = reinterpret_cast<unsigned long long*>(zs);
+        }
         int accumulate = zero_first ? 0 : 1;
         if (vec) {
             auto k = scatter_lds_kernel<kBlock, true, Src>;
@@ -359,7 +383,72 @@ inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 
 }  // namespace
 }  // namespace hyd
 
+// ---- the no-allocation histogram's merge scratch --------------------------
+// A file-scope device array, one copy per GPU the library runs on, held until
+// the process ends.  In host code `scratch` names a host-side stand-in, not a
+// device address: host code uses the per-device address that
+// noalloc_scratch_setup() looks up, never the name.
+constexpr int64_t kNoAllocScratchWords = 8192;  // 64 KiB: bins up to 8192 in the LDS form
+static __device__ __align__(256) unsigned long long scratch[kNoAllocScratchWords];
+
+namespace hyd {
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+unsigned long long* g_noalloc_ptr[kMaxDevices];  // device address of `scratch`, per device
+}  // namespace
+
+// Looks up the device address of `scratch` on `device` (which must be the
+// current device) once; later calls return at once.
+int noalloc_scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices)
+        return set_error(HYD_E_INVALID, "no-allocation scratch: device %d out of range", device);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return HYD_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return set_error(HYD_E_HIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", device,
+                         hipGetErrorString(e));
+    g_noalloc_ptr[device] = static_cast<unsigned long long*>(p);
+    return HYD_OK;
+}
+}  // namespace hyd
+
 using namespace hyd;
+
+// hydhip_weighted_histogram_i64 with the LDS form's merge scratch in the
+// file-scope `scratch` array instead of the stream's allocated block: no
+// device memory is allocated by this call, nor by the library on its behalf
+// (the library must have been initialised: hyd_hip_init or any earlier call).
+extern "C" int hydhip_weighted_histogram_i64_noalloc(const int64_t* data, int64_t n, int64_t nbins,
+                                                     int64_t* hist, int strategy, void* stream) {
+    if (!library_initialized())
+        return set_error(HYD_E_INVALID,
+                         "hydhip_weighted_histogram_i64_noalloc: the library is not initialised "
+                         "(call hyd_hip_init first: this entry point performs no set-up)");
+    if (n < 0 || nbins < 1 || nbins > (int64_t)1 << 31 || !hist || (n > 0 && !data))
+        return set_error(HYD_E_INVALID,
+                         "hydhip_weighted_histogram_i64_noalloc: bad arguments (n=%lld nbins=%lld)",
+                         (long long)n, (long long)nbins);
+    if (strategy == HYD_SCATTER_LDS && nbins > kNoAllocScratchWords)
+        return set_error(HYD_E_INVALID,
+                         "hydhip_weighted_histogram_i64_noalloc: the LDS form's merge needs %lld words of "
+                         "scratch, more than the %lld the no-allocation scratch holds",
+                         (long long)nbins, (long long)kNoAllocScratchWords);
+    int dev = 0;
+    HYD_TRY_HIP(hipGetDevice(&dev));
+    HYD_TRY(noalloc_scratch_setup(dev));  // done at initialisation for the library's device
+    hipStream_t s = resolve_stream(stream);
+    if (n == 0) return fill_zero(hist, (size_t)nbins * 8, s);
+    const MergeScratch own{g_noalloc_ptr[dev], kNoAllocScratchWords, true};
+    const ModConst mod = make_mod_const(nbins);
+    if (mod.mask)
+        return launch_scatter(FusedSrc<true>{data, ModSquare<true>{mod}}, aligned16(data), n, nbins,
+                              hist, strategy, true, s, own);
+    return launch_scatter(FusedSrc<false>{data, ModSquare<false>{mod}}, aligned16(data), n, nbins,
+                          hist, strategy, true, s, own);
+}
 
 extern "C" int hydhip_weighted_histogram_i64(const int64_t* data, int64_t n, int64_t nbins,
                                              int64_t* hist, int strategy, void* stream) {

@@ -136,6 +136,7 @@ _SIGS = {
     "hydhip_fill_splitmix_i64": (_INT, [_P, _I64, _U64, _U64, _INT, _P]),
     "hydhip_fill_splitmix_f64": (_INT, [_P, _I64, _U64, _U64, _D, _D, _P]),
     "hydhip_weighted_histogram_i64": (_INT, [_P, _I64, _I64, _P, _INT, _P]),
+    "hydhip_weighted_histogram_i64_noalloc": (_INT, [_P, _I64, _I64, _P, _INT, _P]),
     "hydhip_scatter_add_i64": (_INT, [_P, _P, _I64, _P, _I64, _INT, _P]),
     "hydhip_scatter_guarded_add_i64": (_INT, [_P, _P, _P, _I64, _P, _I64, _INT, _P]),
     "hydhip_scatter_chain_add_i64": (_INT, [_I64, _P, _P, _P, _P, _P, _I64, _INT, _P]),

@@ -86,6 +86,18 @@ int hydhip_fill_splitmix_f64(double* dst, int64_t n, uint64_t seed,
  * Elements whose route falls outside [0, nbins) are dropped. */
 int hydhip_weighted_histogram_i64(const int64_t* data, int64_t n, int64_t nbins,
                                   int64_t* hist, int strategy, void* stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, hosts that
+ * forbid hipMalloc): hydhip_weighted_histogram_i64 with the same arguments
+ * and results, whose LDS form merges through a fixed 64 KiB device array
+ * (8192 words, one per GPU, held until the process ends) instead of an
+ * allocated block.  It calls no hipMalloc/hipFree or allocating routine; the
+ * library must be initialised first (hyd_hip_init), else HYD_E_INVALID.
+ * HYD_SCATTER_LDS with nbins > 8192 is rejected (HYD_E_INVALID); the automatic
+ * choice then takes a form without scratch.  Every call on one GPU shares the
+ * array: calls that overlap on one GPU (other streams, other threads) race. */
+int hydhip_weighted_histogram_i64_noalloc(const int64_t* data, int64_t n, int64_t nbins,
+                                          int64_t* hist, int strategy, void* stream);
 /* Guarded scatter (+) on int64, scatter_guarded of
  * src/Language/Hydrangea/Lowering.hs:1432-1517: dst holds the defaults and
  * receives dst[idx[i]] += vals[i] for every i with guard[i] != 0 (guard NULL:
