@@ -54,6 +54,58 @@ def _run(cmd):
     return out
 
 
+HYDGEN_SRC = os.path.join(HERE, "hydgen")
+HYDGEN = os.path.join(HERE, "bin", "hydgen")
+PROGRAM_SITES = os.path.join(HERE, "..", "tests", "golden", "cfg")
+PROGRAMS_LIB = os.path.join(LIBDIR, "libhydgen_programs.so")
+PROGRAMS_SIG = os.path.join(LIBDIR, "libhydgen_programs.sig")
+
+
+def build_hydgen(verbose=False):
+    """The kernel generator (host C++): bin/hydgen."""
+    os.makedirs(os.path.dirname(HYDGEN), exist_ok=True)
+    srcs = sorted(os.path.join(HYDGEN_SRC, f) for f in os.listdir(HYDGEN_SRC) if f.endswith(".cpp"))
+    tmp = HYDGEN + ".tmp"
+    out = _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-o", tmp] + srcs)
+    if verbose and out.strip():
+        print(out)
+    os.replace(tmp, HYDGEN)
+    return HYDGEN
+
+
+def build_programs(verbose=False, jobs=None):
+    """The HIP backend's output for the example programs: every
+    tests/golden/cfg/*.sites through hydgen, compiled for gfx950 and linked
+    into lib/libhydgen_programs.so (with the launchers' signatures beside it)."""
+    gen = os.path.join(BUILD, "gen")
+    os.makedirs(gen, exist_ok=True)
+    cc = hipcc()
+    files = sorted(f for f in os.listdir(PROGRAM_SITES) if f.endswith(".sites"))
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    sigs = []
+
+    def one(f):
+        base = os.path.join(gen, f[:-6])
+        _run([HYDGEN, os.path.join(PROGRAM_SITES, f), "-o", base + ".hip", "--signatures", base + ".sig"])
+        _run([cc] + HIPCC_FLAGS + ["-c", base + ".hip", "-o", base + ".o"])
+        return base
+
+    with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
+        bases = list(ex.map(one, files))
+    for b in bases:
+        with open(b + ".sig") as fh:
+            sigs.append(fh.read())
+    tmp = PROGRAMS_LIB + ".tmp"
+    _run([cc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + [b + ".o" for b in bases] +
+         ["-L" + LIBDIR, "-lhydhip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, PROGRAMS_LIB)
+    with open(PROGRAMS_SIG, "w") as fh:
+        fh.write("".join(sigs))
+    if verbose:
+        print("built", PROGRAMS_LIB, "(%d programs)" % len(bases))
+    return PROGRAMS_LIB
+
+
 def build(verbose=False, jobs=None):
     """Compile every HIP source and link lib/libhydhip.so.  Returns its path."""
     os.makedirs(BUILD, exist_ok=True)
@@ -78,6 +130,8 @@ def build(verbose=False, jobs=None):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_hydgen(verbose)
+    build_programs(verbose, jobs)
     return LIB
 
 

@@ -393,6 +393,22 @@ void* hyd_array_device_data(hyd_array_t* arr) {
     return (unsigned char*)st->dev + (arr->data - st->base.data);
 }
 
+void* hyd_array_device_data_keep(hyd_array_t* arr) {
+    HipStorage* st = storage_of(arr);
+    void* d = hyd_array_device_data(arr);
+    if (!st->device_resident && st->bytes) {  // just uploaded: both copies are current now
+        st->device_resident = true;
+        st->host_valid = true;
+    }
+    return d;
+}
+
+int hyd_array_enter_host(hyd_array_t* arr) {
+    HipStorage* st = storage_of(arr);
+    if (st->device_resident && st->host_valid) st->device_resident = false;  // the host bytes may have changed
+    return HYD_OK;
+}
+
 int hyd_array_sync_host(hyd_array_t* arr) {
     HipStorage* st = storage_of(arr);
     if (!st->device_resident || st->host_valid || st->bytes == 0) return HYD_OK;
@@ -424,6 +440,12 @@ int hyd_array_mark_device_written(hyd_array_t* arr) {
 void hyd_hip_set_host_sync(int on) { g_host_sync = on ? 1 : 0; }
 int hyd_hip_get_host_sync(void) { return g_host_sync; }
 void hyd_hip_set_stream(void* stream) { g_export_stream = stream; }
+void* hyd_hip_get_stream(void) {
+    if (ensure_init() != HYD_OK) return nullptr;
+    return export_stream();
+}
+
+hyd_array_t* hyd_array_alloc_device8(hyd_tuple_t shape) { return hyd_array_alloc_device(shape, 8); }
 
 }  // extern "C"
 

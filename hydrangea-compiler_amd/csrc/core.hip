@@ -109,6 +109,29 @@ void* stream_zero_scratch(hipStream_t stream) {
     return p;
 }
 
+// Per (stream, slot) buffers for generated launchers (hydhip_scratch): kept
+// apart from stream_workspace, which the library's own calls use on the same
+// stream in between.
+static std::unordered_map<hipStream_t, Workspace> g_slots[4];
+
+void* stream_slot(hipStream_t stream, int slot, size_t bytes) {
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    Workspace& w = g_slots[slot][stream];
+    if (w.bytes >= bytes && w.ptr) return w.ptr;
+    if (w.ptr) {
+        if (hipStreamSynchronize(stream) != hipSuccess) return nullptr;
+        (void)hipFree(w.ptr);
+        w.ptr = nullptr;
+        w.bytes = 0;
+    }
+    size_t want = bytes < (1u << 16) ? (1u << 16) : bytes;
+    void* p = nullptr;
+    if (hipMalloc(&p, want) != hipSuccess) return nullptr;
+    w.ptr = p;
+    w.bytes = want;
+    return p;
+}
+
 static void* g_scalar = nullptr;
 
 void* scalar_slot() {
@@ -366,6 +389,24 @@ int hydhip_device_sync(void) {
 void* hydhip_default_stream(void) {
     if (ensure_init() != HYD_OK) return nullptr;
     return g_stream;
+}
+
+void* hydhip_scratch(int slot, int64_t bytes, void* stream) {
+    if (ensure_init() != HYD_OK) return nullptr;
+    if (slot < 0 || slot >= 4 || bytes < 0) {
+        set_error(HYD_E_INVALID, "hydhip_scratch: slot %d, %lld bytes", slot, (long long)bytes);
+        return nullptr;
+    }
+    void* p = stream_slot(resolve_stream(stream), slot, (size_t)bytes);
+    if (!p) set_error(HYD_E_OOM, "hydhip_scratch: %lld bytes failed", (long long)bytes);
+    return p;
+}
+
+void* hydhip_zero_scratch(void* stream) {
+    if (ensure_init() != HYD_OK) return nullptr;
+    void* p = stream_zero_scratch(resolve_stream(stream));
+    if (!p) set_error(HYD_E_OOM, "hydhip_zero_scratch: allocation failed");
+    return p;
 }
 
 int hydhip_event_create(void** event) {

@@ -33,7 +33,7 @@ def table():
 
 
 def main():
-    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "hyd_exp_table.inc")
+    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "..", "include", "hyd_exp_table.inc")
     with open(out, "w") as f:
         f.write("/* generated by gen_exp_table.py: 2^(k/128) as (tail, scale-bits) pairs */\n")
         for t, h in table():

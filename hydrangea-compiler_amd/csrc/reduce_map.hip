@@ -18,7 +18,7 @@
 
 #include <vector>
 
-#include "hyd_exp.h"
+#include "../../include/hyd_exp.h"
 #include "hyd_internal.hpp"
 
 namespace hyd {

@@ -204,11 +204,17 @@ _SIGS = {
     "hyd_array_length": (_I64, [_PA]),
     "hyd_array_device_data": (_P, [_PA]),
     "hyd_array_sync_host": (_INT, [_PA]),
+    "hyd_array_device_data_keep": (_P, [_PA]),
+    "hyd_array_enter_host": (_INT, [_PA]),
     "hyd_array_mark_host_dirty": (_INT, [_PA]),
     "hyd_array_is_device_resident": (_INT, [_PA]),
     "hyd_hip_set_host_sync": (None, [_INT]),
     "hyd_hip_get_host_sync": (_INT, []),
     "hyd_hip_set_stream": (None, [_P]),
+    "hyd_hip_get_stream": (_P, []),
+    "hyd_array_alloc_device8": (_PA, [HydTuple]),
+    "hydhip_scratch": (_P, [_INT, _I64, _P]),
+    "hydhip_zero_scratch": (_P, [_P]),
     "hyd_sort_indices": (_PA, [_PA]),
     "hyd_read_array_csv": (_PA, [ctypes.c_char_p, HydTuple]),
     "hyd_read_float_array_csv": (_PA, [ctypes.c_char_p, HydTuple]),

@@ -54,6 +54,13 @@ int hydhip_stream_destroy(void* stream);
 int hydhip_stream_sync(void* stream);
 int hydhip_device_sync(void);
 void* hydhip_default_stream(void);
+/* Device buffers for generated kernels (hydgen launchers): a buffer of at
+ * least `bytes` per (stream, slot), slot 0-3, reused by later calls on the
+ * same stream (kernels on one stream run in order).  NULL on failure. */
+void* hydhip_scratch(int slot, int64_t bytes, void* stream);
+/* A 1 MiB device block per stream that is all zero between kernels: a kernel
+ * that adds into it leaves it zero again before it finishes.  NULL on failure. */
+void* hydhip_zero_scratch(void* stream);
 int hydhip_event_create(void** event);
 int hydhip_event_destroy(void* event);
 int hydhip_event_record(void* event, void* stream);

@@ -30,6 +30,15 @@ hyd_array_t* hyd_array_alloc_device(hyd_tuple_t shape, size_t elem_size);
 /* HBM address of arr->data; uploads the host bytes first when the array is
  * not device-resident.  Exits with a message on failure (drop-in errors). */
 void* hyd_array_device_data(hyd_array_t* arr);
+/* hyd_array_device_data for code that reports its host writes (the kernels
+ * the HIP backend generates, whose printed C marks every host write with
+ * hyd_array_mark_host_dirty): after an upload both copies count as current,
+ * so later kernels read the HBM copy without uploading again. */
+void* hyd_array_device_data_keep(hyd_array_t* arr);
+/* The host may have rewritten arr->data since the last kernel (an array
+ * handed in by a caller): when the host copy is current, the next kernel
+ * uploads it again; an HBM copy that is newer stays authoritative. */
+int hyd_array_enter_host(hyd_array_t* arr);
 /* Copies the HBM contents back into arr->data if they are newer. */
 int hyd_array_sync_host(hyd_array_t* arr);
 /* The caller rewrote arr->data of a device-resident array: the next kernel
@@ -46,6 +55,14 @@ void hyd_hip_set_host_sync(int on);
 int hyd_hip_get_host_sync(void);
 /* Stream used by hyd_export_* (NULL = the library's stream). */
 void hyd_hip_set_stream(void* stream);
+/* The stream hyd_export_*, the uploads of hyd_array_device_data and the
+ * generated launchers run on (the library's stream unless set). */
+void* hyd_hip_get_stream(void);
+/* hyd_array_alloc_bytes(shape, 8) zero in HBM and on the host, both copies
+ * valid: the allocation the HIP backend prints for an array of 8-byte
+ * elements (int64, fp64, bool), so that a kernel writing it first uploads
+ * nothing. */
+hyd_array_t* hyd_array_alloc_device8(hyd_tuple_t shape);
 
 #ifdef __cplusplus
 }

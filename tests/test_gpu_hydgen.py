@@ -1,0 +1,292 @@
+"""GPU parity of the kernels hydgen generates for the example programs
+(lib/libhydgen_programs.so), called through their C-ABI launchers with
+hyd_array_t arguments, as the printed C calls them.
+
+Oracle: the serial CFG interpreter (oracle/cfg_interp.py, pinned on the CPU)
+and, where the C oracle has the same program, the C oracle at the benchmark
+sizes.  Bars: integer and index results bit-exact; elementwise and stencil
+fp64 bit-exact; fp64 reductions within 1e-12 relative; the order-exact fp64
+scatter bit-exact to the serial loop.
+"""
+import numpy as np
+import pytest
+
+import cfg_harness as H
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL_REDUCTION = 1e-12
+
+
+@pytest.fixture(scope="module")
+def prog(hyd):
+    import hydgen_lib
+
+    return hydgen_lib.programs()
+
+
+class Arrays:
+    """hyd_array_t* arguments made from numpy arrays (host copies, uploaded
+    by the first launcher that reads them), read back after the launchers."""
+
+    def __init__(self, hyd):
+        self.hyd = hyd
+        self.L = hyd.lib()
+        self.made = []
+
+    def host(self, a):
+        p = self.hyd.host_array(np.ascontiguousarray(a))
+        self.made.append(p)
+        return p
+
+    def device(self, shape):
+        p = self.L.hyd_array_alloc_device8(self.hyd.hyd_tuple(*shape))
+        self.made.append(p)
+        return p
+
+    def read(self, p, dtype):
+        assert self.L.hyd_array_sync_host(p) == 0
+        return self.hyd.read_array(p, dtype)
+
+    def free(self):
+        for p in self.made:
+            self.L.hyd_array_free(p)
+        self.made = []
+
+
+@pytest.fixture
+def arrays(hyd):
+    a = Arrays(hyd)
+    yield a
+    a.free()
+
+
+def call(hyd, prog, s, env):
+    """The site's launcher with env's values in its argument order."""
+    args = []
+    for v, t in s.get("hsArgs"):
+        x = env[v]
+        if t.name == "CTTuple":
+            x = hyd.hyd_tuple(*x)
+        args.append(x)
+    return getattr(prog, s.get("hsName"))(*args)
+
+
+def bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.uint64) if a.dtype == np.float64 else a
+
+
+SIZES = [0, 1, 1023, 1024, 1025, 100_003]
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_map_chain(hyd, orc, prog, arrays, n):
+    xs = orc.splitmix_f64(n, 901 + n, symmetric=True)
+    ys = orc.splitmix_f64(n, 902 + n, symmetric=True)
+    out = arrays.device((n,))
+    call(hyd, prog, H.site("map_chain"), dict(n=n, out=out, xs=arrays.host(xs), ys=arrays.host(ys)))
+    assert hyd.lib().hyd_array_is_device_resident(out) == 1  # left in HBM: no copy back per loop
+    assert np.array_equal(bits(arrays.read(out, np.float64)), bits(orc.map_chain(xs, ys)))
+
+
+def test_map_chain_full_size(hyd, orc, prog, arrays):
+    c = workloads.DOTMAP
+    n = c["n"]
+    xs = orc.splitmix_f64(n, c["seed"], 0, symmetric=True)
+    ys = orc.splitmix_f64(n, c["seed"], n, symmetric=True)
+    out = arrays.device((n,))
+    print("generated map chain n=%d" % n, flush=True)
+    call(hyd, prog, H.site("map_chain"), dict(n=n, out=out, xs=arrays.host(xs), ys=arrays.host(ys)))
+    assert np.array_equal(bits(arrays.read(out, np.float64)), bits(orc.map_chain(xs, ys)))
+
+
+@pytest.mark.parametrize("n", [1, 1025, 30_001])
+def test_map_chain2(hyd, prog, arrays, n):
+    xs, ks = H.map_chain2_inputs(n, 77 + n)
+    of, oi = arrays.device((n,)), arrays.device((n,))
+    env = dict(n=n, xs=arrays.host(xs), ks=arrays.host(ks), of=of, oi=oi, scale=7.25, m=11)
+    call(hyd, prog, H.site("map_chain2"), env)
+    u, g2 = H.map_chain2_numpy(xs, ks, 7.25, 11)
+    assert np.array_equal(arrays.read(oi, np.int64), g2)
+    assert np.array_equal(bits(arrays.read(of, np.float64)), bits(u))
+    if n <= 1025:  # and the interpreter (slow in Python)
+        ienv = dict(n=n, xs=xs, ks=ks, of=np.zeros(n), oi=np.zeros(n, np.int64), scale=7.25, m=11)
+        H.run_interp(H.site("map_chain2"), ienv)
+        assert np.array_equal(bits(ienv["of"]), bits(u)) and np.array_equal(ienv["oi"], g2)
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_dot(hyd, orc, prog, arrays, n):
+    xs = orc.splitmix_f64(n, 911 + n, symmetric=True)
+    ys = orc.splitmix_f64(n, 912 + n, symmetric=True)
+    env = dict(acc=0.5, red_dim=n, xs=arrays.host(xs), ys=arrays.host(ys))
+    got = call(hyd, prog, H.site("dot"), env)
+    exact = orc.dot_dd(xs, ys, init=0.5)
+    assert abs(got - exact) <= REL_TOL_REDUCTION * max(abs(exact), 1e-300)
+    assert call(hyd, prog, H.site("dot"), env) == got  # deterministic
+    if n == 0:
+        assert got == 0.5
+
+
+def test_dot_full_size(hyd, orc, prog, arrays):
+    c = workloads.DOTMAP
+    n = c["n"]
+    xs = orc.splitmix_f64(n, c["seed"], 0, symmetric=True)
+    ys = orc.splitmix_f64(n, c["seed"], n, symmetric=True)
+    print("generated dot n=%d" % n, flush=True)
+    got = call(hyd, prog, H.site("dot"), dict(acc=0.0, red_dim=n, xs=arrays.host(xs), ys=arrays.host(ys)))
+    exact = orc.dot_dd(xs, ys)
+    assert abs(got - exact) <= REL_TOL_REDUCTION * abs(exact)
+    ones = arrays.host(np.ones(n))
+    assert call(hyd, prog, H.site("dot"), dict(acc=0.0, red_dim=n, xs=ones, ys=ones)) == float(n)
+
+
+@pytest.mark.parametrize("n", [0, 1, 777, 1025, 300_007])
+def test_integer_and_product_reductions(hyd, prog, arrays, n):
+    ks = np.random.default_rng(n).integers(-(1 << 62), 1 << 62, n, dtype=np.int64)
+    dk = arrays.host(ks)
+    got = call(hyd, prog, H.site("reduce_i64", 0), dict(acc=5, ks=dk, n=n))
+    with np.errstate(over="ignore"):
+        sq = (ks.astype(np.uint64) * ks.astype(np.uint64)).sum(dtype=np.uint64) + np.uint64(5)
+    assert got == int(np.int64(sq.astype(np.int64)))
+    got = call(hyd, prog, H.site("reduce_i64", 1), dict(prod=1, ks=dk, n=n))
+    f = (np.fmod(ks, 5) + 3).astype(np.uint64)
+    p = np.uint64(1)
+    with np.errstate(over="ignore"):
+        for x in f:  # exact mod 2^64
+            p = p * x
+    assert got == int(np.array([p]).astype(np.int64)[0])
+    xs = np.random.default_rng(n + 1).uniform(-1, 1, n)
+    got = call(hyd, prog, H.site("reduce_i64", 2), dict(n=n, pf=1.0, xs=arrays.host(xs)))
+    ref = float(np.prod(np.longdouble(1.0) + np.longdouble(xs * 1.0e-3))) if n else 1.0
+    assert abs(got - ref) <= REL_TOL_REDUCTION * abs(ref)
+
+
+def run_jacobi(hyd, prog, arrays, u_arr, out_arr, shape):
+    h, w = shape
+    env = dict(u=u_arr, out=out_arr, shp=(h, w), n0=h, n1=w, n0m1=h - 1, n1m1=w - 1, bd_start_r=h - 1,
+               bd_start_c=w - 1)
+    for s in H.SITES["jacobi_step"]:
+        call(hyd, prog, s, env)
+
+
+@pytest.mark.parametrize("shape", [(2, 2), (3, 5), (64, 130), (257, 1000), (1000, 63)])
+def test_jacobi_regions_one_step(hyd, orc, prog, arrays, shape):
+    u = orc.splitmix_f64(shape[0] * shape[1], 931 + shape[1]).reshape(shape)
+    out = arrays.device(shape)
+    run_jacobi(hyd, prog, arrays, arrays.host(u), out, shape)
+    assert np.array_equal(bits(arrays.read(out, np.float64)), bits(orc.stencil2d(u, 0, 0.1)))
+
+
+def test_jacobi_regions_iterate(hyd, orc, prog, arrays):
+    # the iterate's ping-pong on the host around device-resident buffers
+    shape, steps = (700, 513), 7
+    u = orc.splitmix_f64(shape[0] * shape[1], 933).reshape(shape)
+    a, b = arrays.host(u), arrays.device(shape)
+    for _ in range(steps):
+        run_jacobi(hyd, prog, arrays, a, b, shape)
+        a, b = b, a
+    assert np.array_equal(bits(arrays.read(a, np.float64)), bits(orc.iterate_stencil2d(u, steps, 0, 0.1)))
+
+
+def test_jacobi_regions_config_grid(hyd, orc, prog, arrays):
+    c = workloads.JACOBI
+    h, w = c["h"], c["w"]
+    u = orc.splitmix_f64(h * w, c["seed"]).reshape(h, w)
+    out = arrays.device((h, w))
+    print("generated jacobi step %dx%d" % (h, w), flush=True)
+    run_jacobi(hyd, prog, arrays, arrays.host(u), out, (h, w))
+    assert np.array_equal(bits(arrays.read(out, np.float64)), bits(orc.stencil2d(u, 0, 0.1)))
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (2, 3), (5, 4), (33, 65), (512, 300)])
+def test_wrap_stencil(hyd, prog, arrays, shape):
+    u = np.random.default_rng(shape[1]).uniform(0, 1, shape)
+    out = arrays.device(shape)
+    call(hyd, prog, H.site("wrap_stencil"), dict(n=u.size, out=out, shp=shape, u=arrays.host(u)))
+    assert np.array_equal(bits(arrays.read(out, np.float64)), bits(H.wrap_numpy(u)))
+
+
+@pytest.mark.parametrize("n", [3, 4, 1025, 100_001])
+def test_mirror_1d(hyd, prog, arrays, n):
+    v = np.random.default_rng(n).uniform(-1, 1, n)
+    out = arrays.device((n,))
+    call(hyd, prog, H.site("mirror_1d"), dict(n=n, out=out, v=arrays.host(v)))
+    assert np.array_equal(bits(arrays.read(out, np.float64)), bits(H.mirror_numpy(v)))
+
+
+@pytest.mark.parametrize("n", [0, 1, 5000, 1_000_003])
+@pytest.mark.parametrize("nb", [1, 7, 64, 4096, 8192, 65536])
+def test_histogram(hyd, orc, prog, arrays, n, nb):
+    data = orc.splitmix_i64(n, 941 + n + nb, bits=31)
+    if n > 10:
+        data[::13] = -data[::13]  # negative remainders: routes outside the bins, dropped
+    base = np.random.default_rng(nb).integers(-1000, 1000, nb, dtype=np.int64)
+    hist = arrays.host(base)  # scatter accumulates into the defaults
+    call(hyd, prog, H.site("histogram"), dict(data=arrays.host(data), hist=hist, n=n, nb=nb))
+    # C remainders: a negative x lands in no bin unless x % nb == 0
+    with np.errstate(over="ignore"):
+        ref = (base.astype(np.uint64) + orc.weighted_histogram(data, nb).astype(np.uint64)).astype(np.int64)
+    assert np.array_equal(arrays.read(hist, np.int64), ref)
+
+
+def test_histogram_full_size(hyd, orc, prog, arrays):
+    c = workloads.HIST
+    data = orc.splitmix_i64(c["n"], c["seed"], bits=c["bits"])
+    hist = arrays.device((c["bins"],))
+    call(hyd, prog, H.site("histogram"), dict(data=arrays.host(data), hist=hist, n=c["n"], nb=c["bins"]))
+    assert np.array_equal(arrays.read(hist, np.int64), orc.weighted_histogram(data, c["bins"]))
+
+
+@pytest.mark.parametrize("n,nvox", [(1, 5), (20_000, 997), (1_000_000, 4096), (300_000, 300_000)])
+def test_splat_f64_order_exact(hyd, orc, prog, arrays, n, nvox):
+    idx, val, keep = H.splat_inputs(n, nvox)
+    base = np.random.default_rng(nvox).uniform(0, 1, nvox)
+    dst = arrays.host(base)
+    call(hyd, prog, H.site("splat_f64"), dict(dst=dst, n=n, nvox=nvox))
+    ref = orc.scatter_guarded_add(idx, val, keep, base)
+    assert np.array_equal(bits(arrays.read(dst, np.float64)), bits(ref))
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (37, 53), (4096, 300)])
+def test_row_sums_serial_order(hyd, prog, arrays, shape):
+    m = np.random.default_rng(shape[0]).uniform(-1, 1, shape)
+    out = arrays.device((shape[0],))
+    call(hyd, prog, H.site("row_sums"), dict(m=arrays.host(m), out=out, out_n=shape[0], red_dim=shape[1]))
+    # each row folded left to right from 0.0 by one thread, as the C loop does
+    ref = np.cumsum(np.concatenate([np.zeros((shape[0], 1)), m], axis=1), axis=1)[:, -1]
+    assert np.array_equal(bits(arrays.read(out, np.float64)), bits(ref))
+
+
+def test_gather_and_map3d(hyd, prog, arrays):
+    n, m = 1_000_003, 70_001
+    src = np.random.default_rng(1).integers(-(1 << 50), 1 << 50, m, dtype=np.int64)
+    perm = np.random.default_rng(2).integers(0, m, n, dtype=np.int64)
+    out = arrays.device((n,))
+    call(hyd, prog, H.site("gather"), dict(n=n, out=out, perm=arrays.host(perm), src=arrays.host(src)))
+    assert np.array_equal(arrays.read(out, np.int64), src[perm])
+    s3 = np.random.default_rng(3).uniform(-1, 1, (17, 33, 65))
+    out3 = arrays.device(s3.shape)
+    call(hyd, prog, H.site("map3d"), dict(n0=17, n1=33, n2=65, out=out3, shp=s3.shape, src=arrays.host(s3)))
+    assert np.array_equal(bits(arrays.read(out3, np.float64)), bits(H.map3d_numpy(s3)))
+
+
+def test_chained_sites_stay_in_hbm(hyd, orc, prog, arrays):
+    # map chain -> dot over its output: the second launcher reads the first's
+    # result in place (no host copy in between), the host copy refreshed only
+    # when asked
+    n = 200_003
+    xs = orc.splitmix_f64(n, 951, symmetric=True)
+    ys = orc.splitmix_f64(n, 952, symmetric=True)
+    out = arrays.device((n,))
+    call(hyd, prog, H.site("map_chain"), dict(n=n, out=out, xs=arrays.host(xs), ys=arrays.host(ys)))
+    L = hyd.lib()
+    assert L.hyd_array_is_device_resident(out) == 1
+    got = call(hyd, prog, H.site("dot"), dict(acc=0.0, red_dim=n, xs=out, ys=out))
+    m = orc.map_chain(xs, ys)
+    exact = orc.dot_dd(m, m)
+    assert abs(got - exact) <= REL_TOL_REDUCTION * abs(exact)
+    assert L.hyd_array_is_device_resident(out) == 1
+    assert np.array_equal(bits(arrays.read(out, np.float64)), bits(m))

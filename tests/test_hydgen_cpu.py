@@ -1,0 +1,258 @@
+"""CPU tests of the kernel generator (hydgen) and of its oracle.
+
+* the committed site files equal what tests/golden/make_cfg_sites.py prints;
+* the schedule pass picks the expected form for every example program, and
+  refuses (exit status 1, a message) loops the device cannot run as the
+  emitted C does;
+* the serial CFG interpreter (oracle/cfg_interp.py), the oracle of the
+  generated kernels, is pinned program by program against independent
+  restatements: the C oracle (map chain, stencil, histogram, serial guarded
+  scatter), numpy with the C semantics spelled out, exact integer arithmetic,
+  and a correctly rounded sum;
+* a planted mistake in a site (a clamp folded to the wrong edge) fails.
+"""
+import copy
+import math
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+HYDGEN = os.path.join(ROOT, "hydrangea-compiler_amd", "bin", "hydgen")
+CFG_DIR = os.path.join(HERE, "golden", "cfg")
+sys.path.insert(0, os.path.join(HERE, "golden"))
+
+import cfg_harness as H  # noqa: E402
+import make_cfg_sites  # noqa: E402
+from cfgshow import C, R, show  # noqa: E402
+
+
+def hydgen(*args, text=None):
+    return subprocess.run([HYDGEN] + list(args), input=text, capture_output=True, text=True)
+
+
+@pytest.mark.parametrize("name", sorted(H.SITES))
+def test_committed_sites_match_generator(name):
+    with open(os.path.join(CFG_DIR, name + ".sites")) as f:
+        assert f.read() == make_cfg_sites.render(name)
+
+
+EXPECTED_FORM = {
+    "map_chain": "form=map1d", "map_chain2": "form=map1d", "dot": "form=reduce", "reduce_i64": "form=reduce",
+    "jacobi_step": "form=map2d", "wrap_stencil": "form=map1d", "mirror_1d": "form=map1d",
+    "histogram": "form=scatter_i64", "splat_f64": "form=scatter_f64", "row_sums": "form=map1d",
+    "gather": "form=map1d", "map3d": "form=mapnd",
+}
+
+
+@pytest.mark.parametrize("name", sorted(H.SITES))
+def test_schedule_forms(name):
+    r = hydgen(os.path.join(CFG_DIR, name + ".sites"), "--plan")
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().splitlines()
+    assert len(lines) == len(H.SITES[name])
+    for line in lines:
+        assert EXPECTED_FORM[name] in line, line
+
+
+def test_schedule_details():
+    plan = {l.split(":")[0]: l for l in hydgen(os.path.join(CFG_DIR, "map_chain.sites"), "--plan").stdout.splitlines()}
+    p = plan["hydk_map_chain_0"]
+    assert "deferred=1" in p and "exp_lds=1" in p and "xs:r(stream)" in p and "out:w(stream)" in p
+    p = hydgen(os.path.join(CFG_DIR, "gather.sites"), "--plan").stdout
+    assert "perm:r(stream)" in p and "src:r" in p and "src:r(stream)" not in p  # gathered: cached loads
+    p = hydgen(os.path.join(CFG_DIR, "dot.sites"), "--plan").stdout
+    assert "wrapper=1" in p and "op=add acc=acc" in p
+
+
+def test_generated_header_compiles_as_c99(tmp_path):
+    sites = os.path.join(CFG_DIR, "jacobi_step.sites")
+    hdr = tmp_path / "sites.h"
+    assert hydgen(sites, "--header", str(hdr)).returncode == 0
+    c = tmp_path / "use.c"
+    c.write_text('#include "sites.h"\nint main(void) { return 0; }\n')
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"), "-I" + str(tmp_path),
+                        "-c", str(c), "-o", str(tmp_path / "use.o")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    text = hdr.read_text()
+    assert "int64_t hydk_jacobi_step_interior_interior(int64_t v_n0, int64_t v_n1, hyd_array_t* v_out," in text
+
+
+def test_signatures():
+    r = hydgen(os.path.join(CFG_DIR, "dot.sites"), "--signatures", "/dev/stdout")
+    assert r.stdout.strip() == "hydk_dot_0 f64 : f64 i64 array array"
+
+
+# ---- refusals --------------------------------------------------------------------
+
+def _site_with(body, args, exec_=None, result=None):
+    lp = C("SLoop", R("LoopSpec", [("lsIters", ["i"]), ("lsBounds", [C("IVar", "n")]),
+                                   ("lsExec", exec_ or H.cfg_programs.par()), ("lsRed", C("Nothing")),
+                                   ("lsRole", C("LoopMap")), ("lsOrigins", [])]), body)
+    return show(H.cfg_programs.site("hydk_bad", args, lp, result)) + "\n"
+
+
+P = H.cfg_programs
+ARGS = [("n", P.T_I), ("out", P.arr(P.T_F)), ("xs", P.arr(P.T_F))]
+
+
+@pytest.mark.parametrize("body,exec_,why", [
+    ([P.assign("x", P.load("xs", "i")), P.assign("y", P.uop("CLog", "x")), P.write("out", "i", "y")], None,
+     "glibc-exact"),
+    ([P.assign("x", P.load("xs", "i")), P.assign("y", C("RCall", "foo", [C("AVar", "x")])), P.write("out", "i", "y")],
+     None, "not supported"),
+    ([P.assign("x", P.load("xs", "i")), C("SBreak"), P.write("out", "i", "x")], None, "break"),
+    ([P.assign("x", P.load("xs", "i")), P.assign("old", P.load("out", "i")), P.assign("new", P.bop("CMulF", "x", "old")),
+      P.write("out", "i", "new")], P.par("ParallelScatterAtomicAddFloat"), "combine other than"),
+    ([P.assign("x", P.load("xs", "i")), P.write("out", "i", "z")], None, "neither an argument nor local"),
+])
+def test_refused_loops(tmp_path, body, exec_, why):
+    f = tmp_path / "bad.sites"
+    f.write_text(_site_with(body, ARGS, exec_))
+    r = hydgen(str(f), "-o", str(tmp_path / "bad.hip"))
+    assert r.returncode == 1 and why in r.stderr, r.stderr
+
+
+@pytest.mark.parametrize("text,why", [
+    ("HipSite {hsName = \"x\"", "expected"),
+    ("Foo 1 2", "expected a HipSite"),
+    ("HipSite {hsName = \"x\", hsArgs = [], hsResult = Nothing, hsTypes = fromList [], hsLoop = SBreak}",
+     "hsLoop must be an SLoop"),
+])
+def test_malformed_sites(tmp_path, text, why):
+    f = tmp_path / "bad.sites"
+    f.write_text(text + "\n")
+    r = hydgen(str(f), "--plan")
+    assert r.returncode == 1 and why in r.stderr, r.stderr
+
+
+# ---- the oracle pinned ----------------------------------------------------------------
+
+def test_interp_map_chain_equals_c_oracle(orc):
+    n = 3000
+    xs = orc.splitmix_f64(n, 11, symmetric=True)
+    ys = orc.splitmix_f64(n, 12, symmetric=True)
+    xs[:4] = [0.0, -0.0, 1e308, -1e-300]
+    ys[:4] = [-0.0, 0.0, 1e308, 1.0]
+    env = dict(n=n, xs=xs, ys=ys, out=np.zeros(n))
+    H.run_interp(H.site("map_chain"), env)
+    ref = orc.map_chain(xs, ys)
+    assert np.array_equal(env["out"].view(np.uint64), ref.view(np.uint64))
+
+
+def test_interp_map_chain2_equals_numpy():
+    n = 4000
+    xs, ks = H.map_chain2_inputs(n, 5)
+    env = dict(n=n, xs=xs, ks=ks, of=np.zeros(n), oi=np.zeros(n, np.int64), scale=7.25, m=11)
+    H.run_interp(H.site("map_chain2"), env)
+    u, g2 = H.map_chain2_numpy(xs, ks, 7.25, 11)
+    assert np.array_equal(env["oi"], g2)
+    assert np.array_equal(env["of"].view(np.uint64), u.view(np.uint64))
+
+
+def test_interp_dot_within_bound():
+    rng = np.random.default_rng(3)
+    xs, ys = rng.uniform(-1, 1, 5000), rng.uniform(-1, 1, 5000)
+    env = dict(acc=0.25, red_dim=5000, xs=xs, ys=ys)
+    got = H.run_interp(H.site("dot"), env)
+    exact = math.fsum([0.25] + [float(a) * float(b) for a, b in zip(xs, ys)])  # products exact-rounded, then exact sum
+    assert abs(got - exact) <= 1e-12 * abs(exact)
+    # the serial order is the C loop's: a left fold
+    s = 0.25
+    for a, b in zip(xs, ys):
+        s = s + a * b
+    assert got == s
+
+
+def test_interp_integer_reductions_exact():
+    ks = np.random.default_rng(4).integers(-(1 << 62), 1 << 62, 777, dtype=np.int64)
+    sq = H.run_interp(H.site("reduce_i64", 0), dict(acc=5, ks=ks, n=777))
+    assert sq == H.cfg_interp.wrap(5 + sum(int(x) * int(x) for x in ks))
+    pr = H.run_interp(H.site("reduce_i64", 1), dict(prod=1, ks=ks, n=777))
+    p = 1
+    for x in ks:
+        r = (abs(int(x)) % 5) * (1 if x >= 0 else -1)  # C %: the sign of the dividend
+        p = (p * (r + 3)) % (1 << 64)
+    assert pr == H.cfg_interp.wrap(p)
+
+
+@pytest.mark.parametrize("shape", [(2, 2), (3, 5), (17, 9), (64, 130)])
+def test_interp_jacobi_regions_equal_c_oracle(orc, shape):
+    u = orc.splitmix_f64(shape[0] * shape[1], 21 + shape[1]).reshape(shape)
+    got = H.jacobi_interp(u)
+    assert np.array_equal(got.view(np.uint64), orc.stencil2d(u, 0, 0.1).view(np.uint64))
+
+
+def test_planted_clamp_mistake_fails(orc):
+    # the right-column segment clamping its east neighbour to column 0 instead of n1 - 1
+    u = orc.splitmix_f64(9 * 7, 31).reshape(9, 7)
+    sites = copy.deepcopy(H.SITES["jacobi_step"])
+    bad = [s for s in sites if s.get("hsName").endswith("interior_right")][0]
+    body = bad.get("hsLoop").args[1]
+    for k, st in enumerate(body):
+        if st.name == "SAssign" and st.args[0] == "idx_a_e":
+            body[k] = C("SAssign", "idx_a_e", C("RTuple", [st.args[1].args[0][0], C("AInt", 0)]))
+    env = H.jacobi_env(u.copy())
+    for s in sites:
+        H.run_interp(s, env)
+    assert not np.array_equal(env["out"], orc.stencil2d(u, 0, 0.1))
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (2, 3), (5, 4), (33, 65)])
+def test_interp_wrap_stencil_equals_numpy(shape):
+    u = np.random.default_rng(shape[0]).uniform(0, 1, shape)
+    env = dict(n=u.size, out=np.zeros(u.size), shp=shape, u=u)
+    H.run_interp(H.site("wrap_stencil"), env)
+    assert np.array_equal(env["out"].reshape(shape).view(np.uint64), H.wrap_numpy(u).view(np.uint64))
+
+
+@pytest.mark.parametrize("n", [3, 4, 10, 257])
+def test_interp_mirror_equals_numpy(n):
+    v = np.random.default_rng(n).uniform(-1, 1, n)
+    env = dict(n=n, out=np.zeros(n), v=v)
+    H.run_interp(H.site("mirror_1d"), env)
+    assert np.array_equal(env["out"].view(np.uint64), H.mirror_numpy(v).view(np.uint64))
+
+
+def test_interp_histogram_equals_c_oracle(orc):
+    data = orc.splitmix_i64(5000, 41, bits=31)
+    env = dict(data=data, hist=np.zeros(64, np.int64), n=5000, nb=64)
+    H.run_interp(H.site("histogram"), env)
+    assert np.array_equal(env["hist"], orc.weighted_histogram(data, 64))
+
+
+def test_interp_splat_equals_serial_c(orc):
+    n, nvox = 20000, 997
+    idx, val, keep = H.splat_inputs(n, nvox)
+    base = np.random.default_rng(1).uniform(0, 1, nvox)
+    env = dict(dst=base.copy(), n=n, nvox=nvox)
+    H.run_interp(H.site("splat_f64"), env)
+    ref = orc.scatter_guarded_add(idx, val, keep, base)
+    assert np.array_equal(env["dst"].view(np.uint64), ref.view(np.uint64))
+
+
+def test_interp_row_sums_left_fold():
+    m = np.random.default_rng(2).uniform(-1, 1, (37, 53))
+    env = dict(m=m, out=np.zeros(37), out_n=37, red_dim=53)
+    H.run_interp(H.site("row_sums"), env)
+    for j in range(37):
+        s = 0.0
+        for x in m[j]:
+            s = s + x
+        assert env["out"][j] == s
+
+
+def test_interp_gather_and_map3d():
+    src = np.arange(100, dtype=np.int64) * 3 - 7
+    perm = np.random.default_rng(5).integers(0, 100, 300, dtype=np.int64)
+    env = dict(n=300, out=np.zeros(300, np.int64), perm=perm, src=src)
+    H.run_interp(H.site("gather"), env)
+    assert np.array_equal(env["out"], src[perm])
+    s3 = np.random.default_rng(6).uniform(-1, 1, (4, 5, 6))
+    env = dict(n0=4, n1=5, n2=6, out=np.zeros(s3.size), shp=s3.shape, src=s3)
+    H.run_interp(H.site("map3d"), env)
+    assert np.array_equal(env["out"].reshape(s3.shape).view(np.uint64), H.map3d_numpy(s3).view(np.uint64))
