@@ -106,6 +106,32 @@ def build_programs(verbose=False, jobs=None):
     return PROGRAMS_LIB
 
 
+HYDHIP_CC = os.path.join(HERE, "hydhip-cc")
+E2E_DIR = os.path.join(BUILD, "e2e")
+E2E_PROGRAM = os.path.join(E2E_DIR, "dotmap_program")
+
+
+def build_e2e(verbose=False):
+    """An end-to-end program: tests/golden/cfg/dotmap_program.c (C in the shapes
+    the C backend prints) with the map-chain and dot sites, built by hydhip-cc
+    as the HIP backend builds a program (host code for x86-64-v3, so that it
+    runs on any current x86-64 host)."""
+    os.makedirs(E2E_DIR, exist_ok=True)
+    sites = os.path.join(E2E_DIR, "dotmap.sites")
+    with open(sites, "w") as out:
+        for f in ("map_chain.sites", "dot.sites"):
+            with open(os.path.join(PROGRAM_SITES, f)) as fh:
+                out.write(fh.read())
+    env = dict(os.environ, HYD_HOST_MARCH="x86-64-v3")
+    p = subprocess.run([HYDHIP_CC, os.path.join(PROGRAM_SITES, "dotmap_program.c"), sites, "-o", E2E_PROGRAM,
+                        "--cc", "gcc"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, env=env)
+    if p.returncode != 0:
+        raise RuntimeError("hydhip-cc failed:\n" + p.stdout)
+    if verbose:
+        print("built", E2E_PROGRAM)
+    return E2E_PROGRAM
+
+
 def build(verbose=False, jobs=None):
     """Compile every HIP source and link lib/libhydhip.so.  Returns its path."""
     os.makedirs(BUILD, exist_ok=True)
@@ -132,6 +158,7 @@ def build(verbose=False, jobs=None):
         print("built", LIB)
     build_hydgen(verbose)
     build_programs(verbose, jobs)
+    build_e2e(verbose)
     return LIB
 
 

@@ -1,19 +1,27 @@
 -- |
 -- Module: HIPBackend
 --
--- Compiles and runs what Language.Hydrangea.CodegenHIP printed: the same
--- flow as the reference CLI's C backend (app/CBackend.hs:78-106, compiler
--- chosen by --cc, then $CC, then cc), with the program linked against
--- libhydhip instead of compiled with the runtime's C file and OpenMP.
--- HYD_HIP_HOME names the root of the MI355X backend (include/ and
--- hydrangea-compiler_amd/lib/libhydhip.so).
+-- The CLI's MI355X backend, beside app/CBackend.hs: it writes what
+-- Language.Hydrangea.CodegenHIP produced - the printed C and the offloaded
+-- loops, one HipSite per line - to a temporary directory and hands both to
+-- hydhip-cc ($HYD_HIP_HOME/hydrangea-compiler_amd/hydhip-cc), which runs the
+-- kernel generator (hydgen), compiles the kernels with
+-- `hipcc --offload-arch=gfx950 -O3 -ffp-contract=off`, compiles the C with
+-- the chosen compiler as the C backend does (-O3 -march=native -std=c99, and
+-- -fopenmp after the same probe, app/CBackend.hs:29-49, for the loops that
+-- stay on the host), and links libhydhip.  Then, as compileAndRunC does
+-- (app/CBackend.hs:78-106), it keeps the artifacts if asked and runs the
+-- program unless compile-only.  The compiler is chosen in the C backend's
+-- order: --cc, then $CC, then cc.
 --
--- UNTESTED: GHC is not available where this module was written.  The
--- compile line it builds is the one the repository's tests use for their
--- gcc-built hosts.
-module HIPBackend (compileAndRunHIP, hipCompileArgs) where
+-- UNTESTED here: GHC is not available where this module was written.  The
+-- command it runs is tested: tests/test_hydhip_cc.py builds programs with
+-- exactly these arguments, and tests/test_gpu_hydhip_cc.py runs them on the
+-- GPU against the oracle.
+module HIPBackend (compileAndRunHIP, hydhipCcArgs) where
 
 import Control.Monad (when)
+import Language.Hydrangea.CodegenHIP (HIPArtifacts (..), renderSites)
 import System.Directory (copyFile)
 import System.Environment (lookupEnv)
 import System.Exit (ExitCode (..))
@@ -22,19 +30,16 @@ import System.IO (hPutStrLn, stderr)
 import System.IO.Temp (withSystemTempDirectory)
 import System.Process (createProcess, proc, readProcessWithExitCode, waitForProcess)
 
--- | Compiler arguments for one generated C file.
-hipCompileArgs :: FilePath -> FilePath -> FilePath -> [String]
-hipCompileArgs home cFile exe =
-  [ "-std=c99", "-O3", "-I" ++ (home </> "include"), cFile, "-o", exe
-  , "-L" ++ libDir, "-lhydhip", "-Wl,-rpath," ++ libDir, "-lm"
-  ]
-  where
-    libDir = home </> "hydrangea-compiler_amd" </> "lib"
+-- | The arguments of hydhip-cc for one program.
+hydhipCcArgs :: Maybe String -> Bool -> FilePath -> FilePath -> FilePath -> [String]
+hydhipCcArgs mcc parallel cFile sitesFile exe =
+  [cFile, sitesFile, "-o", exe] ++ maybe [] (\cc -> ["--cc", cc]) mcc ++ ["--no-openmp" | not parallel]
 
--- | Writes the source, compiles it, runs it; keepC copies the source to
--- the working directory as the C backend's --keep-c does.
-compileAndRunHIP :: Maybe String -> Bool -> String -> IO ExitCode
-compileAndRunHIP mcc keepC src = do
+-- | Compile the program and (unless compileOnly) run it.  keepC copies the
+-- C, the sites and the executable to the working directory as
+-- hydrangea_out.c, hydrangea_out.sites and hydrangea_out.
+compileAndRunHIP :: Maybe String -> HIPArtifacts -> Bool -> Bool -> Bool -> IO ExitCode
+compileAndRunHIP mcc arts keepC compileOnly parallel = do
   mhome <- lookupEnv "HYD_HIP_HOME"
   envCC <- lookupEnv "CC"
   case mhome of
@@ -42,17 +47,27 @@ compileAndRunHIP mcc keepC src = do
       hPutStrLn stderr "HIPBackend: set HYD_HIP_HOME to the MI355X backend's root"
       pure (ExitFailure 1)
     Just home -> withSystemTempDirectory "hydrangea-hip" $ \dir -> do
-      let cFile = dir </> "program.c"
-          exe = dir </> "program"
-          cc = pick mcc (pick envCC "cc")
-      writeFile cFile src
-      when keepC (copyFile cFile "program.c")
-      (code, _, err) <- readProcessWithExitCode cc (hipCompileArgs home cFile exe) ""
+      let cFile = dir </> "out.c"
+          sitesFile = dir </> "out.sites"
+          exe = dir </> "hydrangea_out"
+          driver = home </> "hydrangea-compiler_amd" </> "hydhip-cc"
+          cc = pick mcc envCC
+      writeFile cFile (hipC arts)
+      writeFile sitesFile (renderSites (hipSites arts))
+      (code, _, err) <- readProcessWithExitCode driver (hydhipCcArgs cc parallel cFile sitesFile exe) ""
       case code of
-        ExitFailure _ -> hPutStrLn stderr err >> pure code
+        ExitFailure _ -> putStrLn "HIP compilation failed:" >> putStrLn err >> pure code
         ExitSuccess -> do
-          (_, _, _, ph) <- createProcess (proc exe [])
-          waitForProcess ph
+          when keepC $ do
+            copyFile cFile "hydrangea_out.c"
+            copyFile sitesFile "hydrangea_out.sites"
+            copyFile exe "hydrangea_out"
+          if compileOnly
+            then pure ExitSuccess
+            else do
+              (_, _, _, ph) <- createProcess (proc exe [])
+              waitForProcess ph
   where
-    pick (Just s) _ | not (null s) = s
-    pick _ d = d
+    pick (Just s) _ | not (null s) = Just s
+    pick _ (Just s) | not (null s) = Just s
+    pick _ _ = Nothing

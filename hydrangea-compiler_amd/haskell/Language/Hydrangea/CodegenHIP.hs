@@ -2,175 +2,393 @@
 -- |
 -- Module: Language.Hydrangea.CodegenHIP
 --
--- The HIP offload for the C backend.  A CFG-to-CFG pass replaces each loop of
--- one of three shapes with a single runtime call into libhydhip (MI355X
--- kernels).  The C backend then prints the program as usual: it prints an
--- 'RCall' as a plain C call (src/Language/Hydrangea/CodegenC.hs:2340-2344),
--- and include/hyd_hip_glue.h defines the called helpers.  Every other loop
--- stays the C backend's OpenMP loop, so the result is always a complete
--- program.
+-- The MI355X backend beside CodegenC and CodegenMSL.  After the Metal-style
+-- pipeline (optimize + parallelize without SIMD vectors, CFGPipeline.hs:
+-- 224-226) an offload pass picks, in every procedure, the outermost loops
+-- whose iterations run independently and whose bodies the device can run
+-- exactly as the printed C does; each becomes one call of a generated
+-- launcher (an 'RCall', printed by CodegenC.hs:2340-2344 as a plain C call).
+-- The loops themselves go out as 'HipSite' records, one `show` per line: the
+-- kernel generator (hydrangea-compiler_amd/hydgen) reads that text, runs the
+-- GPU-schedule pass (grid, workgroup, elements per thread, reduction tree,
+-- scatter merge) and prints the HIP kernels and launchers.  Every other
+-- statement stays CodegenC's host C.
 --
--- Shapes (single iterator from 0, bound a variable or a constant):
+-- Offloaded (the kernel loop of a site):
 --
--- * scatter-add, the loop CodegenC.hs:1576-1611 detects, unguarded, whose
---   index and value are loads at the iterator.  The parallelizer tagged it
---   'ParallelScatterAtomicAddInt' / 'ParallelScatterPrivatizedIntAdd'
---   (int64, 'CAdd') or the Float variants (fp64, 'CAddF'), CFG.hs:184-191.
---   It becomes @_discarded = hydg_scatter_add_{i64,f64}(dst, idx, vals, n)@.
--- * the fused weighted histogram, the same scatter loop (int64) whose body
---   loads @x@ at the iterator, routes it by @x % nbins@ ('CMod') and adds
---   @x * x@ ('CMul').  It becomes
---   @_discarded = hydg_weighted_histogram_i64(dst, data, nbins, n)@.
--- * dot: a reduction loop ('lsRed' with 'RAdd') whose body loads @x@ and @y@
---   at the iterator and adds @x *. y@ to the accumulator.  It becomes
---   @acc = hydg_dot_f64(xs, ys, n, acc)@.
+-- * a loop with a ReductionSpec (CFG.hs:207-212), or a LoopReductionWrapper
+--   of trip count 1 around one (Lowering.hs:1150-1182): the site returns the
+--   accumulator, which must be assigned before the loop;
+-- * a loop the parallelizer marked 'Parallel' (CFG.hs:239-244) or whose role
+--   is 'LoopMap' / 'LoopMapReduction' (independent iterations by
+--   construction); a combining scatter strategy only with the (+) / (+.)
+--   combine of Lowering.hs:3553-3557;
+-- and only when every statement of its body is one the device runs as C
+-- does (three-address arithmetic, loads and stores, ifs, inner loops; no
+-- log/sin/cos/erf, whose glibc results a device has no exact form of; no
+-- calls, allocation, records or pairs), when no variable it assigns is read
+-- after it (the accumulator aside), and when it is not inside a parallel host
+-- loop or region.
 --
--- UNTESTED: GHC is not available where this module was written.  It follows
--- the interfaces of CFG.hs (LoopSpec :251-259, ExecPolicy :239-244,
--- ParallelSpec :199-204, Stmt :262-270, rewriteStmtsWith :278-288),
--- CFGCore.hs (RHS :140-169, BinOp :33-74, Atom :77-85, Redop :236),
--- CFGPipeline.hs (metalPipelineWithTiling :224-226) and CodegenC.hs
--- (codegenProgramWithOptions :161-162, CodegenArtifacts :52-56).  The C side
--- of every call it emits is tested from a gcc-built host.
+-- Residency: arrays stay in HBM between sites.  The pass brings an array's
+-- host copy up to date (hyd_array_sync_host) before a host statement reads or
+-- writes its elements and before it is returned, marks host writes
+-- (hyd_array_mark_host_dirty), marks array parameters on entry
+-- (hyd_array_enter_host: the caller may have rewritten them), and allocates
+-- the 8-byte-element arrays a site touches with hyd_array_alloc_device8, so
+-- that a kernel writing a fresh array uploads nothing.  The runtime calls are
+-- no-ops when nothing is out of date.
+--
+-- UNTESTED here: GHC is not available where this module was written.  It
+-- follows CFG.hs (Stmt :262-270, LoopSpec :251-259, rewriteStmtsWith
+-- :278-288), CFGCore.hs (RHS :140-169, Atom :77-85, CType :103-113),
+-- CFGTyping.hs (recoverProcTypeEnv :363, inferProgramReturnTypes :341,
+-- buildCallParamTypes :354) and CodegenC.hs (codegenProgramWithOptions :161,
+-- CodegenArtifacts :52-56).  Everything downstream of the site text -
+-- hydgen, the generated kernels, the compile driver - is tested against the
+-- C oracle on the GPU (tests/test_gpu_hydgen.py, tests/test_gpu_hydhip_cc.py).
 module Language.Hydrangea.CodegenHIP
   ( HipSite (..)
+  , HIPArtifacts (..)
   , hipPipeline
   , hipOffloadProgram
-  , hipOffloadSites
+  , offloadableLoop
   , codegenHIPWithOptions
+  , renderSites
   ) where
 
-import Control.Applicative ((<|>))
-import Control.Monad (guard)
-import Data.ByteString (ByteString)
+import Control.Monad.State.Strict (State, get, put, runState)
+import qualified Data.ByteString.Lazy.Char8 as BS
+import Data.List (nub, sort)
 import qualified Data.Map.Strict as M
+import Data.Maybe (isJust, mapMaybe)
+import qualified Data.Set as S
 import Language.Hydrangea.CFG
 import Language.Hydrangea.CFGCore
 import Language.Hydrangea.CFGPipeline (metalPipelineWithTiling)
+import Language.Hydrangea.CFGTyping (TypeEnv, buildCallParamTypes, inferProgramReturnTypes, recoverProcTypeEnv)
 import Language.Hydrangea.CodegenC (CodegenArtifacts (..), CodegenOptions, codegenProgramWithOptions)
 
--- | One offloaded loop: the helper it calls and the arrays it passes.
+-- | One offloaded loop, as hydgen reads it (hydgen/cfg.cpp).
 data HipSite = HipSite
-  { hsHelper :: ByteString
-  , hsArrays :: [CVar]
+  { hsName :: CVar
+  , hsArgs :: [(CVar, CType)]
+  , hsResult :: Maybe (CVar, CType)
+  , hsTypes :: M.Map CVar CType
+  , hsLoop :: Stmt
   }
   deriving (Eq, Show)
 
--- | Optimize and parallelize without SIMD vectorization (the GPU supplies
--- the lanes, and vector loops would hide the shapes), then offload.
+data HIPArtifacts = HIPArtifacts
+  { hipC :: String         -- ^ the printed C, launchers called by name
+  , hipSites :: [HipSite]  -- ^ the offloaded loops
+  , hipHeader :: Maybe String
+  }
+
+-- | Optimize and parallelize without SIMD vectors: the GPU supplies the
+-- lanes, and vector loops would hide the loop shapes.
 hipPipeline :: Program -> Program
-hipPipeline = hipOffloadProgram . metalPipelineWithTiling False
+hipPipeline = metalPipelineWithTiling False
 
-hipOffloadProgram :: Program -> Program
-hipOffloadProgram (Program procs) =
-  Program [p {procBody = rewriteStmtsWith () id (const rewrite) (procBody p)} | p <- procs]
-  where
-    rewrite s = maybe [s] snd (offload s)
+renderSites :: [HipSite] -> String
+renderSites = unlines . map show
 
--- | The loops 'hipOffloadProgram' replaces, in program order.
-hipOffloadSites :: Program -> [HipSite]
-hipOffloadSites (Program procs) = concatMap (goStmts . procBody) procs
-  where
-    goStmts = concatMap goStmt
-    goStmt s = case offload s of
-      Just (site, _) -> [site]
-      Nothing -> case s of
-        SLoop _ body -> goStmts body
-        SParallelRegion body -> goStmts body
-        SIf _ t e -> goStmts t ++ goStmts e
-        _ -> []
-
--- | The C backend's output for the offloaded program, with the helpers'
--- header included.
-codegenHIPWithOptions :: CodegenOptions -> Program -> Either String CodegenArtifacts
+-- | The C backend's output for the offloaded program; the printed C includes
+-- the residency runtime, and the launchers' declarations come from hydgen
+-- (--header), force-included by the compile driver.
+codegenHIPWithOptions :: CodegenOptions -> Program -> Either String HIPArtifacts
 codegenHIPWithOptions opts prog = do
-  arts <- codegenProgramWithOptions opts (hipOffloadProgram prog)
-  pure arts {codegenSource = "#include \"hyd_hip_glue.h\"\n" ++ codegenSource arts}
+  let (prog', sites) = hipOffloadProgram prog
+  arts <- codegenProgramWithOptions opts prog'
+  pure HIPArtifacts
+    { hipC = "#include \"hyd_hip_runtime.h\"\n" ++ codegenSource arts
+    , hipSites = sites
+    , hipHeader = codegenHeader arts
+    }
 
-offload :: Stmt -> Maybe (HipSite, [Stmt])
-offload (SLoop spec body) = do
-  [i] <- Just (lsIters spec)
-  [bound] <- Just (lsBounds spec)
-  guard (all (== IConst 0) (lsOrigins spec))
-  n <- boundAtom bound
-  scatterAdd spec i n body <|> weightedHistogram spec i n body <|> dot spec i n body
-offload _ = Nothing
+-- ---------------------------------------------------------------------------
+-- The pass
 
-boundAtom :: IndexExpr -> Maybe Atom
-boundAtom (IVar v) = Just (AVar v)
-boundAtom (IConst k) = Just (AInt k)
-boundAtom _ = Nothing
+-- | Once any loop of the program is offloaded, every procedure gets the
+-- residency calls: a procedure without sites may still read or write, on
+-- the host, arrays that another procedure's kernels left in HBM.
+hipOffloadProgram :: Program -> (Program, [HipSite])
+hipOffloadProgram prog@(Program procs) =
+  let callTypes = inferProgramReturnTypes prog
+      callParams = buildCallParamTypes callTypes procs
+      results = map (offloadProc callTypes callParams) procs
+      sites = concatMap snd results
+   in if null sites then (prog, []) else (Program (map fst results), sites)
 
--- | Variables bound to a load at the iterator, and the array each reads.
-loadsAt :: CVar -> [Stmt] -> Maybe (M.Map CVar CVar)
-loadsAt i = fmap M.fromList . mapM one
+discarded :: CVar
+discarded = "_discarded"
+
+callStmt :: CVar -> [Atom] -> Stmt
+callStmt fn args = SAssign discarded (RCall fn args)
+
+offloadProc :: M.Map CVar CType -> M.Map CVar [Maybe CType] -> Proc -> (Proc, [HipSite])
+offloadProc callTypes callParams p =
+  let env = recoverProcTypeEnv callTypes callParams p
+      isArr v = case M.lookup v env of
+        Just (CTArray _) -> True
+        _ -> False
+      body = procBody p
+      (body', (_, sites)) = runState (goBlock env isArr (procName p) body body) (0 :: Int, [])
+      siteArrays = S.fromList [v | s <- sites, (v, CTArray _) <- hsArgs s]
+      entry = [callStmt "hyd_array_enter_host" [AVar v] | v <- procParams p, isArr v]
+      body'' = entry ++ map (deviceAllocs env siteArrays) body'
+      env' = M.insert discarded CTInt64 (procTypeEnv p)
+   in (p {procBody = body'', procTypeEnv = env'}, reverse sites)
+
+-- | Fresh 8-byte-element arrays that a site touches are allocated in HBM.
+deviceAllocs :: TypeEnv -> S.Set CVar -> Stmt -> Stmt
+deviceAllocs env siteArrays = go
   where
-    one (SAssign v (RArrayLoad (AVar arr) (AVar j))) | j == i = Just (v, arr)
-    one _ = Nothing
+    go s = case s of
+      SAssign v (RArrayAlloc shp)
+        | v `S.member` siteArrays, Just (CTArray t) <- M.lookup v env, t `elem` [CTInt64, CTDouble, CTBool] ->
+            SAssign v (RCall "hyd_array_alloc_device8" [shp])
+      SLoop spec b -> SLoop spec (map go b)
+      SIf c t e -> SIf c (map go t) (map go e)
+      SParallelRegion b -> SParallelRegion (map go b)
+      _ -> s
 
-scatterAdd :: LoopSpec -> CVar -> Atom -> [Stmt] -> Maybe (HipSite, [Stmt])
-scatterAdd spec i n body = do
-  Parallel ps <- Just (lsExec spec)
-  (op, helper) <- strategyHelper (psStrategy ps)
-  [l1, l2, SAssign old (RArrayLoad (AVar dst) (AVar ix)), SAssign new (RBinOp op' a b), SArrayWrite (AVar dst') (AVar ix') (AVar new')] <-
-    Just body
-  loads <- loadsAt i [l1, l2]
-  guard (op' == op && dst' == dst && ix' == ix && new' == new)
-  v <- case (a, b) of
-    (AVar x, AVar y) | x == old -> Just y
-                     | y == old -> Just x
-    _ -> Nothing
-  idxArr <- M.lookup ix loads
-  valArr <- M.lookup v loads
-  guard (ix /= v && dst /= idxArr && dst /= valArr)
-  let call = RCall helper [AVar dst, AVar idxArr, AVar valArr, n]
-  pure (HipSite helper [dst, idxArr, valArr], [SAssign "_discarded" call])
+type Pass = State (Int, [HipSite])
+
+-- | Statements of one block of procedure `pname`.  `whole` is the
+-- procedure body (for the live-out check).
+goBlock :: TypeEnv -> (CVar -> Bool) -> CVar -> [Stmt] -> [Stmt] -> Pass [Stmt]
+goBlock env isArr pname whole = fmap concat . mapM one
   where
-    strategyHelper s = case s of
-      ParallelScatterAtomicAddInt -> Just (CAdd, "hydg_scatter_add_i64")
-      ParallelScatterPrivatizedIntAdd -> Just (CAdd, "hydg_scatter_add_i64")
-      ParallelScatterAtomicAddFloat -> Just (CAddF, "hydg_scatter_add_f64")
-      ParallelScatterPrivatizedFloatAdd -> Just (CAddF, "hydg_scatter_add_f64")
+    one s = case offloadableLoop env whole s of
+      Just mk -> do
+        (k, sites) <- get
+        let site = mk (siteName k)
+        put (k + 1, site : sites)
+        pure [siteCall site]
+      Nothing -> case s of
+        -- host loops and branches that hold sites: recurse (the iterate
+        -- loop around its step is the common case); a parallel host loop or
+        -- region is never entered, so no launch runs on several threads
+        SLoop spec b | lsExec spec == Serial, containsSite b -> do
+          b' <- goBlock env isArr pname whole b
+          pure [SLoop spec b']
+        SIf c t e | containsSite t || containsSite e -> do
+          t' <- goBlock env isArr pname whole t
+          e' <- goBlock env isArr pname whole e
+          pure [SIf c t' e']
+        SReturn a -> pure (syncReturned a ++ [s])
+        _ -> pure (hostStatement isArr s)
+    containsSite = any (\t -> isJust (offloadableLoop env whole t) || nested t)
+    nested t = case t of
+      SLoop spec b | lsExec spec == Serial -> containsSite b
+      SIf _ a b -> containsSite a || containsSite b
+      _ -> False
+    siteName k = BS.pack "hydk_" <> sanitizeName pname <> BS.pack ("_" ++ show k)
+    syncReturned a = case a of
+      AVar v | isArr v -> [callStmt "hyd_array_sync_host" [AVar v]]
+      AVar v -> [callStmt "hyd_array_sync_host" [AVar f] | f <- recordArrays v]
+      _ -> []
+    recordArrays v = nub (concat [ [x | AVar x <- map snd fs, isArr x] | SAssign w (RRecord fs) <- flatten whole, w == v ]
+                          ++ concat [ [x | AVar x <- [a, b], isArr x] | SAssign w (RPairMake _ _ a b) <- flatten whole, w == v ])
+
+sanitizeName :: CVar -> CVar
+sanitizeName = BS.map (\c -> if c `elem` (['a' .. 'z'] ++ ['A' .. 'Z'] ++ ['0' .. '9'] ++ "_") then c else '_')
+
+-- | A host statement with the residency calls around it: host copies
+-- brought up to date before it touches array elements or hands arrays to a
+-- procedure of the program (the library's own hyd_* functions bring device
+-- copies back themselves), host writes marked after it.
+hostStatement :: (CVar -> Bool) -> Stmt -> [Stmt]
+hostStatement isArr s =
+  let touched = sort (nub [v | v <- elementAccesses s ++ procCallArrays s, isArr v])
+      written = sort (nub [v | v <- elementWrites s, isArr v])
+   in [callStmt "hyd_array_sync_host" [AVar v] | v <- touched]
+        ++ [s]
+        ++ [callStmt "hyd_array_mark_host_dirty" [AVar v] | v <- written]
+
+elementAccesses :: Stmt -> [CVar]
+elementAccesses s = [v | t <- flatten [s], v <- acc t]
+  where
+    acc t = case t of
+      SAssign _ (RArrayLoad (AVar a) _) -> [a]
+      SAssign _ (RVecLoad (AVar a) _) -> [a]
+      SAssign _ (RVecStore (AVar a) _ _) -> [a]
+      SArrayWrite (AVar a) _ _ -> [a]
+      _ -> []
+
+procCallArrays :: Stmt -> [CVar]
+procCallArrays s = [v | SAssign _ (RCall fn as) <- flatten [s], not ("hyd" `BS.isPrefixOf` fn), AVar v <- as]
+
+elementWrites :: Stmt -> [CVar]
+elementWrites s = [v | t <- flatten [s], v <- wr t]
+  where
+    wr t = case t of
+      SArrayWrite (AVar a) _ _ -> [a]
+      SAssign _ (RVecStore (AVar a) _ _) -> [a]
+      _ -> []
+
+flatten :: [Stmt] -> [Stmt]
+flatten = concatMap f
+  where
+    f s = s : case s of
+      SLoop _ b -> flatten b
+      SIf _ t e -> flatten t ++ flatten e
+      SParallelRegion b -> flatten b
+      _ -> []
+
+siteCall :: HipSite -> Stmt
+siteCall site =
+  let args = map (AVar . fst) (hsArgs site)
+   in case hsResult site of
+        Just (acc, _) -> SAssign acc (RCall (hsName site) args)
+        Nothing -> callStmt (hsName site) args
+
+-- ---------------------------------------------------------------------------
+-- The offload predicate
+
+-- | A site builder (given the site's name) when the statement is a loop the
+-- device runs as the printed C does.
+offloadableLoop :: TypeEnv -> [Stmt] -> Stmt -> Maybe (CVar -> HipSite)
+offloadableLoop env whole s@(SLoop spec body) = do
+  (kspec, kbody, macc) <- kernelLoop spec body
+  guard' (all (supportedStmt False) kbody)
+  guard' (scatterOk kspec kbody)
+  let assigned = S.fromList (assignedVars [s])
+      used = S.fromList (usedVars [s])
+      free = S.toList ((used `S.difference` assigned) `S.union` maybe S.empty S.singleton macc)
+      outside = S.fromList (usedVars (removeStmt s whole))
+      liveOut = (assigned `S.intersection` outside) `S.difference` maybe S.empty S.singleton macc
+  guard' (S.null liveOut)
+  args <- mapM (\v -> (,) v <$> argType v) (sort free)
+  accT <- case macc of
+    Nothing -> Just Nothing
+    Just acc -> (\t -> Just (acc, t)) <$> scalarType acc
+  let vars = S.toList (assigned `S.union` used)
+      types = M.fromList [(v, t) | v <- vars, Just t <- [M.lookup v env], t /= CTUnknown]
+  pure $ \name -> HipSite {hsName = name, hsArgs = args, hsResult = accT, hsTypes = types, hsLoop = s}
+  where
+    argType v = case M.lookup v env of
+      Just t@(CTArray e) | e `elem` [CTInt64, CTDouble, CTBool] -> Just t
+      Just t | t `elem` [CTInt64, CTDouble, CTBool, CTTuple] -> Just t
       _ -> Nothing
+    scalarType v = case M.lookup v env of
+      Just t | t `elem` [CTInt64, CTDouble, CTBool] -> Just t
+      _ -> Nothing
+offloadableLoop _ _ _ = Nothing
 
-weightedHistogram :: LoopSpec -> CVar -> Atom -> [Stmt] -> Maybe (HipSite, [Stmt])
-weightedHistogram spec i n body = do
-  Parallel ps <- Just (lsExec spec)
-  guard (psStrategy ps `elem` [ParallelScatterAtomicAddInt, ParallelScatterPrivatizedIntAdd])
-  [s1, s2, s3, SAssign old (RArrayLoad (AVar dst) (AVar r)), SAssign new (RBinOp CAdd a b), SArrayWrite (AVar dst') (AVar r') (AVar new')] <-
-    Just body
-  guard (dst' == dst && r' == r && new' == new)
-  w <- case (a, b) of
-    (AVar p, AVar q) | p == old -> Just q
-                     | q == old -> Just p
-    _ -> Nothing
-  -- the load of x at i, the route r = x % nbins and the weight w = x * x,
-  -- the last two in either order
-  (x, dataArr) <- case s1 of
-    SAssign v (RArrayLoad (AVar arr) (AVar j)) | j == i -> Just (v, arr)
-    _ -> Nothing
-  let route (SAssign v (RBinOp CMod (AVar x') nb)) | v == r && x' == x = Just nb
-      route _ = Nothing
-      weight (SAssign v (RBinOp CMul (AVar p) (AVar q))) = v == w && p == x && q == x
-      weight _ = False
-  nbins <- (route s2 <* guard (weight s3)) <|> (route s3 <* guard (weight s2))
-  guard (dst /= dataArr && r /= w)
-  let helper = "hydg_weighted_histogram_i64"
-  pure (HipSite helper [dst, dataArr], [SAssign "_discarded" (RCall helper [AVar dst, AVar dataArr, nbins, n])])
+guard' :: Bool -> Maybe ()
+guard' b = if b then Just () else Nothing
 
-dot :: LoopSpec -> CVar -> Atom -> [Stmt] -> Maybe (HipSite, [Stmt])
-dot spec i n body = do
-  rs <- lsRed spec
-  guard (rsRedop rs == RAdd)
-  case lsExec spec of
-    Vector _ -> Nothing
-    _ -> Just ()
-  let acc = rsAccVar rs
-  [l1, l2, SAssign p (RBinOp CMulF (AVar x) (AVar y)), SAssign acc' (RBinOp CAddF a b)] <- Just body
-  guard (acc' == acc && (a, b) `elem` [(AVar acc, AVar p), (AVar p, AVar acc)])
-  loads <- loadsAt i [l1, l2]
-  xs <- M.lookup x loads
-  ys <- M.lookup y loads
-  guard (x /= y && p /= acc)
-  let helper = "hydg_dot_f64"
-  pure (HipSite helper [xs, ys], [SAssign acc (RCall helper [AVar xs, AVar ys, n, AVar acc])])
+-- | The loop whose iterations the kernel distributes, its body, and the
+-- reduction accumulator if any.
+kernelLoop :: LoopSpec -> [Stmt] -> Maybe (LoopSpec, [Stmt], Maybe CVar)
+kernelLoop spec body
+  | lsRole spec == LoopReductionWrapper
+  , [_] <- lsIters spec
+  , [IConst 1] <- lsBounds spec
+  , [SLoop inner ib] <- body
+  , Just rs <- lsRed inner
+  , [_] <- lsIters inner =
+      Just (inner, ib, Just (rsAccVar rs))
+  | Just rs <- lsRed spec, [_] <- lsIters spec = Just (spec, body, Just (rsAccVar rs))
+  | Nothing <- lsRed spec, independent = Just (spec, body, Nothing)
+  | otherwise = Nothing
+  where
+    independent = case lsExec spec of
+      Parallel _ -> True
+      _ -> lsRole spec `elem` [LoopMap, LoopMapReduction]
+
+-- | A combining scatter strategy needs the (+) / (+.) combine.
+scatterOk :: LoopSpec -> [Stmt] -> Bool
+scatterOk spec body = case lsExec spec of
+  Parallel ps | psStrategy ps `elem` [ParallelGeneric, ParallelScatterDirect] -> True
+  Parallel _ -> length (lsIters spec) == 1 && hasAddCombine body
+  _ -> True
+  where
+    hasAddCombine ss = any triple (tails3 ss) || any inIf ss
+    inIf t = case t of
+      SIf _ a b -> hasAddCombine a || hasAddCombine b
+      _ -> False
+    tails3 (a : b : c : rest) = (a, b, c) : tails3 (b : c : rest)
+    tails3 _ = []
+    triple (SAssign old (RArrayLoad (AVar d) r), SAssign new (RBinOp op x y), SArrayWrite (AVar d') r' (AVar new')) =
+      op `elem` [CAdd, CAddF] && d == d' && r == r' && new == new' && ((x == AVar old) /= (y == AVar old))
+    triple _ = False
+
+supportedStmt :: Bool -> Stmt -> Bool
+supportedStmt inLoop s = case s of
+  SAssign _ r -> supportedRHS r
+  SArrayWrite a i v -> all simpleAtom [a, i, v]
+  SIf c t e -> simpleAtom c && all (supportedStmt inLoop) t && all (supportedStmt inLoop) e
+  SLoop _ b -> all (supportedStmt True) b
+  SBreak -> inLoop
+  _ -> False
+
+supportedRHS :: RHS -> Bool
+supportedRHS r = all simpleAtom (rhsAtoms r) && case r of
+  RAtom _ -> True
+  RBinOp _ _ _ -> True
+  RUnOp op _ -> op `notElem` [CLog, CSin, CCos, CErf]
+  RArrayLoad _ _ -> True
+  RTuple as -> length as <= 8
+  RProj _ _ -> True
+  RArrayShape _ -> True
+  RShapeSize _ -> True
+  RShapeInit _ -> True
+  RShapeLast _ -> True
+  RFlatToNd _ _ -> True
+  RNdToFlat _ _ -> True
+  R2DToFlat _ _ -> True
+  _ -> False
+
+simpleAtom :: Atom -> Bool
+simpleAtom a = case a of
+  AString _ -> False
+  AVecVar _ -> False
+  _ -> True
+
+assignedVars :: [Stmt] -> [CVar]
+assignedVars ss = concat [f t | t <- flatten ss]
+  where
+    f t = case t of
+      SAssign v _ -> [v]
+      SLoop spec _ -> lsIters spec
+      _ -> []
+
+usedVars :: [Stmt] -> [CVar]
+usedVars ss = concat [f t | t <- flatten ss]
+  where
+    atomVars as = [v | AVar v <- as]
+    f t = case t of
+      SAssign _ r -> atomVars (rhsAtoms r)
+      SArrayWrite a i v -> atomVars [a, i, v]
+      SIf c _ _ -> atomVars [c]
+      SReturn a -> atomVars [a]
+      SLoop spec _ ->
+        concatMap ixVars (lsBounds spec ++ lsOrigins spec) ++ maybe [] (ixVars . rsInit) (lsRed spec)
+      _ -> []
+    ixVars e = case e of
+      IVar v -> [v]
+      IConst _ -> []
+      IAdd a b -> ixVars a ++ ixVars b
+      ISub a b -> ixVars a ++ ixVars b
+      IMul a b -> ixVars a ++ ixVars b
+      IDiv a b -> ixVars a ++ ixVars b
+      ITuple es -> concatMap ixVars es
+      IProj _ x -> ixVars x
+      IFlatToNd a b -> ixVars a ++ ixVars b
+      INdToFlat a b -> ixVars a ++ ixVars b
+      ICall _ es -> concatMap ixVars es
+
+-- | The procedure body without one statement (for the live-out check).
+removeStmt :: Stmt -> [Stmt] -> [Stmt]
+removeStmt target = mapMaybe go
+  where
+    go t
+      | t == target = Nothing
+      | otherwise = Just $ case t of
+          SLoop spec b -> SLoop spec (removeStmt target b)
+          SIf c a b -> SIf c (removeStmt target a) (removeStmt target b)
+          SParallelRegion b -> SParallelRegion (removeStmt target b)
+          _ -> t

@@ -19,7 +19,7 @@ def declared_functions():
     names = set()
     for f in sorted(os.listdir(INCLUDE)):
         # headers of static inline helpers, not library symbols
-        if not f.endswith(".h") or f in ("hyd_hip_glue.h", "hyd_exp.h", "hydgen_rt.h"):
+        if not f.endswith(".h") or f in ("hyd_exp.h", "hydgen_rt.h"):
             continue
         text = open(os.path.join(INCLUDE, f)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
