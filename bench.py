@@ -251,6 +251,172 @@ def bench_guarded(steps, warmup, cpu=True):
     return res
 
 
+def _device_array(shape, fill=None):
+    """A hyd_array_t* in HBM (zero, or filled on the device by fill(ptr, n));
+    its HBM copy is the current one."""
+    L = hydhip.lib()
+    arr = L.hyd_array_alloc_device8(hydhip.hyd_tuple(*shape))
+    if fill is not None:
+        n = int(np.prod(shape))
+        fill(L.hyd_array_device_data_keep(arr), n)
+        L.hyd_array_mark_device_written(arr)
+    return arr
+
+
+def _host_copy(arr, dtype):
+    L = hydhip.lib()
+    hydhip._check(L.hyd_array_sync_host(arr))
+    return hydhip.read_array(arr, dtype)
+
+
+def bench_generated(steps, warmup, cpu=True):
+    """The kernels hydgen generates from the example programs' offloaded
+    loops (lib/libhydgen_programs.so), on the workloads the hand-written
+    kernels run: the 8-op map chain and the dot over 2^27 fp64 (the dot's
+    launcher returns the value, so its time includes the 8-byte read-back),
+    the 64-bin weighted histogram over 2^24 int64, and one clamp heat step on
+    the 8192 x 8192 grid as the reference splits it (9 region loops).  Each
+    is checked against the oracle in the same run."""
+    import hydgen_lib
+
+    prog = hydgen_lib.programs()
+    L = hydhip.lib()
+    res = {}
+    c = workloads.DOTMAP
+    n = c["n"]
+
+    def fill_f64(seed, counter0):
+        return lambda ptr, m: hydhip._check(L.hydhip_fill_splitmix_f64(ptr, m, seed, counter0, 2.0, -1.0, None))
+
+    xs = _device_array((n,), fill_f64(c["seed"], 0))
+    ys = _device_array((n,), fill_f64(c["seed"], n))
+    out = _device_array((n,))
+    made = [xs, ys, out]
+    env = dict(n=n, out=out, xs=xs, ys=ys, acc=0.0, red_dim=n)
+
+    def timed(name, nbytes, launchers):
+        def step(ev):
+            if ev:
+                ev[0].record()
+            r = None
+            for lname in launchers:
+                r = prog.call(lname, env)
+            if ev:
+                ev[1].record()
+            return r
+
+        _, ev = timed_loop(step, steps, warmup, events=True)
+        ms = float(np.mean(ev))
+        res[name] = dict(ms=ms, bytes=nbytes, gbs=nbytes / ms / 1e6, frac_of_8tbs=nbytes / ms / 1e6 / HBM_PEAK_GBS,
+                         launchers=len(launchers))
+        return step
+
+    timed("map_chain", 24 * n, ["hydk_map_chain_0"])
+    dot_step = timed("dot", 16 * n, ["hydk_dot_0"])
+    dot_val = dot_step(None)
+    if cpu:
+        import pyoracle
+
+        hx = pyoracle.splitmix_f64(n, c["seed"], 0, symmetric=True)
+        hy = pyoracle.splitmix_f64(n, c["seed"], n, symmetric=True)
+        res["map_chain"]["matches_oracle"] = bits_equal(_host_copy(out, np.float64), pyoracle.map_chain(hx, hy))
+        ref = pyoracle.dot_dd(hx, hy)
+        res["dot"]["matches_oracle"] = bool(abs(dot_val - ref) <= 1e-12 * abs(ref))
+        del hx, hy
+    h = workloads.HIST
+    data = _device_array((h["n"],), lambda ptr, m: hydhip._check(
+        L.hydhip_fill_splitmix_i64(ptr, m, h["seed"], 0, 64 - h["bits"], None)))
+    hist = _device_array((h["bins"],))
+    made += [data, hist]
+    env.update(data=data, hist=hist, nb=h["bins"])
+    env["n"] = h["n"]
+    timed("weighted_histogram", 8 * h["n"] + 8 * h["bins"], ["hydk_histogram_0"])
+    if cpu:
+        import pyoracle
+
+        hist1 = _device_array((h["bins"],))
+        made.append(hist1)
+        prog.call("hydk_histogram_0", dict(env, hist=hist1))
+        hd = pyoracle.splitmix_i64(h["n"], h["seed"], bits=h["bits"])
+        res["weighted_histogram"]["matches_oracle"] = bits_equal(_host_copy(hist1, np.int64),
+                                                                 pyoracle.weighted_histogram(hd, h["bins"]))
+    j = workloads.JACOBI
+    hh, ww = j["h"], j["w"]
+    u = _device_array((hh, ww), lambda ptr, m: hydhip._check(L.hydhip_fill_splitmix_f64(ptr, m, j["seed"], 0, 1.0, 0.0,
+                                                                                           None)))
+    uo = _device_array((hh, ww))
+    made += [u, uo]
+    env.update(u=u, out=uo, shp=(hh, ww), n0=hh, n1=ww, n0m1=hh - 1, n1m1=ww - 1, bd_start_r=hh - 1,
+               bd_start_c=ww - 1)
+    regions = sorted(k for k in prog.launchers() if k.startswith("hydk_jacobi_step_"))
+    timed("jacobi_heat_step", 16 * hh * ww, regions)
+    if cpu:
+        import pyoracle
+
+        hu = pyoracle.splitmix_f64(hh * ww, j["seed"]).reshape(hh, ww)
+        res["jacobi_heat_step"]["matches_oracle"] = bits_equal(_host_copy(uo, np.float64).reshape(hh, ww),
+                                                               pyoracle.stencil2d(hu, 0, j["alpha"]))
+    for a in made:
+        L.hyd_array_free(a)
+    return res
+
+
+def bench_coo_sort(steps, warmup, cpu=True):
+    """COO -> CSR of 1e7 nonzeros over 2^20 x 2^20 (the served mix's largest
+    build: rows, cols and vals uniform over 2^20) and sort_indices of 1e7
+    40-bit keys (the width of the build's packed keys), standalone.  Rates
+    against the compulsory bytes: COO->CSR reads rows, cols, vals and writes
+    col_idx, vals (40 B per nonzero) and row_ptr (8 B per row); the sort reads
+    the keys and writes the permutation (16 B per key).  Beside them, the
+    oracle's serial loops on the same inputs."""
+    nnz, nrows = 10_000_000, 1 << 20
+    seed = workloads.SERVE["seed"]
+    rows = hydhip.splitmix_i64(nnz, seed + 13, bits=20)
+    cols = hydhip.splitmix_i64(nnz, seed + 14, bits=20)
+    vals = hydhip.splitmix_i64(nnz, seed + 15, bits=20)
+    keys = hydhip.splitmix_i64(nnz, seed + 16, bits=40)
+    perm = hydhip.DeviceArray((nnz,), np.int64)
+    got = {}
+
+    def coo(ev):
+        if ev:
+            ev[0].record()
+        got["coo"] = hydhip.coo_to_csr(nrows, nrows, rows, cols, vals)
+        if ev:
+            ev[1].record()
+
+    def srt(ev):
+        if ev:
+            ev[0].record()
+        hydhip.sort_indices(keys, out=perm)
+        if ev:
+            ev[1].record()
+
+    cms, cev = timed_loop(coo, max(3, steps // 4), 1, events=True)
+    sms, sev = timed_loop(srt, max(3, steps // 4), 1, events=True)
+    cb = 40 * nnz + 8 * (nrows + 1)
+    res = dict(
+        coo_to_csr=dict(nnz=nnz, nrows=nrows, ms_per_call=cms, event_ms=float(np.mean(cev)), compulsory_bytes=cb,
+                        gbs=cb / float(np.mean(cev)) / 1e6, merged_nnz=int(got["coo"][3])),
+        sort_indices=dict(keys=nnz, key_bits=40, ms_per_call=sms, event_ms=float(np.mean(sev)),
+                          compulsory_bytes=16 * nnz, gbs=16 * nnz / float(np.mean(sev)) / 1e6))
+    if cpu:
+        import pyoracle
+
+        hr, hc, hv, hk = rows.numpy(), cols.numpy(), vals.numpy(), keys.numpy()
+        t, ref = cpu_time(lambda: pyoracle.coo_to_csr(nrows, nrows, hr, hc, hv), warmup=0, iters=1)
+        rp, ci, cv, m = got["coo"]
+        res["coo_to_csr"]["matches_oracle"] = bool(m == ref[3] and bits_equal(rp.numpy(), ref[0]) and
+                                                   bits_equal(ci.numpy(), ref[1]) and bits_equal(cv.numpy(), ref[2]))
+        res["coo_to_csr"]["cpu_oracle"] = dict(t, threads=1, note="the serial loops the reference emits")
+        t, (sref, _) = cpu_time(lambda: pyoracle.sort_indices(hk), warmup=0, iters=1)
+        res["sort_indices"]["matches_oracle"] = bits_equal(perm.numpy(), sref)
+        res["sort_indices"]["cpu_oracle"] = dict(t, threads=1, note="hyd_radix_sort_entries, serial")
+    for a in (rows, cols, vals, keys, perm) + tuple(got["coo"][:3]):
+        a.free()
+    return res
+
+
 def csv_file_round_trip(x, n):
     """The drop-in file functions end to end on a device-resident array of n
     fp64 values: hyd_write_array_csv_float (format on the GPU, text back over
@@ -700,6 +866,13 @@ def main():
     if not only or "jacobi" in only:
         log("jacobi ...")
         kernels["jacobi_heat"] = bench_jacobi(args.steps, args.warmup, cpu=not args.no_cpu)
+
+    if not only or "generated" in only:
+        log("generated kernels ...")
+        kernels["generated"] = bench_generated(args.steps, args.warmup, cpu=not args.no_cpu)
+    if not only or "coo" in only:
+        log("coo -> csr and sort ...")
+        kernels["coo_sort"] = bench_coo_sort(args.steps, args.warmup, cpu=not args.no_cpu)
 
     if not only or "batch" in only:
         log("batched histograms ...")

@@ -26,11 +26,20 @@
 //     statement and no written array is read, the round computes all its
 //     elements first and stores afterwards (the loads of later elements
 //     issue before the stores of earlier ones); loads and stores at the loop
-//     index of arrays touched nowhere else are non-temporal;
-//   * map, 2-D   - 64 x 4 workgroups, x over the inner (contiguous) index, a
-//     grid-stride walk over rows (the clamp-stencil region loops,
-//     Lowering.hs:2159-2172);
+//     index of arrays touched nowhere else are non-temporal, and when all of
+//     those arrays are 16-byte aligned each thread moves two adjacent
+//     elements per access (a 16-byte load or store);
+//   * map, 2-D   - workgroups of 256 threads over 256 consecutive inner
+//     (contiguous) indices; grid.y splits the outer index into bands of
+//     consecutive rows, each walked R = 4 rows per round (computed first,
+//     stored after) so a row's neighbours are read while still in L2 (the
+//     clamp-stencil region loops, Lowering.hs:2159-2172);
 //   * map, N-D   - flattened, the indices recovered by division.
+// Loop-invariant statements at the head of the body (the shape and bound
+// computations Lowering emits there) run once in the host launcher; an
+// integer division or remainder by a loop invariant becomes a multiply-shift
+// (Granlund-Montgomery) with the exact quotient of C's truncating `/` and
+// `%`; nd_to_flat of a tuple known element by element is a closed form.
 // Every fp64 operation is emitted as its own IEEE operation in source order
 // (three-address form, CodegenC.hs:2293-2400) and the sources are compiled
 // with -ffp-contract=off; integer + - * wrap (the oracle is built with -fwrapv).
@@ -39,6 +48,7 @@
 
 #include <algorithm>
 #include <functional>
+#include <set>
 #include <sstream>
 
 #include "hydgen.hpp"
@@ -140,7 +150,17 @@ struct Plan {
     bool red_mul = false;
     // scatter
     Triple tri;
-    bool scatter_lds = false;  // decided at run time for ScatterInt; both kernels are emitted
+    // statements the launcher evaluates once (top level, one assignment,
+    // operands that do not change over the loop), in order
+    std::vector<const Stmt*> invariant_stmts;
+    std::set<std::string> invariants;
+    // tuples built once from components that do not change after (RTuple):
+    // RNdToFlat / RProj of them are printed in closed form, as CodegenC.hs:
+    // 2336-2346 prints RNdToFlat of a known tuple
+    std::map<std::string, std::vector<Atom>> tuples;
+    // integer divisors that do not change over the loop: / and % by them use
+    // multiply-shift constants the launcher computes
+    std::set<std::string> divisors;
 };
 
 const char* form_name(Plan::Form f) {
@@ -389,7 +409,7 @@ struct Analyzer {
             if (uses) throw Error("site " + site.name + ": the scatter target is used outside its combine");
         }
         // Deferred stores: top-level writes only, and no written array read.
-        if (p.form == Plan::Map1D || p.form == Plan::MapND) {
+        if (p.form == Plan::Map1D || p.form == Plan::MapND || p.form == Plan::Map2D) {
             bool ok = true;
             auto no_write = [&](const Stmt& t) {
                 if (t.kind == Stmt::SArrayWrite) ok = false;
@@ -407,6 +427,67 @@ struct Analyzer {
             if (a.first != p.acc && p.locals.count(a.first))
                 throw Error("site " + site.name + ": argument " + a.first + " is assigned inside the loop");
         check_free_vars();
+        find_invariants();
+    }
+
+    // Invariant statements, known tuples and invariant divisors.
+    void find_invariants() {
+        std::map<std::string, int> count;
+        for_each_stmt(p.kloop->body, [&](const Stmt& s) {
+            if (s.kind == Stmt::SAssign) count[s.var]++;
+            if (s.kind == Stmt::SLoop)
+                for (const auto& it : s.loop.iters) count[it] += 2;
+        });
+        std::set<std::string> iters(p.iters.begin(), p.iters.end());
+        if (p.wrapper)
+            for (const auto& it : p.wrapper->loop.iters) iters.insert(it);
+        auto scalar_arg = [&](const std::string& v) {
+            bool f;
+            CType t = arg_type(site, v, &f);
+            return f && !t.is_array() && v != p.acc;
+        };
+        auto inv_atom = [&](const Atom& a) {
+            if (!a.is_var()) return a.kind == Atom::Int || a.kind == Atom::Float || a.kind == Atom::Bool ||
+                                    a.kind == Atom::Unit;
+            return p.invariants.count(a.var) > 0 || scalar_arg(a.var);
+        };
+        for (const Stmt& s : p.kloop->body) {
+            if (s.kind != Stmt::SAssign || count[s.var] != 1 || s.var == p.acc || iters.count(s.var)) continue;
+            bool ok = false;
+            switch (s.rhs.kind) {
+                case RHS::RAtom: case RHS::RBinOp: case RHS::RUnOp: case RHS::RProj: case RHS::RTuple:
+                case RHS::RShapeSize: case RHS::RShapeInit: case RHS::RShapeLast: case RHS::R2DToFlat:
+                case RHS::RNdToFlat: case RHS::RFlatToNd:
+                    ok = std::all_of(s.rhs.args.begin(), s.rhs.args.end(), inv_atom);
+                    break;
+                case RHS::RArrayShape: ok = true; break;
+                default: break;
+            }
+            if (ok) {
+                p.invariants.insert(s.var);
+                p.invariant_stmts.push_back(&s);
+            }
+        }
+        // a tuple whose components cannot change after it is built
+        auto fixed = [&](const Atom& a) {
+            return !a.is_var() || p.invariants.count(a.var) || scalar_arg(a.var) || iters.count(a.var) ||
+                   count[a.var] == 1;
+        };
+        for_each_stmt(p.kloop->body, [&](const Stmt& s) {
+            if (s.kind == Stmt::SAssign && s.rhs.kind == RHS::RTuple && count[s.var] == 1 &&
+                std::all_of(s.rhs.args.begin(), s.rhs.args.end(), fixed))
+                p.tuples[s.var] = s.rhs.args;
+        });
+        for_each_stmt(p.kloop->body, [&](const Stmt& s) {
+            if (s.kind == Stmt::SAssign && s.rhs.kind == RHS::RBinOp &&
+                (s.rhs.bop == BinOp::CDiv || s.rhs.bop == BinOp::CMod) && s.rhs.args[1].is_var()) {
+                const std::string& d = s.rhs.args[1].var;
+                if (p.invariants.count(d) || scalar_arg(d)) {
+                    CType t = type_of(d);
+                    if (t.is_int_like()) p.divisors.insert(d);
+                }
+            }
+        });
     }
 
     void check_free_vars() {
@@ -449,11 +530,16 @@ Plan make_plan(const Site& site) {
 
 // ---- printing ----------------------------------------------------------------------
 struct Printer {
-    Printer(const Site& s, const Plan& plan, std::ostringstream& out) : site(s), p(plan), o(out) {}
+    Printer(const Site& s, const Plan& plan, std::ostringstream& out) : site(s), p(plan), o(out) {
+        for (const Stmt* st : p.invariant_stmts) hoisted.insert(st);
+    }
     const Site& site;
     const Plan& p;
     std::ostringstream& o;
-    std::string acc_expr;  // how the kernel names the reduction accumulator
+    std::set<const Stmt*> hoisted;
+    bool host = false;          // printing for the launcher (host C++), not the kernel
+    bool preloaded = false;     // streamed loads come from the round's 16-byte registers
+    std::string acc_expr;       // how the kernel names the reduction accumulator
     enum WriteMode { Direct, Deferred } wmode = Direct;
     std::map<const Stmt*, int> write_slot;  // deferred writes: slot per top-level write
     mutable int loop_depth = 0;             // inner loops around the statement being printed
@@ -490,16 +576,19 @@ struct Printer {
     }
     const ArrayUse& use(const std::string& a) const { return p.arrays.at(a); }
     std::string arr(const std::string& a) const { return "hk_a_" + sanitize(a).substr(2); }
+    bool one_d() const {
+        return p.form == Plan::Map1D || p.form == Plan::Reduce || p.form == Plan::ScatterInt ||
+               p.form == Plan::ScatterFloat;
+    }
     bool stream_load(const std::string& a) const {
         const ArrayUse& u = use(a);
-        return (p.form == Plan::Map1D || p.form == Plan::Reduce || p.form == Plan::ScatterInt ||
-                p.form == Plan::ScatterFloat) &&
-               u.loads_at_iter && !u.written;
+        return one_d() && u.read && u.loads_at_iter && !u.written;
     }
     bool stream_store(const std::string& a) const {
         const ArrayUse& u = use(a);
-        return (p.form == Plan::Map1D) && u.writes_at_iter && !u.read;
+        return p.form == Plan::Map1D && u.written && u.writes_at_iter && !u.read;
     }
+    bool known_tuple(const Atom& a) const { return a.is_var() && p.tuples.count(a.var); }
 
     std::string rhs(const RHS& r, const CType& target) const {
         const auto& A = r.args;
@@ -511,7 +600,11 @@ struct Printer {
                 if (is_compare(r.bop)) return "((int64_t)(" + a + " " + c_binop(r.bop) + " " + b + "))";
                 if (r.bop == BinOp::CAdd || r.bop == BinOp::CSub || r.bop == BinOp::CMul)
                     return "((int64_t)((uint64_t)" + a + " " + c_binop(r.bop) + " (uint64_t)" + b + "))";
-                return "(" + a + " " + c_binop(r.bop) + " " + b + ")";  // CDiv, CMod: C truncation
+                // CDiv, CMod: C truncation; by a divisor fixed over the loop, multiply-shift
+                if (!host && A[1].is_var() && p.divisors.count(A[1].var))
+                    return std::string(r.bop == BinOp::CDiv ? "hydk_cdiv(" : "hydk_cmod(") + a + ", hk_dc_" +
+                           sanitize(A[1].var).substr(2) + ")";
+                return "(" + a + " " + c_binop(r.bop) + " " + b + ")";
             }
             case RHS::RUnOp: {
                 const std::string a = atom(A[0]);
@@ -521,7 +614,7 @@ struct Printer {
                         if (target.kind == CType::Double) return "(-" + a + ")";
                         return "((int64_t)(0ull - (uint64_t)" + a + "))";
                     case UnOp::CSqrt: return "__builtin_sqrt(" + a + ")";
-                    case UnOp::CExpF: return "hyd_exp_with(" + a + ", hk_exptab)";
+                    case UnOp::CExpF: return host ? "hyd_exp(" + a + ")" : "hyd_exp_with(" + a + ", hk_exptab)";
                     case UnOp::CAbsF: return "__builtin_fabs(" + a + ")";
                     case UnOp::CFloorF: return "__builtin_floor(" + a + ")";
                     case UnOp::CCeilF: return "__builtin_ceil(" + a + ")";
@@ -535,7 +628,13 @@ struct Printer {
             case RHS::RArrayLoad: {
                 const std::string& a = A[0].var;
                 const std::string ix = atom(A[1]);
-                if (stream_load(a)) return "__builtin_nontemporal_load(&" + arr(a) + "[" + ix + "])";
+                if (stream_load(a)) {
+                    if (preloaded) {
+                        const std::string reg = "hk_p_" + sanitize(a).substr(2) + "[hk_u >> 1]";
+                        return "((hk_u & 1) ? " + reg + ".y : " + reg + ".x)";
+                    }
+                    return "__builtin_nontemporal_load(&" + arr(a) + "[" + ix + "])";
+                }
                 return arr(a) + "[" + ix + "]";
             }
             case RHS::RTuple: {
@@ -544,15 +643,32 @@ struct Printer {
                 for (const Atom& a : A) s += ", " + atom(a);
                 return s + ")";
             }
-            case RHS::RProj: return "(" + atom(A[0]) + ".elems[" + std::to_string(r.proj) + "])";
-            case RHS::RArrayShape: return "hk_shape_" + sanitize(A[0].var).substr(2);
+            case RHS::RProj:
+                if (known_tuple(A[0])) {
+                    const auto& comp = p.tuples.at(A[0].var);
+                    if (r.proj >= 0 && r.proj < (int64_t)comp.size()) return atom(comp[(size_t)r.proj]);
+                }
+                return "(" + atom(A[0]) + ".elems[" + std::to_string(r.proj) + "])";
+            case RHS::RArrayShape:
+                return host ? sanitize(A[0].var) + "->shape" : "hk_shape_" + sanitize(A[0].var).substr(2);
             case RHS::RShapeSize: return "hydk_shape_size(" + atom(A[0]) + ")";
             case RHS::RShapeInit: return "hydk_shape_init(" + atom(A[0]) + ")";
             case RHS::RShapeLast: return "hydk_shape_last(" + atom(A[0]) + ")";
             case RHS::RFlatToNd: return "hydk_flat_to_nd(" + atom(A[0]) + ", " + atom(A[1]) + ")";
-            case RHS::RNdToFlat:
+            case RHS::RNdToFlat: {
                 if (A[0].kind == Atom::Unit) return "((int64_t)0)";
+                if (known_tuple(A[0]) && p.tuples.at(A[0].var).size() >= 2) {
+                    // row-major: ((e0 * s1 + e1) * s2 + e2) ..., wrapping
+                    const auto& comp = p.tuples.at(A[0].var);
+                    const std::string shp = atom(A[1]);
+                    std::string acc = "(uint64_t)" + atom(comp[0]);
+                    for (size_t k = 1; k < comp.size(); ++k)
+                        acc = "(" + acc + " * (uint64_t)" + shp + ".elems[" + std::to_string(k) + "] + (uint64_t)" +
+                              atom(comp[k]) + ")";
+                    return "((int64_t)" + acc + ")";
+                }
                 return "hydk_nd_to_flat(" + atom(A[0]) + ", " + atom(A[1]) + ")";
+            }
             case RHS::R2DToFlat: return "((int64_t)((uint64_t)" + atom(A[0]) + " * (uint64_t)" + atom(A[1]) + "))";
             default: throw Error("site " + site.name + ": right-hand side not supported in a kernel");
         }
@@ -568,12 +684,25 @@ struct Printer {
     void line(int ind, const std::string& s) const { o << std::string((size_t)ind * 4, ' ') << s << "\n"; }
 
     void declare_locals(int ind) const {
-        for (const auto& l : p.locals) line(ind, std::string(c_type(l.second)) + " " + name(l.first) + ";");
+        for (const auto& l : p.locals)
+            if (!p.invariants.count(l.first)) line(ind, std::string(c_type(l.second)) + " " + name(l.first) + ";");
+    }
+
+    // The launcher's evaluation of the invariant statements (host C++).
+    void hoisted_stmts(int ind) {
+        host = true;
+        for (const Stmt* s : p.invariant_stmts) {
+            const CType t = var_type(s->var);
+            line(ind, std::string("const ") + c_type(t) + " " + name(s->var) + " = (" + c_type(t) + ")" +
+                          rhs(s->rhs, t) + ";");
+        }
+        host = false;
     }
 
     void stmts(const std::vector<Stmt>& ss, int ind, std::set<std::string>& assigned) const {
         for (size_t k = 0; k < ss.size(); ++k) {
             const Stmt& s = ss[k];
+            if (hoisted.count(&s)) continue;  // a kernel parameter
             if (smode != None && &s == p.tri.load) {
                 scatter_contribution(ind);
                 k += 2;  // the combine and its write
@@ -683,6 +812,14 @@ std::vector<Param> kernel_params(const Site& site, const Plan& p) {
             ps.push_back({c_type(a.second), sanitize(v), sanitize(v)});
         }
     }
+    for (const Stmt* s : p.invariant_stmts) {
+        auto it = p.locals.find(s->var);
+        ps.push_back({c_type(it->second), sanitize(s->var), sanitize(s->var)});
+    }
+    for (const std::string& d : p.divisors) {
+        const std::string base = sanitize(d).substr(2);
+        ps.push_back({"hydk_divc_t", "hk_dc_" + base, "hk_dc_" + base});
+    }
     return ps;
 }
 
@@ -704,8 +841,8 @@ void exp_table_prologue(std::ostringstream& o, const Plan& p) {
          "    __syncthreads();\n";
 }
 
-// One element of a 1-D form: declarations, the wrapper's index, the kernel
-// index, the body.
+// One element of a 1-D or N-D form: declarations, the wrapper's index, the
+// kernel index, the body.
 void element_body(Printer& pr, const Plan& p, int ind, const std::string& index_expr) {
     pr.declare_locals(ind);
     if (p.wrapper)
@@ -727,6 +864,103 @@ void element_body(Printer& pr, const Plan& p, int ind, const std::string& index_
     pr.stmts(p.kloop->body, ind, assigned);
 }
 
+const char* vec2_type(const CType& t) { return t.kind == CType::Double ? "hydk_v2d" : "hydk_v2i"; }
+
+// The streaming element loop of the 1-D and N-D forms.  A round gives each
+// thread `U` elements; with `paired` (U even, a 1-D form) elements 2q and
+// 2q+1 of the thread are adjacent, so the streamed arrays move in 16-byte
+// accesses when PAIRS (a template parameter the launcher sets when they are
+// 16-byte aligned) and in two 8-byte accesses otherwise, with the same
+// element-to-thread assignment either way (a reduction's order does not
+// depend on alignment).
+void stream_loop(std::ostringstream& o, Printer& pr, const Plan& p, int U, int B, bool paired, bool deferred) {
+    const int T = U * B;
+    const std::string emap = paired ? "hk_base + 2 * ((int64_t)(hk_u >> 1) * " + std::to_string(B) +
+                                          " + threadIdx.x) + (hk_u & 1)"
+                                    : "hk_base + (int64_t)hk_u * " + std::to_string(B) + " + threadIdx.x";
+    std::vector<std::string> sload, sstore;  // streamed arrays
+    for (const auto& a : p.arrays) {
+        if (pr.stream_load(a.first)) sload.push_back(a.first);
+        if (deferred && pr.stream_store(a.first)) sstore.push_back(a.first);
+    }
+    auto full_tile = [&](bool pairs_io) {
+        if (pairs_io)
+            for (const auto& a : sload) {
+                const std::string reg = "hk_p_" + sanitize(a).substr(2);
+                const char* vt = vec2_type(pr.use(a).elem);
+                o << "            " << vt << " " << reg << "[" << U / 2 << "];\n";
+                o << "#pragma unroll\n";
+                o << "            for (int hk_q = 0; hk_q < " << U / 2 << "; ++hk_q)\n";
+                o << "                " << reg << "[hk_q] = __builtin_nontemporal_load((const " << vt << "*)&" << pr.arr(a)
+                  << "[hk_lo + hk_base + 2 * ((int64_t)hk_q * " << B << " + threadIdx.x)]);\n";
+            }
+        if (deferred) {
+            int slot = 0;
+            pr.write_slot.clear();
+            for (const Stmt& s : p.kloop->body)
+                if (s.kind == Stmt::SArrayWrite) {
+                    pr.write_slot[&s] = slot;
+                    o << "            int64_t hk_wi" << slot << "[" << U << "];\n";
+                    o << "            " << c_type(p.arrays.at(s.a0.var).elem) << " hk_wv" << slot << "[" << U << "];\n";
+                    ++slot;
+                }
+            pr.wmode = Printer::Deferred;
+        }
+        pr.preloaded = pairs_io;
+        o << "#pragma unroll\n";
+        o << "            for (int hk_u = 0; hk_u < " << U << "; ++hk_u) {\n";
+        o << "                const int64_t hk_e = " << emap << ";\n";
+        element_body(pr, p, 4, "hk_e");
+        o << "            }\n";
+        pr.preloaded = false;
+        if (deferred) {
+            pr.wmode = Printer::Direct;
+            for (const auto& ws : pr.write_slot) {
+                const std::string& a = ws.first->a0.var;
+                const std::string sl = std::to_string(ws.second);
+                const bool streamed = std::find(sstore.begin(), sstore.end(), a) != sstore.end();
+                if (streamed && pairs_io) {
+                    const char* vt = vec2_type(pr.use(a).elem);
+                    o << "#pragma unroll\n";
+                    o << "            for (int hk_q = 0; hk_q < " << U / 2 << "; ++hk_q) {\n";
+                    o << "                " << vt << " hk_w2 = {hk_wv" << sl << "[2 * hk_q], hk_wv" << sl << "[2 * hk_q + 1]};\n";
+                    o << "                __builtin_nontemporal_store(hk_w2, (" << vt << "*)&" << pr.arr(a) << "[hk_wi" << sl
+                      << "[2 * hk_q]]);\n";
+                    o << "            }\n";
+                } else {
+                    o << "#pragma unroll\n";
+                    o << "            for (int hk_u = 0; hk_u < " << U << "; ++hk_u)\n";
+                    if (streamed)
+                        o << "                __builtin_nontemporal_store(hk_wv" << sl << "[hk_u], &" << pr.arr(a) << "[hk_wi"
+                          << sl << "[hk_u]]);\n";
+                    else
+                        o << "                " << pr.arr(a) << "[hk_wi" << sl << "[hk_u]] = hk_wv" << sl << "[hk_u];\n";
+                }
+            }
+        }
+    };
+    o << "    for (int64_t hk_base = (int64_t)blockIdx.x * " << T << "; hk_base < hk_n; hk_base += (int64_t)gridDim.x * " << T
+      << ") {\n";
+    o << "        if (hk_base + " << T << " <= hk_n) {\n";
+    if (paired && (!sload.empty() || !sstore.empty())) {
+        o << "          if constexpr (PAIRS) {\n";
+        full_tile(true);
+        o << "          } else {\n";
+        full_tile(false);
+        o << "          }\n";
+    } else {
+        full_tile(false);
+    }
+    o << "        } else {\n";
+    o << "            for (int hk_u = 0; hk_u < " << U << "; ++hk_u) {\n";
+    o << "                const int64_t hk_e = " << emap << ";\n";
+    o << "                if (hk_e >= hk_n) continue;\n";
+    element_body(pr, p, 4, "hk_e");
+    o << "            }\n";
+    o << "        }\n";
+    o << "    }\n";
+}
+
 void emit_site(std::ostringstream& o, const Site& site, const Options& opt) {
     const Plan p = make_plan(site);
     const std::string K = "hk_kernel_" + site.name;
@@ -736,6 +970,13 @@ void emit_site(std::ostringstream& o, const Site& site, const Options& opt) {
     const std::string plist = param_list(ps);
     const std::string sep = plist.empty() ? "" : ", ";
     Printer pr(site, p, o);
+    const bool one_d = pr.one_d();
+    const bool paired = one_d && U % 2 == 0;
+    std::vector<std::string> streamed;  // arrays the PAIRS variant moves in 16-byte accesses
+    for (const auto& a : p.arrays)
+        if (pr.stream_load(a.first) || (p.deferred && pr.stream_store(a.first))) streamed.push_back(a.first);
+    const bool has_pairs = paired && !streamed.empty();
+    const std::string tmpl = has_pairs ? "template <bool PAIRS>\n" : "";
 
     o << "// ---- " << site.name << ": " << describe_plan(site, opt) << "\n";
     std::string idx_params = p.form == Plan::MapND ? "" : "int64_t hk_lo, int64_t hk_n";
@@ -746,85 +987,84 @@ void emit_site(std::ostringstream& o, const Site& site, const Options& opt) {
         idx_params += ", int64_t hk_n";
     }
 
-    // -- the streaming 1-D element loop, shared by map / reduce / scatter
-    auto stream_loop = [&](bool with_deferred) {
-        const int T = U * B;
-        o << "    for (int64_t hk_base = (int64_t)blockIdx.x * " << T << "; hk_base < hk_n; hk_base += (int64_t)gridDim.x * " << T
-          << ") {\n";
-        o << "        if (hk_base + " << T << " <= hk_n) {\n";
-        if (with_deferred) {
-            int slot = 0;
-            for (const Stmt& s : p.kloop->body)
-                if (s.kind == Stmt::SArrayWrite) {
-                    pr.write_slot[&s] = slot;
-                    o << "            int64_t hk_wi" << slot << "[" << U << "];\n";
-                    o << "            " << c_type(p.arrays.at(s.a0.var).elem) << " hk_wv" << slot << "[" << U << "];\n";
-                    ++slot;
-                }
-            pr.wmode = Printer::Deferred;
-        }
-        o << "#pragma unroll\n";
-        o << "            for (int hk_u = 0; hk_u < " << U << "; ++hk_u) {\n";
-        o << "                const int64_t hk_e = hk_base + (int64_t)hk_u * " << B << " + threadIdx.x;\n";
-        element_body(pr, p, 4, "hk_e");
-        o << "            }\n";
-        if (with_deferred) {
-            pr.wmode = Printer::Direct;
-            o << "#pragma unroll\n";
-            o << "            for (int hk_u = 0; hk_u < " << U << "; ++hk_u) {\n";
-            for (const auto& ws : pr.write_slot) {
-                const std::string& a = ws.first->a0.var;
-                const std::string sl = std::to_string(ws.second);
-                if (pr.stream_store(a))
-                    o << "                __builtin_nontemporal_store(hk_wv" << sl << "[hk_u], &" << pr.arr(a) << "[hk_wi" << sl
-                      << "[hk_u]]);\n";
-                else
-                    o << "                " << pr.arr(a) << "[hk_wi" << sl << "[hk_u]] = hk_wv" << sl << "[hk_u];\n";
-            }
-            o << "            }\n";
-        }
-        o << "        } else {\n";
-        o << "            for (int hk_u = 0; hk_u < " << U << "; ++hk_u) {\n";
-        o << "                const int64_t hk_e = hk_base + (int64_t)hk_u * " << B << " + threadIdx.x;\n";
-        o << "                if (hk_e >= hk_n) break;\n";
-        element_body(pr, p, 4, "hk_e");
-        o << "            }\n";
-        o << "        }\n";
-        o << "    }\n";
-    };
-
     if (p.form == Plan::Map1D || p.form == Plan::MapND) {
-        o << "__global__ __launch_bounds__(" << B << ") void " << K << "(" << plist << sep << idx_params << ") {\n";
+        o << tmpl << "__global__ __launch_bounds__(" << B << ") void " << K << "(" << plist << sep << idx_params << ") {\n";
         exp_table_prologue(o, p);
-        stream_loop(p.deferred);
+        stream_loop(o, pr, p, U, B, paired, p.deferred);
         o << "}\n\n";
     } else if (p.form == Plan::Map2D) {
+        // 256-thread workgroups spanning 256 consecutive columns (the inner,
+        // contiguous index: each row access of a workgroup is 2 KiB of one
+        // row); workgroup row y owns the band of rows [y*band, (y+1)*band)
+        // and walks it R rows per round, so that the rows it reads as
+        // neighbours it has just read (L1 hits); with deferred stores a round
+        // computes its R rows before storing them, so loads of one address in
+        // a round are loaded once
+        const int R = p.deferred ? 4 : 1;
         o << "__global__ __launch_bounds__(256) void " << K << "(" << plist << sep
           << "int64_t hk_lo0, int64_t hk_n0, int64_t hk_lo1, int64_t hk_n1) {\n";
         exp_table_prologue(o, p);
-        o << "    const int64_t hk_c = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);\n";
+        o << "    const int64_t hk_c = (int64_t)blockIdx.x * 256 + threadIdx.x;\n";
         o << "    if (hk_c >= hk_n1) return;\n";
         o << "    const int64_t " << sanitize(p.iters[1]) << " = hk_lo1 + hk_c;\n";
-        o << "    for (int64_t hk_r = (int64_t)blockIdx.y * 4 + (threadIdx.x >> 6); hk_r < hk_n0; hk_r += (int64_t)gridDim.y * 4) {\n";
-        o << "        const int64_t " << sanitize(p.iters[0]) << " = hk_lo0 + hk_r;\n";
-        pr.declare_locals(2);
-        std::set<std::string> assigned;
-        pr.stmts(p.kloop->body, 2, assigned);
+        o << "    const int64_t hk_band = (hk_n0 + gridDim.y - 1) / gridDim.y;\n";
+        o << "    const int64_t hk_r1 = min(hk_n0, ((int64_t)blockIdx.y + 1) * hk_band);\n";
+        o << "    for (int64_t hk_r = (int64_t)blockIdx.y * hk_band; hk_r < hk_r1; hk_r += " << R << ") {\n";
+        auto row_body = [&](int ind) {
+            pr.declare_locals(ind);
+            o << std::string((size_t)ind * 4, ' ') << "const int64_t " << sanitize(p.iters[0]) << " = hk_lo0 + hk_r + hk_u;\n";
+            std::set<std::string> assigned;
+            pr.stmts(p.kloop->body, ind, assigned);
+        };
+        if (R > 1) {
+            o << "        if (hk_r + " << R << " <= hk_r1) {\n";
+            int slot = 0;
+            pr.write_slot.clear();
+            for (const Stmt& st : p.kloop->body)
+                if (st.kind == Stmt::SArrayWrite) {
+                    pr.write_slot[&st] = slot;
+                    o << "            int64_t hk_wi" << slot << "[" << R << "];\n";
+                    o << "            " << c_type(p.arrays.at(st.a0.var).elem) << " hk_wv" << slot << "[" << R << "];\n";
+                    ++slot;
+                }
+            pr.wmode = Printer::Deferred;
+            o << "#pragma unroll\n";
+            o << "            for (int hk_u = 0; hk_u < " << R << "; ++hk_u) {\n";
+            row_body(4);
+            o << "            }\n";
+            pr.wmode = Printer::Direct;
+            for (const auto& ws : pr.write_slot) {
+                const std::string sl = std::to_string(ws.second);
+                o << "#pragma unroll\n";
+                o << "            for (int hk_u = 0; hk_u < " << R << "; ++hk_u) " << pr.arr(ws.first->a0.var) << "[hk_wi" << sl
+                  << "[hk_u]] = hk_wv" << sl << "[hk_u];\n";
+            }
+            o << "        } else {\n";
+            o << "            for (int hk_u = 0; hk_u < " << R << " && hk_r + hk_u < hk_r1; ++hk_u) {\n";
+            row_body(4);
+            o << "            }\n";
+            o << "        }\n";
+        } else {
+            o << "        {\n";
+            o << "            const int hk_u = 0;\n";
+            row_body(3);
+            o << "        }\n";
+        }
         o << "    }\n}\n\n";
     } else if (p.form == Plan::Reduce) {
         const std::string T = c_type(p.acc_type);
         const bool fl = p.acc_type.kind == CType::Double;
         const std::string ident = p.red_mul ? (fl ? "1.0" : "((int64_t)1)") : (fl ? "(-0.0)" : "((int64_t)0)");
         const std::string comb = p.red_mul ? (fl ? "hydk_mul_f64" : "hydk_mul_i64") : (fl ? "hydk_add_f64" : "hydk_add_i64");
-        o << "__global__ __launch_bounds__(" << B << ") void " << K << "(" << plist << sep << idx_params << ", " << T
+        o << tmpl << "__global__ __launch_bounds__(" << B << ") void " << K << "(" << plist << sep << idx_params << ", " << T
           << "* hk_partials) {\n";
         exp_table_prologue(o, p);
         o << "    " << T << " hk_acc[" << U << "];\n";
         o << "#pragma unroll\n    for (int hk_u = 0; hk_u < " << U << "; ++hk_u) hk_acc[hk_u] = " << ident << ";\n";
         pr.acc_expr = "hk_acc[hk_u]";
-        stream_loop(false);
+        stream_loop(o, pr, p, U, B, paired, false);
         pr.acc_expr.clear();
-        // fixed combine: the thread's accumulators pairwise, the wave by xor
+        // fixed combine: the thread's accumulators in order, the wave by xor
         // butterfly, the waves in order
         o << "    " << T << " hk_v = hk_acc[0];\n";
         o << "#pragma unroll\n    for (int hk_u = 1; hk_u < " << U << "; ++hk_u) hk_v = " << comb << "(hk_v, hk_acc[hk_u]);\n";
@@ -853,7 +1093,7 @@ void emit_site(std::ostringstream& o, const Site& site, const Options& opt) {
         o << "    }\n}\n\n";
     } else if (p.form == Plan::ScatterInt) {
         // LDS-private bins, merged through spread copies of a zeroed scratch
-        o << "__global__ __launch_bounds__(" << B << ") void " << K << "_lds(" << plist << sep << idx_params
+        o << tmpl << "__global__ __launch_bounds__(" << B << ") void " << K << "_lds(" << plist << sep << idx_params
           << ", int64_t hk_ndst, int hk_copies, int hk_stride, int hk_ncopy, unsigned long long* hk_spread) {\n";
         exp_table_prologue(o, p);
         o << "    extern __shared__ unsigned long long hk_bins[];\n";
@@ -861,7 +1101,7 @@ void emit_site(std::ostringstream& o, const Site& site, const Options& opt) {
         o << "    __syncthreads();\n";
         o << "    unsigned long long* hk_mine = hk_bins + (threadIdx.x % hk_copies) * hk_stride;\n";
         pr.smode = Printer::LdsBins;
-        stream_loop(false);
+        stream_loop(o, pr, p, U, B, paired, false);
         o << "    __syncthreads();\n";
         o << "    unsigned long long* hk_dst_copy = hk_spread + (size_t)(blockIdx.x % hk_ncopy) * hk_ndst;\n";
         o << "    for (int64_t hk_b = threadIdx.x; hk_b < hk_ndst; hk_b += " << B << ") {\n";
@@ -869,19 +1109,19 @@ void emit_site(std::ostringstream& o, const Site& site, const Options& opt) {
         o << "        for (int hk_c = 0; hk_c < hk_copies; ++hk_c) hk_s += hk_bins[hk_c * hk_stride + hk_b];\n";
         o << "        if (hk_s) hydk_global_add(&hk_dst_copy[hk_b], hk_s);\n";
         o << "    }\n}\n\n";
-        o << "__global__ __launch_bounds__(" << B << ") void " << K << "_global(" << plist << sep << idx_params
+        o << tmpl << "__global__ __launch_bounds__(" << B << ") void " << K << "_global(" << plist << sep << idx_params
           << ", int64_t hk_ndst) {\n";
         exp_table_prologue(o, p);
         pr.smode = Printer::GlobalAtomics;
-        stream_loop(false);
+        stream_loop(o, pr, p, U, B, paired, false);
         o << "}\n\n";
         pr.smode = Printer::None;
     } else if (p.form == Plan::ScatterFloat) {
-        o << "__global__ __launch_bounds__(" << B << ") void " << K << "(" << plist << sep << idx_params
+        o << tmpl << "__global__ __launch_bounds__(" << B << ") void " << K << "(" << plist << sep << idx_params
           << ", int64_t* hk_sidx, double* hk_sval, int64_t* hk_skeep) {\n";
         exp_table_prologue(o, p);
         pr.smode = Printer::Materialize;
-        stream_loop(false);
+        stream_loop(o, pr, p, U, B, paired, false);
         pr.smode = Printer::None;
         o << "}\n\n";
     }
@@ -902,94 +1142,111 @@ void emit_site(std::ostringstream& o, const Site& site, const Options& opt) {
         o << "    " << c_type(a.second.elem) << "* hk_a_" << base << " = (" << c_type(a.second.elem)
           << "*)hydk_device_data(" << sanitize(a.first) << ");\n";
     }
+    if (p.form == Plan::Reduce) o << "    " << c_type(p.acc_type) << " hk_result = " << sanitize(p.acc) << ";\n";
     const LoopSpec& ks = p.kloop->loop;
+    std::string nonempty;
     if (p.form == Plan::Map2D) {
         o << "    const int64_t hk_lo0 = " << pr.index_expr(ks.origins[0]) << ", hk_n0 = " << pr.index_expr(ks.bounds[0]) << ";\n";
         o << "    const int64_t hk_lo1 = " << pr.index_expr(ks.origins[1]) << ", hk_n1 = " << pr.index_expr(ks.bounds[1]) << ";\n";
-        o << "    if (hk_n0 > 0 && hk_n1 > 0) {\n";
-        o << "        const int64_t hk_gx = (hk_n1 + 63) / 64;\n";
-        o << "        int64_t hk_gy = std::min<int64_t>((hk_n0 + 3) / 4, std::max<int64_t>(1, (int64_t)hydk_num_cus() * "
+        nonempty = "hk_n0 > 0 && hk_n1 > 0";
+    } else if (p.form == Plan::MapND) {
+        o << "    int64_t hk_n = 1;\n";
+        for (size_t d = 0; d < p.iters.size(); ++d) {
+            o << "    const int64_t hk_lo" << d << " = " << pr.index_expr(ks.origins[d]) << ", hk_n" << d << " = "
+              << pr.index_expr(ks.bounds[d]) << ";\n";
+            o << "    hk_n = hk_n" << d << " > 0 ? hk_n * hk_n" << d << " : 0;\n";
+        }
+        nonempty = "hk_n > 0";
+    } else {
+        o << "    const int64_t hk_lo = " << pr.index_expr(ks.origins[0]) << ", hk_n = " << pr.index_expr(ks.bounds[0]) << ";\n";
+        nonempty = "hk_n > 0";
+    }
+    if (p.form == Plan::ScatterInt || p.form == Plan::ScatterFloat) {
+        o << "    const int64_t hk_ndst = hyd_shape_size(" << sanitize(p.tri.dst) << "->shape);\n";
+        nonempty += " && hk_ndst > 0";
+    }
+    o << "    if (" << nonempty << ") {\n";
+    // the loop's invariant statements, once, and the divisors' constants
+    pr.hoisted_stmts(2);
+    for (const std::string& d : p.divisors)
+        o << "        const hydk_divc_t hk_dc_" << sanitize(d).substr(2) << " = hydk_make_divc(" << sanitize(d) << ");\n";
+    std::string targs;  // template argument of the paired kernels
+    if (has_pairs) {
+        o << "        const bool hk_pairs = true";
+        for (const auto& a : streamed) o << " && hydk_aligned16(hk_a_" << sanitize(a).substr(2) << " + hk_lo)";
+        o << ";\n";
+    }
+    auto launch = [&](const std::string& kern, const std::string& grid, const std::string& block, const std::string& lds,
+                      const std::string& args, int ind) {
+        const std::string pad((size_t)ind * 4, ' ');
+        if (has_pairs) {
+            o << pad << "if (hk_pairs) " << kern << "<true><<<" << grid << ", " << block << ", " << lds << ", hk_s>>>(" << args
+              << ");\n";
+            o << pad << "else " << kern << "<false><<<" << grid << ", " << block << ", " << lds << ", hk_s>>>(" << args << ");\n";
+        } else {
+            o << pad << kern << "<<<" << grid << ", " << block << ", " << lds << ", hk_s>>>(" << args << ");\n";
+        }
+        o << pad << "hydk_check_launch(hk_site);\n";
+    };
+    const std::string al = arg_list(ps);
+    if (p.form == Plan::Map2D) {
+        o << "        const int64_t hk_gx = (hk_n1 + 255) / 256;\n";
+        o << "        int64_t hk_gy = std::min<int64_t>(hk_n0, std::max<int64_t>(1, (int64_t)hydk_num_cus() * "
           << 8 * opt.waves_per_cu / 4 << " / hk_gx));\n";
         o << "        hk_gy = std::min<int64_t>(hk_gy, 65535);\n";
         o << "        if (hk_gx > 0x7fffffff) hydk_fail(hk_site, \"inner extent too large\");\n";
-        o << "        " << K << "<<<dim3((unsigned)hk_gx, (unsigned)hk_gy), 256, 0, hk_s>>>(" << arg_list(ps) << sep
-          << "hk_lo0, hk_n0, hk_lo1, hk_n1);\n";
-        o << "        hydk_check_launch(hk_site);\n";
-        o << "    }\n";
+        launch(K, "dim3((unsigned)hk_gx, (unsigned)hk_gy)", "256", "0", al + sep + "hk_lo0, hk_n0, hk_lo1, hk_n1", 2);
     } else {
         std::string idx_args;
         if (p.form == Plan::MapND) {
-            o << "    int64_t hk_n = 1;\n";
-            for (size_t d = 0; d < p.iters.size(); ++d) {
-                o << "    const int64_t hk_lo" << d << " = " << pr.index_expr(ks.origins[d]) << ", hk_n" << d << " = "
-                  << pr.index_expr(ks.bounds[d]) << ";\n";
-                o << "    hk_n = hk_n" << d << " > 0 ? hk_n * hk_n" << d << " : 0;\n";
+            for (size_t d = 0; d < p.iters.size(); ++d)
                 idx_args += "hk_lo" + std::to_string(d) + ", hk_n" + std::to_string(d) + ", ";
-            }
             idx_args += "hk_n";
         } else {
-            o << "    const int64_t hk_lo = " << pr.index_expr(ks.origins[0]) << ", hk_n = " << pr.index_expr(ks.bounds[0]) << ";\n";
             idx_args = "hk_lo, hk_n";
         }
-        const int T = U * B;
-        o << "    const int64_t hk_grid = hydk_stream_grid(hk_n, " << T << ", " << opt.waves_per_cu * 64 / B << ");\n";
+        const std::string A = al + sep + idx_args;
+        const bool scatter = p.form == Plan::ScatterInt || p.form == Plan::ScatterFloat;
+        o << "        const int64_t hk_grid = hydk_stream_grid(hk_n, " << U * B << ", "
+          << (scatter ? 2 : 1) * opt.waves_per_cu * 64 / B << ");\n";
         if (p.form == Plan::Reduce) {
             const std::string T = c_type(p.acc_type);
-            o << "    " << T << " hk_result = " << sanitize(p.acc) << ";\n";
-            o << "    if (hk_n > 0) {\n";
             o << "        " << T << "* hk_part = (" << T << "*)hydk_scratch(0, (hk_grid + 1) * 8, hk_s);\n";
-            o << "        " << K << "<<<(unsigned)hk_grid, " << B << ", 0, hk_s>>>(" << arg_list(ps) << sep << idx_args
-              << ", hk_part);\n";
-            o << "        hydk_check_launch(hk_site);\n";
+            launch(K, "(unsigned)hk_grid", std::to_string(B), "0", A + ", hk_part", 2);
             o << "        " << K << "_final<<<1, 256, 0, hk_s>>>(hk_part, hk_grid, " << sanitize(p.acc) << ", hk_part + hk_grid);\n";
             o << "        hydk_check_launch(hk_site);\n";
             o << "        hydk_read_back(&hk_result, hk_part + hk_grid, sizeof hk_result, hk_s, hk_site);\n";
-            o << "    }\n";
         } else if (p.form == Plan::ScatterInt) {
-            const std::string dst = sanitize(p.tri.dst);
-            o << "    const int64_t hk_ndst = hyd_shape_size(" << dst << "->shape);\n";
-            o << "    if (hk_n > 0 && hk_ndst > 0) {\n";
             o << "        if (hk_ndst <= 8192) {\n";
             o << "            const int hk_stride = (int)(hk_ndst | 1);\n";
             o << "            const int hk_copies = std::max(1, std::min(16, 8192 / hk_stride));\n";
             o << "            const int hk_ncopy = (int)std::max<int64_t>(1, std::min<int64_t>(32, 4096 / hk_ndst));\n";
             o << "            unsigned long long* hk_spread = (unsigned long long*)hydk_zero_scratch(hk_s);\n";
-            o << "            " << K << "_lds<<<(unsigned)hk_grid, " << B
-              << ", (size_t)hk_copies * hk_stride * 8, hk_s>>>(" << arg_list(ps) << sep << idx_args
-              << ", hk_ndst, hk_copies, hk_stride, hk_ncopy, hk_spread);\n";
-            o << "            hydk_check_launch(hk_site);\n";
+            launch(K + "_lds", "(unsigned)hk_grid", std::to_string(B), "(size_t)hk_copies * hk_stride * 8",
+                   A + ", hk_ndst, hk_copies, hk_stride, hk_ncopy, hk_spread", 3);
             o << "            const int hk_used = (int)std::min<int64_t>(hk_ncopy, hk_grid);\n";
-            o << "            hydk_spread_fold<<<(unsigned)((hk_ndst + 255) / 256), 256, 0, hk_s>>>(hk_spread, hk_used, hk_ndst, "
-                 "(unsigned long long*)hk_a_"
+            o << "            int hk_lanes = 1;\n";
+            o << "            while (hk_lanes < hk_used) hk_lanes <<= 1;\n";
+            o << "            hydk_spread_fold<<<(unsigned)((hk_ndst * hk_lanes + 255) / 256), 256, 0, hk_s>>>(hk_spread, hk_used, "
+                 "hk_lanes, hk_ndst, (unsigned long long*)hk_a_"
               << sanitize(p.tri.dst).substr(2) << ");\n";
             o << "            hydk_check_launch(hk_site);\n";
             o << "        } else {\n";
-            o << "            " << K << "_global<<<(unsigned)hk_grid, " << B << ", 0, hk_s>>>(" << arg_list(ps) << sep
-              << idx_args << ", hk_ndst);\n";
-            o << "            hydk_check_launch(hk_site);\n";
+            launch(K + "_global", "(unsigned)hk_grid", std::to_string(B), "0", A + ", hk_ndst", 3);
             o << "        }\n";
-            o << "    }\n";
         } else if (p.form == Plan::ScatterFloat) {
-            const std::string dst = sanitize(p.tri.dst);
-            o << "    const int64_t hk_ndst = hyd_shape_size(" << dst << "->shape);\n";
-            o << "    if (hk_n > 0 && hk_ndst > 0) {\n";
             o << "        unsigned char* hk_m = (unsigned char*)hydk_scratch(2, hk_n * 24, hk_s);\n";
             o << "        int64_t* hk_sidx = (int64_t*)hk_m;\n";
             o << "        double* hk_sval = (double*)(hk_m + hk_n * 8);\n";
             o << "        int64_t* hk_skeep = (int64_t*)(hk_m + hk_n * 16);\n";
-            o << "        " << K << "<<<(unsigned)hk_grid, " << B << ", 0, hk_s>>>(" << arg_list(ps) << sep << idx_args
-              << ", hk_sidx, hk_sval, hk_skeep);\n";
-            o << "        hydk_check_launch(hk_site);\n";
+            launch(K, "(unsigned)hk_grid", std::to_string(B), "0", A + ", hk_sidx, hk_sval, hk_skeep", 2);
             o << "        hydk_check(hydhip_scatter_guarded_add_f64(hk_sidx, hk_sval, hk_skeep, hk_n, hk_a_"
               << sanitize(p.tri.dst).substr(2) << ", hk_ndst, hk_s), hk_site);\n";
-            o << "    }\n";
         } else {
-            o << "    if (hk_n > 0) {\n";
-            o << "        " << K << "<<<(unsigned)hk_grid, " << B << ", 0, hk_s>>>(" << arg_list(ps) << sep << idx_args << ");\n";
-            o << "        hydk_check_launch(hk_site);\n";
-            o << "    }\n";
+            launch(K, "(unsigned)hk_grid", std::to_string(B), "0", A, 2);
         }
     }
+    o << "    }\n";
     for (const auto& a : p.arrays)
         if (a.second.written) o << "    hydk_device_written(" << sanitize(a.first) << ");\n";
     o << "    return " << (p.form == Plan::Reduce ? "hk_result" : "0") << ";\n";
@@ -1002,20 +1259,32 @@ std::string describe_plan(const Site& site, const Options& opt) {
     const Plan p = make_plan(site);
     std::ostringstream o;
     o << "form=" << form_name(p.form) << " iters=" << p.iters.size();
+    const bool one_d = p.form == Plan::Map1D || p.form == Plan::Reduce || p.form == Plan::ScatterInt ||
+                       p.form == Plan::ScatterFloat;
     if (p.form != Plan::Map2D) o << " block=" << opt.block << " unroll=" << opt.unroll;
-    else o << " block=64x4";
+    else o << " block=256x1 rows=" << (p.deferred ? 4 : 1);
+    if (one_d && opt.unroll % 2 == 0) o << " paired=1";
     if (p.wrapper) o << " wrapper=1";
     if (p.form == Plan::Map1D || p.form == Plan::MapND) o << " deferred=" << (p.deferred ? 1 : 0);
     if (p.uses_exp) o << " exp_lds=1";
+    if (!p.invariants.empty()) {
+        o << " hoisted=";
+        bool f = true;
+        for (const Stmt* s : p.invariant_stmts) o << (f ? "" : ",") << s->var, f = false;
+    }
+    if (!p.divisors.empty()) {
+        o << " divisors=";
+        bool f = true;
+        for (const auto& d : p.divisors) o << (f ? "" : ",") << d, f = false;
+    }
+    if (!p.tuples.empty()) o << " tuples=" << p.tuples.size();
     o << " arrays=";
     bool first = true;
     for (const auto& a : p.arrays) {
         o << (first ? "" : ",") << a.first << ":" << (a.second.read ? "r" : "") << (a.second.written ? "w" : "");
-        const bool sl = (p.form == Plan::Map1D || p.form == Plan::Reduce || p.form == Plan::ScatterInt ||
-                         p.form == Plan::ScatterFloat) && a.second.loads_at_iter && !a.second.written && a.second.read;
+        const bool sl = one_d && a.second.loads_at_iter && !a.second.written && a.second.read;
         const bool ss = p.form == Plan::Map1D && a.second.writes_at_iter && !a.second.read && a.second.written;
-        if (sl) o << "(stream)";
-        if (ss) o << "(stream)";
+        if (sl || ss) o << "(stream)";
         first = false;
     }
     if (p.form == Plan::Reduce) o << " op=" << (p.red_mul ? "mul" : "add") << " acc=" << p.acc;
@@ -1047,7 +1316,7 @@ std::string emit_signatures(const std::vector<Site>& sites) {
     std::ostringstream o;
     for (const Site& s : sites) {
         o << s.name << " " << (s.has_result ? tag(s.result.second) : std::string("i64")) << " :";
-        for (const auto& a : s.args) o << " " << tag(a.second);
+        for (const auto& a : s.args) o << " " << tag(a.second) << ":" << a.first;
         o << "\n";
     }
     return o.str();

@@ -141,7 +141,7 @@ std::vector<Site> read_sites(const std::string& text);
 // ---- the GPU-schedule pass and the emitter ----------------------------------------
 struct Options {
     int block = 256;          // threads per workgroup of the streaming forms
-    int unroll = 4;           // elements per thread per round (map and reduction forms)
+    int unroll = 8;           // elements per thread per round (map and reduction forms; even: paired)
     int waves_per_cu = 4;     // workgroups per CU of the streaming grid, x block/64
 };
 
@@ -149,8 +149,8 @@ struct Options {
 std::string emit_sites(const std::vector<Site>& sites, const Options& opt, const std::string& origin);
 // The C declarations of the launchers (for the printed C to include).
 std::string emit_header(const std::vector<Site>& sites);
-// One line per launcher, "name result : arg arg ...", each type one of i64
-// f64 tuple array: for hosts that bind the launchers at run time (ctypes).
+// One line per launcher, "name result : type:arg type:arg ...", each type one
+// of i64 f64 tuple array: for hosts that bind the launchers at run time.
 std::string emit_signatures(const std::vector<Site>& sites);
 
 // What the schedule pass decided for one site (also printed by --plan).

@@ -1,10 +1,11 @@
 """ctypes bindings of a library of hydgen-generated launchers.
 
 hydgen writes, beside the HIP source of a program's offloaded loops, one line
-per launcher: `name result : arg arg ...` with each type one of i64, f64,
-tuple (hyd_tuple_t by value) and array (hyd_array_t*).  Program() loads the
-compiled library and declares every launcher from those lines, so a Python
-host calls them exactly as the printed C does.  Arrays are the library's
+per launcher: `name result : type:arg type:arg ...` with each type one of
+i64, f64, tuple (hyd_tuple_t by value) and array (hyd_array_t*).  Program()
+loads the compiled library and declares every launcher from those lines, so
+a Python host calls them exactly as the printed C does, positionally or by
+argument name (Program.call).  Arrays are the library's
 hyd_array_t (hydhip.host_array / hyd_array_alloc_device); a launcher leaves
 the arrays it writes in HBM, and hyd_array_sync_host brings them back.
 """
@@ -26,14 +27,15 @@ _TYPES = {
 
 
 def parse_signatures(text):
-    """{name: (result type, [argument types])} from the signature lines."""
+    """{name: (result type, [(argument type, argument name)])} from the
+    signature lines."""
     out = {}
     for line in text.splitlines():
         if not line.strip():
             continue
-        head, _, args = line.partition(":")
+        head, _, args = line.partition(" : ")
         name, res = head.split()
-        out[name] = (res, args.split())
+        out[name] = (res, [tuple(a.split(":", 1)) for a in args.split()])
     return out
 
 
@@ -50,8 +52,17 @@ class Program:
         for name, (res, args) in self.signatures.items():
             fn = getattr(self._cdll, name)
             fn.restype = _TYPES[res]
-            fn.argtypes = [_TYPES[a] for a in args]
+            fn.argtypes = [_TYPES[t] for t, _ in args]
             self._fns[name] = fn
+
+    def call(self, name, env):
+        """Launcher `name` with its arguments taken from env by name (tuples
+        as Python sequences of ints)."""
+        args = []
+        for t, a in self.signatures[name][1]:
+            x = env[a]
+            args.append(hydhip.hyd_tuple(*x) if t == "tuple" else x)
+        return self._fns[name](*args)
 
     def __getattr__(self, name):
         fns = self.__dict__.get("_fns", {})

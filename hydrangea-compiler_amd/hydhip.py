@@ -205,6 +205,7 @@ _SIGS = {
     "hyd_array_device_data": (_P, [_PA]),
     "hyd_array_sync_host": (_INT, [_PA]),
     "hyd_array_device_data_keep": (_P, [_PA]),
+    "hyd_array_mark_device_written": (_INT, [_PA]),
     "hyd_array_enter_host": (_INT, [_PA]),
     "hyd_array_mark_host_dirty": (_INT, [_PA]),
     "hyd_array_is_device_resident": (_INT, [_PA]),

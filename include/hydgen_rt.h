@@ -25,52 +25,134 @@
 #include "hyd_exp.h"
 #include "hyd_hip_runtime.h"
 
-/* ---- device helpers ---------------------------------------------------- */
-__device__ static inline hyd_tuple_t hydk_tuple(int n, int64_t e0 = 0, int64_t e1 = 0, int64_t e2 = 0,
-                                                int64_t e3 = 0, int64_t e4 = 0, int64_t e5 = 0,
-                                                int64_t e6 = 0, int64_t e7 = 0) {
+/* ---- shape tuples (host and device) --------------------------------------
+ * The loops run over all HYD_MAX_DIMS slots with the rank as a guard, so
+ * that a tuple stays in registers (a loop bound by the rank would index the
+ * tuple dynamically, which puts it in scratch memory). */
+__host__ __device__ static inline hyd_tuple_t hydk_tuple(int n, int64_t e0 = 0, int64_t e1 = 0, int64_t e2 = 0,
+                                                         int64_t e3 = 0, int64_t e4 = 0, int64_t e5 = 0,
+                                                         int64_t e6 = 0, int64_t e7 = 0) {
     hyd_tuple_t t;
-    t.elems[0] = e0, t.elems[1] = e1, t.elems[2] = e2, t.elems[3] = e3;
-    t.elems[4] = e4, t.elems[5] = e5, t.elems[6] = e6, t.elems[7] = e7;
-    for (int i = n; i < HYD_MAX_DIMS; ++i) t.elems[i] = 0;
+    t.elems[0] = n > 0 ? e0 : 0, t.elems[1] = n > 1 ? e1 : 0, t.elems[2] = n > 2 ? e2 : 0;
+    t.elems[3] = n > 3 ? e3 : 0, t.elems[4] = n > 4 ? e4 : 0, t.elems[5] = n > 5 ? e5 : 0;
+    t.elems[6] = n > 6 ? e6 : 0, t.elems[7] = n > 7 ? e7 : 0;
     t.ndims = n;
     return t;
 }
 
-__device__ static inline int64_t hydk_shape_size(hyd_tuple_t s) {
+__host__ __device__ static inline int64_t hydk_shape_size(hyd_tuple_t s) {
     uint64_t size = 1;
-    for (int i = 0; i < s.ndims; ++i) size *= (uint64_t)s.elems[i];
+#pragma unroll
+    for (int i = 0; i < HYD_MAX_DIMS; ++i)
+        if (i < s.ndims) size *= (uint64_t)s.elems[i];
     return (int64_t)size;
 }
 
-__device__ static inline hyd_tuple_t hydk_shape_init(hyd_tuple_t s) {
+__host__ __device__ static inline hyd_tuple_t hydk_shape_init(hyd_tuple_t s) {
     hyd_tuple_t o = hydk_tuple(0);
     if (s.ndims <= 0) return o;
     o.ndims = s.ndims - 1;
-    for (int i = 0; i < o.ndims; ++i) o.elems[i] = s.elems[i];
+#pragma unroll
+    for (int i = 0; i < HYD_MAX_DIMS; ++i) o.elems[i] = i < o.ndims ? s.elems[i] : 0;
     return o;
 }
 
-__device__ static inline int64_t hydk_shape_last(hyd_tuple_t s) { return s.ndims <= 0 ? 0 : s.elems[s.ndims - 1]; }
+__host__ __device__ static inline int64_t hydk_shape_last(hyd_tuple_t s) {
+    int64_t last = 0;
+#pragma unroll
+    for (int i = 0; i < HYD_MAX_DIMS; ++i)
+        if (i == s.ndims - 1) last = s.elems[i];
+    return last;
+}
 
-__device__ static inline hyd_tuple_t hydk_flat_to_nd(int64_t flat, hyd_tuple_t s) {
+__host__ __device__ static inline hyd_tuple_t hydk_flat_to_nd(int64_t flat, hyd_tuple_t s) {
     hyd_tuple_t idx = hydk_tuple(s.ndims);
     int64_t rem = flat;
-    for (int i = s.ndims - 1; i >= 0; --i) {
-        idx.elems[i] = rem % s.elems[i];
-        rem /= s.elems[i];
-    }
+#pragma unroll
+    for (int i = HYD_MAX_DIMS - 1; i >= 0; --i)
+        if (i < s.ndims) {
+            idx.elems[i] = rem % s.elems[i];
+            rem /= s.elems[i];
+        }
     return idx;
 }
 
-__device__ static inline int64_t hydk_nd_to_flat(hyd_tuple_t nd, hyd_tuple_t s) {
+__host__ __device__ static inline int64_t hydk_nd_to_flat(hyd_tuple_t nd, hyd_tuple_t s) {
     uint64_t flat = 0, stride = 1;
-    for (int i = s.ndims - 1; i >= 0; --i) {
-        flat += (uint64_t)nd.elems[i] * stride;
-        stride *= (uint64_t)s.elems[i];
-    }
+#pragma unroll
+    for (int i = HYD_MAX_DIMS - 1; i >= 0; --i)
+        if (i < s.ndims) {
+            flat += (uint64_t)nd.elems[i] * stride;
+            stride *= (uint64_t)s.elems[i];
+        }
     return (int64_t)flat;
 }
+
+/* ---- division by a divisor that does not change over a loop ----------------
+ * C's truncating q = x / m and r = x % m (CodegenC.hs:2473-2474) by
+ * multiplication: Granlund & Montgomery, "Division by invariant integers
+ * using multiplication" (PLDI 1994), fig. 4.1, for the unsigned |x| / |m|:
+ * l = ceil(log2 |m|), magic = floor(2^64 (2^l - |m|) / |m|) + 1,
+ * t = mulhi(magic, n), q = (t + ((n - t) >> min(l, 1))) >> max(l - 1, 0),
+ * exact for every 0 <= n < 2^64; then the signs of C (the quotient negative
+ * when exactly one operand is, the remainder the sign of x).  The launcher
+ * computes the constants; m = 0 and m = INT64_MIN keep the plain operators. */
+typedef struct {
+    int64_t m;
+    uint64_t am, magic;
+    uint64_t mask;  /* |m| - 1 when |m| is a power of two (>= 2): mask and shift */
+    uint32_t sh1, sh2, log2;
+    int neg, fast;
+} hydk_divc_t;
+
+static inline hydk_divc_t hydk_make_divc(int64_t m) {
+    hydk_divc_t d;
+    d.m = m;
+    d.fast = m != 0 && m != INT64_MIN;
+    d.neg = m < 0;
+    d.am = d.fast ? (uint64_t)(m < 0 ? -m : m) : 1;
+    uint32_t l = 0;
+    while (l < 64 && ((uint64_t)1 << l) < d.am) ++l;
+    const unsigned __int128 num = ((unsigned __int128)(((unsigned __int128)1 << l) - d.am)) << 64;
+    d.magic = (uint64_t)(num / d.am) + 1;
+    d.sh1 = l < 1 ? l : 1;
+    d.sh2 = l > 0 ? l - 1 : 0;
+    d.mask = (d.am >= 2 && (d.am & (d.am - 1)) == 0) ? d.am - 1 : 0;
+    d.log2 = l;
+    return d;
+}
+
+__host__ __device__ static inline uint64_t hydk_udiv(uint64_t n, const hydk_divc_t& d) {
+    if (d.mask) return n >> d.log2;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint64_t t = __umul64hi(d.magic, n);
+#else
+    const uint64_t t = (uint64_t)(((unsigned __int128)d.magic * n) >> 64);
+#endif
+    return (t + ((n - t) >> d.sh1)) >> d.sh2;
+}
+
+__host__ __device__ static inline int64_t hydk_cdiv(int64_t x, const hydk_divc_t& d) {
+    if (!d.fast) return x / d.m;
+    const uint64_t n = x < 0 ? 0 - (uint64_t)x : (uint64_t)x;
+    const uint64_t q = hydk_udiv(n, d);
+    return ((x < 0) != (d.neg != 0)) ? (int64_t)(0 - q) : (int64_t)q;
+}
+
+__host__ __device__ static inline int64_t hydk_cmod(int64_t x, const hydk_divc_t& d) {
+    if (!d.fast) return x % d.m;
+    const uint64_t n = x < 0 ? 0 - (uint64_t)x : (uint64_t)x;
+    const uint64_t r = d.mask ? (n & d.mask) : n - hydk_udiv(n, d) * d.am;
+    return x < 0 ? (int64_t)(0 - r) : (int64_t)r;
+}
+
+/* ---- 16-byte accesses of streamed arrays -------------------------------- */
+typedef double hydk_v2d __attribute__((ext_vector_type(2)));
+typedef long long hydk_v2i __attribute__((ext_vector_type(2)));
+
+static inline int hydk_aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+
+/* ---- device helpers ---------------------------------------------------- */
 
 /* reduction combines: fp64 IEEE, int64 wrapping */
 __device__ static inline double hydk_add_f64(double a, double b) { return a + b; }
@@ -100,19 +182,24 @@ __device__ static inline void hydk_global_add(unsigned long long* p, uint64_t v)
 }
 
 /* dst[b] += the `used` spread copies of bin b (mod 2^64); the copies are
- * zeroed again for the next launch on the stream. */
+ * zeroed again for the next launch on the stream.  One thread per (bin,
+ * copy): `lanes` (a power of two >= used, <= 32) consecutive lanes hold the
+ * copies of one bin and add them by a shuffle butterfly, so the fold costs
+ * one memory round trip rather than `used` dependent ones. */
 template <int D = 0>
-__global__ __launch_bounds__(256) void hydk_spread_fold_t(unsigned long long* spread, int used, int64_t ndst,
+__global__ __launch_bounds__(256) void hydk_spread_fold_t(unsigned long long* spread, int used, int lanes, int64_t ndst,
                                                           unsigned long long* dst) {
-    const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (b >= ndst) return;
-    unsigned long long s = 0;
-    for (int c = 0; c < used; ++c) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t b = t / lanes;
+    const int c = (int)(t % lanes);
+    unsigned long long v = 0;
+    if (b < ndst && c < used) {
         unsigned long long* q = spread + (size_t)c * ndst + b;
-        s += *q;
+        v = *q;
         *q = 0;
     }
-    dst[b] += s;
+    for (int off = lanes >> 1; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (c == 0 && b < ndst) dst[b] += v;
 }
 #define hydk_spread_fold hydk_spread_fold_t<0>
 

@@ -84,7 +84,7 @@ def test_generated_header_compiles_as_c99(tmp_path):
 
 def test_signatures():
     r = hydgen(os.path.join(CFG_DIR, "dot.sites"), "--signatures", "/dev/stdout")
-    assert r.stdout.strip() == "hydk_dot_0 f64 : f64 i64 array array"
+    assert r.stdout.strip() == "hydk_dot_0 f64 : f64:acc i64:red_dim array:xs array:ys"
 
 
 # ---- refusals --------------------------------------------------------------------
