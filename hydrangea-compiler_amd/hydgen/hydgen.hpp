@@ -142,7 +142,9 @@ std::vector<Site> read_sites(const std::string& text);
 struct Options {
     int block = 256;          // threads per workgroup of the streaming forms
     int unroll = 8;           // elements per thread per round (map and reduction forms; even: paired)
-    int waves_per_cu = 4;     // workgroups per CU of the streaming grid, x block/64
+    int waves_per_cu = 8;     // workgroups per CU of the streaming grid, x block/64
+    int rows = 4;             // rows per round of the 2-D form (when its stores are deferred)
+    int band_rounds = 1;      // rounds per workgroup of the 2-D form
 };
 
 // The HIP source of every site (kernels and extern "C" launchers).

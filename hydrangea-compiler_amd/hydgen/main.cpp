@@ -1,12 +1,14 @@
 // main.cpp - the hydgen command line.
 //
 //   hydgen SITES [-o OUT.hip] [--header OUT.h] [--signatures OUT.sig] [--plan]
-//          [--unroll U] [--block B] [--waves-per-cu W]
+//          [--unroll U] [--block B] [--waves-per-cu W] [--rows R] [--band-rounds K]
 //
 // SITES holds one HipSite per line (the derived Show text Language.Hydrangea.
 // CodegenHIP writes).  -o writes the HIP source of every site, --header the C
 // declarations of their launchers, --signatures one line per launcher for
 // hosts that bind them at run time, --plan prints each site's schedule.
+// --unroll, --block and --waves-per-cu shape the streaming forms; --rows
+// (1..8) and --band-rounds (rounds per workgroup) the 2-D form.
 // Errors (an unsupported construct, a malformed line) print a message and
 // exit with status 1: the HIP backend then stops, as the C backend stops on
 // a C compiler error.
@@ -36,7 +38,7 @@ void spit(const std::string& path, const std::string& text) {
 int usage() {
     fprintf(stderr,
             "usage: hydgen SITES [-o OUT.hip] [--header OUT.h] [--signatures OUT.sig] [--plan] [--unroll U] [--block B] "
-            "[--waves-per-cu W]\n");
+            "[--waves-per-cu W] [--rows R] [--band-rounds K]\n");
     return 2;
 }
 
@@ -64,12 +66,15 @@ int main(int argc, char** argv) {
             sigs = v;
         } else if (!strcmp(a, "--plan")) {
             plan = true;
-        } else if (!strcmp(a, "--unroll") || !strcmp(a, "--block") || !strcmp(a, "--waves-per-cu")) {
+        } else if (!strcmp(a, "--unroll") || !strcmp(a, "--block") || !strcmp(a, "--waves-per-cu") ||
+                   !strcmp(a, "--rows") || !strcmp(a, "--band-rounds")) {
             const char* v = next();
             if (!v) return usage();
             const int x = atoi(v);
             if (!strcmp(a, "--unroll")) opt.unroll = x;
             else if (!strcmp(a, "--block")) opt.block = x;
+            else if (!strcmp(a, "--rows")) opt.rows = x;
+            else if (!strcmp(a, "--band-rounds")) opt.band_rounds = x;
             else opt.waves_per_cu = x;
         } else if (a[0] == '-' && a[1]) {
             return usage();
@@ -78,7 +83,8 @@ int main(int argc, char** argv) {
         }
     }
     if (!in || opt.unroll < 1 || opt.unroll > 16 || (opt.block != 64 && opt.block != 128 && opt.block != 256 &&
-                                                     opt.block != 512) || opt.waves_per_cu < 1)
+                                                     opt.block != 512) || opt.waves_per_cu < 1 || opt.rows < 1 ||
+        opt.rows > 8 || opt.band_rounds < 1)
         return usage();
     try {
         const std::vector<hydgen::Site> sites = hydgen::read_sites(slurp(in));
