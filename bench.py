@@ -608,6 +608,28 @@ def bench_batched(steps, warmup, cpu=True):
         ms, ev = timed_loop(step, max(3, steps // 4), 1, events=True)
         res[name] = dict(ms_per_batch=ms, event_ms=float(np.mean(ev)),
                          gbs=res["bytes"] / ms / 1e6, req_s=lens.size / ms * 1e3)
+    # the instances of one bin class alone (one batched launch per class, in
+    # the order of BATCH["bins"]): where the batch's time goes
+    by_class = {}
+    for nb in c["bins"]:
+        m = bins == nb
+        if not m.any():
+            continue
+        lc, bc, oc, hc = lens[m], bins[m], off[m], hoff[m]
+        cb = workloads.batch_algorithmic_bytes(lc, bc)
+
+        def step(ev, lc=lc, bc=bc, oc=oc, hc=hc):
+            if ev:
+                ev[0].record()
+            hydhip.batched_histogram(d, oc, lc, bc, hc, hist=hist, policy=hydhip.BATCH_TUNED)
+            if ev:
+                ev[1].record()
+
+        _, ev = timed_loop(step, max(3, steps // 4), 1, events=True)
+        ms = float(np.mean(ev))
+        by_class[str(nb)] = dict(instances=int(m.sum()), elements=int(lc.sum()), bytes=cb, event_ms=ms,
+                                 gbs=cb / ms / 1e6)
+    res["by_class"] = by_class
     # per-call plain launches: one synchronous hydhip_weighted_histogram_i64 per instance
     lib = hydhip.lib()
     s = hydhip.default_stream()

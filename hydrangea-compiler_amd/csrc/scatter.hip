@@ -250,6 +250,16 @@ __global__ __launch_bounds__(BLOCK) void scatter_global_kernel(Src src, int64_t 
 }
 
 constexpr int kBlock = 512;
+
+// Lets kernel K take up to `bytes` of dynamic LDS: set once per kernel (the
+// first call's bytes must be the largest any launch asks for), not per
+// launch - the attribute call costs host time on every histogram otherwise.
+template <auto K>
+hipError_t allow_lds(size_t bytes) {
+    static const hipError_t e =
+        hipFuncSetAttribute((const void*)K, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    return e;
+}
 constexpr size_t kLdsTarget = 64 * 1024;  // per workgroup: leaves room for 2 per CU
 
 // The LDS form's merge scratch: by default the stream's zeroed scratch
@@ -319,15 +329,13 @@ int launch_scatter(const Src& src, bool vec, int64_t n, int64_t nbins, int64_t* 
         }
         int accumulate = zero_first ? 0 : 1;
         if (vec) {
-            auto k = scatter_lds_kernel<kBlock, true, Src>;
-            HYD_TRY_HIP(hipFuncSetAttribute((const void*)k,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            k<<<(unsigned)blocks, kBlock, lds, s>>>(src, n, (int)nbins, copies, stride, ncopy, spread);
+            HYD_TRY_HIP((allow_lds<scatter_lds_kernel<kBlock, true, Src>>(lds_cap)));
+            scatter_lds_kernel<kBlock, true, Src><<<(unsigned)blocks, kBlock, lds, s>>>(src, n, (int)nbins, copies,
+                                                                                        stride, ncopy, spread);
         } else {
-            auto k = scatter_lds_kernel<kBlock, false, Src>;
-            HYD_TRY_HIP(hipFuncSetAttribute((const void*)k,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            k<<<(unsigned)blocks, kBlock, lds, s>>>(src, n, (int)nbins, copies, stride, ncopy, spread);
+            HYD_TRY_HIP((allow_lds<scatter_lds_kernel<kBlock, false, Src>>(lds_cap)));
+            scatter_lds_kernel<kBlock, false, Src><<<(unsigned)blocks, kBlock, lds, s>>>(src, n, (int)nbins, copies,
+                                                                                         stride, ncopy, spread);
         }
         HYD_CHECK_LAUNCH();
         const int used = (int)std::min<int64_t>(ncopy, blocks);
@@ -355,15 +363,11 @@ int launch_scatter(const Src& src, bool vec, int64_t n, int64_t nbins, int64_t* 
         auto ssrc = src.shared();
         using SSrc = decltype(ssrc);
         if (vec) {
-            auto k = scatter_split_kernel<kBlock, true, SSrc>;
-            HYD_TRY_HIP(hipFuncSetAttribute((const void*)k,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            k<<<grid, kBlock, lds, s>>>(ssrc, n, nbins, range, dst);
+            HYD_TRY_HIP((allow_lds<scatter_split_kernel<kBlock, true, SSrc>>(lds_cap)));
+            scatter_split_kernel<kBlock, true, SSrc><<<grid, kBlock, lds, s>>>(ssrc, n, nbins, range, dst);
         } else {
-            auto k = scatter_split_kernel<kBlock, false, SSrc>;
-            HYD_TRY_HIP(hipFuncSetAttribute((const void*)k,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            k<<<grid, kBlock, lds, s>>>(ssrc, n, nbins, range, dst);
+            HYD_TRY_HIP((allow_lds<scatter_split_kernel<kBlock, false, SSrc>>(lds_cap)));
+            scatter_split_kernel<kBlock, false, SSrc><<<grid, kBlock, lds, s>>>(ssrc, n, nbins, range, dst);
         }
     } else if (strategy == HYD_SCATTER_GLOBAL) {
         int64_t blocks = std::min<int64_t>(want_blocks, (int64_t)di.num_cus * 8);
