@@ -12,6 +12,10 @@
 //              writes the bin range with plain stores - no atomics, no zeroing.
 //              Instances with more bins than LDS holds get one tile per
 //              16384-bin range; each of them streams the instance.
+//   * MERGE:   OWNER over one piece of an instance much longer than a
+//              workgroup's share of the batch (one workgroup streams at a
+//              fraction of HBM's rate): the piece's bins are added into the
+//              pre-zeroed output with one global atomic per nonzero bin.
 //   * GLOBAL:  one 64-bit global atomic per element of an element chunk into
 //              the pre-zeroed output (the autotuner, tune.hip, can pick it per
 //              (bin class, size class); global atomics on gfx950 run at tens of
@@ -48,7 +52,7 @@ int tune_class_n(int64_t n);
 
 namespace {
 
-enum TileMode : int32_t { kModeOwner = 0, kModeGlobal = 2 };
+enum TileMode : int32_t { kModeOwner = 0, kModeMerge = 1, kModeGlobal = 2 };
 
 // Self-contained: one load gives a workgroup everything about its tile.
 struct Tile {
@@ -68,6 +72,8 @@ constexpr int kWords = 16384;                 // LDS words per workgroup: one bi
 constexpr int kXcds = 8;                      // XCDs of the MI355X
 constexpr int64_t kChunk = (int64_t)1 << 18;  // elements per GLOBAL tile
 constexpr int64_t kTileOverhead = 4096;       // fixed cost of a tile, in elements
+constexpr int64_t kMinPiece = (int64_t)1 << 16;  // elements: the shortest piece of a split instance
+constexpr int kPiecesPerQueue = 4;            // split so that each queue gets about this many pieces
 constexpr int kTeam = 4;                      // workgroups that stream one instance together
 
 __device__ __forceinline__ void lds_add(unsigned long long* p, uint64_t v) {
@@ -90,7 +96,7 @@ typedef const __attribute__((address_space(1))) int64_t GlobalElem;
 // stall the address path for about half the cycles).  Otherwise plain loads,
 // so that the other range tiles of the instance find the elements in L2.
 template <int BLOCK, bool ONCE, class Add>
-__device__ __forceinline__ void tile_elements(const int64_t* p_generic, int64_t len, Add add) {
+__device__ __forceinline__ void tile_elements(const int64_t* p_generic, int64_t len, const int64_t* spare, Add add) {
     if (len <= 0) return;
     GlobalElem* p = (GlobalElem*)p_generic;
     if (reinterpret_cast<uintptr_t>(p_generic) & 15) {
@@ -106,13 +112,21 @@ __device__ __forceinline__ void tile_elements(const int64_t* p_generic, int64_t 
     // flight): the next four pairs load while the current eight elements are
     // routed and added
     v2i A[4], B[4];
-    auto load = [&](v2i* r, int64_t at) {
+    // `spare`: 16 bytes the workgroup has just read (so in L2), the address
+    // of the set loaded after the last one - it keeps the load count static
+    // without fetching anything from HBM (a read-once stream's lines are not
+    // kept in L2, so reloading the current set's addresses would fetch them
+    // again: measured, 8.5% more HBM reads)
+    GlobalPair* dummy = (GlobalPair*)spare;
+    auto load = [&](v2i* r, bool real, int64_t at) {
+        GlobalPair* base = real ? p2 + at : dummy;
+        const int64_t step = real ? BLOCK : 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             if (ONCE)
-                r[k] = __builtin_nontemporal_load(&p2[at + k * BLOCK]);
+                r[k] = __builtin_nontemporal_load(&base[k * step]);
             else
-                r[k] = p2[at + k * BLOCK];
+                r[k] = base[k * step];
         }
     };
     auto run = [&](const v2i* r) {
@@ -122,20 +136,20 @@ __device__ __forceinline__ void tile_elements(const int64_t* p_generic, int64_t 
             add(r[k].y);
         }
     };
-    // The next set is loaded unconditionally (on the last round from the
-    // current set's own, valid, addresses; the values are then unused) so
-    // that the wait before the arithmetic counts only the current set.
+    // The next set is loaded unconditionally (after the last set from
+    // `spare`; the values are then unused) so that the wait before the
+    // arithmetic counts only the current set.
     if (q + 3 * BLOCK < npairs) {
-        load(A, q);
+        load(A, true, q);
         for (;;) {
             const bool nb = q + 7 * BLOCK < npairs;  // B = the four pairs after A
-            load(B, nb ? q + 4 * BLOCK : q);
+            load(B, nb, q + 4 * BLOCK);
             __builtin_amdgcn_sched_barrier(0);
             run(A);
             q += 4 * BLOCK;
             if (!nb) break;
             const bool na = q + 7 * BLOCK < npairs;  // A = the four pairs after B
-            load(A, na ? q + 4 * BLOCK : q);
+            load(A, na, q + 4 * BLOCK);
             __builtin_amdgcn_sched_barrier(0);
             run(B);
             q += 4 * BLOCK;
@@ -226,9 +240,9 @@ __global__ __launch_bounds__(kBlock) void batched_hist_kernel(const Tile* __rest
                 if ((uint64_t)b < (uint64_t)nb) global_add(&h[b], (uint64_t)x * (uint64_t)x);
             };
             if (pow2)
-                tile_elements<kBlock, true>(tile.src, tile.len, [&](int64_t x) { add(c_mod_pow2(x, tile.mod), x); });
+                tile_elements<kBlock, true>(tile.src, tile.len, qseg, [&](int64_t x) { add(c_mod_pow2(x, tile.mod), x); });
             else
-                tile_elements<kBlock, true>(tile.src, tile.len, [&](int64_t x) { add(c_mod(x, tile.mod), x); });
+                tile_elements<kBlock, true>(tile.src, tile.len, qseg, [&](int64_t x) { add(c_mod(x, tile.mod), x); });
             continue;
         }
         const int64_t lo = tile.lo;
@@ -259,24 +273,32 @@ __global__ __launch_bounds__(kBlock) void batched_hist_kernel(const Tile* __rest
                 if ((x >= 0 || r == 0) && i < w32) lds_add(&mine[i], (uint64_t)x * (uint64_t)x);
             };
             if (tile.once)
-                tile_elements<kBlock, true>(tile.src, tile.len, f);
+                tile_elements<kBlock, true>(tile.src, tile.len, qseg, f);
             else
-                tile_elements<kBlock, false>(tile.src, tile.len, f);
+                tile_elements<kBlock, false>(tile.src, tile.len, qseg, f);
         } else {
             auto f = [&](int64_t x) {
                 const int64_t b = c_mod(x, tile.mod) - lo;
                 if ((uint64_t)b < (uint64_t)width) lds_add(&mine[b], (uint64_t)x * (uint64_t)x);
             };
             if (tile.once)
-                tile_elements<kBlock, true>(tile.src, tile.len, f);
+                tile_elements<kBlock, true>(tile.src, tile.len, qseg, f);
             else
-                tile_elements<kBlock, false>(tile.src, tile.len, f);
+                tile_elements<kBlock, false>(tile.src, tile.len, qseg, f);
         }
         __syncthreads();
-        for (int b = threadIdx.x; b < width; b += kBlock) {
-            unsigned long long s = 0;
-            for (int c = 0; c < copies; ++c) s += lds[c * stride + b];
-            h[lo + b] = s;
+        if (tile.mode == kModeMerge) {
+            for (int b = threadIdx.x; b < width; b += kBlock) {
+                unsigned long long s = 0;
+                for (int c = 0; c < copies; ++c) s += lds[c * stride + b];
+                if (s) global_add(&h[lo + b], s);
+            }
+        } else {
+            for (int b = threadIdx.x; b < width; b += kBlock) {
+                unsigned long long s = 0;
+                for (int c = 0; c < copies; ++c) s += lds[c * stride + b];
+                h[lo + b] = s;
+            }
         }
         __syncthreads();  // LDS reused by the next tile
     }
@@ -305,7 +327,7 @@ struct Group {  // tiles that should run side by side on one XCD (an instance's 
 std::once_flag g_attr_once;
 hipError_t g_attr_err = hipSuccess;
 
-int64_t tile_cost(const Tile& t) { return t.len + (t.mode == kModeOwner ? t.width : 0) + kTileOverhead; }
+int64_t tile_cost(const Tile& t) { return t.len + (t.mode != kModeGlobal ? t.width : 0) + kTileOverhead; }
 
 }  // namespace
 
@@ -316,6 +338,10 @@ int batched_histogram_ptrs(const int64_t* const* src, const int64_t* len, const 
                            int64_t* out_tiles) {
     std::vector<Tile> flat;
     std::vector<Group> groups;
+    int64_t total_len = 0;
+    for (int64_t k = 0; k < count; ++k) total_len += std::max<int64_t>(len[k], 0);
+    const int64_t split_at =
+        std::max<int64_t>(kMinPiece, total_len / ((int64_t)device_info().num_cus * kPiecesPerQueue));
     std::vector<int64_t> zero;  // (dst pointer, bins) pairs
     TuneSnapshot tune;
     if (policy != 1 && policy != 2) tuned_batch_snapshot(&tune);
@@ -347,7 +373,6 @@ int batched_histogram_ptrs(const int64_t* const* src, const int64_t* len, const 
             mode = tune.mode[tune_class_bins(nb)][tune_class_n(n)];
             if (mode < 0) mode = rule_mode(n, nb);
         }
-        const int64_t first = (int64_t)flat.size();
         if (mode == 1) {
             zero.push_back((int64_t)reinterpret_cast<uintptr_t>(dst[k]));
             zero.push_back(nb);
@@ -357,14 +382,31 @@ int batched_histogram_ptrs(const int64_t* const* src, const int64_t* len, const 
             }
             continue;
         }
-        int64_t cost = 0;
+        // An instance much longer than a queue's share is cut into pieces,
+        // each its own LDS tiles whose bins are added into the zeroed
+        // destination: one workgroup streams at a fraction of HBM's rate,
+        // so one long instance on one workgroup would set the batch's end.
+        // Only for bins that fit one range: a piece of a multi-range
+        // instance would stream the elements once per range again and add
+        // every bin of every range with an atomic (measured: slower).
+        const int64_t pieces = (n > split_at && nb <= kWords) ? (n + split_at - 1) / split_at : 1;
         const int32_t once = nb <= kWords ? 1 : 0;  // one range tile reads the elements
-        for (int64_t lo = 0; lo < nb; lo += kWords) {
-            const int32_t w = (int32_t)std::min<int64_t>(kWords, nb - lo);
-            flat.push_back(Tile{src[k], d, n, nb, mod, (int32_t)lo, w, kModeOwner, once});
-            cost += tile_cost(flat.back());
+        if (pieces > 1) {
+            zero.push_back((int64_t)reinterpret_cast<uintptr_t>(dst[k]));
+            zero.push_back(nb);
         }
-        groups.push_back(Group{cost, first, (int64_t)flat.size() - first});
+        for (int64_t pc = 0; pc < pieces; ++pc) {
+            const int64_t st = n * pc / pieces, en = n * (pc + 1) / pieces;
+            const int64_t pfirst = (int64_t)flat.size();
+            int64_t cost = 0;
+            for (int64_t lo = 0; lo < nb; lo += kWords) {
+                const int32_t w = (int32_t)std::min<int64_t>(kWords, nb - lo);
+                flat.push_back(Tile{src[k] + st, d, en - st, nb, mod, (int32_t)lo, w,
+                                    pieces > 1 ? kModeMerge : kModeOwner, once});
+                cost += tile_cost(flat.back());
+            }
+            groups.push_back(Group{cost, pfirst, (int64_t)flat.size() - pfirst});
+        }
     }
     const int nq = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)flat.size(), device_info().num_cus));
     // Gang scheduling: teams of kTeam workgroups on one XCD (queue q runs on
