@@ -56,6 +56,27 @@ def test_batched_chunks_and_bin_ranges(hyd, orc, policy):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("policy", [0, 1])
+def test_batched_split_pieces(hyd, orc, policy):
+    # instances far longer than a queue's share with bins in one LDS range
+    # (up to exactly 16384) are cut into pieces whose bins are added into the
+    # zeroed output; pieces start at unaligned elements, and the batch mixes
+    # them with short and multi-range instances
+    lens = np.array([3_000_001, 17, 2_500_003, 1 << 21, 999_999, 5, 1_000_000], np.int64)
+    bins = np.array([5000, 64, 16384, 1, 64, 3, 65536], np.int64)
+    off, hoff = hyd.batch_offsets(lens, bins)
+    data = orc.splitmix_i64(int(lens.sum()), 31337, bits=40)
+    data[::13] = -data[::13]
+    d = hyd.DeviceArray.from_numpy(data)
+    print("batched split pieces policy=%d" % policy, flush=True)
+    # the output starts as garbage and is reused: each call must overwrite it
+    hist = hyd.DeviceArray.from_numpy(np.full(int(bins.sum()), 12345, np.int64))
+    for _ in range(2):
+        got = hyd.batched_histogram(d, off, lens, bins, hoff, hist=hist, policy=policy).numpy()
+    ref, _ = orc.batched_histograms(data, off, lens, bins)
+    assert np.array_equal(got, ref)
+
+
 @pytest.mark.parametrize("policy", [0, 1, 2])
 def test_batched_wide_values(hyd, orc, policy):
     # 63-bit values, half negated: routes through both remainder forms (mask
