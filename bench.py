@@ -173,6 +173,52 @@ def bench_copy(steps, warmup):
     return dict(ms=ms, bytes=16 * n, gbs=16 * n / ms / 1e6)
 
 
+def bench_shapes(steps, warmup):
+    """The streaming grid of the map chain and the copy over 2^27 fp64 at 1, 2,
+    4 and 8 workgroups per CU (set by hand, then removed), and what the
+    autotuner picks per size class.  Map chain and copy results do not depend on
+    the grid; the dot keeps its fixed grid (its partial count fixes the order of
+    its sum) and is not swept."""
+    n = 1 << 27
+    c = workloads.DOTMAP
+    L = hydhip.lib()
+    xs = hydhip.splitmix_f64(n, c["seed"], 0, symmetric=True)
+    ys = hydhip.splitmix_f64(n, c["seed"], n, symmetric=True)
+    out = hydhip.DeviceArray((n,), np.float64)
+    res = {}
+    for kind, name in ((hydhip.SHAPE_MAP_CHAIN, "map_chain"), (hydhip.SHAPE_COPY, "copy")):
+        row = {}
+        for pc in (1, 2, 4, 8):
+            hydhip.tune_shape_set(kind, n, pc)
+
+            def step(ev):
+                if ev:
+                    ev[0].record()
+                if kind == hydhip.SHAPE_MAP_CHAIN:
+                    hydhip._check(L.hydhip_map_chain_f64(xs.ptr, ys.ptr, n, out.ptr, None))
+                else:
+                    hydhip._check(L.hydhip_copy_f64(xs.ptr, n, out.ptr, None))
+                if ev:
+                    ev[1].record()
+
+            _, ev = timed_loop(step, steps, warmup, events=True)
+            row[str(pc)] = round(float(np.mean(ev)) * 1e3, 1)
+            hydhip.tune_shape_set(kind, n, 0)
+        best = min(row.values())
+        res[name] = dict(us_by_workgroups_per_cu=row, default=1, default_vs_best=round(row["1"] / best, 4))
+    t0 = time.perf_counter()
+    classes = hydhip.tune_shapes()
+    res["tuner"] = dict(classes=classes, seconds=round(time.perf_counter() - t0, 3),
+                        table=[dict(kind=["map_chain", "copy"][k], n_log2=nl, per_cu=pc)
+                               for k, nl, pc in hydhip.tune_shape_table()])
+    for k, nl, _ in hydhip.tune_shape_table():  # leave the defaults for the rest of the run
+        hydhip.tune_shape_set(k, 1 << nl, 0)
+    xs.free()
+    ys.free()
+    out.free()
+    return res
+
+
 def bench_dropin_host(reps=4):
     """The dot + map chain step through the drop-in exports with host-resident
     inputs (hyd_export_dot, hyd_export_map_chain; host sync on): each call
@@ -879,6 +925,9 @@ def main():
     if not only or "dropin" in only:
         log("drop-in with host arrays ...")
         kernels["dropin_host_arrays"] = bench_dropin_host()
+    if "shapes" in only:
+        log("launch shapes ...")
+        kernels["launch_shapes"] = bench_shapes(args.steps, args.warmup)
     if not only or "hist" in only:
         log("histogram ...")
         kernels["weighted_histogram"] = bench_hist(args.steps, args.warmup, cpu=not args.no_cpu)

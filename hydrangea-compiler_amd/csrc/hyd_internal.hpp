@@ -75,6 +75,10 @@ struct TuneSnapshot {
     int8_t mode[64][64];
 };
 
+// The tuned streaming grid (workgroups per CU) of a launch-shape kind
+// (HYD_SHAPE_*) for n elements, or dflt when that class is untuned.
+int tuned_per_cu(int kind, int64_t n, int dflt);
+
 }  // namespace hyd
 
 #define HYD_TRY_HIP(expr)                                                          \

@@ -361,7 +361,7 @@ extern "C" int hydhip_copy_f64(const double* src, int64_t n, double* dst, void* 
         return set_error(HYD_E_INVALID, "hydhip_copy_f64: needs an even count and 16-byte aligned pointers");
     if (n == 0) return HYD_OK;
     const int64_t npairs = n >> 1;
-    const int64_t blocks = stream_grid(npairs, kStreamDepth, kCopyPerCU);
+    const int64_t blocks = stream_grid(npairs, kStreamDepth, tuned_per_cu(HYD_SHAPE_COPY, n, kCopyPerCU));
     copy_kernel<kStreamDepth><<<(unsigned)blocks, kBlock, 0, resolve_stream(stream)>>>(src, npairs, dst);
     HYD_CHECK_LAUNCH();
     return HYD_OK;
@@ -403,7 +403,8 @@ extern "C" int hydhip_map_chain_f64(const double* xs, const double* ys, int64_t 
     if (n == 0) return HYD_OK;
     hipStream_t s = resolve_stream(stream);
     const bool vec = aligned16(xs) && aligned16(ys) && aligned16(out);
-    const int64_t blocks = stream_grid(vec ? (n >> 1) : n, vec ? kMapDepth : 1, kMapPerCU);
+    const int64_t blocks =
+        stream_grid(vec ? (n >> 1) : n, vec ? kMapDepth : 1, tuned_per_cu(HYD_SHAPE_MAP_CHAIN, n, kMapPerCU));
     if (vec)
         map_chain_kernel<true, kMapDepth><<<(unsigned)blocks, kBlock, 0, s>>>(xs, ys, n, out);
     else

@@ -5,8 +5,13 @@
 // <= 262144 elements, Parallelize.hs:466-503; fixed 32x32 polyhedral tiles,
 // Polyhedral.hs:200-204) with measurements on the device at hand.
 //
-// Table file (text, one class per line: "bin_log2 n_log2 mode"), default
-// $HYD_TUNE_CACHE or $HOME/.cache/hydhip/tune_<arch>.txt.
+// It also tunes launch shapes: the streaming grid (workgroups per CU) of the
+// map chain and the copy per size class, where results do not depend on the
+// grid.
+//
+// Table file (text, one class per line: "bin_log2 n_log2 mode", or for a
+// launch shape "-1-kind n_log2 workgroups_per_cu"), default $HYD_TUNE_CACHE or
+// $HOME/.cache/hydhip/tune_<arch>.txt.
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -79,6 +84,19 @@ int load_locked(const std::string& path) {
 int tune_class_bins(int64_t bins) { return ilog2_ceil(bins); }
 int tune_class_n(int64_t n) { return ilog2_floor(std::max<int64_t>(n, 1)); }
 
+// Launch-shape entries share the table: key (-1 - kind, floor log2 n), value
+// the streaming grid's workgroups per CU.  A call of size n takes the entry
+// of the largest tuned size class <= n's; without one, `dflt`.
+int tuned_per_cu(int kind, int64_t n, int dflt) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_loaded) load_locked(default_path());
+    auto it = g_table.upper_bound({-1 - kind, ilog2_floor(std::max<int64_t>(n, 1))});
+    if (it == g_table.begin()) return dflt;
+    --it;
+    if (it->first.first != -1 - kind || it->second < 1) return dflt;
+    return it->second;
+}
+
 // The table as a dense array (-1: untuned), under one lock.
 void tuned_batch_snapshot(TuneSnapshot* out) {
     memset(out->mode, -1, sizeof out->mode);
@@ -104,6 +122,21 @@ int hydhip_tune_set(int64_t bins, int64_t n, int mode) {
         g_table.erase(key);
     else
         g_table[key] = mode;
+    return HYD_OK;
+}
+
+// Sets the streaming grid of one launch-shape class by hand (per_cu <= 0
+// removes it).
+int hydhip_tune_shape_set(int kind, int64_t n, int per_cu) {
+    if (kind < 0 || kind >= HYD_SHAPE_KINDS || n < 1)
+        return set_error(HYD_E_INVALID, "hydhip_tune_shape_set: bad kind %d or size %lld", kind, (long long)n);
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_loaded) load_locked(default_path());
+    auto key = std::make_pair(-1 - kind, ilog2_floor(n));
+    if (per_cu <= 0)
+        g_table.erase(key);
+    else
+        g_table[key] = per_cu;
     return HYD_OK;
 }
 
@@ -195,12 +228,14 @@ static int tune_impl(const char* path, void* stream, int* measured) {
     return HYD_OK;
 }
 
-// Copies the table into out[3*i..] (bin_log2, n_log2, mode); returns entries.
+// Copies the batched-scatter classes into out[3*i..] (bin_log2, n_log2,
+// mode); returns entries.
 int hydhip_tune_table(int* out, int max_entries) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_loaded) load_locked(default_path());
     int i = 0;
     for (auto& kv : g_table) {
+        if (kv.first.first < 0) continue;  // launch shapes
         if (i >= max_entries) break;
         out[3 * i] = kv.first.first;
         out[3 * i + 1] = kv.first.second;
@@ -208,6 +243,91 @@ int hydhip_tune_table(int* out, int max_entries) {
         ++i;
     }
     return i;
+}
+
+// Copies the launch-shape classes into out[3*i..] (kind, n_log2, per_cu).
+int hydhip_tune_shape_table(int* out, int max_entries) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_loaded) load_locked(default_path());
+    int i = 0;
+    for (auto& kv : g_table) {
+        if (kv.first.first >= 0) continue;
+        if (i >= max_entries) break;
+        out[3 * i] = -1 - kv.first.first;
+        out[3 * i + 1] = kv.first.second;
+        out[3 * i + 2] = kv.second;
+        ++i;
+    }
+    return i;
+}
+
+// Measures the streaming grid (1, 2, 4 or 8 workgroups per CU) of the map
+// chain and the copy at 2^20, 2^23 and 2^26 elements, keeps the fastest of
+// each (best of 3 after a warm-up) and stores the table like
+// hydhip_tune_batched.  The dot is not tuned: its partial count fixes the
+// order of its sum, so its grid stays a fixed function of n and the device.
+// Returns the number of classes measured, or minus a status code.
+int hydhip_tune_shapes(const char* path, void* stream) {
+    int rc = ensure_init();
+    if (rc) return -rc;
+    hipStream_t s = resolve_stream(stream);
+    const int logs[] = {20, 23, 26};
+    const int per_cu[] = {1, 2, 4, 8};
+    const int64_t nmax = (int64_t)1 << 26;
+    double* buf = nullptr;
+    if (hipMalloc(&buf, (size_t)nmax * 3 * 8) != hipSuccess) {
+        set_error(HYD_E_OOM, "tune shapes: allocation failed");
+        return -HYD_E_OOM;
+    }
+    double *xs = buf, *ys = buf + nmax, *out = buf + 2 * nmax;
+    rc = hydhip_fill_splitmix_f64(xs, 2 * nmax, 0x73686170ULL, 0, 2.0, -1.0, s);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        rc = set_error(HYD_E_HIP, "tune shapes: event creation failed");
+    int measured = 0;
+    for (int kind = 0; kind < HYD_SHAPE_KINDS && !rc; ++kind) {
+        for (int nl : logs) {
+            const int64_t n = (int64_t)1 << nl;
+            float best = 1e30f;
+            int pick = 1;
+            for (int pc : per_cu) {
+                if ((rc = hydhip_tune_shape_set(kind, n, pc))) break;
+                float tbest = 1e30f;
+                for (int rep = 0; rep < 4 && !rc; ++rep) {
+                    if (hipEventRecord(e0, s) != hipSuccess) rc = set_error(HYD_E_HIP, "tune shapes: event");
+                    if (!rc) rc = kind == HYD_SHAPE_MAP_CHAIN ? hydhip_map_chain_f64(xs, ys, n, out, s)
+                                                             : hydhip_copy_f64(xs, n, out, s);
+                    if (!rc && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+                        rc = set_error(HYD_E_HIP, "tune shapes: event");
+                    float ms = 0;
+                    if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = set_error(HYD_E_HIP, "tune shapes: event");
+                    if (!rc && rep > 0 && ms < tbest) tbest = ms;
+                }
+                if (rc) break;
+                if (tbest < best) {
+                    best = tbest;
+                    pick = pc;
+                }
+            }
+            if (rc) break;
+            rc = hydhip_tune_shape_set(kind, n, pick);
+            ++measured;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(buf);
+    if (rc) return -rc;
+    std::lock_guard<std::mutex> lk(g_mu);
+    std::string p = path ? std::string(path) : default_path();
+    mkdirs(p);
+    FILE* f = fopen(p.c_str(), "w");
+    if (f) {
+        for (auto& kv : g_table) fprintf(f, "%d %d %d\n", kv.first.first, kv.first.second, kv.second);
+        fclose(f);
+    }
+    return measured;
 }
 
 }  // extern "C"

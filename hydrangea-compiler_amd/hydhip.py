@@ -165,6 +165,9 @@ _SIGS = {
     "hydhip_tune_set": (_INT, [_I64, _I64, _INT]),
     "hydhip_tune_clear": (_INT, []),
     "hydhip_tune_table": (_INT, [ctypes.POINTER(_INT), _INT]),
+    "hydhip_tune_shapes": (_INT, [ctypes.c_char_p, _P]),
+    "hydhip_tune_shape_set": (_INT, [_INT, _I64, _INT]),
+    "hydhip_tune_shape_table": (_INT, [ctypes.POINTER(_INT), _INT]),
     "hydhip_batched_dot_f64": (_INT, [_P, _P, _P, _I64, _D, _P, _P]),
     "hydhip_csv_parse_i64": (_INT, [_P, _I64, _I64, _P, ctypes.POINTER(CsvStatus), _P]),
     "hydhip_csv_parse_f64": (_INT, [_P, _I64, _I64, _P, ctypes.POINTER(CsvStatus), _P]),
@@ -528,6 +531,28 @@ def tune_batched(path=None, stream=None):
 def tune_table():
     buf = (_INT * (3 * 256))()
     n = lib().hydhip_tune_table(buf, 256)
+    return [(buf[3 * i], buf[3 * i + 1], buf[3 * i + 2]) for i in range(n)]
+
+
+SHAPE_MAP_CHAIN, SHAPE_COPY = 0, 1
+
+
+def tune_shapes(path=None, stream=None):
+    """Measure the streaming grid of the map chain and the copy per size
+    class; returns the number of classes measured."""
+    rc = lib().hydhip_tune_shapes(path.encode() if path else None, stream)
+    if rc < 0:
+        _check(-rc)
+    return rc
+
+
+def tune_shape_set(kind, n, per_cu):
+    _check(lib().hydhip_tune_shape_set(kind, n, per_cu))
+
+
+def tune_shape_table():
+    buf = (_INT * (3 * 64))()
+    n = lib().hydhip_tune_shape_table(buf, 64)
     return [(buf[3 * i], buf[3 * i + 1], buf[3 * i + 2]) for i in range(n)]
 
 

@@ -406,3 +406,27 @@ def test_coo_to_csr_outside_the_box(hyd, orc, nrows, ncols, colrange):
     rrp, rci, rcv, rm = orc.coo_to_csr(nrows, ncols, rows, cols, vals)
     assert m == rm
     assert bits_equal(rp.numpy(), rrp) and bits_equal(ci.numpy(), rci) and bits_equal(cv.numpy(), rcv)
+
+
+def test_launch_shapes_leave_results_unchanged(hyd, orc):
+    # the map chain and the copy at every streaming grid the tuner may pick
+    # give the same bits; the tuner stores one class per (kind, size) and the
+    # classes are removed again
+    n = (1 << 20) + 6
+    xs = hyd.splitmix_f64(n, 91, 0, symmetric=True)
+    ys = hyd.splitmix_f64(n, 91, n, symmetric=True)
+    ref = orc.map_chain(xs.numpy(), ys.numpy())
+    out = hyd.DeviceArray((n,), np.float64)
+    lib = hyd.lib()
+    for pc in (1, 2, 4, 8):
+        print("launch shape per_cu=%d" % pc, flush=True)
+        hyd.tune_shape_set(hyd.SHAPE_MAP_CHAIN, n, pc)
+        hyd.tune_shape_set(hyd.SHAPE_COPY, n, pc)
+        hyd.map_chain(xs, ys, out=out)
+        assert np.array_equal(out.numpy().view(np.uint64), ref.view(np.uint64))
+        hyd._check(lib.hydhip_copy_f64(xs.ptr, n, out.ptr, None))
+        assert np.array_equal(out.numpy(), xs.numpy())
+    assert (hyd.SHAPE_COPY, 20, 8) in hyd.tune_shape_table()
+    for k in (hyd.SHAPE_MAP_CHAIN, hyd.SHAPE_COPY):
+        hyd.tune_shape_set(k, n, 0)
+    assert not [e for e in hyd.tune_shape_table() if e[1] == 20]
