@@ -321,8 +321,9 @@ def bench_generated(steps, warmup, cpu=True):
     kernels run: the 8-op map chain and the dot over 2^27 fp64 (the dot's
     launcher returns the value, so its time includes the 8-byte read-back),
     the 64-bin weighted histogram over 2^24 int64, and one clamp heat step on
-    the 8192 x 8192 grid as the reference splits it (9 region loops).  Each
-    is checked against the oracle in the same run."""
+    the 8192 x 8192 grid as the reference splits it (9 region loops, through
+    their group launcher, and one launcher per region).  Each is checked
+    against the oracle in the same run."""
     import hydgen_lib
 
     prog = hydgen_lib.programs()
@@ -394,8 +395,11 @@ def bench_generated(steps, warmup, cpu=True):
     made += [u, uo]
     env.update(u=u, out=uo, shp=(hh, ww), n0=hh, n1=ww, n0m1=hh - 1, n1m1=ww - 1, bd_start_r=hh - 1,
                bd_start_c=ww - 1)
+    # the 9 region loops through their group launcher (one launch: their
+    # writes are disjoint boxes of out), and, for comparison, one by one
     regions = sorted(k for k in prog.launchers() if k.startswith("hydk_jacobi_step_"))
-    timed("jacobi_heat_step", 16 * hh * ww, regions)
+    timed("jacobi_heat_step_regions", 16 * hh * ww, regions)
+    timed("jacobi_heat_step", 16 * hh * ww, ["hydk_jacobi_step"])
     if cpu:
         import pyoracle
 

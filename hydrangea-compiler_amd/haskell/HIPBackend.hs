@@ -53,7 +53,7 @@ compileAndRunHIP mcc arts keepC compileOnly parallel = do
           driver = home </> "hydrangea-compiler_amd" </> "hydhip-cc"
           cc = pick mcc envCC
       writeFile cFile (hipC arts)
-      writeFile sitesFile (renderSites (hipSites arts))
+      writeFile sitesFile (renderSites (hipSites arts) (hipGroups arts))
       (code, _, err) <- readProcessWithExitCode driver (hydhipCcArgs cc parallel cFile sitesFile exe) ""
       case code of
         ExitFailure _ -> putStrLn "HIP compilation failed:" >> putStrLn err >> pure code

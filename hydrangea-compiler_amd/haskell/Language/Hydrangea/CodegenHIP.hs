@@ -30,6 +30,11 @@
 -- after it (the accumulator aside), and when it is not inside a parallel host
 -- loop or region.
 --
+-- Adjacent calls of map sites (loops without a result, nothing between
+-- them: the region loops of a split stencil) become one call of a group
+-- launcher, listed as a 'HipGroup' line; hydgen runs the members in one
+-- launch when it finds their accesses independent at run time.
+--
 -- Residency: arrays stay in HBM between sites.  The pass brings an array's
 -- host copy up to date (hyd_array_sync_host) before a host statement reads or
 -- writes its elements and before it is returned, marks host writes
@@ -49,6 +54,7 @@
 -- C oracle on the GPU (tests/test_gpu_hydgen.py, tests/test_gpu_hydhip_cc.py).
 module Language.Hydrangea.CodegenHIP
   ( HipSite (..)
+  , HipGroup (..)
   , HIPArtifacts (..)
   , hipPipeline
   , hipOffloadProgram
@@ -61,13 +67,23 @@ import Control.Monad.State.Strict (State, get, put, runState)
 import qualified Data.ByteString.Lazy.Char8 as BS
 import Data.List (nub, sort)
 import qualified Data.Map.Strict as M
-import Data.Maybe (isJust, mapMaybe)
+import Data.Maybe (isJust, isNothing, mapMaybe)
 import qualified Data.Set as S
 import Language.Hydrangea.CFG
 import Language.Hydrangea.CFGCore
 import Language.Hydrangea.CFGPipeline (metalPipelineWithTiling)
 import Language.Hydrangea.CFGTyping (TypeEnv, buildCallParamTypes, inferProgramReturnTypes, recoverProcTypeEnv)
 import Language.Hydrangea.CodegenC (CodegenArtifacts (..), CodegenOptions, codegenProgramWithOptions)
+
+-- | A run of adjacent map sites the printed C calls through one launcher
+-- (hydgen/cfg.cpp read_groups): hydgen runs them in one launch when their
+-- accesses are independent at run time - the split stencil's region loops
+-- are the common case - and in order otherwise.
+data HipGroup = HipGroup
+  { hgName :: CVar
+  , hgSites :: [CVar]
+  }
+  deriving (Eq, Show)
 
 -- | One offloaded loop, as hydgen reads it (hydgen/cfg.cpp).
 data HipSite = HipSite
@@ -80,8 +96,9 @@ data HipSite = HipSite
   deriving (Eq, Show)
 
 data HIPArtifacts = HIPArtifacts
-  { hipC :: String         -- ^ the printed C, launchers called by name
-  , hipSites :: [HipSite]  -- ^ the offloaded loops
+  { hipC :: String           -- ^ the printed C, launchers called by name
+  , hipSites :: [HipSite]    -- ^ the offloaded loops
+  , hipGroups :: [HipGroup]  -- ^ runs of adjacent map sites called as one
   , hipHeader :: Maybe String
   }
 
@@ -90,19 +107,20 @@ data HIPArtifacts = HIPArtifacts
 hipPipeline :: Program -> Program
 hipPipeline = metalPipelineWithTiling False
 
-renderSites :: [HipSite] -> String
-renderSites = unlines . map show
+renderSites :: [HipSite] -> [HipGroup] -> String
+renderSites sites groups = unlines (map show sites ++ map show groups)
 
 -- | The C backend's output for the offloaded program; the printed C includes
 -- the residency runtime, and the launchers' declarations come from hydgen
 -- (--header), force-included by the compile driver.
 codegenHIPWithOptions :: CodegenOptions -> Program -> Either String HIPArtifacts
 codegenHIPWithOptions opts prog = do
-  let (prog', sites) = hipOffloadProgram prog
+  let (prog', sites, groups) = hipOffloadProgram prog
   arts <- codegenProgramWithOptions opts prog'
   pure HIPArtifacts
     { hipC = "#include \"hyd_hip_runtime.h\"\n" ++ codegenSource arts
     , hipSites = sites
+    , hipGroups = groups
     , hipHeader = codegenHeader arts
     }
 
@@ -112,13 +130,14 @@ codegenHIPWithOptions opts prog = do
 -- | Once any loop of the program is offloaded, every procedure gets the
 -- residency calls: a procedure without sites may still read or write, on
 -- the host, arrays that another procedure's kernels left in HBM.
-hipOffloadProgram :: Program -> (Program, [HipSite])
+hipOffloadProgram :: Program -> (Program, [HipSite], [HipGroup])
 hipOffloadProgram prog@(Program procs) =
   let callTypes = inferProgramReturnTypes prog
       callParams = buildCallParamTypes callTypes procs
       results = map (offloadProc callTypes callParams) procs
-      sites = concatMap snd results
-   in if null sites then (prog, []) else (Program (map fst results), sites)
+      sites = concat [ss | (_, ss, _) <- results]
+      groups = concat [gs | (_, _, gs) <- results]
+   in if null sites then (prog, [], []) else (Program [q | (q, _, _) <- results], sites, groups)
 
 discarded :: CVar
 discarded = "_discarded"
@@ -126,19 +145,19 @@ discarded = "_discarded"
 callStmt :: CVar -> [Atom] -> Stmt
 callStmt fn args = SAssign discarded (RCall fn args)
 
-offloadProc :: M.Map CVar CType -> M.Map CVar [Maybe CType] -> Proc -> (Proc, [HipSite])
+offloadProc :: M.Map CVar CType -> M.Map CVar [Maybe CType] -> Proc -> (Proc, [HipSite], [HipGroup])
 offloadProc callTypes callParams p =
   let env = recoverProcTypeEnv callTypes callParams p
       isArr v = case M.lookup v env of
         Just (CTArray _) -> True
         _ -> False
       body = procBody p
-      (body', (_, sites)) = runState (goBlock env isArr (procName p) body body) (0 :: Int, [])
+      (body', (_, sites, groups)) = runState (goBlock env isArr (procName p) body body) (0 :: Int, [], [])
       siteArrays = S.fromList [v | s <- sites, (v, CTArray _) <- hsArgs s]
       entry = [callStmt "hyd_array_enter_host" [AVar v] | v <- procParams p, isArr v]
       body'' = entry ++ map (deviceAllocs env siteArrays) body'
       env' = M.insert discarded CTInt64 (procTypeEnv p)
-   in (p {procBody = body'', procTypeEnv = env'}, reverse sites)
+   in (p {procBody = body'', procTypeEnv = env'}, reverse sites, reverse groups)
 
 -- | Fresh 8-byte-element arrays that a site touches are allocated in HBM.
 deviceAllocs :: TypeEnv -> S.Set CVar -> Stmt -> Stmt
@@ -153,18 +172,18 @@ deviceAllocs env siteArrays = go
       SParallelRegion b -> SParallelRegion (map go b)
       _ -> s
 
-type Pass = State (Int, [HipSite])
+type Pass = State (Int, [HipSite], [HipGroup])
 
 -- | Statements of one block of procedure `pname`.  `whole` is the
 -- procedure body (for the live-out check).
 goBlock :: TypeEnv -> (CVar -> Bool) -> CVar -> [Stmt] -> [Stmt] -> Pass [Stmt]
-goBlock env isArr pname whole = fmap concat . mapM one
+goBlock env isArr pname whole stmts = mapM one stmts >>= groupRuns . concat
   where
     one s = case offloadableLoop env whole s of
       Just mk -> do
-        (k, sites) <- get
+        (k, sites, groups) <- get
         let site = mk (siteName k)
-        put (k + 1, site : sites)
+        put (k + 1, site : sites, groups)
         pure [siteCall site]
       Nothing -> case s of
         -- host loops and branches that hold sites: recurse (the iterate
@@ -191,6 +210,34 @@ goBlock env isArr pname whole = fmap concat . mapM one
       _ -> []
     recordArrays v = nub (concat [ [x | AVar x <- map snd fs, isArr x] | SAssign w (RRecord fs) <- flatten whole, w == v ]
                           ++ concat [ [x | AVar x <- [a, b], isArr x] | SAssign w (RPairMake _ _ a b) <- flatten whole, w == v ])
+
+-- | Runs of two or more adjacent calls of map sites (no result, nothing
+-- between them) become one call of a group launcher whose arguments are the
+-- members' arguments, each once, in order of first appearance - the order
+-- hydgen gives the group launcher's parameters.
+groupRuns :: [Stmt] -> Pass [Stmt]
+groupRuns stmts = do
+  (_, sites, _) <- get
+  let mapSites = S.fromList [hsName s | s <- sites, isNothing (hsResult s)]
+      isMapCall t = case t of
+        SAssign v (RCall fn _) -> v == discarded && fn `S.member` mapSites
+        _ -> False
+      go [] = pure []
+      go xs@(y : ys)
+        | isMapCall y =
+            let (run, rest) = span isMapCall xs
+             in case run of
+                  (_ : _ : _) -> do
+                    (k, ss, gs) <- get
+                    let names = [fn | SAssign _ (RCall fn _) <- run]
+                        gname = head names <> BS.pack "_group"
+                        args = nub (concat [as | SAssign _ (RCall _ as) <- run])
+                    put (k, ss, HipGroup gname names : gs)
+                    rest' <- go rest
+                    pure (callStmt gname args : rest')
+                  _ -> (run ++) <$> go rest
+        | otherwise = (y :) <$> go ys
+  go stmts
 
 sanitizeName :: CVar -> CVar
 sanitizeName = BS.map (\c -> if c `elem` (['a' .. 'z'] ++ ['A' .. 'Z'] ++ ['0' .. '9'] ++ "_") then c else '_')

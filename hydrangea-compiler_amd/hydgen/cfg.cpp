@@ -352,7 +352,21 @@ Site site_from_value(const Value& v) {
 
 std::vector<Site> read_sites(const std::string& text) {
     std::vector<Site> out;
-    for (const VPtr& v : parse_show_lines(text)) out.push_back(site_from_value(*v));
+    for (const VPtr& v : parse_show_lines(text))
+        if (!v->is_con("HipGroup")) out.push_back(site_from_value(*v));
+    return out;
+}
+
+std::vector<Group> read_groups(const std::string& text) {
+    std::vector<Group> out;
+    for (const VPtr& v : parse_show_lines(text)) {
+        if (!v->is_con("HipGroup")) continue;
+        if (v->kind != Value::Record) bad(*v, "expected a HipGroup record");
+        Group g;
+        g.name = str(v->field("hgName"));
+        for (const VPtr& m : list(v->field("hgSites"))) g.members.push_back(str(*m));
+        out.push_back(g);
+    }
     return out;
 }
 

@@ -35,6 +35,13 @@
 //     stored after) so a row's neighbours are read while still in L2 (the
 //     clamp-stencil region loops, Lowering.hs:2159-2172);
 //   * map, N-D   - flattened, the indices recovered by division.
+// A map form's body is a device function of its workgroup's grid position,
+// its arguments a struct the launcher's prep function fills; a group of
+// adjacent sites (HipGroup) gets one kernel that runs each member's body on
+// its own range of workgroups, launched when the members' accesses are
+// independent at run time (no member reads bytes another writes; shared
+// written arrays only through disjoint boxes of the iteration space), else
+// the members' launchers run in order.
 // Loop-invariant statements at the head of the body (the shape and bound
 // computations Lowering emits there) run once in the host launcher; an
 // integer division or remainder by a loop invariant becomes a multiply-shift
@@ -528,6 +535,126 @@ Plan make_plan(const Site& site) {
     return p;
 }
 
+// A written array whose every write (outside inner loops) lands at
+// (iter_0 + off_0, ..., iter_k + off_k) under one shape - a flat index of one
+// iterator for the 1-D form - with loop-invariant offsets: the elements the
+// site writes are then the box of its iteration space shifted by the offsets.
+// Group launchers use the boxes to prove two sites' writes disjoint.
+struct BoxWrite {
+    std::vector<Atom> offs;  // per iteration dimension
+    Atom shape;              // the flat index's shape (more than one dimension)
+};
+
+bool same_atom(const Atom& a, const Atom& b) {
+    if (a.kind != b.kind) return false;
+    if (a.kind == Atom::Var) return a.var == b.var;
+    if (a.kind == Atom::Int) return a.i == b.i;
+    return false;
+}
+
+std::map<std::string, BoxWrite> box_writes(const Site& site, const Plan& p) {
+    std::map<std::string, BoxWrite> out;
+    if (p.form != Plan::Map1D && p.form != Plan::Map2D && p.form != Plan::MapND) return out;
+    const Stmt& loop = *p.kloop;
+    std::map<std::string, int> count;
+    for_each_stmt(loop.body, [&](const Stmt& st) {
+        if (st.kind == Stmt::SAssign) count[st.var]++;
+        if (st.kind == Stmt::SLoop)
+            for (const auto& it : st.loop.iters) count[it] += 2;
+    });
+    std::map<std::string, const Stmt*> top;  // single top-level assignments
+    for (const Stmt& st : loop.body)
+        if (st.kind == Stmt::SAssign && count[st.var] == 1) top[st.var] = &st;
+    auto inv = [&](const Atom& a) {
+        if (a.kind == Atom::Int) return true;
+        if (!a.is_var()) return false;
+        if (p.invariants.count(a.var)) return true;
+        auto it = p.scalars.find(a.var);
+        return it != p.scalars.end() && a.var != p.acc;
+    };
+    // the offset of `a` from iterator d, when a = iter_d (+ invariant)
+    auto iter_off = [&](const Atom& a, size_t d, Atom* off) {
+        if (!a.is_var()) return false;
+        if (a.var == p.iters[d]) {
+            off->kind = Atom::Int;
+            off->i = 0;
+            return true;
+        }
+        auto it = top.find(a.var);
+        if (it == top.end()) return false;
+        const RHS& r = it->second->rhs;
+        if (r.kind != RHS::RBinOp || r.bop != BinOp::CAdd) return false;
+        if (r.args[0].is_var() && r.args[0].var == p.iters[d] && inv(r.args[1])) {
+            *off = r.args[1];
+            return true;
+        }
+        if (r.args[1].is_var() && r.args[1].var == p.iters[d] && inv(r.args[0])) {
+            *off = r.args[0];
+            return true;
+        }
+        return false;
+    };
+    const size_t nd = p.form == Plan::Map1D ? 1 : p.iters.size();
+    std::set<std::string> bad;
+    std::function<void(const std::vector<Stmt>&, int)> walk = [&](const std::vector<Stmt>& ss, int depth) {
+        for (const Stmt& st : ss) {
+            if (st.kind == Stmt::SArrayWrite) {
+                const std::string& arr = st.a0.var;
+                BoxWrite bw;
+                bool ok = depth == 0 && st.a1.is_var();
+                if (ok && nd == 1) {
+                    Atom off;
+                    ok = iter_off(st.a1, 0, &off);
+                    bw.offs.push_back(off);
+                } else if (ok) {
+                    auto it = top.find(st.a1.var);
+                    ok = it != top.end() && it->second->rhs.kind == RHS::RNdToFlat && it->second->rhs.args.size() == 2 &&
+                         it->second->rhs.args[0].is_var() && inv(it->second->rhs.args[1]);
+                    if (ok) {
+                        bw.shape = it->second->rhs.args[1];
+                        auto tt = top.find(it->second->rhs.args[0].var);
+                        ok = tt != top.end() && tt->second->rhs.kind == RHS::RTuple && tt->second->rhs.args.size() == nd;
+                        for (size_t d = 0; ok && d < nd; ++d) {
+                            Atom off;
+                            ok = iter_off(tt->second->rhs.args[d], d, &off);
+                            bw.offs.push_back(off);
+                        }
+                    }
+                }
+                if (!ok) {
+                    bad.insert(arr);
+                } else {
+                    auto prev = out.find(arr);
+                    if (prev == out.end()) {
+                        out[arr] = bw;
+                    } else {
+                        bool same = prev->second.offs.size() == bw.offs.size() && (nd == 1 || same_atom(prev->second.shape, bw.shape));
+                        for (size_t d = 0; same && d < bw.offs.size(); ++d) same = same_atom(prev->second.offs[d], bw.offs[d]);
+                        if (!same) bad.insert(arr);
+                    }
+                }
+            } else if (st.kind == Stmt::SIf) {
+                walk(st.body, depth);
+                walk(st.els, depth);
+            } else if (st.kind == Stmt::SLoop) {
+                walk(st.body, depth + 1);
+            }
+        }
+    };
+    walk(loop.body, 0);
+    for (const auto& b : bad) out.erase(b);
+    (void)site;
+    return out;
+}
+
+// A site whose body a group kernel can run: a map form without the exp
+// table (its LDS prologue) in 256-thread workgroups.
+bool groupable(const Plan& p, const Options& opt) {
+    if (p.uses_exp) return false;
+    if (p.form == Plan::Map2D) return true;
+    return (p.form == Plan::Map1D || p.form == Plan::MapND) && opt.block == 256;
+}
+
 // ---- printing ----------------------------------------------------------------------
 struct Printer {
     Printer(const Site& s, const Plan& plan, std::ostringstream& out) : site(s), p(plan), o(out) {
@@ -833,6 +960,30 @@ std::string arg_list(const std::vector<Param>& ps) {
     for (const auto& q : ps) s += (s.empty() ? "" : ", ") + q.launcher_arg;
     return s;
 }
+std::string name_list(const std::vector<Param>& ps, const std::string& prefix = "") {
+    std::string s;
+    for (const auto& q : ps) s += (s.empty() ? "" : ", ") + prefix + q.name;
+    return s;
+}
+// The index-range arguments of a map form's kernel, in order.
+std::vector<std::string> map_idx_vars(const Plan& p) {
+    std::vector<std::string> v;
+    if (p.form == Plan::Map2D) return {"hk_lo0", "hk_n0", "hk_lo1", "hk_n1"};
+    if (p.form == Plan::MapND) {
+        for (size_t d = 0; d < p.iters.size(); ++d) {
+            v.push_back("hk_lo" + std::to_string(d));
+            v.push_back("hk_n" + std::to_string(d));
+        }
+        v.push_back("hk_n");
+        return v;
+    }
+    return {"hk_lo", "hk_n"};
+}
+std::string map_idx_names(const Plan& p, const std::string& prefix = "") {
+    std::string s;
+    for (const auto& v : map_idx_vars(p)) s += (s.empty() ? "" : ", ") + prefix + v;
+    return s;
+}
 
 void exp_table_prologue(std::ostringstream& o, const Plan& p) {
     if (!p.uses_exp) return;
@@ -873,7 +1024,8 @@ const char* vec2_type(const CType& t) { return t.kind == CType::Double ? "hydk_v
 // 16-byte aligned) and in two 8-byte accesses otherwise, with the same
 // element-to-thread assignment either way (a reduction's order does not
 // depend on alignment).
-void stream_loop(std::ostringstream& o, Printer& pr, const Plan& p, int U, int B, bool paired, bool deferred) {
+void stream_loop(std::ostringstream& o, Printer& pr, const Plan& p, int U, int B, bool paired, bool deferred,
+                 const std::string& bx = "blockIdx.x", const std::string& gx = "gridDim.x") {
     const int T = U * B;
     const std::string emap = paired ? "hk_base + 2 * ((int64_t)(hk_u >> 1) * " + std::to_string(B) +
                                           " + threadIdx.x) + (hk_u & 1)"
@@ -939,8 +1091,8 @@ void stream_loop(std::ostringstream& o, Printer& pr, const Plan& p, int U, int B
             }
         }
     };
-    o << "    for (int64_t hk_base = (int64_t)blockIdx.x * " << T << "; hk_base < hk_n; hk_base += (int64_t)gridDim.x * " << T
-      << ") {\n";
+    o << "    for (int64_t hk_base = (int64_t)" << bx << " * " << T << "; hk_base < hk_n; hk_base += (int64_t)" << gx << " * "
+      << T << ") {\n";
     o << "        if (hk_base + " << T << " <= hk_n) {\n";
     if (paired && (!sload.empty() || !sstore.empty())) {
         o << "          if constexpr (PAIRS) {\n";
@@ -959,6 +1111,119 @@ void stream_loop(std::ostringstream& o, Printer& pr, const Plan& p, int U, int B
     o << "            }\n";
     o << "        }\n";
     o << "    }\n";
+}
+
+// Map forms: the kernel's arguments are computed by hk_prep_<site> into a
+// struct (hk_m_<site>, passed to the kernel) plus host-side facts
+// (hk_x_<site>: whether there is work, the 16-byte form, the write boxes), so
+// that the site's launcher and a group launcher share them.
+void emit_map_launcher(std::ostringstream& o, const Site& site, const Plan& p, const Options& opt, Printer& pr,
+                       const std::vector<Param>& ps, bool has_pairs, const std::vector<std::string>& streamed,
+                       const std::string& lparams, const std::string& largs) {
+    const std::string M = "hk_m_" + site.name, X = "hk_x_" + site.name, K = "hk_kernel_" + site.name;
+    const std::vector<std::string> iv = map_idx_vars(p);
+    const std::map<std::string, BoxWrite> boxes = box_writes(site, p);
+    const size_t nd = p.form == Plan::Map1D ? 1 : p.iters.size();
+    const int U = std::max(1, opt.unroll), B = opt.block;
+    o << "struct " << M << " {\n";
+    for (const auto& q : ps) o << "    " << q.ctype << " " << q.name << ";\n";
+    for (const auto& v : iv) o << "    int64_t " << v << ";\n";
+    o << "    int64_t hk_gx, hk_gy;\n};\n";
+    o << "struct " << X << " {\n    int hk_live, hk_pairs;\n";
+    for (const auto& b : boxes) {
+        const std::string base = sanitize(b.first).substr(2);
+        o << "    int64_t hk_bl_" << base << "[" << nd << "], hk_bh_" << base << "[" << nd << "];\n";
+        o << "    hyd_tuple_t hk_bs_" << base << ";\n";
+    }
+    o << "};\n";
+    o << "static inline " << M << " hk_prep_" << site.name << "(" << lparams << (lparams.empty() ? "" : ", ") << X
+      << "* hk_x) {\n";
+    o << "    const char* hk_site = \"" << site.name << "\";\n";
+    o << "    (void)hk_site;\n";
+    o << "    " << M << " hk_m;\n";
+    o << "    memset(&hk_m, 0, sizeof hk_m);\n";
+    o << "    memset(hk_x, 0, sizeof *hk_x);\n";
+    for (const auto& a : p.arrays) {
+        const std::string base = sanitize(a.first).substr(2);
+        o << "    " << c_type(a.second.elem) << "* hk_a_" << base << " = (" << c_type(a.second.elem)
+          << "*)hydk_device_data(" << sanitize(a.first) << ");\n";
+    }
+    const LoopSpec& ks = p.kloop->loop;
+    std::string nonempty;
+    if (p.form == Plan::Map2D) {
+        o << "    const int64_t hk_lo0 = " << pr.index_expr(ks.origins[0]) << ", hk_n0 = " << pr.index_expr(ks.bounds[0]) << ";\n";
+        o << "    const int64_t hk_lo1 = " << pr.index_expr(ks.origins[1]) << ", hk_n1 = " << pr.index_expr(ks.bounds[1]) << ";\n";
+        nonempty = "hk_n0 > 0 && hk_n1 > 0";
+    } else if (p.form == Plan::MapND) {
+        o << "    int64_t hk_n = 1;\n";
+        for (size_t d = 0; d < p.iters.size(); ++d) {
+            o << "    const int64_t hk_lo" << d << " = " << pr.index_expr(ks.origins[d]) << ", hk_n" << d << " = "
+              << pr.index_expr(ks.bounds[d]) << ";\n";
+            o << "    hk_n = hk_n" << d << " > 0 ? hk_n * hk_n" << d << " : 0;\n";
+        }
+        nonempty = "hk_n > 0";
+    } else {
+        o << "    const int64_t hk_lo = " << pr.index_expr(ks.origins[0]) << ", hk_n = " << pr.index_expr(ks.bounds[0]) << ";\n";
+        nonempty = "hk_n > 0";
+    }
+    o << "    if (" << nonempty << ") {\n";
+    pr.hoisted_stmts(2);
+    for (const std::string& d : p.divisors)
+        o << "        const hydk_divc_t hk_dc_" << sanitize(d).substr(2) << " = hydk_make_divc(" << sanitize(d) << ");\n";
+    if (has_pairs) {
+        o << "        const bool hk_pairs = true";
+        for (const auto& a : streamed) o << " && hydk_aligned16(hk_a_" << sanitize(a).substr(2) << " + hk_lo)";
+        o << ";\n";
+    }
+    if (p.form == Plan::Map2D) {
+        const int R = p.deferred ? opt.rows : 1;
+        const int64_t rows_per_band = (int64_t)R * opt.band_rounds;
+        o << "        const int64_t hk_gx = (hk_n1 + 255) / 256;\n";
+        o << "        const int64_t hk_gy = std::min<int64_t>(65535, (hk_n0 + " << rows_per_band - 1 << ") / " << rows_per_band
+          << ");\n";
+        o << "        if (hk_gx > 0x7fffffff) hydk_fail(hk_site, \"inner extent too large\");\n";
+    } else {
+        o << "        const int64_t hk_gx = hydk_stream_grid(hk_n, " << U * B << ", " << opt.waves_per_cu * 64 / B
+          << "), hk_gy = 1;\n";
+    }
+    for (const auto& q : ps) o << "        hk_m." << q.name << " = " << q.launcher_arg << ";\n";
+    for (const auto& v : iv) o << "        hk_m." << v << " = " << v << ";\n";
+    o << "        hk_m.hk_gx = hk_gx;\n        hk_m.hk_gy = hk_gy;\n";
+    o << "        hk_x->hk_live = 1;\n        hk_x->hk_pairs = " << (has_pairs ? "hk_pairs" : "0") << ";\n";
+    for (const auto& b : boxes) {
+        const std::string base = sanitize(b.first).substr(2);
+        for (size_t d = 0; d < nd; ++d) {
+            const std::string lo = nd == 1 ? "hk_lo" : "hk_lo" + std::to_string(d);
+            const std::string n = nd == 1 ? "hk_n" : "hk_n" + std::to_string(d);
+            o << "        hk_x->hk_bl_" << base << "[" << d << "] = " << lo << " + (int64_t)" << pr.atom(b.second.offs[d]) << ";\n";
+            o << "        hk_x->hk_bh_" << base << "[" << d << "] = hk_x->hk_bl_" << base << "[" << d << "] + " << n << ";\n";
+        }
+        if (nd > 1) o << "        hk_x->hk_bs_" << base << " = " << pr.atom(b.second.shape) << ";\n";
+    }
+    o << "    }\n";
+    o << "    return hk_m;\n}\n\n";
+
+    o << "extern \"C\" int64_t " << site.name << "(" << lparams << ") {\n";
+    o << "    const char* hk_site = \"" << site.name << "\";\n";
+    o << "    hipStream_t hk_s = hydk_stream();\n";
+    o << "    " << X << " hk_x;\n";
+    o << "    const " << M << " hk_m = hk_prep_" << site.name << "(" << largs << (largs.empty() ? "" : ", ") << "&hk_x);\n";
+    o << "    if (hk_x.hk_live) {\n";
+    const std::string grid = p.form == Plan::Map2D ? "dim3((unsigned)hk_m.hk_gx, (unsigned)hk_m.hk_gy)" : "(unsigned)hk_m.hk_gx";
+    const std::string block = p.form == Plan::Map2D ? "256" : std::to_string(B);
+    const std::string pl = name_list(ps, "hk_m.");
+    const std::string args = pl + (pl.empty() ? "" : ", ") + map_idx_names(p, "hk_m.");
+    if (has_pairs) {
+        o << "        if (hk_x.hk_pairs) " << K << "<true><<<" << grid << ", " << block << ", 0, hk_s>>>(" << args << ");\n";
+        o << "        else " << K << "<false><<<" << grid << ", " << block << ", 0, hk_s>>>(" << args << ");\n";
+    } else {
+        o << "        " << K << "<<<" << grid << ", " << block << ", 0, hk_s>>>(" << args << ");\n";
+    }
+    o << "        hydk_check_launch(hk_site);\n";
+    o << "    }\n";
+    for (const auto& a : p.arrays)
+        if (a.second.written) o << "    hydk_device_written(" << sanitize(a.first) << ");\n";
+    o << "    return 0;\n}\n\n";
 }
 
 void emit_site(std::ostringstream& o, const Site& site, const Options& opt) {
@@ -987,11 +1252,24 @@ void emit_site(std::ostringstream& o, const Site& site, const Options& opt) {
         idx_params += ", int64_t hk_n";
     }
 
-    if (p.form == Plan::Map1D || p.form == Plan::MapND) {
-        o << tmpl << "__global__ __launch_bounds__(" << B << ") void " << K << "(" << plist << sep << idx_params << ") {\n";
-        exp_table_prologue(o, p);
-        stream_loop(o, pr, p, U, B, paired, p.deferred);
+    // Map forms: the body is a device function of the workgroup's grid
+    // position, so that a group launcher can run several sites' bodies in one
+    // launch; the site's own kernel calls it with blockIdx / gridDim.
+    const std::string Kb = "hk_body_" + site.name;
+    const std::string gparams = "unsigned hk_bx, unsigned hk_by, unsigned hk_gx, unsigned hk_gy";
+    auto map_wrapper = [&](int block, const std::string& iparams, const std::string& inames) {
+        o << tmpl << "__global__ __launch_bounds__(" << block << ") void " << K << "(" << plist << sep << iparams << ") {\n";
+        o << "    " << Kb << (has_pairs ? "<PAIRS>" : "") << "(" << name_list(ps) << sep << inames
+          << ", blockIdx.x, blockIdx.y, gridDim.x, gridDim.y);\n";
         o << "}\n\n";
+    };
+    if (p.form == Plan::Map1D || p.form == Plan::MapND) {
+        o << tmpl << "__device__ __forceinline__ void " << Kb << "(" << plist << sep << idx_params << ", " << gparams
+          << ") {\n";
+        exp_table_prologue(o, p);
+        stream_loop(o, pr, p, U, B, paired, p.deferred, "hk_bx", "hk_gx");
+        o << "}\n\n";
+        map_wrapper(B, idx_params, map_idx_names(p));
     } else if (p.form == Plan::Map2D) {
         // 256-thread workgroups spanning 256 consecutive columns (the inner,
         // contiguous index: each row access of a workgroup is 2 KiB of one
@@ -1003,16 +1281,16 @@ void emit_site(std::ostringstream& o, const Site& site, const Options& opt) {
         // workgroups, balanced by the dispatcher, ran the 8192 x 8192 heat
         // step in 244 us against 292 us for one resident round of long bands
         const int R = p.deferred ? opt.rows : 1;
-        o << "__global__ __launch_bounds__(256) void " << K << "(" << plist << sep
-          << "int64_t hk_lo0, int64_t hk_n0, int64_t hk_lo1, int64_t hk_n1) {\n";
+        o << "__device__ __forceinline__ void " << Kb << "(" << plist << sep
+          << "int64_t hk_lo0, int64_t hk_n0, int64_t hk_lo1, int64_t hk_n1, " << gparams << ") {\n";
         exp_table_prologue(o, p);
-        o << "    const int64_t hk_c = (int64_t)blockIdx.x * 256 + threadIdx.x;\n";
+        o << "    const int64_t hk_c = (int64_t)hk_bx * 256 + threadIdx.x;\n";
         o << "    if (hk_c >= hk_n1) return;\n";
         o << "    const int64_t " << sanitize(p.iters[1]) << " = hk_lo1 + hk_c;\n";
-        o << "    const int64_t hk_band = ((hk_n0 + " << R << " - 1) / " << R << " + gridDim.y - 1) / gridDim.y * " << R
-          << ";\n";
-        o << "    const int64_t hk_r1 = min(hk_n0, ((int64_t)blockIdx.y + 1) * hk_band);\n";
-        o << "    for (int64_t hk_r = (int64_t)blockIdx.y * hk_band; hk_r < hk_r1; hk_r += " << R << ") {\n";
+        o << "    const int64_t hk_band = ((hk_n0 + " << R << " - 1) / " << R << " + (int64_t)hk_gy - 1) / (int64_t)hk_gy * "
+          << R << ";\n";
+        o << "    const int64_t hk_r1 = min(hk_n0, ((int64_t)hk_by + 1) * hk_band);\n";
+        o << "    for (int64_t hk_r = (int64_t)hk_by * hk_band; hk_r < hk_r1; hk_r += " << R << ") {\n";
         auto row_body = [&](int ind) {
             pr.declare_locals(ind);
             o << std::string((size_t)ind * 4, ' ') << "const int64_t " << sanitize(p.iters[0]) << " = hk_lo0 + hk_r + hk_u;\n";
@@ -1054,6 +1332,7 @@ void emit_site(std::ostringstream& o, const Site& site, const Options& opt) {
             o << "        }\n";
         }
         o << "    }\n}\n\n";
+        map_wrapper(256, "int64_t hk_lo0, int64_t hk_n0, int64_t hk_lo1, int64_t hk_n1", "hk_lo0, hk_n0, hk_lo1, hk_n1");
     } else if (p.form == Plan::Reduce) {
         const std::string T = c_type(p.acc_type);
         const bool fl = p.acc_type.kind == CType::Double;
@@ -1131,11 +1410,16 @@ void emit_site(std::ostringstream& o, const Site& site, const Options& opt) {
 
     // ---- launcher
     const std::string ret = site.has_result ? c_type(site.result.second) : "int64_t";
-    std::string lparams;
+    std::string lparams, largs;
     for (const auto& a : site.args) {
         lparams += lparams.empty() ? "" : ", ";
         lparams += a.second.is_array() ? "hyd_array_t* " + sanitize(a.first)
                                        : std::string(c_type(a.second)) + " " + sanitize(a.first);
+        largs += (largs.empty() ? "" : ", ") + sanitize(a.first);
+    }
+    if (p.form == Plan::Map1D || p.form == Plan::Map2D || p.form == Plan::MapND) {
+        emit_map_launcher(o, site, p, opt, pr, ps, has_pairs, streamed, lparams, largs);
+        return;
     }
     o << "extern \"C\" " << ret << " " << site.name << "(" << lparams << ") {\n";
     o << "    const char* hk_site = \"" << site.name << "\";\n";
@@ -1146,24 +1430,11 @@ void emit_site(std::ostringstream& o, const Site& site, const Options& opt) {
           << "*)hydk_device_data(" << sanitize(a.first) << ");\n";
     }
     if (p.form == Plan::Reduce) o << "    " << c_type(p.acc_type) << " hk_result = " << sanitize(p.acc) << ";\n";
+    // reductions and scatters (map forms: emit_map_launcher): one index
     const LoopSpec& ks = p.kloop->loop;
     std::string nonempty;
-    if (p.form == Plan::Map2D) {
-        o << "    const int64_t hk_lo0 = " << pr.index_expr(ks.origins[0]) << ", hk_n0 = " << pr.index_expr(ks.bounds[0]) << ";\n";
-        o << "    const int64_t hk_lo1 = " << pr.index_expr(ks.origins[1]) << ", hk_n1 = " << pr.index_expr(ks.bounds[1]) << ";\n";
-        nonempty = "hk_n0 > 0 && hk_n1 > 0";
-    } else if (p.form == Plan::MapND) {
-        o << "    int64_t hk_n = 1;\n";
-        for (size_t d = 0; d < p.iters.size(); ++d) {
-            o << "    const int64_t hk_lo" << d << " = " << pr.index_expr(ks.origins[d]) << ", hk_n" << d << " = "
-              << pr.index_expr(ks.bounds[d]) << ";\n";
-            o << "    hk_n = hk_n" << d << " > 0 ? hk_n * hk_n" << d << " : 0;\n";
-        }
-        nonempty = "hk_n > 0";
-    } else {
-        o << "    const int64_t hk_lo = " << pr.index_expr(ks.origins[0]) << ", hk_n = " << pr.index_expr(ks.bounds[0]) << ";\n";
-        nonempty = "hk_n > 0";
-    }
+    o << "    const int64_t hk_lo = " << pr.index_expr(ks.origins[0]) << ", hk_n = " << pr.index_expr(ks.bounds[0]) << ";\n";
+    nonempty = "hk_n > 0";
     if (p.form == Plan::ScatterInt || p.form == Plan::ScatterFloat) {
         o << "    const int64_t hk_ndst = hyd_shape_size(" << sanitize(p.tri.dst) << "->shape);\n";
         nonempty += " && hk_ndst > 0";
@@ -1192,24 +1463,8 @@ void emit_site(std::ostringstream& o, const Site& site, const Options& opt) {
         o << pad << "hydk_check_launch(hk_site);\n";
     };
     const std::string al = arg_list(ps);
-    if (p.form == Plan::Map2D) {
-        const int R = p.deferred ? opt.rows : 1;
-        const int64_t rows_per_band = (int64_t)R * opt.band_rounds;
-        o << "        const int64_t hk_gx = (hk_n1 + 255) / 256;\n";
-        o << "        const int64_t hk_gy = std::min<int64_t>(65535, (hk_n0 + " << rows_per_band - 1 << ") / " << rows_per_band
-          << ");\n";
-        o << "        if (hk_gx > 0x7fffffff) hydk_fail(hk_site, \"inner extent too large\");\n";
-        launch(K, "dim3((unsigned)hk_gx, (unsigned)hk_gy)", "256", "0", al + sep + "hk_lo0, hk_n0, hk_lo1, hk_n1", 2);
-    } else {
-        std::string idx_args;
-        if (p.form == Plan::MapND) {
-            for (size_t d = 0; d < p.iters.size(); ++d)
-                idx_args += "hk_lo" + std::to_string(d) + ", hk_n" + std::to_string(d) + ", ";
-            idx_args += "hk_n";
-        } else {
-            idx_args = "hk_lo, hk_n";
-        }
-        const std::string A = al + sep + idx_args;
+    {
+        const std::string A = al + sep + "hk_lo, hk_n";
         const bool scatter = p.form == Plan::ScatterInt || p.form == Plan::ScatterFloat;
         o << "        const int64_t hk_grid = hydk_stream_grid(hk_n, " << U * B << ", "
           << (scatter ? 2 : 1) * opt.waves_per_cu * 64 / B << ");\n";
@@ -1246,8 +1501,6 @@ void emit_site(std::ostringstream& o, const Site& site, const Options& opt) {
             launch(K, "(unsigned)hk_grid", std::to_string(B), "0", A + ", hk_sidx, hk_sval, hk_skeep", 2);
             o << "        hydk_check(hydhip_scatter_guarded_add_f64(hk_sidx, hk_sval, hk_skeep, hk_n, hk_a_"
               << sanitize(p.tri.dst).substr(2) << ", hk_ndst, hk_s), hk_site);\n";
-        } else {
-            launch(K, "(unsigned)hk_grid", std::to_string(B), "0", A, 2);
         }
     }
     o << "    }\n";
@@ -1298,15 +1551,182 @@ std::string describe_plan(const Site& site, const Options& opt) {
     return o.str();
 }
 
-std::string emit_sites(const std::vector<Site>& sites, const Options& opt, const std::string& origin) {
+namespace {
+
+// The 16-byte (PAIRS) variant exists for a 1-D map whose streamed arrays
+// are all 16-byte aligned at run time; a group kernel runs the plain one.
+bool site_has_pairs(const Plan& p, const Options& opt) {
+    if (p.form != Plan::Map1D || std::max(1, opt.unroll) % 2 != 0) return false;
+    for (const auto& a : p.arrays) {
+        const ArrayUse& u = a.second;
+        if (u.read && u.loads_at_iter && !u.written) return true;
+        if (p.deferred && u.written && u.writes_at_iter && !u.read) return true;
+    }
+    return false;
+}
+
+void emit_group(std::ostringstream& o, const Group& g, const std::map<std::string, const Site*>& by_name,
+                const Options& opt) {
+    std::vector<const Site*> ms;
+    for (const auto& m : g.members) ms.push_back(by_name.at(m));
+    std::vector<Plan> plans;
+    bool fusable = true;
+    for (const Site* m : ms) {
+        plans.push_back(make_plan(*m));
+        fusable = fusable && groupable(plans.back(), opt);
+    }
+    std::string lparams;
+    {
+        // the union of the members' arguments (group_site checked their types)
+        std::map<std::string, CType> seen;
+        std::vector<std::pair<std::string, CType>> args;
+        for (const Site* m : ms)
+            for (const auto& a : m->args)
+                if (seen.emplace(a.first, a.second).second) args.push_back(a);
+        for (const auto& a : args) {
+            lparams += lparams.empty() ? "" : ", ";
+            lparams += a.second.is_array() ? "hyd_array_t* " + sanitize(a.first)
+                                           : std::string(c_type(a.second)) + " " + sanitize(a.first);
+        }
+    }
+    auto member_args = [&](const Site* m) {
+        std::string s;
+        for (const auto& a : m->args) s += (s.empty() ? "" : ", ") + sanitize(a.first);
+        return s;
+    };
+    const size_t N = ms.size();
+    o << "// ---- group " << g.name << ":";
+    for (const auto& m : g.members) o << " " << m;
+    o << (fusable ? " (one launch when independent)" : " (in order: a member is not a 256-thread map form)") << "\n";
+    if (fusable) {
+        o << "__global__ __launch_bounds__(256) void hk_group_" << g.name << "(";
+        for (size_t k = 0; k < N; ++k) o << "hk_m_" << ms[k]->name << " hk_m" << k << ", ";
+        for (size_t k = 0; k < N; ++k) o << "int64_t hk_e" << k << (k + 1 < N ? ", " : "");
+        o << ") {\n";
+        o << "    const int64_t hk_b = blockIdx.x;\n";
+        for (size_t k = 0; k < N; ++k) {
+            const std::string mk = "hk_m" + std::to_string(k);
+            const std::vector<Param> ps = kernel_params(*ms[k], plans[k]);
+            const std::string pl = name_list(ps, mk + ".");
+            o << "    if (hk_b < hk_e" << k << ") {\n";
+            o << "        const unsigned hk_l = (unsigned)(hk_b" << (k ? " - hk_e" + std::to_string(k - 1) : std::string()) << ");\n";
+            o << "        hk_body_" << ms[k]->name << (site_has_pairs(plans[k], opt) ? "<false>" : "") << "(" << pl
+              << (pl.empty() ? "" : ", ") << map_idx_names(plans[k], mk + ".") << ", hk_l % (unsigned)" << mk
+              << ".hk_gx, hk_l / (unsigned)" << mk << ".hk_gx, (unsigned)" << mk << ".hk_gx, (unsigned)" << mk << ".hk_gy);\n";
+            o << "        return;\n    }\n";
+        }
+        o << "}\n\n";
+    }
+    o << "extern \"C\" int64_t " << g.name << "(" << lparams << ") {\n";
+    if (!fusable) {
+        for (const Site* m : ms) o << "    (void)" << m->name << "(" << member_args(m) << ");\n";
+        o << "    return 0;\n}\n\n";
+        return;
+    }
+    o << "    const char* hk_site = \"" << g.name << "\";\n";
+    o << "    hipStream_t hk_s = hydk_stream();\n";
+    size_t naccs = 0;
+    for (size_t k = 0; k < N; ++k) {
+        const std::string ma = member_args(ms[k]);
+        o << "    hk_x_" << ms[k]->name << " hk_x" << k << ";\n";
+        o << "    const hk_m_" << ms[k]->name << " hk_m" << k << " = hk_prep_" << ms[k]->name << "(" << ma
+          << (ma.empty() ? "" : ", ") << "&hk_x" << k << ");\n";
+        naccs += plans[k].arrays.size();
+    }
+    o << "    hydk_acc_t hk_acc[" << std::max<size_t>(naccs, 1) << "];\n";
+    o << "    int hk_na = 0;\n";
+    for (size_t k = 0; k < N; ++k) {
+        const Plan& p = plans[k];
+        const auto boxes = box_writes(*ms[k], p);
+        const size_t nd = p.form == Plan::Map1D ? 1 : p.iters.size();
+        const std::string xk = "hk_x" + std::to_string(k), mk = "hk_m" + std::to_string(k);
+        o << "    if (" << xk << ".hk_live) {\n";
+        for (const auto& a : p.arrays) {
+            if (!a.second.read && !a.second.written) continue;
+            const std::string base = sanitize(a.first).substr(2);
+            const bool boxed = a.second.written && boxes.count(a.first);
+            o << "        hk_acc[hk_na++] = hydk_acc(" << k << ", " << mk << ".hk_a_" << base << ", hyd_shape_size("
+              << sanitize(a.first) << "->shape) * (int64_t)sizeof(" << c_type(a.second.elem) << "), "
+              << (a.second.read ? 1 : 0) << ", " << (a.second.written ? 1 : 0) << ", " << (boxed ? 1 : 0) << ", " << nd
+              << ", " << (boxed ? xk + ".hk_bl_" + base : std::string("nullptr")) << ", "
+              << (boxed ? xk + ".hk_bh_" + base : std::string("nullptr")) << ", "
+              << (boxed ? "&" + xk + ".hk_bs_" + base : std::string("nullptr")) << ");\n";
+        }
+        o << "    }\n";
+    }
+    for (size_t k = 0; k < N; ++k) {
+        const std::string xk = "hk_x" + std::to_string(k), mk = "hk_m" + std::to_string(k);
+        o << "    const int64_t hk_e" << k << " = " << (k ? "hk_e" + std::to_string(k - 1) + " + " : std::string()) << "("
+          << xk << ".hk_live ? " << mk << ".hk_gx * " << mk << ".hk_gy : 0);\n";
+    }
+    o << "    const bool hk_fuse = hydk_accesses_independent(hk_acc, hk_na) && hk_e" << N - 1 << " <= 0x7fffffff && (";
+    for (size_t k = 0; k < N; ++k) o << (k ? " + " : "") << "sizeof(hk_m" << k << ")";
+    o << ") <= 3072;\n";
+    o << "    if (!hk_fuse) {\n";
+    for (const Site* m : ms) o << "        (void)" << m->name << "(" << member_args(m) << ");\n";
+    o << "        return 0;\n    }\n";
+    o << "    if (hk_e" << N - 1 << " > 0) {\n";
+    o << "        hk_group_" << g.name << "<<<(unsigned)hk_e" << N - 1 << ", 256, 0, hk_s>>>(";
+    for (size_t k = 0; k < N; ++k) o << "hk_m" << k << ", ";
+    for (size_t k = 0; k < N; ++k) o << "hk_e" << k << (k + 1 < N ? ", " : "");
+    o << ");\n";
+    o << "        hydk_check_launch(hk_site);\n    }\n";
+    std::set<std::string> written;
+    for (size_t k = 0; k < N; ++k)
+        for (const auto& a : plans[k].arrays)
+            if (a.second.written && written.insert(a.first).second)
+                o << "    hydk_device_written(" << sanitize(a.first) << ");\n";
+    o << "    return 0;\n}\n\n";
+}
+
+}  // namespace
+
+Site group_site(const Group& g, const std::vector<Site>& sites) {
+    Site gs;
+    gs.name = g.name;
+    if (sites.empty()) return gs;
+    std::map<std::string, const Site*> by_name;
+    for (const Site& s : sites) by_name[s.name] = &s;
+    std::map<std::string, CType> seen;
+    for (const auto& m : g.members) {
+        auto it = by_name.find(m);
+        if (it == by_name.end()) throw Error("group " + g.name + ": no site named " + m);
+        for (const auto& a : it->second->args) {
+            auto s = seen.find(a.first);
+            if (s == seen.end()) {
+                seen[a.first] = a.second;
+                gs.args.push_back(a);
+            } else if (s->second.show() != a.second.show()) {
+                throw Error("group " + g.name + ": argument " + a.first + " has two types");
+            }
+        }
+    }
+    return gs;
+}
+
+std::string emit_sites(const std::vector<Site>& sites, const std::vector<Group>& groups, const Options& opt,
+                       const std::string& origin) {
     std::ostringstream o;
     o << "// Generated by hydgen from " << origin << ": one kernel and launcher per offloaded loop.\n"
          "// Compile with hipcc --offload-arch=gfx950 -O3 -ffp-contract=off and link with -lhydhip.\n"
          "#include \"hydgen_rt.h\"\n\n";
     std::set<std::string> names;
+    std::map<std::string, const Site*> by_name;
     for (const Site& s : sites) {
         if (!names.insert(s.name).second) throw Error("two sites named " + s.name);
+        by_name[s.name] = &s;
         emit_site(o, s, opt);
+    }
+    std::set<std::string> grouped;
+    for (const Group& g : groups) {
+        if (!names.insert(g.name).second) throw Error("group " + g.name + ": the name is taken");
+        if (g.members.empty()) throw Error("group " + g.name + ": no members");
+        for (const auto& m : g.members) {
+            if (!by_name.count(m)) throw Error("group " + g.name + ": no site named " + m);
+            if (!grouped.insert(m).second) throw Error("group " + g.name + ": site " + m + " is in two groups");
+        }
+        (void)group_site(g, sites);  // the members' arguments agree
+        emit_group(o, g, by_name, opt);
     }
     return o.str();
 }

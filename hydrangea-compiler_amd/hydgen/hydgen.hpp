@@ -138,6 +138,19 @@ struct Site {
 Site site_from_value(const Value& v);
 std::vector<Site> read_sites(const std::string& text);
 
+// Consecutive sites the printed C calls as one launcher (a HipGroup line:
+// hgName, hgSites in program order).  The group launcher runs the members in
+// one launch when they are map forms whose accesses it finds independent at
+// run time, and calls them in order otherwise.
+struct Group {
+    std::string name;
+    std::vector<std::string> members;
+};
+std::vector<Group> read_groups(const std::string& text);
+// The group's launcher as a site (its name and the members' arguments, each
+// once, in order of first appearance), for the header and signatures.
+Site group_site(const Group& g, const std::vector<Site>& sites);
+
 // ---- the GPU-schedule pass and the emitter ----------------------------------------
 struct Options {
     int block = 256;          // threads per workgroup of the streaming forms
@@ -148,7 +161,8 @@ struct Options {
 };
 
 // The HIP source of every site (kernels and extern "C" launchers).
-std::string emit_sites(const std::vector<Site>& sites, const Options& opt, const std::string& origin);
+std::string emit_sites(const std::vector<Site>& sites, const std::vector<Group>& groups, const Options& opt,
+                       const std::string& origin);
 // The C declarations of the launchers (for the printed C to include).
 std::string emit_header(const std::vector<Site>& sites);
 // One line per launcher, "name result : type:arg type:arg ...", each type one

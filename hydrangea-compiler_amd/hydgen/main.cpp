@@ -4,7 +4,8 @@
 //          [--unroll U] [--block B] [--waves-per-cu W] [--rows R] [--band-rounds K]
 //
 // SITES holds one HipSite per line (the derived Show text Language.Hydrangea.
-// CodegenHIP writes).  -o writes the HIP source of every site, --header the C
+// CodegenHIP writes), and a HipGroup line per run of sites the printed C
+// calls through one group launcher.  -o writes the HIP source of every site, --header the C
 // declarations of their launchers, --signatures one line per launcher for
 // hosts that bind them at run time, --plan prints each site's schedule.
 // --unroll, --block and --waves-per-cu shape the streaming forms; --rows
@@ -87,12 +88,22 @@ int main(int argc, char** argv) {
         opt.rows > 8 || opt.band_rounds < 1)
         return usage();
     try {
-        const std::vector<hydgen::Site> sites = hydgen::read_sites(slurp(in));
-        if (plan)
+        const std::string text = slurp(in);
+        const std::vector<hydgen::Site> sites = hydgen::read_sites(text);
+        const std::vector<hydgen::Group> groups = hydgen::read_groups(text);
+        std::vector<hydgen::Site> callable = sites;  // what the printed C may call
+        for (const auto& g : groups) callable.push_back(hydgen::group_site(g, sites));
+        if (plan) {
             for (const auto& s : sites) printf("%s: %s\n", s.name.c_str(), hydgen::describe_plan(s, opt).c_str());
-        if (!out.empty()) spit(out, hydgen::emit_sites(sites, opt, in));
-        if (!header.empty()) spit(header, hydgen::emit_header(sites));
-        if (!sigs.empty()) spit(sigs, hydgen::emit_signatures(sites));
+            for (const auto& g : groups) {
+                printf("%s: group of", g.name.c_str());
+                for (const auto& m : g.members) printf(" %s", m.c_str());
+                printf("\n");
+            }
+        }
+        if (!out.empty()) spit(out, hydgen::emit_sites(sites, groups, opt, in));
+        if (!header.empty()) spit(header, hydgen::emit_header(callable));
+        if (!sigs.empty()) spit(sigs, hydgen::emit_signatures(callable));
     } catch (const std::exception& e) {
         fprintf(stderr, "hydgen: %s\n", e.what());
         return 1;

@@ -19,6 +19,7 @@
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include <algorithm>
 
@@ -216,6 +217,67 @@ static inline void hydk_check(int rc, const char* site) {
 static inline void hydk_check_launch(const char* site) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) hydk_fail(site, hipGetErrorString(e));
+}
+
+/* ---- group launches --------------------------------------------------------
+ * One array access of one member of a group: its bytes [lo, hi), whether
+ * the member reads and writes it, and - when every write of the member lands
+ * in a box of its iteration space - that box per dimension, [blo, bhi), and
+ * (for more than one dimension) the shape of the flat index. */
+typedef struct {
+    int member;
+    uintptr_t lo, hi;
+    int read, written, boxed, nd;
+    int64_t blo[HYD_MAX_DIMS], bhi[HYD_MAX_DIMS];
+    hyd_tuple_t shape;
+} hydk_acc_t;
+
+static inline hydk_acc_t hydk_acc(int member, const void* p, int64_t bytes, int read, int written, int boxed, int nd,
+                                  const int64_t* blo, const int64_t* bhi, const hyd_tuple_t* shape) {
+    hydk_acc_t a;
+    memset(&a, 0, sizeof a);
+    a.member = member;
+    a.lo = (uintptr_t)p;
+    a.hi = a.lo + (uintptr_t)(bytes > 0 ? bytes : 0);
+    a.read = read;
+    a.written = written;
+    a.boxed = boxed && nd >= 1 && nd <= HYD_MAX_DIMS;
+    a.nd = nd;
+    for (int d = 0; a.boxed && d < nd; ++d) {
+        a.blo[d] = blo[d];
+        a.bhi[d] = bhi[d];
+    }
+    if (shape) a.shape = *shape;
+    /* more than one dimension: the flat index is one-to-one only inside the shape */
+    if (a.boxed && nd > 1) {
+        if (a.shape.ndims != nd) a.boxed = 0;
+        for (int d = 0; a.boxed && d < nd; ++d)
+            if (a.blo[d] < 0 || a.bhi[d] > a.shape.elems[d]) a.boxed = 0;
+    }
+    return a;
+}
+
+/* True when the members' kernels may run in one launch in any order: no
+ * member reads bytes another writes, and two members write the same bytes
+ * only through disjoint boxes of one array under one shape. */
+static inline bool hydk_accesses_independent(const hydk_acc_t* a, int n) {
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            const hydk_acc_t &x = a[i], &y = a[j];
+            if (x.member == y.member) continue;
+            if (x.hi <= y.lo || y.hi <= x.lo) continue; /* other bytes */
+            if (!x.written && !y.written) continue;     /* both read */
+            if (x.read || y.read) return false;         /* one reads what the other writes */
+            if (!x.boxed || !y.boxed || x.lo != y.lo || x.nd != y.nd) return false;
+            bool same_shape = x.nd == 1 || x.shape.ndims == y.shape.ndims;
+            for (int d = 0; same_shape && x.nd > 1 && d < x.nd; ++d) same_shape = x.shape.elems[d] == y.shape.elems[d];
+            if (!same_shape) return false;
+            bool apart = false;
+            for (int d = 0; d < x.nd; ++d)
+                if (x.bhi[d] <= y.blo[d] || y.bhi[d] <= x.blo[d]) apart = true;
+            if (!apart) return false;
+        }
+    return true;
 }
 
 /* The stream of the drop-in layer (the array uploads run on it too). */

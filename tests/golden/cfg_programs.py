@@ -432,5 +432,43 @@ PROGRAMS = {
 }
 
 
+def group_overlap():
+    """Three maps the printed C would call back to back: out = xs + 1 over
+    [0, n), out[i + sh] = 2 xs[i + sh] over [0, n - sh) (its writes overlap
+    the first's), and ys = out + xs (it reads what the others write).  Their
+    group launcher must find them dependent and run them in order."""
+    a = loop(["i"], ["n"], [assign("x", load("xs", "i")), assign("y", bop("CAddF", "x", 1.0)), write("out", "i", "y")],
+             par())
+    b = loop(["i"], [C("ISub", C("IVar", "n"), C("IVar", "sh"))],
+             [assign("ii", bop("CAdd", "i", "sh")), assign("x", load("xs", "ii")), assign("y", bop("CMulF", "x", 2.0)),
+              write("out", "ii", "y")], par())
+    c = loop(["i"], ["n"], [assign("o", load("out", "i")), assign("x", load("xs", "i")), assign("y", bop("CAddF", "o", "x")),
+                            write("ys", "i", "y")], par())
+    return [site("hydk_overlap_add1", [("n", T_I), ("out", arr(T_F)), ("xs", arr(T_F))], a),
+            site("hydk_overlap_twice", [("n", T_I), ("out", arr(T_F)), ("sh", T_I), ("xs", arr(T_F))], b),
+            site("hydk_overlap_sum", [("n", T_I), ("out", arr(T_F)), ("xs", arr(T_F)), ("ys", arr(T_F))], c)]
+
+
+PROGRAMS["group_overlap"] = group_overlap
+
+# Runs of sites the printed C calls through one group launcher (HipGroup
+# lines): the split stencil's 9 region loops, which follow one another with
+# nothing between them, and the overlap program's three maps.
+GROUPS = {
+    "jacobi_step": [("hydk_jacobi_step", None)],
+    "group_overlap": [("hydk_overlap", None)],
+}
+
+
+def groups(name):
+    """HipGroup records of a program: each group takes every site of the
+    program, in order."""
+    out = []
+    for gname, members in GROUPS.get(name, []):
+        ms = members or [s.get("hsName") for s in sites(name)]
+        out.append(R("HipGroup", [("hgName", gname), ("hgSites", list(ms))]))
+    return out
+
+
 def sites(name):
     return PROGRAMS[name]()

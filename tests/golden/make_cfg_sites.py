@@ -19,7 +19,8 @@ OUT = os.path.join(HERE, "cfg")
 
 
 def render(name):
-    return "".join(show(s) + "\n" for s in cfg_programs.sites(name))
+    recs = cfg_programs.sites(name) + cfg_programs.groups(name)
+    return "".join(show(s) + "\n" for s in recs)
 
 
 def main():

@@ -4,7 +4,7 @@ checks tie their text to the tested parts:
 
 * every runtime function the offload pass emits as an RCall is declared in
   include/hyd_hip_runtime.h and exported by libhydhip.so;
-* the HipSite record's fields are the ones hydgen reads;
+* the HipSite and HipGroup records' fields are the ones hydgen reads;
 * the unary operators the pass keeps on the host are the ones hydgen refuses;
 * HIPBackend passes hydhip-cc the arguments tests/test_hydhip_cc.py builds
   programs with.
@@ -43,6 +43,19 @@ def test_site_fields_match_hydgen():
     assert fields == ["hsName", "hsArgs", "hsResult", "hsTypes", "hsLoop"]
     read_fields = set(re.findall(r'field\("(hs\w+)"\)', read(HYDGEN, "cfg.cpp")))
     assert read_fields == set(fields)
+
+
+def test_group_fields_match_hydgen():
+    # HipGroup lines: the fields the pass shows are the ones hydgen reads,
+    # and the group's call passes the members' arguments once each in order
+    # of first appearance, as hydgen orders the group launcher's parameters
+    fields = re.findall(r"\b(hg[A-Z]\w*) ::", CODEGEN)
+    assert fields == ["hgName", "hgSites"]
+    cfg = read(HYDGEN, "cfg.cpp")
+    assert set(re.findall(r'field\("(hg\w+)"\)', cfg)) == set(fields) and 'is_con("HipGroup")' in cfg
+    assert "args = nub (concat [as | SAssign _ (RCall _ as) <- run])" in CODEGEN
+    assert "seen.emplace(a.first, a.second).second) args.push_back(a)" in read(HYDGEN, "codegen.cpp")
+    assert "renderSites (hipSites arts) (hipGroups arts)" in BACKEND
 
 
 def test_host_only_unary_operators_match():

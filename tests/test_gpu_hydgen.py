@@ -201,6 +201,47 @@ def test_jacobi_regions_config_grid(hyd, orc, prog, arrays):
     assert np.array_equal(bits(arrays.read(out, np.float64)), bits(orc.stencil2d(u, 0, 0.1)))
 
 
+def jacobi_env(u_arr, out_arr, shape):
+    h, w = shape
+    return dict(u=u_arr, out=out_arr, shp=(h, w), n0=h, n1=w, n0m1=h - 1, n1m1=w - 1, bd_start_r=h - 1,
+                bd_start_c=w - 1)
+
+
+@pytest.mark.parametrize("shape", [(2, 2), (3, 5), (64, 130), (257, 1000), (1000, 63), (8192, 8192)])
+def test_jacobi_group_one_launch(hyd, orc, prog, arrays, shape):
+    # the 9 regions through their group launcher: one launch (their writes
+    # are disjoint boxes of out, and none reads out), the same bits
+    print("jacobi group %dx%d" % shape, flush=True)
+    u = orc.splitmix_f64(shape[0] * shape[1], 977 + shape[1]).reshape(shape)
+    out = arrays.device(shape)
+    prog.call("hydk_jacobi_step", jacobi_env(arrays.host(u), out, shape))
+    assert np.array_equal(bits(arrays.read(out, np.float64)), bits(orc.stencil2d(u, 0, 0.1)))
+
+
+def test_jacobi_group_iterate(hyd, orc, prog, arrays):
+    shape, steps = (700, 513), 9
+    u = orc.splitmix_f64(shape[0] * shape[1], 979).reshape(shape)
+    a, b = arrays.host(u), arrays.device(shape)
+    for _ in range(steps):
+        prog.call("hydk_jacobi_step", jacobi_env(a, b, shape))
+        a, b = b, a
+    assert np.array_equal(bits(arrays.read(a, np.float64)), bits(orc.iterate_stencil2d(u, steps, 0, 0.1)))
+
+
+@pytest.mark.parametrize("n,sh", [(1000, 10), (1000, 0), (1000, 1000), (5, 7), (100_003, 517)])
+def test_group_in_order_when_dependent(hyd, prog, arrays, n, sh):
+    # out = xs + 1; out[sh:] = 2 xs[sh:] (overlapping writes); ys = out + xs
+    # (reads what the others write): the group launcher must run them in order
+    xs = np.random.default_rng(n + sh).uniform(-1, 1, n)
+    out, ys = arrays.device((n,)), arrays.device((n,))
+    prog.call("hydk_overlap", dict(n=n, out=out, xs=arrays.host(xs), sh=sh, ys=ys))
+    ref = xs + 1.0
+    if sh < n:
+        ref[sh:] = xs[sh:] * 2.0
+    assert np.array_equal(bits(arrays.read(out, np.float64)), bits(ref))
+    assert np.array_equal(bits(arrays.read(ys, np.float64)), bits(ref + xs))
+
+
 @pytest.mark.parametrize("shape", [(1, 1), (2, 3), (5, 4), (33, 65), (512, 300)])
 def test_wrap_stencil(hyd, prog, arrays, shape):
     u = np.random.default_rng(shape[1]).uniform(0, 1, shape)

@@ -45,7 +45,7 @@ EXPECTED_FORM = {
     "map_chain": "form=map1d", "map_chain2": "form=map1d", "dot": "form=reduce", "reduce_i64": "form=reduce",
     "jacobi_step": "form=map2d", "wrap_stencil": "form=map1d", "mirror_1d": "form=map1d",
     "histogram": "form=scatter_i64", "splat_f64": "form=scatter_f64", "row_sums": "form=map1d",
-    "gather": "form=map1d", "map3d": "form=mapnd",
+    "gather": "form=map1d", "map3d": "form=mapnd", "group_overlap": "form=map1d",
 }
 
 
@@ -53,10 +53,30 @@ EXPECTED_FORM = {
 def test_schedule_forms(name):
     r = hydgen(os.path.join(CFG_DIR, name + ".sites"), "--plan")
     assert r.returncode == 0, r.stderr
-    lines = r.stdout.strip().splitlines()
+    lines = [l for l in r.stdout.strip().splitlines() if ": group of " not in l]
     assert len(lines) == len(H.SITES[name])
     for line in lines:
         assert EXPECTED_FORM[name] in line, line
+
+
+def test_group_lines():
+    # a HipGroup line names a run of sites; the group's launcher is declared
+    # beside theirs, with the members' arguments once each
+    p = hydgen(os.path.join(CFG_DIR, "jacobi_step.sites"), "--plan").stdout
+    g = [l for l in p.splitlines() if ": group of " in l]
+    assert len(g) == 1 and g[0].startswith("hydk_jacobi_step: group of") and len(g[0].split()) == 3 + 9
+    import tempfile
+    with tempfile.TemporaryDirectory() as t:
+        sig = os.path.join(t, "s.sig")
+        assert hydgen(os.path.join(CFG_DIR, "group_overlap.sites"), "--signatures", sig).returncode == 0
+        lines = open(sig).read().splitlines()
+        assert "hydk_overlap i64 : i64:n array:out array:xs i64:sh array:ys" in lines
+        bad = os.path.join(t, "bad.sites")
+        text = open(os.path.join(CFG_DIR, "group_overlap.sites")).read()
+        with open(bad, "w") as f:
+            f.write(text.replace('hgSites = ["hydk_overlap_add1"', 'hgSites = ["hydk_nosuch"'))
+        r = hydgen(bad, "-o", os.path.join(t, "x.hip"))
+        assert r.returncode == 1 and "no site named hydk_nosuch" in r.stderr
 
 
 def test_schedule_details():
