@@ -37,7 +37,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <functional>
 #include <mutex>
+#include <queue>
 #include <vector>
 
 #include "hyd_internal.hpp"
@@ -338,6 +340,8 @@ int batched_histogram_ptrs(const int64_t* const* src, const int64_t* len, const 
                            int64_t* out_tiles) {
     std::vector<Tile> flat;
     std::vector<Group> groups;
+    flat.reserve((size_t)count + 64);
+    groups.reserve((size_t)count + 64);
     int64_t total_len = 0;
     for (int64_t k = 0; k < count; ++k) total_len += std::max<int64_t>(len[k], 0);
     const int64_t split_at =
@@ -427,46 +431,59 @@ int batched_histogram_ptrs(const int64_t* const* src, const int64_t* len, const 
                 for (int i = 0; i < team && q0 + kXcds * i < nq; ++i) m.push_back(q0 + kXcds * i);
                 teams.push_back(m);
             }
+        // Jobs hold at most `team` tile indices inline (no allocation per
+        // job), and the least-loaded team comes from a heap ordered by
+        // (load, team index): the same choice as scanning the teams for the
+        // first minimal load, in O(log teams) - the planner runs on the host
+        // before the launch, so its time adds to the batch's (4096
+        // instances: 0.45 ms with a scan and per-job vectors).
         struct Job {
             int64_t cost;
-            std::vector<int64_t> tiles;
+            int32_t n;
+            int64_t tiles[kTeam];
         };
+        std::vector<int64_t> tcost(flat.size());
+        for (size_t t = 0; t < flat.size(); ++t) tcost[t] = tile_cost(flat[t]);
         std::vector<Job> jobs;
+        jobs.reserve(flat.size() / (size_t)team + groups.size() + 1);
         std::vector<int64_t> singles;
+        singles.reserve(groups.size());
         for (const Group& g : groups) {
             if (g.count == 1) {
                 singles.push_back(g.first);
                 continue;
             }
             for (int64_t i = 0; i < g.count; i += team) {
-                Job jb{0, {}};
+                Job jb{0, 0, {}};
                 for (int64_t k = i; k < std::min<int64_t>(g.count, i + team); ++k) {
-                    jb.tiles.push_back(g.first + k);
-                    jb.cost = std::max(jb.cost, tile_cost(flat[(size_t)(g.first + k)]));
+                    jb.tiles[jb.n++] = g.first + k;
+                    jb.cost = std::max(jb.cost, tcost[(size_t)(g.first + k)]);
                 }
-                jobs.push_back(std::move(jb));
+                jobs.push_back(jb);
             }
         }
-        std::stable_sort(singles.begin(), singles.end(), [&](int64_t a, int64_t b) {
-            return tile_cost(flat[(size_t)a]) > tile_cost(flat[(size_t)b]);
-        });
+        std::stable_sort(singles.begin(), singles.end(),
+                         [&](int64_t a, int64_t b) { return tcost[(size_t)a] > tcost[(size_t)b]; });
         for (size_t i = 0; i < singles.size(); i += (size_t)team) {
-            Job jb{0, {}};
+            Job jb{0, 0, {}};
             for (size_t k = i; k < std::min(singles.size(), i + (size_t)team); ++k) {
-                jb.tiles.push_back(singles[k]);
-                jb.cost = std::max(jb.cost, tile_cost(flat[(size_t)singles[k]]));
+                jb.tiles[jb.n++] = singles[k];
+                jb.cost = std::max(jb.cost, tcost[(size_t)singles[k]]);
             }
-            jobs.push_back(std::move(jb));
+            jobs.push_back(jb);
         }
         std::stable_sort(jobs.begin(), jobs.end(), [](const Job& a, const Job& b) { return a.cost > b.cost; });
-        std::vector<int64_t> tload(teams.size(), 0);
+        for (auto& q : qtiles) q.reserve(flat.size() / (size_t)nq + 8);
+        std::priority_queue<std::pair<int64_t, int>, std::vector<std::pair<int64_t, int>>,
+                            std::greater<std::pair<int64_t, int>>>
+            heap;
+        for (int t = 0; t < (int)teams.size(); ++t) heap.push({0, t});
         for (const Job& jb : jobs) {
-            size_t t = 0;
-            for (size_t u = 1; u < teams.size(); ++u)
-                if (tload[u] < tload[t]) t = u;
-            for (size_t k = 0; k < jb.tiles.size() && k < teams[t].size(); ++k)
-                qtiles[(size_t)teams[t][k]].push_back(jb.tiles[k]);
-            tload[t] += jb.cost;
+            const std::pair<int64_t, int> top = heap.top();
+            heap.pop();
+            const std::vector<int>& members = teams[(size_t)top.second];
+            for (int k = 0; k < jb.n && k < (int)members.size(); ++k) qtiles[(size_t)members[(size_t)k]].push_back(jb.tiles[k]);
+            heap.push({top.first + jb.cost, top.second});
         }
     }
     std::vector<int64_t> qseg((size_t)nq + 1);

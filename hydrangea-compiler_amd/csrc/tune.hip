@@ -38,19 +38,9 @@ std::mutex g_mu;
 std::map<std::pair<int, int>, int> g_table;
 bool g_loaded = false;
 
-int ilog2_ceil(int64_t v) {
-    int r = 0;
-    while (((int64_t)1 << r) < v) ++r;
-    return r;
-}
-int ilog2_floor(int64_t v) {
-    int r = 0;
-    while (v > 1) {
-        v >>= 1;
-        ++r;
-    }
-    return r;
-}
+// ceil / floor of log2 (0 for v <= 1)
+int ilog2_ceil(int64_t v) { return v <= 1 ? 0 : 64 - __builtin_clzll((unsigned long long)(v - 1)); }
+int ilog2_floor(int64_t v) { return v <= 1 ? 0 : 63 - __builtin_clzll((unsigned long long)v); }
 
 std::string default_path() {
     const char* e = getenv("HYD_TUNE_CACHE");
