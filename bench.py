@@ -426,12 +426,15 @@ def bench_coo_sort(steps, warmup, cpu=True):
     vals = hydhip.splitmix_i64(nnz, seed + 15, bits=20)
     keys = hydhip.splitmix_i64(nnz, seed + 16, bits=40)
     perm = hydhip.DeviceArray((nnz,), np.int64)
+    outs = (hydhip.DeviceArray((nrows + 1,), np.int64), hydhip.DeviceArray((nnz,), np.int64),
+            hydhip.DeviceArray((nnz,), np.int64))
     got = {}
 
     def coo(ev):
+        # outputs allocated once, outside the timed calls (as the sort's)
         if ev:
             ev[0].record()
-        got["coo"] = hydhip.coo_to_csr(nrows, nrows, rows, cols, vals)
+        got["coo"] = hydhip.coo_to_csr(nrows, nrows, rows, cols, vals, out=outs)
         if ev:
             ev[1].record()
 
@@ -442,14 +445,26 @@ def bench_coo_sort(steps, warmup, cpu=True):
         if ev:
             ev[1].record()
 
+    scan_out = hydhip.DeviceArray((nnz,), np.int64)
+
+    def scn(ev):
+        if ev:
+            ev[0].record()
+        hydhip.scan(vals, 0, True, False, out=scan_out)
+        if ev:
+            ev[1].record()
+
     cms, cev = timed_loop(coo, max(3, steps // 4), 1, events=True)
     sms, sev = timed_loop(srt, max(3, steps // 4), 1, events=True)
+    qms, qev = timed_loop(scn, max(3, steps // 2), 1, events=True)
     cb = 40 * nnz + 8 * (nrows + 1)
     res = dict(
         coo_to_csr=dict(nnz=nnz, nrows=nrows, ms_per_call=cms, event_ms=float(np.mean(cev)), compulsory_bytes=cb,
                         gbs=cb / float(np.mean(cev)) / 1e6, merged_nnz=int(got["coo"][3])),
         sort_indices=dict(keys=nnz, key_bits=40, ms_per_call=sms, event_ms=float(np.mean(sev)),
-                          compulsory_bytes=16 * nnz, gbs=16 * nnz / float(np.mean(sev)) / 1e6))
+                          compulsory_bytes=16 * nnz, gbs=16 * nnz / float(np.mean(sev)) / 1e6),
+        scan_i64=dict(n=nnz, kind="inclusive (+), single pass with decoupled look-back", ms_per_call=qms,
+                      event_ms=float(np.mean(qev)), bytes=16 * nnz, gbs=16 * nnz / float(np.mean(qev)) / 1e6))
     if cpu:
         import pyoracle
 
@@ -462,7 +477,10 @@ def bench_coo_sort(steps, warmup, cpu=True):
         t, (sref, _) = cpu_time(lambda: pyoracle.sort_indices(hk), warmup=0, iters=1)
         res["sort_indices"]["matches_oracle"] = bits_equal(perm.numpy(), sref)
         res["sort_indices"]["cpu_oracle"] = dict(t, threads=1, note="hyd_radix_sort_entries, serial")
-    for a in (rows, cols, vals, keys, perm) + tuple(got["coo"][:3]):
+        t, qref = cpu_time(lambda: pyoracle.scan(hv, 0, True, False), warmup=0, iters=1)
+        res["scan_i64"]["matches_oracle"] = bits_equal(scan_out.numpy(), qref)
+        res["scan_i64"]["cpu_oracle"] = dict(t, threads=1, note="the reference's serial scan loop")
+    for a in (rows, cols, vals, keys, perm, scan_out) + outs:
         a.free()
     return res
 

@@ -481,11 +481,18 @@ def csr_from_sorted_coo(nrows, nnz, rows, cols, vals, stream=None):
     return rp, ci, cv
 
 
-def coo_to_csr(nrows, ncols, rows, cols, vals, nnz=None, stream=None):
+def coo_to_csr(nrows, ncols, rows, cols, vals, nnz=None, out=None, stream=None):
+    """out: (row_ptr[nrows+1], col_idx[nnz], vals[nnz]) device arrays to
+    write into (default: fresh ones)."""
     nnz = rows.size if nnz is None else nnz
-    rp = DeviceArray((nrows + 1,), np.int64)
-    ci = DeviceArray((nnz,), np.int64)
-    cv = DeviceArray((nnz,), np.int64)
+    if out is None:
+        rp = DeviceArray((nrows + 1,), np.int64)
+        ci = DeviceArray((nnz,), np.int64)
+        cv = DeviceArray((nnz,), np.int64)
+    else:
+        rp, ci, cv = out
+        if rp.size < nrows + 1 or ci.size < nnz or cv.size < nnz:
+            raise ValueError("coo_to_csr: output arrays too small")
     cnt = _I64()
     _check(lib().hydhip_coo_to_csr_i64(nrows, ncols, nnz, rows.ptr, cols.ptr, vals.ptr, rp.ptr, ci.ptr,
                                        cv.ptr, ctypes.byref(cnt), stream))

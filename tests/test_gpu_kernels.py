@@ -295,7 +295,7 @@ def test_jacobi_200_steps_graph_equals_eager(hyd):
 
 # ---- scans, sort, gather, COO ---------------------------------------------------------------
 
-@pytest.mark.parametrize("n", [0, 1, 2047, 2048, 2049, 1_000_007])
+@pytest.mark.parametrize("n", [0, 1, 2047, 4095, 4096, 4097, 8193, 1_000_007, (1 << 24) + 5])
 @pytest.mark.parametrize("inclusive", [False, True])
 @pytest.mark.parametrize("reverse", [False, True])
 def test_scan(hyd, orc, n, inclusive, reverse):
@@ -387,6 +387,21 @@ def test_coo_to_csr_random(hyd, orc, nnz, nrows, ncols, dupbits):
     # invariants: row_ptr monotone, row_ptr[nrows] == merged count
     r = rp.numpy()
     assert np.all(np.diff(r) >= 0) and r[-1] == m
+
+
+@pytest.mark.parametrize("nnz,nrows,ncols", [(50_000, 2, 2), (20_000, 1, 1), (70_001, 3, 7), (4096 * 3, 1, 2)])
+def test_coo_to_csr_long_runs(hyd, orc, nnz, nrows, ncols):
+    # runs of equal (row, col) much longer than a 4096-element tile: tiles
+    # with no run head at all, run sums carried across tiles, wrapping int64
+    # sums of values near +-2^62
+    rows = orc.splitmix_i64(nnz, 821, bits=40) % nrows
+    cols = orc.splitmix_i64(nnz, 822, bits=40) % ncols
+    vals = orc.splitmix_i64(nnz, 823, bits=63) - (1 << 62)
+    print("coo long runs nnz=%d nrows=%d ncols=%d" % (nnz, nrows, ncols), flush=True)
+    rp, ci, cv, m = hyd.coo_to_csr(nrows, ncols, dev(hyd, rows), dev(hyd, cols), dev(hyd, vals))
+    rrp, rci, rcv, rm = orc.coo_to_csr(nrows, ncols, rows, cols, vals)
+    assert m == rm
+    assert bits_equal(rp.numpy(), rrp) and bits_equal(ci.numpy(), rci) and bits_equal(cv.numpy(), rcv)
 
 
 @pytest.mark.parametrize("nrows,ncols,colrange", [(1000, 10, (-5, 25)), (4, 1 << 61, (0, 1 << 61))])
