@@ -296,10 +296,14 @@ __global__ __launch_bounds__(kBlock) void batched_hist_kernel(const Tile* __rest
                 if (s) global_add(&h[lo + b], s);
             }
         } else {
+            // non-temporal stores: the bins are not read again here, and plain
+            // stores would take L2 lines from the elements the other range
+            // tiles of an instance still read (measured on MI355X: the
+            // 4096-instance batch 1.17 -> 1.10-1.12 ms)
             for (int b = threadIdx.x; b < width; b += kBlock) {
                 unsigned long long s = 0;
                 for (int c = 0; c < copies; ++c) s += lds[c * stride + b];
-                h[lo + b] = s;
+                __builtin_nontemporal_store(s, &h[lo + b]);
             }
         }
         __syncthreads();  // LDS reused by the next tile
