@@ -394,10 +394,21 @@ int batched_histogram_ptrs(const int64_t* const* src, const int64_t* len, const 
         // each its own LDS tiles whose bins are added into the zeroed
         // destination: one workgroup streams at a fraction of HBM's rate,
         // so one long instance on one workgroup would set the batch's end.
-        // Only for bins that fit one range: a piece of a multi-range
-        // instance would stream the elements once per range again and add
-        // every bin of every range with an atomic (measured: slower).
-        const int64_t pieces = (n > split_at && nb <= kWords) ? (n + split_at - 1) / split_at : 1;
+        // An instance with more bins than one range is split only when it is
+        // far longer than a queue's share (16x), into at most as many pieces as
+        // fill the CUs with its range tiles: each piece streams the elements
+        // once per range again and adds every bin of every range with an
+        // atomic (measured: slower for instances of up to 2^20 elements;
+        // unsplit, one 2^27-element instance with 65536 bins kept 4
+        // workgroups busy for 15 ms).
+        int64_t pieces = 1;
+        if (nb <= kWords) {
+            if (n > split_at) pieces = (n + split_at - 1) / split_at;
+        } else if (n > 16 * split_at) {
+            const int64_t ranges = (nb + kWords - 1) / kWords;
+            pieces = std::min<int64_t>((n + split_at - 1) / split_at,
+                                       std::max<int64_t>(1, (int64_t)device_info().num_cus / ranges));
+        }
         const int32_t once = nb <= kWords ? 1 : 0;  // one range tile reads the elements
         if (pieces > 1) {
             zero.push_back((int64_t)reinterpret_cast<uintptr_t>(dst[k]));

@@ -77,6 +77,26 @@ def test_batched_split_pieces(hyd, orc, policy):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("policy", [0, 1])
+def test_batched_split_multi_range(hyd, orc, policy):
+    # instances with bins beyond one LDS range and far longer than a queue's
+    # share are cut into pieces too (at most as many as fill the CUs with
+    # their range tiles); power-of-two and other bin counts, unaligned piece
+    # starts, negative elements, a garbage output reused twice
+    lens = np.array([1 << 24, 3, 12_000_001, 50_000], np.int64)
+    bins = np.array([65536, 64, 40_000, 65536], np.int64)
+    off, hoff = hyd.batch_offsets(lens, bins)
+    data = orc.splitmix_i64(int(lens.sum()), 5151, bits=40)
+    data[::7] = -data[::7]
+    d = hyd.DeviceArray.from_numpy(data)
+    print("batched split multi-range policy=%d" % policy, flush=True)
+    hist = hyd.DeviceArray.from_numpy(np.full(int(bins.sum()), -99, np.int64))
+    for _ in range(2):
+        got = hyd.batched_histogram(d, off, lens, bins, hoff, hist=hist, policy=policy).numpy()
+    ref, _ = orc.batched_histograms(data, off, lens, bins)
+    assert np.array_equal(got, ref)
+
+
 @pytest.mark.parametrize("policy", [0, 1, 2])
 def test_batched_wide_values(hyd, orc, policy):
     # 63-bit values, half negated: routes through both remainder forms (mask
