@@ -698,6 +698,33 @@ def bench_batched(steps, warmup, cpu=True):
         by_class[str(nb)] = dict(instances=int(m.sum()), elements=int(lc.sum()), bytes=cb, event_ms=ms,
                                  gbs=cb / ms / 1e6)
     res["by_class"] = by_class
+    # uniform batches over the first 2^27 elements (equal-length instances of
+    # one bin count, LDS policy): the streaming rate of single-range tiles
+    # against that of range tiles (bins beyond one 16384-word LDS range, each
+    # range a workgroup streaming the whole instance through L2)
+    uniform = {}
+    nu = 1 << 27
+    for nb, count in ((4096, 64), (16384, 64), (65536, 64), (65536, 1024)):
+        lu = np.full(count, nu // count, np.int64)
+        bu = np.full(count, nb, np.int64)
+        ou, hu = hydhip.batch_offsets(lu, bu)
+        hu_dev = hydhip.DeviceArray((int(bu.sum()),), np.int64)
+
+        def step(ev, lu=lu, bu=bu, ou=ou, hu=hu, hu_dev=hu_dev):
+            if ev:
+                ev[0].record()
+            hydhip.batched_histogram(d, ou, lu, bu, hu, hist=hu_dev, policy=hydhip.BATCH_LDS)
+            if ev:
+                ev[1].record()
+
+        _, ev = timed_loop(step, max(3, steps // 4), 1, events=True)
+        ms = float(np.mean(ev))
+        ub = workloads.batch_algorithmic_bytes(lu, bu)
+        uniform["%d_x_%d_bins_%d" % (count, nu // count, nb)] = dict(
+            event_ms=ms, bytes=ub, gbs=ub / ms / 1e6, element_gbs=8 * nu / ms / 1e6,
+            ranges=-(-nb // 16384))
+        hu_dev.free()
+    res["uniform"] = uniform
     # per-call plain launches: one synchronous hydhip_weighted_histogram_i64 per instance
     lib = hydhip.lib()
     s = hydhip.default_stream()
