@@ -21,8 +21,8 @@
 // device scope, which cross the XCDs' non-coherent L2s) and 54 us for three
 // launches.  The keyed COO merge works the same way over run-head counts.
 // The radix sort
-// runs per 8-bit digit: per-tile digit counts (LDS atomics), one scan over
-// the digit-major count table, and a stable scatter that ranks each element
+// runs per 8-bit digit: per-tile digit counts (LDS atomics), the run starts
+// from the count table (two launches over blocks of tiles), and a stable scatter that ranks each element
 // inside its tile with wave64 ballots (a lane's rank among equal digits is
 // the popcount of the equal-digit lanes below it) plus per-wave LDS counts.
 #include <algorithm>
@@ -345,24 +345,101 @@ __global__ __launch_bounds__(kSortBlock) void radix_count_kernel(const int64_t* 
         counts[(int64_t)blockIdx.x * D + d] = (int64_t)hist[0][d] + hist[1][d] + hist[2][d] + hist[3][d];
 }
 
-// out[c * rows + r] = in[r * cols + c]: 32 x 32 blocks through LDS, both
-// sides coalesced; a 1-D grid of ceil(rows/32) * ceil(cols/32) workgroups.
-__global__ __launch_bounds__(256) void transpose_i64_kernel(const int64_t* __restrict__ in, int64_t rows,
-                                                            int64_t cols, int64_t* __restrict__ out) {
-    __shared__ int64_t t[32][33];
-    const int64_t bcols = (cols + 31) / 32;
-    const int64_t r0 = ((int64_t)blockIdx.x / bcols) * 32, c0 = ((int64_t)blockIdx.x % bcols) * 32;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+// Run starts from the tile-major count table, in two launches over blocks of
+// consecutive tiles (at most kSortBlocks blocks; thread i holds digits i,
+// i + 256, ...: every access to a table row is coalesced):
+//   radix_block_sums_kernel: bsum[b][d] = the block's count of digit d;
+//   radix_offsets_kernel:    per block, the digit's start (the column totals,
+//     then an exclusive scan over digits), plus the earlier blocks' counts
+//     (each block adds those up itself), then down the block's rows:
+//     offsets[t][d] = start[d] + counts of digit d in the tiles before t.
+// `offsets` may be `counts` (each thread reads an entry before it writes it).
+// No workgroup waits on another, and no table is transposed: measured on
+// MI355X over 1e7 keys, 13.6 us per pass against 28 us for two transposes
+// around a scan of the digit-major table.
+constexpr int kSortBlocks = 64;
+
+template <int BITS>
+__global__ __launch_bounds__(kSortBlock) void radix_block_sums_kernel(const int64_t* __restrict__ counts,
+                                                                      int64_t ntiles, int64_t per_block,
+                                                                      int64_t* __restrict__ bsum) {
+    constexpr int D = 1 << BITS, PER = D / kSortBlock;
+    const int64_t t0 = (int64_t)blockIdx.x * per_block;
+    const int64_t t1 = t0 + per_block < ntiles ? t0 + per_block : ntiles;
+    int64_t acc[PER];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int64_t r = r0 + ty + 8 * k, c = c0 + tx;
-        if (r < rows && c < cols) t[ty + 8 * k][tx] = in[r * cols + c];
+    for (int q = 0; q < PER; ++q) acc[q] = 0;
+    constexpr int U = 16;  // rows in flight at a time
+    for (int64_t tb = t0; tb < t1; tb += U) {
+        int64_t v[U][PER];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int q = 0; q < PER; ++q) v[u][q] = tb + u < t1 ? counts[(tb + u) * D + threadIdx.x + q * kSortBlock] : 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int q = 0; q < PER; ++q) acc[q] += v[u][q];
     }
-    __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int64_t c = c0 + ty + 8 * k, r = r0 + tx;
-        if (r < rows && c < cols) out[c * rows + r] = t[tx][ty + 8 * k];
+    for (int q = 0; q < PER; ++q) bsum[(int64_t)blockIdx.x * D + threadIdx.x + q * kSortBlock] = acc[q];
+}
+
+template <int BITS>
+__global__ __launch_bounds__(kSortBlock) void radix_offsets_kernel(const int64_t* counts, int64_t ntiles,
+                                                                   int64_t per_block, int64_t nblocks,
+                                                                   const int64_t* __restrict__ bsum,
+                                                                   int64_t* offsets) {
+    constexpr int D = 1 << BITS, PER = D / kSortBlock;
+    const int64_t b = blockIdx.x;
+    int64_t total[PER], before[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) total[q] = before[q] = 0;
+    // sixteen rows in flight at a time (a loop of dependent loads would wait
+    // out one L2 round trip per block)
+    constexpr int U = 16;
+    for (int64_t c0 = 0; c0 < nblocks; c0 += U) {
+        int64_t v[U][PER];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int q = 0; q < PER; ++q)
+                v[u][q] = c0 + u < nblocks ? bsum[(c0 + u) * D + threadIdx.x + q * kSortBlock] : 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                total[q] += v[u][q];
+                before[q] += c0 + u < b ? v[u][q] : 0;
+            }
+    }
+    // digit d's start: the totals of the digits below it (digits q * 256 + i
+    // in order: the scan over threads for q = 0, then q = 1 after all of them)
+    int64_t start[PER];
+    int64_t carry = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        int64_t all;
+        const int64_t ex = block_excl_scan(total[q], &all);
+        start[q] = carry + ex + before[q];
+        carry += all;
+    }
+    const int64_t t0 = b * per_block;
+    const int64_t t1 = t0 + per_block < ntiles ? t0 + per_block : ntiles;
+    for (int64_t tb = t0; tb < t1; tb += U) {
+        int64_t v[U][PER];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int q = 0; q < PER; ++q)
+                v[u][q] = tb + u < t1 ? counts[(tb + u) * D + threadIdx.x + q * kSortBlock] : 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                if (tb + u < t1) offsets[(tb + u) * D + threadIdx.x + q * kSortBlock] = start[q];
+                start[q] += v[u][q];
+            }
     }
 }
 
@@ -533,30 +610,25 @@ static int digit_widths(int bits, int* widths) {
     return p8;
 }
 
-static int transpose_i64(const int64_t* in, int64_t rows, int64_t cols, int64_t* out, hipStream_t s) {
-    const int64_t blocks = ceil_div(rows, 32) * ceil_div(cols, 32);
-    if (blocks > 0x7fffffff) return set_error(HYD_E_INVALID, "sort: count table too large");
-    transpose_i64_kernel<<<(unsigned)blocks, 256, 0, s>>>(in, rows, cols, out);
-    HYD_CHECK_LAUNCH();
-    return HYD_OK;
-}
-
 // One pass: per-tile digit counts (tile-major, so each tile writes one
-// contiguous row), transposed to digit-major for the exclusive scan that
-// gives every (digit, tile) run its start, transposed back so that each tile
-// of the scatter reads its starts as one contiguous row.  (Digit-major rows
-// read and written directly cost one memory line per digit and tile: twice
-// the pass's compulsory traffic at 10-bit digits.)
+// contiguous row), every (tile, digit) run's start from them in place (block
+// sums, then offsets), and the scatter, each tile reading its starts as one
+// contiguous row.  (Digit-major rows read and written directly cost one
+// memory line per digit and tile: twice the pass's compulsory traffic at
+// 10-bit digits.)
 template <int BITS>
 static int radix_pass(const int64_t* kin, const int64_t* iin, int64_t n, int shift, int64_t ntiles,
-                      int64_t* counts, int64_t* dmajor, int64_t* scan_ws, int64_t* kout, int64_t* iout,
+                      int64_t* counts, int64_t* dmajor, int64_t* kout, int64_t* iout,
                       hipStream_t s) {
-    constexpr int64_t D = (int64_t)1 << BITS;
     radix_count_kernel<BITS><<<(unsigned)ntiles, kSortBlock, 0, s>>>(kin, n, shift, ntiles, counts);
     HYD_CHECK_LAUNCH();
-    HYD_TRY(transpose_i64(counts, ntiles, D, dmajor, s));
-    HYD_TRY(scan_i64(dmajor, D * ntiles, 0, dmajor, false, false, s, scan_ws));
-    HYD_TRY(transpose_i64(dmajor, D, ntiles, counts, s));
+    const int64_t per_block = ceil_div(ntiles, kSortBlocks), nblocks = ceil_div(ntiles, per_block);
+    int64_t* bsum = dmajor;  // nblocks * D words
+    radix_block_sums_kernel<BITS><<<(unsigned)nblocks, kSortBlock, 0, s>>>(counts, ntiles, per_block, bsum);
+    HYD_CHECK_LAUNCH();
+    radix_offsets_kernel<BITS><<<(unsigned)nblocks, kSortBlock, 0, s>>>(counts, ntiles, per_block, nblocks, bsum,
+                                                                        counts);
+    HYD_CHECK_LAUNCH();
     radix_scatter_kernel<BITS><<<(unsigned)ntiles, kSortBlock, 0, s>>>(kin, iin, n, shift, ntiles, counts,
                                                                        kout, iout);
     HYD_CHECK_LAUNCH();
@@ -614,9 +686,9 @@ int sort_pairs_i64(const int64_t* keys, const int64_t* payload, int64_t n, int64
         int64_t* kout = (last && keys_out) ? keys_out : ((p & 1) ? kbuf1 : kbuf0);
         int64_t* iout = ((passes - 1 - p) & 1) ? ibuf : payload_out;
         if (widths[p] == 8)
-            HYD_TRY(radix_pass<8>(kin, iin, n, shift, ntiles, counts, dmajor, scan_ws, kout, iout, s));
+            HYD_TRY(radix_pass<8>(kin, iin, n, shift, ntiles, counts, dmajor, kout, iout, s));
         else
-            HYD_TRY(radix_pass<9>(kin, iin, n, shift, ntiles, counts, dmajor, scan_ws, kout, iout, s));
+            HYD_TRY(radix_pass<9>(kin, iin, n, shift, ntiles, counts, dmajor, kout, iout, s));
         shift += widths[p];
         kin = kout;
         iin = iout;
