@@ -174,8 +174,9 @@ def bench_copy(steps, warmup):
 
 
 def bench_shapes(steps, warmup):
-    """The streaming grid of the map chain and the copy over 2^27 fp64 at 1, 2,
-    4 and 8 workgroups per CU (set by hand, then removed), and what the
+    """The streaming grid of the map chain and the copy over 2^27 fp64 in the
+    flat form and at 1, 2, 4 and 8 workgroups per CU (set by hand, then
+    removed), and what the
     autotuner picks per size class.  Map chain and copy results do not depend on
     the grid; the dot keeps its fixed grid (its partial count fixes the order of
     its sum) and is not swept."""
@@ -188,7 +189,7 @@ def bench_shapes(steps, warmup):
     res = {}
     for kind, name in ((hydhip.SHAPE_MAP_CHAIN, "map_chain"), (hydhip.SHAPE_COPY, "copy")):
         row = {}
-        for pc in (1, 2, 4, 8):
+        for pc in (hydhip.SHAPE_FLAT, 1, 2, 4, 8):
             hydhip.tune_shape_set(kind, n, pc)
 
             def step(ev):
@@ -202,7 +203,7 @@ def bench_shapes(steps, warmup):
                     ev[1].record()
 
             _, ev = timed_loop(step, steps, warmup, events=True)
-            row[str(pc)] = round(float(np.mean(ev)) * 1e3, 1)
+            row["flat" if pc == hydhip.SHAPE_FLAT else str(pc)] = round(float(np.mean(ev)) * 1e3, 1)
             hydhip.tune_shape_set(kind, n, 0)
         best = min(row.values())
         res[name] = dict(us_by_workgroups_per_cu=row, default=1, default_vs_best=round(row["1"] / best, 4))

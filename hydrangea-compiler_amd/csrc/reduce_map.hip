@@ -200,6 +200,76 @@ __global__ __launch_bounds__(kBlock) void map_chain_kernel(const double* __restr
     }
 }
 
+// The flat form of the map chain and the copy: no loop; workgroup b takes
+// the contiguous block of kBlock * K pairs at b * kBlock * K, thread t the
+// pairs at t + k * kBlock in it, and the grid is sized by the data (2^27
+// fp64 elements: 262144 workgroups).  Measured on MI355X against the
+// persistent grid-stride form (one workgroup per CU, each thread several
+// pairs a grid apart): an x + y stream 511-521 against 539-555 us, a copy
+// 317-322 against 348-355 us (6.7 TB/s) - with the hardware keeping every CU
+// full of short-lived waves and each workgroup's accesses one contiguous
+// block, the DRAM sees more requests in flight and fewer pages open at once.
+#ifndef HYD_FLAT_PAIRS
+#define HYD_FLAT_PAIRS 1
+#endif
+constexpr int kFlatPairs = HYD_FLAT_PAIRS;
+
+template <int K>
+__global__ __launch_bounds__(kBlock) void map_chain_flat_kernel(const double* __restrict__ xs,
+                                                                const double* __restrict__ ys, int64_t npairs,
+                                                                int64_t n, double* __restrict__ out) {
+    __shared__ uint64_t exp_tab[256];
+    static_assert(kBlock == 256, "one table entry per thread");
+    exp_tab[threadIdx.x] = hyd_exp_tab_dev[threadIdx.x];
+    const int64_t base = (int64_t)blockIdx.x * kBlock * K + threadIdx.x;
+    v2d x[K], y[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int64_t q = base + (int64_t)k * kBlock;
+        if (q < npairs) {
+            x[k] = ld_stream2(xs + 2 * q);
+            y[k] = ld_stream2(ys + 2 * q);
+        }
+    }
+    __syncthreads();  // the table is in LDS (the loads above are in flight meanwhile)
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int64_t q = base + (int64_t)k * kBlock;
+        if (q < npairs) {
+            v2d r;
+            r.x = map_chain_op(x[k].x, y[k].x, exp_tab);
+            r.y = map_chain_op(x[k].y, y[k].y, exp_tab);
+            st_stream2(out + 2 * q, r);
+        }
+    }
+    if ((n & 1) && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+        out[n - 1] = map_chain_op(xs[n - 1], ys[n - 1], exp_tab);
+}
+
+template <int K>
+__global__ __launch_bounds__(kBlock) void copy_flat_kernel(const double* __restrict__ src, int64_t npairs,
+                                                           double* __restrict__ dst) {
+    const int64_t base = (int64_t)blockIdx.x * kBlock * K + threadIdx.x;
+    v2d a[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int64_t q = base + (int64_t)k * kBlock;
+        if (q < npairs) a[k] = ld_stream2(src + 2 * q);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int64_t q = base + (int64_t)k * kBlock;
+        if (q < npairs) st_stream2(dst + 2 * q, a[k]);
+    }
+}
+
+// Workgroups of the flat form for `npairs` pairs, or 0 when the grid would be
+// too large for one launch (the grid-stride form then).
+inline int64_t flat_grid(int64_t npairs) {
+    const int64_t g = std::max<int64_t>(1, ceil_div(npairs, (int64_t)kBlock * kFlatPairs));
+    return g <= 0x7fffffff ? g : 0;
+}
+
 inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
 // Plain streaming copy: the bandwidth reference the fused kernels are read
@@ -361,8 +431,15 @@ extern "C" int hydhip_copy_f64(const double* src, int64_t n, double* dst, void* 
         return set_error(HYD_E_INVALID, "hydhip_copy_f64: needs an even count and 16-byte aligned pointers");
     if (n == 0) return HYD_OK;
     const int64_t npairs = n >> 1;
-    const int64_t blocks = stream_grid(npairs, kStreamDepth, tuned_per_cu(HYD_SHAPE_COPY, n, kCopyPerCU));
-    copy_kernel<kStreamDepth><<<(unsigned)blocks, kBlock, 0, resolve_stream(stream)>>>(src, npairs, dst);
+    // untuned classes take the flat form; a tuned class its workgroups per CU
+    const int per_cu = tuned_per_cu(HYD_SHAPE_COPY, n, HYD_SHAPE_FLAT);
+    const int64_t flat = per_cu == HYD_SHAPE_FLAT ? flat_grid(npairs) : 0;
+    if (flat) {
+        copy_flat_kernel<kFlatPairs><<<(unsigned)flat, kBlock, 0, resolve_stream(stream)>>>(src, npairs, dst);
+    } else {
+        const int64_t blocks = stream_grid(npairs, kStreamDepth, per_cu != HYD_SHAPE_FLAT ? per_cu : kCopyPerCU);
+        copy_kernel<kStreamDepth><<<(unsigned)blocks, kBlock, 0, resolve_stream(stream)>>>(src, npairs, dst);
+    }
     HYD_CHECK_LAUNCH();
     return HYD_OK;
 }
@@ -403,8 +480,16 @@ extern "C" int hydhip_map_chain_f64(const double* xs, const double* ys, int64_t 
     if (n == 0) return HYD_OK;
     hipStream_t s = resolve_stream(stream);
     const bool vec = aligned16(xs) && aligned16(ys) && aligned16(out);
+    // untuned classes take the flat form; a tuned class its workgroups per CU
+    const int per_cu = tuned_per_cu(HYD_SHAPE_MAP_CHAIN, n, HYD_SHAPE_FLAT);
+    const int64_t flat = vec && per_cu == HYD_SHAPE_FLAT ? flat_grid(n >> 1) : 0;
+    if (flat) {
+        map_chain_flat_kernel<kFlatPairs><<<(unsigned)flat, kBlock, 0, s>>>(xs, ys, n >> 1, n, out);
+        HYD_CHECK_LAUNCH();
+        return HYD_OK;
+    }
     const int64_t blocks =
-        stream_grid(vec ? (n >> 1) : n, vec ? kMapDepth : 1, tuned_per_cu(HYD_SHAPE_MAP_CHAIN, n, kMapPerCU));
+        stream_grid(vec ? (n >> 1) : n, vec ? kMapDepth : 1, per_cu != HYD_SHAPE_FLAT ? per_cu : kMapPerCU);
     if (vec)
         map_chain_kernel<true, kMapDepth><<<(unsigned)blocks, kBlock, 0, s>>>(xs, ys, n, out);
     else

@@ -262,7 +262,7 @@ int hydhip_tune_shapes(const char* path, void* stream) {
     if (rc) return -rc;
     hipStream_t s = resolve_stream(stream);
     const int logs[] = {20, 23, 26};
-    const int per_cu[] = {1, 2, 4, 8};
+    const int per_cu[] = {HYD_SHAPE_FLAT, 1, 2, 4, 8};
     const int64_t nmax = (int64_t)1 << 26;
     double* buf = nullptr;
     if (hipMalloc(&buf, (size_t)nmax * 3 * 8) != hipSuccess) {

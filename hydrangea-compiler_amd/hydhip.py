@@ -542,6 +542,7 @@ def tune_table():
 
 
 SHAPE_MAP_CHAIN, SHAPE_COPY = 0, 1
+SHAPE_FLAT = 65536  # per_cu value of the flat form (HYD_SHAPE_FLAT)
 
 
 def tune_shapes(path=None, stream=None):

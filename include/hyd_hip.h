@@ -196,8 +196,9 @@ int hydhip_tune_clear(void);
 int hydhip_tune_table(int* out, int max_entries);
 /* Launch shapes: the streaming grid (workgroups per CU) of the map chain and
  * the copy, per size class (floor log2 n); a call takes the largest tuned
- * class <= its own, else one workgroup per CU.  hydhip_tune_shapes measures
- * 1, 2, 4 and 8 per CU at 2^20, 2^23 and 2^26 elements and stores the choices
+ * class <= its own, else the flat form (HYD_SHAPE_FLAT).  hydhip_tune_shapes
+ * measures the flat form and 1, 2, 4 and 8 per CU at 2^20, 2^23 and 2^26
+ * elements and stores the choices
  * in the same table file as hydhip_tune_batched; hydhip_tune_shape_set sets
  * one class (per_cu <= 0 removes it); hydhip_tune_shape_table copies
  * (kind, n_log2, per_cu) triples.  The dot keeps a fixed grid: its partial
@@ -206,6 +207,9 @@ int hydhip_tune_table(int* out, int max_entries);
 #define HYD_SHAPE_MAP_CHAIN 0
 #define HYD_SHAPE_COPY 1
 #define HYD_SHAPE_KINDS 2
+/* per_cu value of the flat form: no loop, one contiguous block of pairs per
+ * workgroup, the grid sized by the data (the default of untuned classes). */
+#define HYD_SHAPE_FLAT 65536
 int hydhip_tune_shapes(const char* path, void* stream);
 int hydhip_tune_shape_set(int kind, int64_t n, int per_cu);
 int hydhip_tune_shape_table(int* out, int max_entries);

@@ -425,15 +425,15 @@ def test_coo_to_csr_outside_the_box(hyd, orc, nrows, ncols, colrange):
 
 def test_launch_shapes_leave_results_unchanged(hyd, orc):
     # the map chain and the copy at every streaming grid the tuner may pick
-    # give the same bits; the tuner stores one class per (kind, size) and the
-    # classes are removed again
+    # (the flat form and 1-8 workgroups per CU) give the same bits; the tuner
+    # stores one class per (kind, size) and the classes are removed again
     n = (1 << 20) + 6
     xs = hyd.splitmix_f64(n, 91, 0, symmetric=True)
     ys = hyd.splitmix_f64(n, 91, n, symmetric=True)
     ref = orc.map_chain(xs.numpy(), ys.numpy())
     out = hyd.DeviceArray((n,), np.float64)
     lib = hyd.lib()
-    for pc in (1, 2, 4, 8):
+    for pc in (hyd.SHAPE_FLAT, 1, 2, 4, 8):
         print("launch shape per_cu=%d" % pc, flush=True)
         hyd.tune_shape_set(hyd.SHAPE_MAP_CHAIN, n, pc)
         hyd.tune_shape_set(hyd.SHAPE_COPY, n, pc)
