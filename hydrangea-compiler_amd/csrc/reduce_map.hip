@@ -55,11 +55,13 @@ __device__ __forceinline__ double block_sum(double v) {
     return s;
 }
 
-// Launch shape of the streaming kernels: a few waves per CU, each thread with
-// kStreamDepth 16-byte pairs (per array) in flight.  Measured on MI355X, the
-// HBM rate of a read or copy stream falls as the number of concurrent waves
-// grows (dot: 7.3 TB/s at one 256-thread workgroup per CU, 5.8 TB/s at eight),
-// so the grid is sized in workgroups per CU, not by the data.
+// Launch shape of the persistent streaming kernels: a few waves per CU, each
+// thread with kStreamDepth 16-byte pairs (per array) in flight.  Measured on
+// MI355X, the HBM rate of a read stream falls as the number of concurrent
+// waves grows (dot: 7.3 TB/s at one 256-thread workgroup per CU, 5.8 TB/s at
+// eight; the flat form below, 6.2-6.3 TB/s at 2-8 pairs per thread), so the
+// dot's grid is sized in workgroups per CU, not by the data.  Streams that
+// write (the map chain, the copy) run faster in the flat form.
 constexpr int kStreamDepth = 4;
 constexpr int kDotPerCU = 1;
 constexpr int kMapPerCU = 1;
