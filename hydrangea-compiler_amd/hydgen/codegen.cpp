@@ -1183,8 +1183,9 @@ void emit_map_launcher(std::ostringstream& o, const Site& site, const Plan& p, c
           << ");\n";
         o << "        if (hk_gx > 0x7fffffff) hydk_fail(hk_site, \"inner extent too large\");\n";
     } else {
-        o << "        const int64_t hk_gx = hydk_stream_grid(hk_n, " << U * B << ", " << opt.waves_per_cu * 64 / B
-          << "), hk_gy = 1;\n";
+        // flat: the grid sized by the data, one round per workgroup (per_cu 0)
+        o << "        const int64_t hk_gx = hydk_stream_grid(hk_n, " << U * B << ", "
+          << (opt.flat_maps ? 0 : opt.waves_per_cu * 64 / B) << "), hk_gy = 1;\n";
     }
     for (const auto& q : ps) o << "        hk_m." << q.name << " = " << q.launcher_arg << ";\n";
     for (const auto& v : iv) o << "        hk_m." << v << " = " << v << ";\n";

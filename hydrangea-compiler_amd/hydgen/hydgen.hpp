@@ -158,6 +158,7 @@ struct Options {
     int waves_per_cu = 8;     // workgroups per CU of the streaming grid, x block/64
     int rows = 4;             // rows per round of the 2-D form (when its stores are deferred)
     int band_rounds = 1;      // rounds per workgroup of the 2-D form
+    bool flat_maps = true;    // 1-D / N-D maps: grid sized by the data (one round per workgroup)
 };
 
 // The HIP source of every site (kernels and extern "C" launchers).

@@ -2,6 +2,7 @@
 //
 //   hydgen SITES [-o OUT.hip] [--header OUT.h] [--signatures OUT.sig] [--plan]
 //          [--unroll U] [--block B] [--waves-per-cu W] [--rows R] [--band-rounds K]
+//          [--map-grid flat|persistent]
 //
 // SITES holds one HipSite per line (the derived Show text Language.Hydrangea.
 // CodegenHIP writes), and a HipGroup line per run of sites the printed C
@@ -9,7 +10,9 @@
 // declarations of their launchers, --signatures one line per launcher for
 // hosts that bind them at run time, --plan prints each site's schedule.
 // --unroll, --block and --waves-per-cu shape the streaming forms; --rows
-// (1..8) and --band-rounds (rounds per workgroup) the 2-D form.
+// (1..8) and --band-rounds (rounds per workgroup) the 2-D form; --map-grid
+// flat (the default) sizes a 1-D or N-D map's grid by the data, one round per
+// workgroup, persistent caps it at --waves-per-cu per CU.
 // Errors (an unsupported construct, a malformed line) print a message and
 // exit with status 1: the HIP backend then stops, as the C backend stops on
 // a C compiler error.
@@ -39,7 +42,7 @@ void spit(const std::string& path, const std::string& text) {
 int usage() {
     fprintf(stderr,
             "usage: hydgen SITES [-o OUT.hip] [--header OUT.h] [--signatures OUT.sig] [--plan] [--unroll U] [--block B] "
-            "[--waves-per-cu W] [--rows R] [--band-rounds K]\n");
+            "[--waves-per-cu W] [--rows R] [--band-rounds K] [--map-grid flat|persistent]\n");
     return 2;
 }
 
@@ -67,6 +70,10 @@ int main(int argc, char** argv) {
             sigs = v;
         } else if (!strcmp(a, "--plan")) {
             plan = true;
+        } else if (!strcmp(a, "--map-grid")) {
+            const char* v = next();
+            if (!v || (strcmp(v, "flat") && strcmp(v, "persistent"))) return usage();
+            opt.flat_maps = !strcmp(v, "flat");
         } else if (!strcmp(a, "--unroll") || !strcmp(a, "--block") || !strcmp(a, "--waves-per-cu") ||
                    !strcmp(a, "--rows") || !strcmp(a, "--band-rounds")) {
             const char* v = next();

@@ -302,7 +302,9 @@ static inline int hydk_num_cus(void) {
  * at most `per_cu` per CU (grid-stride beyond that). */
 static inline int64_t hydk_stream_grid(int64_t n, int64_t tile, int per_cu) {
     const int64_t tiles = (n + tile - 1) / tile;
-    const int64_t cap = (int64_t)hydk_num_cus() * (per_cu > 0 ? per_cu : 1);
+    // per_cu 0: one tile per workgroup (the grid-stride loop then runs one
+    // round), up to the largest grid one launch takes
+    const int64_t cap = per_cu > 0 ? (int64_t)hydk_num_cus() * per_cu : (int64_t)0x7fffffff;
     return std::max<int64_t>(1, std::min<int64_t>(tiles, cap));
 }
 
