@@ -1030,9 +1030,9 @@ def main():
         "config": {"workload": "fp64 dot + fused 8-op map chain over xs, ys [2^27] uniform [-1,1)",
                    "n": dm["n"], "step_bytes": dm["step_bytes"], "device": info["arch"],
                    "cus": info["num_cus"]},
-        "roofline": {"bound": "hbm", "kernel": "map_chain_kernel", "achieved": round(mc["gbs"], 2),
+        "roofline": {"bound": "hbm", "kernel": "map_chain_flat_kernel", "achieved": round(mc["gbs"], 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(mc["gbs"] / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic("map_chain_kernel<true", workloads.DOTMAP["n"])},
+                     "traffic": measured_traffic("map_chain_flat_kernel<1>", workloads.DOTMAP["n"])},
         "cpu_baseline": cpu,
         "kernels": kernels,
     }
