@@ -206,7 +206,7 @@ def bench_shapes(steps, warmup):
             row["flat" if pc == hydhip.SHAPE_FLAT else str(pc)] = round(float(np.mean(ev)) * 1e3, 1)
             hydhip.tune_shape_set(kind, n, 0)
         best = min(row.values())
-        res[name] = dict(us_by_workgroups_per_cu=row, default=1, default_vs_best=round(row["1"] / best, 4))
+        res[name] = dict(us_by_workgroups_per_cu=row, default="flat", default_vs_best=round(row["flat"] / best, 4))
     t0 = time.perf_counter()
     classes = hydhip.tune_shapes()
     res["tuner"] = dict(classes=classes, seconds=round(time.perf_counter() - t0, 3),
