@@ -159,10 +159,19 @@ template <int OP>
 __device__ __forceinline__ uint64_t tiles_before(const int64_t* __restrict__ sums, int64_t tile, bool scanned) {
     __shared__ uint64_t wv[kBlock / 64];
     if (scanned) return (uint64_t)sums[tile];
+    // eight totals in flight per thread at a time (one at a time, each
+    // would wait out an L2 round trip)
     uint64_t v = 0;
-    for (int64_t j = threadIdx.x; j < tile; j += kBlock) {
-        const uint64_t x = (uint64_t)sums[j];
-        v = OP == 0 ? v + x : (x > v ? x : v);
+    constexpr int U = 8;
+    for (int64_t j0 = threadIdx.x; j0 < tile; j0 += (int64_t)U * kBlock) {
+        uint64_t x[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t j = j0 + (int64_t)u * kBlock;
+            x[u] = j < tile ? (uint64_t)sums[j] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) v = OP == 0 ? v + x[u] : (x[u] > v ? x[u] : v);
     }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
@@ -846,11 +855,20 @@ __global__ __launch_bounds__(kBlock) void coo_keyed_merge_kernel(
     const int tile_n = (int)(n - base < kTile ? n - base : kTile);
     const uint64_t ucols = (uint64_t)ncols;
 
-    // keys, through LDS (coalesced), with the neighbours across the tile edges
+    // keys and values loaded together (coalesced: lane = consecutive
+    // element), then each moved through LDS to the thread that owns kItems
+    // consecutive elements; the neighbours across the tile edges as well
+    int64_t kin[kItems], vin[kItems];
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const int j = k * kBlock + t;
-        stage[(j / kItems) * kRow + j % kItems] = j < tile_n ? skeys[base + j] : 0;
+        kin[k] = j < tile_n ? skeys[base + j] : 0;
+        vin[k] = j < tile_n ? svals[base + j] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const int j = k * kBlock + t;
+        stage[(j / kItems) * kRow + j % kItems] = kin[k];
     }
     if (t == 0) {
         edge[0] = base > 0 ? skeys[base - 1] : 0;
@@ -871,7 +889,7 @@ __global__ __launch_bounds__(kBlock) void coo_keyed_merge_kernel(
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const int j = k * kBlock + t;
-        stage[(j / kItems) * kRow + j % kItems] = j < tile_n ? svals[base + j] : 0;
+        stage[(j / kItems) * kRow + j % kItems] = vin[k];
     }
     __syncthreads();
     uint64_t val[kItems];
