@@ -1098,9 +1098,11 @@ __global__ void coo_fix_kernel(const int64_t* __restrict__ cont, int64_t ntiles,
 
 // keys[i] = rows[i] * ncols + cols[i]; *bad = 1 when any element lies
 // outside [0, nrows) x [0, ncols) (one device-scope atomic per workgroup at
-// most).  bad zeroed by the caller.  Pairs of elements per 16-byte access,
-// four pairs in flight per thread; rows and cols are read once
-// (non-temporal), the keys stay cached for the sort that reads them next.
+// most).  bad zeroed by the caller.  Flat form: one pair of elements per
+// thread (16-byte accesses when rows, cols and keys are 16-byte aligned; the
+// grid sized by the data, as the streaming map kernels): rows and cols are
+// read once (non-temporal), the keys stay cached for the sort that reads
+// them next.
 __device__ __forceinline__ bool pack_one(int64_t r, int64_t c, int64_t nrows, int64_t ncols, int64_t* key) {
     *key = (int64_t)((uint64_t)r * (uint64_t)ncols + (uint64_t)c);
     return !(r >= 0 && r < nrows && c >= 0 && c < ncols);
@@ -1112,43 +1114,23 @@ __global__ __launch_bounds__(256) void pack_keys_box_kernel(const int64_t* __res
                                                             int64_t* __restrict__ keys,
                                                             unsigned long long* __restrict__ bad) {
     bool out = false;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    // callers may pass 8-byte aligned arrays: 16-byte accesses only when all
-    // three are 16-byte aligned (keys is workspace, always aligned)
     const bool vec = ((reinterpret_cast<uintptr_t>(rows) | reinterpret_cast<uintptr_t>(cols) |
                        reinterpret_cast<uintptr_t>(keys)) & 15) == 0;
-    const int64_t npairs = vec ? n >> 1 : 0;
-    if (!vec) {
-        for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride)
-            out |= pack_one(rows[j], cols[j], nrows, ncols, &keys[j]);
-    }
-    const v2i* r2 = reinterpret_cast<const v2i*>(rows);
-    const v2i* c2 = reinterpret_cast<const v2i*>(cols);
-    v2i* k2 = reinterpret_cast<v2i*>(keys);
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i + 3 * stride < npairs; i += 4 * stride) {
-        v2i r[4], c[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            r[u] = __builtin_nontemporal_load(&r2[i + u * stride]);
-            c[u] = __builtin_nontemporal_load(&c2[i + u * stride]);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // pair q: elements 2q, 2q + 1
+    if (vec) {
+        if (2 * q + 1 < n) {
+            const v2i r = __builtin_nontemporal_load(reinterpret_cast<const v2i*>(rows) + q);
+            const v2i c = __builtin_nontemporal_load(reinterpret_cast<const v2i*>(cols) + q);
             int64_t a, b;
-            out |= pack_one(r[u].x, c[u].x, nrows, ncols, &a);
-            out |= pack_one(r[u].y, c[u].y, nrows, ncols, &b);
-            k2[i + u * stride] = v2i{a, b};
+            out |= pack_one(r.x, c.x, nrows, ncols, &a);
+            out |= pack_one(r.y, c.y, nrows, ncols, &b);
+            reinterpret_cast<v2i*>(keys)[q] = v2i{a, b};
+        } else if (2 * q < n) {
+            out |= pack_one(rows[2 * q], cols[2 * q], nrows, ncols, &keys[2 * q]);
         }
+    } else {
+        for (int64_t j = 2 * q; j < 2 * q + 2 && j < n; ++j) out |= pack_one(rows[j], cols[j], nrows, ncols, &keys[j]);
     }
-    for (; i < npairs; i += stride) {
-        const v2i r = r2[i], c = c2[i];
-        int64_t a, b;
-        out |= pack_one(r.x, c.x, nrows, ncols, &a);
-        out |= pack_one(r.y, c.y, nrows, ncols, &b);
-        k2[i] = v2i{a, b};
-    }
-    if (vec && (n & 1) && blockIdx.x == 0 && threadIdx.x == 0) out |= pack_one(rows[n - 1], cols[n - 1], nrows, ncols, &keys[n - 1]);
     __shared__ int wbad[4];
     const bool any_out = __any(out);
     if ((threadIdx.x & 63) == 0) wbad[threadIdx.x >> 6] = any_out ? 1 : 0;
@@ -1766,7 +1748,8 @@ int coo_to_csr_ws(int64_t nrows, int64_t ncols, int64_t nnz, const int64_t* rows
                                                          ((unsigned __int128)1 << 62);
     if (nnz > 0 && box_fits) {
         const int64_t tm = ceil_div(nnz, kTile), tr = ceil_div(nrows, kTile);
-        if (tm > 0x7fffffff || tr > 0x7fffffff) return set_error(HYD_E_INVALID, "coo_to_csr: too many elements");
+        if (tm > 0x7fffffff || tr > 0x7fffffff || ceil_div(ceil_div(nnz, 2), 256) > 0x7fffffff)
+            return set_error(HYD_E_INVALID, "coo_to_csr: too many elements");
         int64_t* X = d_cnt + 8;
         int64_t* k_cnt = X + 2;
         unsigned long long* bad = reinterpret_cast<unsigned long long*>(X + 3);
@@ -1775,8 +1758,8 @@ int coo_to_csr_ws(int64_t nrows, int64_t ncols, int64_t nnz, const int64_t* rows
         int64_t* hsums = row_last1 + nrows;
         int64_t* cont = hsums + tm;
         HYD_TRY(fill_zero(X, (size_t)(8 + tr + nrows) * 8, s));
-        pack_keys_box_kernel<<<grid_for(nnz, 256, 2 * device_info().num_cus), 256, 0, s>>>(rows, cols, nnz, nrows,
-                                                                                          ncols, keys, bad);
+        pack_keys_box_kernel<<<(unsigned)ceil_div(ceil_div(nnz, 2), 256), 256, 0, s>>>(rows, cols, nnz, nrows,
+                                                                                    ncols, keys, bad);
         HYD_CHECK_LAUNCH();
         // keys inside the box are at most nrows * ncols - 1: sorting those bits
         // orders them as the reference's byte passes do (extra high passes of
