@@ -464,7 +464,7 @@ def bench_coo_sort(steps, warmup, cpu=True):
                         gbs=cb / float(np.mean(cev)) / 1e6, merged_nnz=int(got["coo"][3])),
         sort_indices=dict(keys=nnz, key_bits=40, ms_per_call=sms, event_ms=float(np.mean(sev)),
                           compulsory_bytes=16 * nnz, gbs=16 * nnz / float(np.mean(sev)) / 1e6),
-        scan_i64=dict(n=nnz, kind="inclusive (+), single pass with decoupled look-back", ms_per_call=qms,
+        scan_i64=dict(n=nnz, kind="inclusive (+), reduce-then-scan in two launches", ms_per_call=qms,
                       event_ms=float(np.mean(qev)), bytes=16 * nnz, gbs=16 * nnz / float(np.mean(qev)) / 1e6))
     if cpu:
         import pyoracle

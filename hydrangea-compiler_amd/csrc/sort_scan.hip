@@ -27,6 +27,7 @@
 // the popcount of the equal-digit lanes below it) plus per-wave LDS counts.
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 
 #include "hyd_internal.hpp"
 
@@ -467,8 +468,15 @@ __global__ __launch_bounds__(kSortBlock) void radix_offsets_kernel(const int64_t
 // The keys are then placed in LDS at their slots and written out in slot
 // order - consecutive threads write consecutive addresses of a digit's run -
 // and the payloads after them the same way.
+//
+// Four workgroups fit on a CU (4 waves per SIMD) with 8-bit digits: 40 KiB of LDS
+// (the digit per slot in a byte, the wave sums kept in that array's
+// space before it is written) and at most 128 VGPRs (the payloads are loaded
+// after the ranking, while the keys are written out).  At three per CU the
+// 2442 tiles of 1e7 keys took four rounds, the last one 18 % full.
 template <int BITS>
-__global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
+__global__ __launch_bounds__(kSortBlock) __attribute__((amdgpu_waves_per_eu(BITS <= 8 ? 4 : 3)))
+void radix_scatter_kernel(
     const int64_t* __restrict__ keys_in, const int64_t* __restrict__ idx_in, int64_t n, int shift,
     int64_t ntiles, const int64_t* __restrict__ offsets, int64_t* __restrict__ keys_out,
     int64_t* __restrict__ idx_out) {
@@ -477,11 +485,14 @@ __global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
     constexpr int kWaves = kSortBlock / 64;
     constexpr int kQuarter = kSortTile / kWaves;  // elements per wave
     static_assert(PER >= 1 && D % kSortBlock == 0 && kQuarter == 64 * kSortRounds, "tile shape");
-    __shared__ int64_t stage[kSortTile];        // keys, then payloads, in slot order
-    __shared__ uint16_t slot_digit[kSortTile];  // the digit of each slot
-    __shared__ int64_t gbase[D];                // global position of slot 0's digit run
-    __shared__ uint16_t wave_run[kWaves][D];    // per wave: its count of each digit, then its start
-    __shared__ unsigned wave_sum[kWaves];
+    using digit_t = typename std::conditional<(BITS <= 8), uint8_t, uint16_t>::type;
+    __shared__ int64_t stage[kSortTile];       // keys, then payloads, in slot order
+    __shared__ __attribute__((aligned(16))) digit_t slot_digit[kSortTile];  // the digit of each slot
+    __shared__ int64_t gbase[D];               // global position of slot 0's digit run
+    __shared__ uint16_t wave_run[kWaves][D];   // per wave: its count of each digit, then its start
+    // the wave sums live in slot_digit's space: read before the barrier
+    // that precedes the first write of slot_digit
+    unsigned* wave_sum = reinterpret_cast<unsigned*>(slot_digit);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t tile = blockIdx.x;
     const int64_t base = tile * kSortTile;
@@ -489,12 +500,11 @@ __global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
 
     for (int i = threadIdx.x; i < kWaves * D; i += kSortBlock) (&wave_run[0][0])[i] = 0;
     const int first = wave * kQuarter + lane;  // this lane's element of round 0
-    int64_t key[kSortRounds], idx[kSortRounds];
+    int64_t key[kSortRounds];
 #pragma unroll
     for (int r = 0; r < kSortRounds; ++r) {
         const int e = first + r * 64;
         key[r] = e < tile_n ? keys_in[base + e] : 0;
-        idx[r] = e < tile_n ? (idx_in ? idx_in[base + e] : base + e) : 0;
     }
     __syncthreads();
 
@@ -559,7 +569,7 @@ __global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
     }
     __syncthreads();
 
-    // keys in slot order
+    // keys in slot order; the payloads' loads go out meanwhile
     int slot[kSortRounds];
 #pragma unroll
     for (int r = 0; r < kSortRounds; ++r) {
@@ -568,6 +578,12 @@ __global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
             slot[r] = mine[digit_of<BITS>(key[r], shift)] + rank[r];
             stage[slot[r]] = key[r];
         }
+    }
+    int64_t idx[kSortRounds];
+#pragma unroll
+    for (int r = 0; r < kSortRounds; ++r) {
+        const int e = first + r * 64;
+        idx[r] = e < tile_n ? (idx_in ? idx_in[base + e] : base + e) : 0;
     }
     __syncthreads();
     for (int i = threadIdx.x; i < tile_n; i += kSortBlock) {
