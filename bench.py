@@ -321,10 +321,11 @@ def bench_generated(steps, warmup, cpu=True):
     loops (lib/libhydgen_programs.so), on the workloads the hand-written
     kernels run: the 8-op map chain and the dot over 2^27 fp64 (the dot's
     launcher returns the value, so its time includes the 8-byte read-back),
-    the 64-bin weighted histogram over 2^24 int64, and one clamp heat step on
-    the 8192 x 8192 grid as the reference splits it (9 region loops, through
-    their group launcher, and one launcher per region).  Each is checked
-    against the oracle in the same run."""
+    the 64-bin weighted histogram over 2^24 int64, one clamp heat step on the
+    8192 x 8192 grid as the reference splits it (9 region loops, through their
+    group launcher, and one launcher per region), and the 200-step iterate of
+    that step through its one launcher.  Each is checked against the oracle in
+    the same run (the iterate at 20 steps)."""
     import hydgen_lib
 
     prog = hydgen_lib.programs()
@@ -407,6 +408,34 @@ def bench_generated(steps, warmup, cpu=True):
         hu = pyoracle.splitmix_f64(hh * ww, j["seed"]).reshape(hh, ww)
         res["jacobi_heat_step"]["matches_oracle"] = bits_equal(_host_copy(uo, np.float64).reshape(hh, ww),
                                                                pyoracle.stencil2d(hu, 0, j["alpha"]))
+    # the whole iterate (200 steps) through one generated launcher: its
+    # temporal form, 8 steps per launch; the call allocates the two state
+    # buffers it ping-pongs (their allocation is inside the timed call), and
+    # the result is freed after the end event
+    iters = j["iters"]
+    ms_it = []
+    form = -1
+    for k in range(max(1, warmup) + steps):
+        e0, e1 = hydhip.Event(), hydhip.Event()
+        e0.record()
+        r = prog.hydk_heat_iterate(iters, u)
+        e1.record()
+        hydhip.device_sync()
+        form = prog._cdll.hydk_heat_iterate__form()
+        if k >= max(1, warmup):
+            ms_it.append(e0.elapsed_ms(e1))
+        if k == 0 and cpu:
+            import pyoracle
+
+            ref20 = prog.hydk_heat_iterate(20, u)
+            res["heat_iterate_check"] = dict(steps=20, matches_oracle=bits_equal(
+                _host_copy(ref20, np.float64).reshape(hh, ww), pyoracle.iterate_stencil2d(hu, 20, 0, j["alpha"])))
+            L.hyd_array_free(ref20)
+        L.hyd_array_free(r)
+    ms = float(np.mean(ms_it))
+    res["heat_iterate"] = dict(steps_per_call=iters, ms=ms, us_per_step=ms * 1e3 / iters,
+                               form="temporal" if form == 2 else "in order" if form == 1 else "none",
+                               gbs_algorithmic=16 * hh * ww * iters / ms / 1e6)
     for a in made:
         L.hyd_array_free(a)
     return res

@@ -30,6 +30,13 @@
 -- after it (the accumulator aside), and when it is not inside a parallel host
 -- loop or region.
 --
+-- An iterate (Lowering.hs:840-857: a serial LoopIterate loop whose step
+-- allocates the new state once, evaluates scalars of the state's shape,
+-- branches on them, runs loops offloadable as above and ends with
+-- cur = next) is offloaded whole, as one site returning the final state:
+-- hydgen runs a split stencil's steps several per launch over LDS tiles
+-- (temporal blocking), and any other step's loops in order.
+--
 -- Adjacent calls of map sites (loops without a result, nothing between
 -- them: the region loops of a split stencil) become one call of a group
 -- launcher, listed as a 'HipGroup' line; hydgen runs the members in one
@@ -59,6 +66,7 @@ module Language.Hydrangea.CodegenHIP
   , hipPipeline
   , hipOffloadProgram
   , offloadableLoop
+  , offloadableIterate
   , codegenHIPWithOptions
   , renderSites
   ) where
@@ -179,7 +187,7 @@ type Pass = State (Int, [HipSite], [HipGroup])
 goBlock :: TypeEnv -> (CVar -> Bool) -> CVar -> [Stmt] -> [Stmt] -> Pass [Stmt]
 goBlock env isArr pname whole stmts = mapM one stmts >>= groupRuns . concat
   where
-    one s = case offloadableLoop env whole s of
+    one s = case offloadableIterate env whole s `orElse` offloadableLoop env whole s of
       Just mk -> do
         (k, sites, groups) <- get
         let site = mk (siteName k)
@@ -198,7 +206,8 @@ goBlock env isArr pname whole stmts = mapM one stmts >>= groupRuns . concat
           pure [SIf c t' e']
         SReturn a -> pure (syncReturned a ++ [s])
         _ -> pure (hostStatement isArr s)
-    containsSite = any (\t -> isJust (offloadableLoop env whole t) || nested t)
+    containsSite = any (\t -> isJust (offloadableLoop env whole t) || isJust (offloadableIterate env whole t) || nested t)
+    orElse a b = maybe b Just a
     nested t = case t of
       SLoop spec b | lsExec spec == Serial -> containsSite b
       SIf _ a b -> containsSite a || containsSite b
@@ -326,6 +335,58 @@ offloadableLoop _ _ _ = Nothing
 
 guard' :: Bool -> Maybe ()
 guard' b = if b then Just () else Nothing
+
+-- | A site builder for an iterate loop (Lowering.hs:840-857) whose whole
+-- step the device runs: the loop is serial with one counter the step never
+-- reads; the step ends with cur = next, allocates next once at its top
+-- level and assigns it nowhere else, and otherwise holds scalar statements
+-- of scalars (no element loads), branches, and map loops offloadable on
+-- their own.  The site takes cur among its arguments and returns the final
+-- state.
+offloadableIterate :: TypeEnv -> [Stmt] -> Stmt -> Maybe (CVar -> HipSite)
+offloadableIterate env whole s@(SLoop spec body) = do
+  guard' (lsRole spec == LoopIterate && lsExec spec == Serial && isNothing (lsRed spec))
+  [t] <- Just (lsIters spec)
+  guard' (not (null body))
+  let step = init body
+  SAssign cur (RAtom (AVar nxt)) <- Just (last body)
+  ct@(CTArray e) <- M.lookup cur env
+  guard' (e `elem` [CTInt64, CTDouble, CTBool])
+  guard' (length [() | SAssign v (RArrayAlloc _) <- step, v == nxt] == 1)
+  guard' (length [() | SAssign v _ <- flatten step, v == nxt] == 1)
+  guard' (cur `notElem` assignedVars step)
+  guard' (t `notElem` usedVars step)
+  guard' (all (stepStmt nxt) step)
+  let assigned = S.fromList (assignedVars [s])
+      used = S.fromList (usedVars [s])
+      free = S.toList ((used `S.difference` assigned) `S.union` S.singleton cur)
+      outside = S.fromList (usedVars (removeStmt s whole))
+      liveOut = (assigned `S.intersection` outside) `S.difference` S.singleton cur
+  guard' (S.null liveOut)
+  args <- mapM (\v -> (,) v <$> argType v) (sort free)
+  let vars = S.toList (assigned `S.union` used)
+      types = M.fromList [(v, ty) | v <- vars, Just ty <- [M.lookup v env], ty /= CTUnknown]
+  pure $ \name -> HipSite {hsName = name, hsArgs = args, hsResult = Just (cur, ct), hsTypes = types, hsLoop = s}
+  where
+    argType v = case M.lookup v env of
+      Just ty@(CTArray el) | el `elem` [CTInt64, CTDouble, CTBool] -> Just ty
+      Just ty | ty `elem` [CTInt64, CTDouble, CTBool, CTTuple] -> Just ty
+      _ -> Nothing
+    scalarVar v = case M.lookup v env of
+      Just ty -> ty `elem` [CTInt64, CTDouble, CTBool, CTTuple]
+      Nothing -> False
+    noLoad r = case r of
+      RArrayLoad _ _ -> False
+      _ -> True
+    stepStmt nxt st = case st of
+      SAssign v (RArrayAlloc _) -> v == nxt
+      SAssign v r -> scalarVar v && supportedRHS r && noLoad r
+      SIf _ a b -> all (stepStmt nxt) a && all (stepStmt nxt) b
+      SLoop _ _ -> case offloadableLoop env whole st of
+        Just mk -> isNothing (hsResult (mk "_"))
+        Nothing -> False
+      _ -> False
+offloadableIterate _ _ _ = Nothing
 
 -- | The loop whose iterations the kernel distributes, its body, and the
 -- reduction accumulator if any.

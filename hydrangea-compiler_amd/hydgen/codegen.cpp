@@ -55,6 +55,7 @@
 
 #include <algorithm>
 #include <functional>
+#include <memory>
 #include <set>
 #include <sstream>
 
@@ -656,6 +657,9 @@ bool groupable(const Plan& p, const Options& opt) {
 }
 
 // ---- printing ----------------------------------------------------------------------
+// the register array of column offset dc in the band form
+std::string dc_name(int64_t dc) { return dc < 0 ? "m" + std::to_string(-dc) : dc > 0 ? "p" + std::to_string(dc) : "0"; }
+
 struct Printer {
     Printer(const Site& s, const Plan& plan, std::ostringstream& out) : site(s), p(plan), o(out) {
         for (const Stmt* st : p.invariant_stmts) hoisted.insert(st);
@@ -671,6 +675,17 @@ struct Printer {
     std::map<const Stmt*, int> write_slot;  // deferred writes: slot per top-level write
     mutable int loop_depth = 0;             // inner loops around the statement being printed
     enum ScatterMode { None, LdsBins, GlobalAtomics, Materialize } smode = None;
+    // The temporal stencil form: loads of `tile_array` read the workgroup's
+    // LDS tile hk_src (the flat index variables in `tile_pos` print as tile
+    // offsets of their (row, column)), and the write of `out_array` sets
+    // hk_val, the cell's new value.
+    std::string tile_array, out_array;
+    const std::map<std::string, std::pair<Atom, Atom>>* tile_pos = nullptr;
+    // ... or, in the band form, from registers: the load at (row + dr,
+    // col + dc) of the cell in row hk_k of the wave's band is
+    // hk_x<dc>[hk_k + reg_rr + dr]
+    const std::map<std::string, std::pair<int64_t, int64_t>>* reg_pos = nullptr;
+    int reg_rr = 0;
 
     std::string name(const std::string& v) const {
         if (!p.acc.empty() && v == p.acc && !acc_expr.empty()) return acc_expr;
@@ -755,6 +770,13 @@ struct Printer {
             case RHS::RArrayLoad: {
                 const std::string& a = A[0].var;
                 const std::string ix = atom(A[1]);
+                if (!tile_array.empty() && a == tile_array) {
+                    if (reg_pos && A[1].is_var() && reg_pos->count(A[1].var)) {
+                        const auto& d = reg_pos->at(A[1].var);
+                        return "hk_x" + dc_name(d.second) + "[hk_k + " + std::to_string(reg_rr + d.first) + "]";
+                    }
+                    return "hk_src[" + ix + "]";
+                }
                 if (stream_load(a)) {
                     if (preloaded) {
                         const std::string reg = "hk_p_" + sanitize(a).substr(2) + "[hk_u >> 1]";
@@ -838,14 +860,24 @@ struct Printer {
             switch (s.kind) {
                 case Stmt::SAssign: {
                     const CType t = var_type(s.var);
-                    line(ind, name(s.var) + " = (" + c_type(t) + ")" + rhs(s.rhs, t) + ";");
+                    if (reg_pos && reg_pos->count(s.var)) {
+                        // the load reads a register: no index
+                    } else if (tile_pos && tile_pos->count(s.var)) {
+                        const auto& rc = tile_pos->at(s.var);
+                        line(ind, name(s.var) + " = (int64_t)((int)(" + atom(rc.first) + " - hk_R0) * hk_LW + (int)(" +
+                                      atom(rc.second) + " - hk_C0));");
+                    } else {
+                        line(ind, name(s.var) + " = (" + c_type(t) + ")" + rhs(s.rhs, t) + ";");
+                    }
                     assigned.insert(s.var);
                     break;
                 }
                 case Stmt::SArrayWrite: {
                     const std::string& a = s.a0.var;
                     const std::string et = c_type(use(a).elem);
-                    if (wmode == Deferred) {
+                    if (!out_array.empty() && a == out_array) {
+                        line(ind, "hk_val = (" + et + ")" + atom(s.a2) + ";");
+                    } else if (wmode == Deferred) {
                         const int slot = write_slot.at(&s);
                         line(ind, "hk_wi" + std::to_string(slot) + "[hk_u] = " + atom(s.a1) + ";");
                         line(ind, "hk_wv" + std::to_string(slot) + "[hk_u] = (" + et + ")" + atom(s.a2) + ";");
@@ -1513,7 +1545,892 @@ void emit_site(std::ostringstream& o, const Site& site, const Options& opt) {
 
 }  // namespace
 
+// ---- an iterate of a stencil step (a LoopIterate site) -----------------------------------
+//
+// The site's loop is the whole `iterate`: Lowering.hs:840-857 lowers it to a
+// serial LoopIterate loop whose body is the step (its statements, one array
+// allocated for the new state, the step's loops writing it) followed by
+// cur = next.  The launcher evaluates the step's scalar statements once (they
+// depend on the state's shape, never on its elements or the counter), finds
+// the step's loops the branches select, and runs the steps:
+//   * temporal form: when the selected loops are one run of 2-index maps that
+//     write the new state at (i + a, j + b), read the old one only at
+//     (i + c, j + d) or at fixed rows/columns near the cell, and exactly tile
+//     the grid, one launch advances K steps over an LDS tile with a K-cell
+//     halo per unit of radius (each cell runs the body of the loop whose box
+//     holds it, reading its neighbours from the tile);
+//   * otherwise every step calls the loops' own launchers in order, into a
+//     zeroed array (the reference allocates a fresh, calloc'd one per step).
+// Two HBM buffers ping-pong; the result is the last one written (the input
+// itself after zero steps, as the reference's loop leaves it).
+namespace {
+
+void rhs_vars(const RHS& r, std::set<std::string>& out) {
+    for (const Atom& a : r.args)
+        if (a.is_var()) out.insert(a.var);
+}
+void index_vars(const IndexExpr& e, std::set<std::string>& out) {
+    if (e.kind == IndexExpr::IVar) out.insert(e.var);
+    for (const auto& k : e.kids) index_vars(k, out);
+}
+// Variables statements read and assign (loop iterators count as assigned).
+void stmt_vars(const std::vector<Stmt>& ss, std::set<std::string>& used, std::set<std::string>& assigned) {
+    for_each_stmt(ss, [&](const Stmt& s) {
+        switch (s.kind) {
+            case Stmt::SAssign: rhs_vars(s.rhs, used); assigned.insert(s.var); break;
+            case Stmt::SArrayWrite:
+                for (const Atom* a : {&s.a0, &s.a1, &s.a2})
+                    if (a->is_var()) used.insert(a->var);
+                break;
+            case Stmt::SIf: case Stmt::SReturn:
+                if (s.a0.is_var()) used.insert(s.a0.var);
+                break;
+            case Stmt::SLoop:
+                for (const auto& b : s.loop.bounds) index_vars(b, used);
+                for (const auto& b : s.loop.origins) index_vars(b, used);
+                if (s.loop.has_red) index_vars(s.loop.red_init, used);
+                for (const auto& it : s.loop.iters) assigned.insert(it);
+                break;
+            default: break;
+        }
+    });
+}
+
+// a*i + b*j + sum(c_v * v) + k over the kernel iterators i, j and
+// loop-invariant integers v
+struct Lin {
+    bool ok = false;
+    int64_t ci = 0, cj = 0, k = 0;
+    std::map<std::string, int64_t> terms;
+};
+Lin lin_sum(const Lin& a, const Lin& b, int64_t sb) {
+    Lin r;
+    if (!a.ok || !b.ok) return r;
+    r.ok = true;
+    r.ci = a.ci + sb * b.ci;
+    r.cj = a.cj + sb * b.cj;
+    r.k = a.k + sb * b.k;
+    r.terms = a.terms;
+    for (const auto& t : b.terms) {
+        r.terms[t.first] += sb * t.second;
+        if (r.terms[t.first] == 0) r.terms.erase(t.first);
+    }
+    return r;
+}
+Lin lin_scale(const Lin& a, int64_t c) {
+    Lin r = a;
+    if (!r.ok) return r;
+    r.ci *= c, r.cj *= c, r.k *= c;
+    for (auto& t : r.terms) t.second *= c;
+    if (c == 0) r.terms.clear();
+    return r;
+}
+
+struct IterMember {
+    Site sub;  // the loop as a site of its own: its arguments are its free variables
+    Plan plan;
+    bool near = false;
+    std::string why;  // why the temporal form does not take it
+    Lin wrow, wcol;   // where it writes the new state
+    std::map<std::string, std::pair<Atom, Atom>> tile_pos;  // flat index variables of its loads
+    std::map<std::string, std::pair<int64_t, int64_t>> rel;  // (row, column) offsets of those that move with the cell
+    int64_t dmin[2] = {0, 0}, dmax[2] = {0, 0};             // offsets of the loads that move with the cell
+    std::vector<Lin> fixed[2];                              // loads at a fixed row / column
+    int radius = 0;
+};
+
+struct IterNode {
+    enum Kind { Host, Member, If } kind = Host;
+    const Stmt* s = nullptr;
+    int member = -1;
+    std::vector<IterNode> then_, else_;
+};
+
+struct IterPlan {
+    std::string t, cur, next;
+    const Stmt* loop = nullptr;
+    const Stmt* alloc = nullptr;
+    CType elem;
+    std::vector<IterNode> nodes;
+    std::vector<std::unique_ptr<IterMember>> members;  // stable: plans point into their sites
+    std::vector<std::vector<int>> leaves;  // runs of member loops with nothing between them
+    std::vector<int> leaf_radius;          // -1: no temporal form for the run
+    std::map<std::string, CType> host_vars;
+    std::vector<std::string> host_order;
+    std::set<std::string> shape_vars;  // host variables holding the state's shape
+};
+
+constexpr int kIterTH = 32, kIterLW = 64, kIterBlock = 512, kIterMaxRadius = 16;
+// output tile width: the 64-column LDS row less the halo on both sides
+int iter_tile_w(int r);
+int iter_steps_per_launch(int r) { return r == 0 ? 8 : std::max(1, 8 / r); }
+int iter_tile_w(int r) { return kIterLW - 2 * iter_steps_per_launch(r) * r; }
+
+bool is_iterate_site(const Site& site) {
+    return site.loop.kind == Stmt::SLoop && site.loop.loop.role == Role::Iterate;
+}
+
+struct IterAnalyzer {
+    const Site& site;
+    IterPlan& ip;
+
+    [[noreturn]] void fail(const std::string& w) const { throw Error("site " + site.name + ": iterate: " + w); }
+
+    CType host_rhs_type(const RHS& r) {
+        auto t = [&](const Atom& a) -> CType {
+            if (a.kind == Atom::Int || a.kind == Atom::Bool || a.kind == Atom::Unit) return CType::scalar(CType::Int64);
+            if (a.kind == Atom::Float) return CType::scalar(CType::Double);
+            if (!a.is_var()) fail("string or vector atom in the step");
+            auto h = ip.host_vars.find(a.var);
+            if (h != ip.host_vars.end()) return h->second;
+            bool found;
+            CType at = arg_type(site, a.var, &found);
+            if (!found) fail("the step reads " + a.var + " before assigning it");
+            return at;
+        };
+        switch (r.kind) {
+            case RHS::RAtom: return t(r.args[0]);
+            case RHS::RBinOp:
+                if (is_compare(r.bop)) return CType::scalar(CType::Bool);
+                return CType::scalar(is_float_op(r.bop) ? CType::Double : CType::Int64);
+            case RHS::RUnOp:
+                switch (r.uop) {
+                    case UnOp::CNot: return CType::scalar(CType::Bool);
+                    case UnOp::CNeg: return t(r.args[0]);
+                    case UnOp::CIntOf: return CType::scalar(CType::Int64);
+                    case UnOp::CLog: case UnOp::CSin: case UnOp::CCos: case UnOp::CErf:
+                        fail("log/sin/cos/erf in the step's scalar statements");
+                    default: return CType::scalar(CType::Double);
+                }
+            case RHS::RTuple: case RHS::RShapeInit: return CType::scalar(CType::Tuple);
+            case RHS::RArrayShape:
+                if (!r.args[0].is_var() || r.args[0].var != ip.cur)
+                    fail("the step takes the shape of an array other than the state");
+                return CType::scalar(CType::Tuple);
+            case RHS::RProj: case RHS::RShapeSize: case RHS::RShapeLast: return CType::scalar(CType::Int64);
+            default: fail("a scalar statement of the step is not one the launcher evaluates");
+        }
+    }
+
+    // The first `count` statements of ss (all of them by default).
+    std::vector<IterNode> block(const std::vector<Stmt>& ss, size_t count = SIZE_MAX) {
+        std::vector<IterNode> out;
+        bool in_run = false;
+        for (size_t si = 0; si < ss.size() && si < count; ++si) {
+            const Stmt& s = ss[si];
+            IterNode n;
+            n.s = &s;
+            if (s.kind == Stmt::SAssign) {
+                if (s.var == ip.next) {
+                    if (&s != ip.alloc) fail(ip.next + " assigned inside a branch");
+                    in_run = false;
+                    continue;
+                }
+                if (ip.host_vars.count(s.var)) fail(s.var + " assigned twice in the step");
+                bool found;
+                (void)arg_type(site, s.var, &found);
+                if (found) fail("the step assigns the argument " + s.var);
+                const CType t = host_rhs_type(s.rhs);
+                ip.host_vars[s.var] = t;
+                ip.host_order.push_back(s.var);
+                if (s.rhs.kind == RHS::RArrayShape ||
+                    (s.rhs.kind == RHS::RAtom && s.rhs.args[0].is_var() && ip.shape_vars.count(s.rhs.args[0].var)))
+                    ip.shape_vars.insert(s.var);
+                n.kind = IterNode::Host;
+                in_run = false;
+            } else if (s.kind == Stmt::SIf) {
+                if (!s.a0.is_var() || !ip.host_vars.count(s.a0.var)) fail("a branch on something other than a step scalar");
+                n.kind = IterNode::If;
+                n.then_ = block(s.body);
+                n.else_ = block(s.els);
+                in_run = false;
+            } else if (s.kind == Stmt::SLoop) {
+                n.kind = IterNode::Member;
+                n.member = member(s);
+                if (!in_run) ip.leaves.emplace_back();
+                ip.leaves.back().push_back(n.member);
+                in_run = true;
+            } else {
+                fail("a statement of the step that is neither scalar, a branch nor a loop");
+            }
+            out.push_back(std::move(n));
+        }
+        return out;
+    }
+
+    int member(const Stmt& loop) {
+        auto mp = std::make_unique<IterMember>();
+        IterMember& m = *mp;
+        m.sub.name = site.name + "_m" + std::to_string(ip.members.size());
+        std::set<std::string> used, assigned;
+        stmt_vars({loop}, used, assigned);
+        for (const std::string& v : used) {
+            if (assigned.count(v)) continue;
+            if (v == ip.t) fail("a loop of the step reads the iteration counter");
+            CType t;
+            auto h = ip.host_vars.find(v);
+            bool found;
+            if (v == ip.next) {
+                t.kind = CType::Array;
+                t.elem = std::make_shared<CType>(ip.elem);
+            } else if (h != ip.host_vars.end()) {
+                t = h->second;
+            } else {
+                t = arg_type(site, v, &found);
+                if (!found) fail("loop reads " + v + ", which is neither an argument nor a step scalar");
+            }
+            if (t.is_array() && v != ip.cur && v != ip.next) {
+                // another array: read or written each step, fine for the in-order form
+            }
+            m.sub.args.push_back({v, t});
+        }
+        m.sub.types = site.types;
+        for (const auto& h : ip.host_vars) m.sub.types[h.first] = h.second;
+        m.sub.loop = loop;
+        m.plan = make_plan(m.sub);
+        if (m.plan.form != Plan::Map1D && m.plan.form != Plan::Map2D && m.plan.form != Plan::MapND)
+            fail("a loop of the step is not a map (" + std::string(form_name(m.plan.form)) + ")");
+        analyze_near(m);
+        ip.members.push_back(std::move(mp));
+        return (int)ip.members.size() - 1;
+    }
+
+    void analyze_near(IterMember& m) {
+        const Plan& p = m.plan;
+        auto no = [&](const std::string& w) { m.near = false, m.why = w; };
+        if (p.form != Plan::Map2D) return no("not a 2-index map");
+        for (const auto& a : p.arrays) {
+            if (a.first == ip.cur) {
+                if (a.second.written) return no("writes the old state");
+            } else if (a.first == ip.next) {
+                if (a.second.read) return no("reads the new state");
+            } else {
+                return no("touches the array " + a.first);
+            }
+        }
+        if (!p.arrays.count(ip.next) || !p.arrays.at(ip.next).written) return no("does not write the new state");
+        const std::vector<Stmt>& body = p.kloop->body;
+        const std::string& I = p.iters[0];
+        const std::string& J = p.iters[1];
+        std::map<std::string, int> count;
+        for_each_stmt(body, [&](const Stmt& s) {
+            if (s.kind == Stmt::SAssign) ++count[s.var];
+            if (s.kind == Stmt::SLoop)
+                for (const auto& it : s.loop.iters) ++count[it];
+        });
+        std::set<std::string> free_ints;
+        for (const auto& a : m.sub.args)
+            if (!a.second.is_array() && a.second.is_int_like() && !count.count(a.first)) free_ints.insert(a.first);
+        std::map<std::string, Lin> form;
+        std::map<std::string, const Stmt*> def;
+        auto atom_lin = [&](const Atom& a) -> Lin {
+            Lin r;
+            if (a.kind == Atom::Int) {
+                r.ok = true, r.k = a.i;
+            } else if (a.is_var()) {
+                if (a.var == I) r.ok = true, r.ci = 1;
+                else if (a.var == J) r.ok = true, r.cj = 1;
+                else if (form.count(a.var)) r = form.at(a.var);
+                else if (free_ints.count(a.var)) r.ok = true, r.terms[a.var] = 1;
+            }
+            return r;
+        };
+        for (const Stmt& s : body) {
+            if (s.kind != Stmt::SAssign || count[s.var] != 1) continue;
+            def[s.var] = &s;
+            const RHS& r = s.rhs;
+            Lin f;
+            if (r.kind == RHS::RAtom) {
+                f = atom_lin(r.args[0]);
+            } else if (r.kind == RHS::RBinOp && (r.bop == BinOp::CAdd || r.bop == BinOp::CSub)) {
+                f = lin_sum(atom_lin(r.args[0]), atom_lin(r.args[1]), r.bop == BinOp::CAdd ? 1 : -1);
+            } else if (r.kind == RHS::RBinOp && r.bop == BinOp::CMul) {
+                if (r.args[0].kind == Atom::Int) f = lin_scale(atom_lin(r.args[1]), r.args[0].i);
+                else if (r.args[1].kind == Atom::Int) f = lin_scale(atom_lin(r.args[0]), r.args[1].i);
+            }
+            if (f.ok) form[s.var] = f;
+        }
+        // (row, column) of a flat index built as nd_to_flat((row, col), shape of the state)
+        auto pos = [&](const Atom& f, Lin& r, Lin& c, std::pair<Atom, Atom>* atoms) -> bool {
+            if (!f.is_var() || !def.count(f.var)) return false;
+            const Stmt* d = def.at(f.var);
+            if (d->rhs.kind != RHS::RNdToFlat) return false;
+            const Atom& tup = d->rhs.args[0];
+            const Atom& shp = d->rhs.args[1];
+            if (!shp.is_var() || !ip.shape_vars.count(shp.var)) return false;
+            if (!tup.is_var() || !def.count(tup.var)) return false;
+            const Stmt* td = def.at(tup.var);
+            if (td->rhs.kind != RHS::RTuple || td->rhs.args.size() != 2) return false;
+            r = atom_lin(td->rhs.args[0]);
+            c = atom_lin(td->rhs.args[1]);
+            if (atoms) *atoms = {td->rhs.args[0], td->rhs.args[1]};
+            return r.ok && c.ok;
+        };
+        int writes = 0;
+        const Stmt* w = nullptr;
+        for_each_stmt(body, [&](const Stmt& s) {
+            if (s.kind == Stmt::SArrayWrite) ++writes;
+        });
+        for (const Stmt& s : body)
+            if (s.kind == Stmt::SArrayWrite) w = &s;
+        if (writes != 1 || !w) return no("not one top-level write of the new state");
+        if (!pos(w->a1, m.wrow, m.wcol, nullptr)) return no("the write is not at nd_to_flat((row, col), shape)");
+        if (!(m.wrow.ci == 1 && m.wrow.cj == 0 && m.wcol.ci == 0 && m.wcol.cj == 1))
+            return no("the write is not at (i + a, j + b)");
+        bool bad = false;
+        std::string why;
+        for_each_stmt(body, [&](const Stmt& s) {
+            if (bad || s.kind != Stmt::SAssign || s.rhs.kind != RHS::RArrayLoad) return;
+            Lin r, c;
+            std::pair<Atom, Atom> rc;
+            if (!pos(s.rhs.args[1], r, c, &rc)) {
+                bad = true, why = "a load index is not nd_to_flat((row, col), shape)";
+                return;
+            }
+            m.tile_pos[s.rhs.args[1].var] = rc;
+            if (r.ci == 1 && c.cj == 1 && r.cj == 0 && c.ci == 0 && r.terms == m.wrow.terms && c.terms == m.wcol.terms)
+                m.rel[s.rhs.args[1].var] = {r.k - m.wrow.k, c.k - m.wcol.k};
+            const Lin* rcw[2] = {&r, &c};
+            const Lin* wl[2] = {&m.wrow, &m.wcol};
+            for (int d = 0; d < 2; ++d) {
+                const Lin& x = *rcw[d];
+                const int64_t own = d == 0 ? x.ci : x.cj, other = d == 0 ? x.cj : x.ci;
+                if (other != 0 || (own != 0 && own != 1)) {
+                    bad = true, why = "a load mixes the row and column iterators";
+                    return;
+                }
+                if (own == 1) {
+                    if (x.terms != wl[d]->terms) {
+                        bad = true, why = "a load's offset from the cell is not a constant";
+                        return;
+                    }
+                    const int64_t delta = x.k - wl[d]->k;
+                    m.dmin[d] = std::min<int64_t>(m.dmin[d], delta);
+                    m.dmax[d] = std::max<int64_t>(m.dmax[d], delta);
+                } else {
+                    m.fixed[d].push_back(x);
+                }
+            }
+        });
+        if (bad) return no(why);
+        // the flat index variables of the loads serve only as load indices
+        // (and possibly the write's): they print as tile offsets
+        for_each_stmt(body, [&](const Stmt& s) {
+            if (bad) return;
+            auto check = [&](const Atom& a, bool ok_here) {
+                if (a.is_var() && m.tile_pos.count(a.var) && !ok_here) bad = true;
+            };
+            if (s.kind == Stmt::SAssign) {
+                for (size_t k = 0; k < s.rhs.args.size(); ++k)
+                    check(s.rhs.args[k], s.rhs.kind == RHS::RArrayLoad && k == 1);
+                if (m.tile_pos.count(s.var) && count[s.var] != 1) bad = true;
+            } else if (s.kind == Stmt::SArrayWrite) {
+                check(s.a0, false), check(s.a1, true), check(s.a2, false);
+            } else if (s.kind == Stmt::SIf) {
+                check(s.a0, false);
+            }
+        });
+        if (bad) return no("a load's index variable is used as a value");
+        int64_t r = 0;
+        for (int d = 0; d < 2; ++d) r = std::max<int64_t>(r, std::max<int64_t>(-m.dmin[d], m.dmax[d]));
+        if (r > kIterMaxRadius) return no("radius above " + std::to_string(kIterMaxRadius));
+        m.radius = (int)r;
+        m.near = true;
+    }
+
+    void run() {
+        const Stmt& L = site.loop;
+        if (L.loop.iters.size() != 1 || L.loop.bounds.size() != 1) fail("one counter expected");
+        if (L.loop.exec != LoopSpec::Serial) fail("the iterate loop must be serial");
+        if (L.loop.has_red) fail("a reduction on the iterate loop");
+        ip.loop = &L;
+        ip.t = L.loop.iters[0];
+        if (!site.has_result || !site.result.second.is_array() || !site.result.second.elem)
+            fail("the site returns the iterated array");
+        ip.cur = site.result.first;
+        bool found;
+        const CType ct = arg_type(site, ip.cur, &found);
+        if (!found || !ct.is_array() || !ct.elem) fail(ip.cur + " is not an array argument of the site");
+        ip.elem = *ct.elem;
+        (void)c_type(ip.elem);
+        const std::vector<Stmt>& body = L.body;
+        if (body.empty()) fail("an empty step");
+        const Stmt& last = body.back();
+        if (!(last.kind == Stmt::SAssign && last.var == ip.cur && last.rhs.kind == RHS::RAtom &&
+              last.rhs.args[0].is_var()))
+            fail("the step does not end with " + ip.cur + " = <new state>");
+        ip.next = last.rhs.args[0].var;
+        const std::vector<Stmt> step(body.begin(), body.end() - 1);  // for the variable checks only
+        std::set<std::string> used, assigned;
+        stmt_vars(step, used, assigned);
+        if (used.count(ip.t)) fail("the step reads the iteration counter");
+        if (assigned.count(ip.cur)) fail("the step assigns " + ip.cur + " before its end");
+        int allocs = 0;
+        for_each_stmt(step, [&](const Stmt& s) {
+            if (s.kind == Stmt::SAssign && s.var == ip.next) ++allocs;
+        });
+        for (size_t k = 0; k + 1 < body.size(); ++k)
+            if (body[k].kind == Stmt::SAssign && body[k].var == ip.next && body[k].rhs.kind == RHS::RArrayAlloc)
+                ip.alloc = &body[k];
+        if (allocs != 1 || !ip.alloc) fail("the step must allocate " + ip.next + " once, at its top level");
+        ip.nodes = block(body, body.size() - 1);  // nodes point into the site's own loop
+        const Atom& shp = ip.alloc->rhs.args[0];
+        if (!shp.is_var() || !ip.shape_vars.count(shp.var))
+            fail(ip.next + " is not allocated with the shape of " + ip.cur);
+        for (const auto& leaf : ip.leaves) {
+            int r = 0;
+            bool ok = leaf.size() <= 16;
+            for (int k : leaf) {
+                ok = ok && ip.members[(size_t)k]->near;
+                r = std::max(r, ip.members[(size_t)k]->radius);
+            }
+            ip.leaf_radius.push_back(ok ? r : -1);
+        }
+    }
+};
+
+IterPlan make_iter_plan(const Site& site) {
+    IterPlan ip;
+    IterAnalyzer{site, ip}.run();
+    return ip;
+}
+
+std::string lin_host(const Lin& l, const std::string& prefix) {
+    std::string s = "(int64_t)" + std::to_string(l.k) + "LL";
+    for (const auto& t : l.terms)
+        s += " + (int64_t)" + std::to_string(t.second) + "LL * " + prefix + sanitize(t.first);
+    return "(" + s + ")";
+}
+
+// The temporal kernel of a run (hk_tk_<run>): tiles whose every computed
+// cell lies in one loop's box (all but the grid's edge tiles of a split
+// stencil) run that loop's body alone in the band form; the edge tiles,
+// listed by the launcher, each cell finding its loop.
+void emit_iter_temporal(std::ostringstream& o, const Site& site, const IterPlan& ip, size_t L) {
+    const std::vector<int>& leaf = ip.leaves[L];
+    const int R = ip.leaf_radius[L];
+    const int K = iter_steps_per_launch(R), H = K * R;
+    const int LW = kIterLW, TW = LW - 2 * H, LH = kIterTH + 2 * H;
+    const int WAVES = kIterBlock / 64, RPW = (LH + WAVES - 1) / WAVES;
+    const size_t M = leaf.size();
+    const std::string T = c_type(ip.elem);
+    const std::string tag = site.name + "_" + std::to_string(L);
+    bool exp = false;
+    for (int k : leaf) exp = exp || ip.members[(size_t)k]->plan.uses_exp;
+    // loops that may own a whole tile: those whose loads all move with the
+    // cell (a loop with loads at a fixed row or column, a clamped boundary,
+    // is at most the halo wide there)
+    std::vector<size_t> uni;
+    for (size_t k = 0; k < M; ++k) {
+        const IterMember& m = *ip.members[(size_t)leaf[k]];
+        if (m.fixed[0].empty() && m.fixed[1].empty()) uni.push_back(k);
+    }
+    o << "// temporal form of " << site.name << " run " << L << ": radius " << R << ", " << K
+      << " steps per launch over " << kIterTH << " x " << TW << " tiles with a " << H << "-cell halo (LDS rows of "
+      << LW << ": a lane per column, a wave per row)\n";
+    o << "struct hk_tb_" << tag << " {\n    int64_t rlo[" << M << "], rhi[" << M << "], clo[" << M << "], chi[" << M
+      << "], wr[" << M << "], wc[" << M << "];\n};\n";
+    // one member's body for the cell (hk_r, hk_c), its value into hk_val
+    auto body = [&](size_t k, int ind) {
+        const IterMember& m = *ip.members[(size_t)leaf[k]];
+        const std::string ks = std::to_string(k);
+        const std::string pad((size_t)ind * 4, ' ');
+        for (const Param& q : kernel_params(m.sub, m.plan))
+            o << pad << "const " << q.ctype << " " << q.name << " = hk_P" << k << "." << q.name << ";\n"
+              << pad << "(void)" << q.name << ";\n";
+        Printer pr(m.sub, m.plan, o);
+        pr.tile_array = ip.cur;
+        pr.out_array = ip.next;
+        pr.tile_pos = &m.tile_pos;
+        pr.declare_locals(ind);
+        o << pad << "const int64_t " << sanitize(m.plan.iters[0]) << " = hk_r - hk_tb.wr[" << ks << "];\n";
+        o << pad << "const int64_t " << sanitize(m.plan.iters[1]) << " = hk_c - hk_tb.wc[" << ks << "];\n";
+        std::set<std::string> assigned;
+        pr.stmts(m.plan.kloop->body, ind, assigned);
+    };
+    // kind 0: the band kernel over every tile (a tile no loop owns whole
+    // returns); kind 1: the edge kernel over the launcher's list of those
+    for (int kind = 0; kind < 2; ++kind) {
+    o << "__global__ __launch_bounds__(" << kIterBlock << ") void hk_tk" << (kind ? "e" : "") << "_" << tag
+      << "(const " << T << "* __restrict__ hk_in, " << T
+      << "* __restrict__ hk_out, int64_t hk_g0, int64_t hk_g1, int hk_steps, hk_tb_" << tag << " hk_tb";
+    for (size_t k = 0; k < M; ++k) o << ", hk_m_" << ip.members[(size_t)leaf[k]]->sub.name << " hk_P" << k;
+    if (kind) {
+        o << ", const uint32_t* __restrict__ hk_tiles, uint32_t hk_ntx) {\n";
+        o << "    const uint32_t hk_tile = hk_tiles[blockIdx.x];\n";
+        o << "    const int64_t hk_bx = hk_tile % hk_ntx, hk_by = hk_tile / hk_ntx;\n";
+    } else {
+        o << ") {\n";
+        o << "    const int64_t hk_bx = blockIdx.x, hk_by = blockIdx.y;\n";
+    }
+    o << "    constexpr int hk_TH = " << kIterTH << ", hk_TW = " << TW << ", hk_H = " << H << ", hk_RR = " << R << ";\n";
+    o << "    constexpr int hk_LH = " << LH << ", hk_LW = " << LW << ", hk_NC = hk_LH * hk_LW, hk_WAVES = " << WAVES
+      << ", hk_RPW = " << RPW << ";\n";
+    o << "    const int64_t hk_R0 = hk_by * hk_TH - hk_H, hk_C0 = hk_bx * hk_TW - hk_H;\n";
+    // the loop whose box holds every cell the workgroup computes (its first
+    // step's region), if one does (the launcher's edge list applies the
+    // same test)
+    o << "    int hk_uni = -1;\n";
+    o << "    {\n";
+    o << "        const int64_t hk_a0 = max(hk_R0 + hk_RR, (int64_t)0), hk_a1 = min(hk_R0 + hk_LH - hk_RR, hk_g0);\n";
+    o << "        const int64_t hk_b0 = max(hk_C0 + hk_RR, (int64_t)0), hk_b1 = min(hk_C0 + hk_LW - hk_RR, hk_g1);\n";
+    bool first_u = true;
+    for (size_t k : uni) {
+        const std::string ks = std::to_string(k);
+        o << "        " << (first_u ? "" : "else ") << "if (hk_a0 >= hk_tb.rlo[" << ks << "] && hk_a1 <= hk_tb.rhi[" << ks
+          << "] && hk_b0 >= hk_tb.clo[" << ks << "] && hk_b1 <= hk_tb.chi[" << ks << "]) hk_uni = " << ks << ";\n";
+        first_u = false;
+    }
+    o << "    }\n";
+    if (kind) o << "    (void)hk_uni;\n";
+    else o << "    if (hk_uni < 0) return;\n";
+    o << "    __shared__ " << T << " hk_tile0[hk_NC], hk_tile1[hk_NC];\n";
+    if (exp)
+        o << "    __shared__ uint64_t hk_exptab[256];\n"
+             "    for (int hk_t = threadIdx.x; hk_t < 256; hk_t += blockDim.x) hk_exptab[hk_t] = hyd_exp_tab_dev[hk_t];\n";
+    o << "    const int hk_lane = threadIdx.x & 63, hk_wave = threadIdx.x >> 6;\n";
+    o << "    const int64_t hk_c = hk_C0 + hk_lane;\n";
+    o << "    const bool hk_cin = hk_c >= 0 && hk_c < hk_g1;\n";
+    o << "#pragma unroll\n";
+    o << "    for (int hk_k = 0; hk_k < hk_RPW; ++hk_k) {\n";
+    o << "        const int hk_lr = hk_wave + hk_k * hk_WAVES;\n";
+    o << "        const int64_t hk_r = hk_R0 + hk_lr;\n";
+    o << "        if (hk_lr < hk_LH && hk_cin && hk_r >= 0 && hk_r < hk_g0) hk_tile0[hk_lr * hk_LW + hk_lane] = hk_in[hk_r * hk_g1 + hk_c];\n";
+    o << "    }\n";
+    o << "    __syncthreads();\n";
+    o << "    " << T << "* hk_src = hk_tile0;\n    " << T << "* hk_dst = hk_tile1;\n";
+    o << "    for (int hk_s = 1; hk_s <= hk_steps; ++hk_s) {\n";
+    o << "        const int64_t hk_vr0 = max(hk_R0 + (int64_t)hk_s * hk_RR, (int64_t)0);\n";
+    o << "        const int64_t hk_vr1 = min(hk_R0 + hk_LH - (int64_t)hk_s * hk_RR, hk_g0);\n";
+    o << "        const bool hk_cok = hk_c >= max(hk_C0 + (int64_t)hk_s * hk_RR, (int64_t)0) &&\n";
+    o << "                            hk_c < min(hk_C0 + hk_LW - (int64_t)hk_s * hk_RR, hk_g1);\n";
+    if (kind) {
+    o << "        {\n";
+    o << "            for (int hk_lr = hk_wave; hk_lr < hk_LH; hk_lr += hk_WAVES) {\n";
+    o << "                const int64_t hk_r = hk_R0 + hk_lr;\n";
+    o << "                if (!hk_cok || hk_r < hk_vr0 || hk_r >= hk_vr1) continue;\n";
+    o << "                const int hk_q = hk_lr * hk_LW + hk_lane;\n";
+    o << "                " << T << " hk_val = hk_src[hk_q];\n";
+    for (size_t k = 0; k < M; ++k) {
+        const std::string ks = std::to_string(k);
+        o << "                " << (k ? "else " : "") << "if (hk_r >= hk_tb.rlo[" << ks << "] && hk_r < hk_tb.rhi[" << ks
+          << "] && hk_c >= hk_tb.clo[" << ks << "] && hk_c < hk_tb.chi[" << ks << "]) {\n";
+        body(k, 5);
+        o << "                }\n";
+    }
+    o << "                hk_dst[hk_q] = hk_val;\n";
+    o << "            }\n";
+    } else {
+    if (uni.empty()) o << "        {\n";
+    // band form: wave w owns rows [w * RPW, (w + 1) * RPW) of the tile; per
+    // step it reads its rows and R above and below into registers once
+    // (rows clamped into the tile: the clamped ones feed only cells that
+    // are not stored), takes the column neighbours by lane shifts, and
+    // computes every row of the band
+    for (size_t u = 0; u < uni.size(); ++u) {
+        const size_t k = uni[u];
+        const IterMember& m = *ip.members[(size_t)leaf[k]];
+        std::set<int64_t> dcs = {0};
+        for (const auto& rp : m.rel) dcs.insert(rp.second.second);
+        o << "        " << (u ? "} else " : "") << "if (hk_uni == " << k << ") {\n";
+        o << "            const int hk_lr0 = hk_wave * hk_RPW;\n";
+        o << "            " << T << " hk_x0[hk_RPW + 2 * hk_RR];\n";
+        o << "#pragma unroll\n";
+        o << "            for (int hk_q = 0; hk_q < hk_RPW + 2 * hk_RR; ++hk_q)\n";
+        o << "                hk_x0[hk_q] = hk_src[min(max(hk_lr0 - hk_RR + hk_q, 0), hk_LH - 1) * hk_LW + hk_lane];\n";
+        for (int64_t dc : dcs) {
+            if (dc == 0) continue;
+            o << "            " << T << " hk_x" << dc_name(dc) << "[hk_RPW + 2 * hk_RR];\n";
+            o << "#pragma unroll\n";
+            o << "            for (int hk_q = 0; hk_q < hk_RPW + 2 * hk_RR; ++hk_q) hk_x" << dc_name(dc)
+              << "[hk_q] = hydk_lane_shift<" << dc << ">(hk_x0[hk_q]);\n";
+        }
+        o << "#pragma unroll\n";
+        o << "            for (int hk_k = 0; hk_k < hk_RPW; ++hk_k) {\n";
+        o << "                const int hk_lr = hk_lr0 + hk_k;\n";
+        o << "                const int64_t hk_r = hk_R0 + hk_lr;\n";
+        o << "                " << T << " hk_val = hk_x0[hk_k + hk_RR];\n";
+        o << "                {\n";
+        {
+            const std::string ks = std::to_string(k);
+            const std::string pad((size_t)5 * 4, ' ');
+            for (const Param& q : kernel_params(m.sub, m.plan))
+                o << pad << "const " << q.ctype << " " << q.name << " = hk_P" << k << "." << q.name << ";\n"
+                  << pad << "(void)" << q.name << ";\n";
+            Printer pr(m.sub, m.plan, o);
+            pr.tile_array = ip.cur;
+            pr.out_array = ip.next;
+            pr.tile_pos = &m.tile_pos;
+            pr.reg_pos = &m.rel;
+            pr.reg_rr = R;
+            pr.declare_locals(5);
+            o << pad << "const int64_t " << sanitize(m.plan.iters[0]) << " = hk_r - hk_tb.wr[" << ks << "];\n";
+            o << pad << "const int64_t " << sanitize(m.plan.iters[1]) << " = hk_c - hk_tb.wc[" << ks << "];\n";
+            std::set<std::string> assigned;
+            pr.stmts(m.plan.kloop->body, 5, assigned);
+        }
+        o << "                }\n";
+        o << "                if (hk_cok && hk_lr < hk_LH && hk_r >= hk_vr0 && hk_r < hk_vr1)\n";
+        o << "                    hk_dst[hk_lr * hk_LW + hk_lane] = hk_val;\n";
+        o << "            }\n";
+    }
+    }
+    o << "        }\n";
+    o << "        __syncthreads();\n";
+    o << "        " << T << "* hk_t = hk_src;\n        hk_src = hk_dst;\n        hk_dst = hk_t;\n";
+    o << "    }\n";
+    o << "    if (hk_lane >= hk_H && hk_lane < hk_H + hk_TW && hk_cin)\n";
+    o << "        for (int hk_lr = hk_H + hk_wave; hk_lr < hk_H + hk_TH; hk_lr += hk_WAVES) {\n";
+    o << "            const int64_t hk_r = hk_R0 + hk_lr;\n";
+    o << "            if (hk_r < hk_g0) hk_out[hk_r * hk_g1 + hk_c] = hk_src[hk_lr * hk_LW + hk_lane];\n";
+    o << "        }\n";
+    o << "}\n\n";
+    }
+}
+
+// The launcher's attempt at the temporal form for run L (inside the
+// launcher, after the step scalars): true when it ran every step.
+void emit_iter_try(std::ostringstream& o, const IterPlan& ip, size_t L, const std::string& name) {
+    const std::vector<int>& leaf = ip.leaves[L];
+    const int R = ip.leaf_radius[L];
+    const int K = iter_steps_per_launch(R), TW = iter_tile_w(R), H = K * R, LH = kIterTH + 2 * H;
+    const size_t M = leaf.size();
+    const std::string tag = name + "_" + std::to_string(L);
+    const std::string T = c_type(ip.elem);
+    o << "    if (!hk_done && hk_nact == " << M;
+    for (size_t k = 0; k < M; ++k) o << " && hk_act[" << k << "] == " << leaf[k];
+    o << ") {\n";
+    o << "        " << sanitize(ip.cur) << " = hk_cur_in;\n        " << sanitize(ip.next) << " = hk_buf[0];\n";
+    for (size_t k = 0; k < M; ++k) {
+        const IterMember& m = *ip.members[(size_t)leaf[k]];
+        std::string args;
+        for (const auto& a : m.sub.args) args += (args.empty() ? "" : ", ") + sanitize(a.first);
+        o << "        hk_x_" << m.sub.name << " hk_x" << k << ";\n";
+        o << "        const hk_m_" << m.sub.name << " hk_P" << k << " = hk_prep_" << m.sub.name << "(" << args
+          << (args.empty() ? "" : ", ") << "&hk_x" << k << ");\n";
+    }
+    o << "        const int64_t hk_g0 = hk_shape.elems[0], hk_g1 = hk_shape.ndims == 2 ? hk_shape.elems[1] : 0;\n";
+    o << "        bool hk_ok = hk_shape.ndims == 2 && hk_g0 > 0 && hk_g1 > 0 && (hk_g0 + " << kIterTH - 1 << ") / "
+      << kIterTH << " <= 65535 && (hk_g1 + " << TW - 1 << ") / " << TW << " <= 0x7fffffff;\n";
+    o << "        hk_tb_" << tag << " hk_tb;\n        memset(&hk_tb, 0, sizeof hk_tb);\n";
+    o << "        int64_t hk_area = 0;\n";
+    for (size_t k = 0; k < M; ++k) {
+        const IterMember& m = *ip.members[(size_t)leaf[k]];
+        const std::string ks = std::to_string(k), P = "hk_P" + ks + ".";
+        o << "        if (hk_x" << ks << ".hk_live) {\n";
+        o << "            hk_tb.wr[" << ks << "] = " << lin_host(m.wrow, P) << ";\n";
+        o << "            hk_tb.wc[" << ks << "] = " << lin_host(m.wcol, P) << ";\n";
+        o << "            hk_tb.rlo[" << ks << "] = " << P << "hk_lo0 + hk_tb.wr[" << ks << "];\n";
+        o << "            hk_tb.rhi[" << ks << "] = hk_tb.rlo[" << ks << "] + " << P << "hk_n0;\n";
+        o << "            hk_tb.clo[" << ks << "] = " << P << "hk_lo1 + hk_tb.wc[" << ks << "];\n";
+        o << "            hk_tb.chi[" << ks << "] = hk_tb.clo[" << ks << "] + " << P << "hk_n1;\n";
+        o << "            hk_ok = hk_ok && hk_tb.rlo[" << ks << "] >= 0 && hk_tb.rhi[" << ks << "] <= hk_g0 && hk_tb.clo[" << ks
+          << "] >= 0 && hk_tb.chi[" << ks << "] <= hk_g1;\n";
+        // loads that move with the cell stay inside the grid
+        o << "            hk_ok = hk_ok && hk_tb.rlo[" << ks << "] + (" << m.dmin[0] << ") >= 0 && hk_tb.rhi[" << ks << "] - 1 + ("
+          << m.dmax[0] << ") < hk_g0 && hk_tb.clo[" << ks << "] + (" << m.dmin[1] << ") >= 0 && hk_tb.chi[" << ks
+          << "] - 1 + (" << m.dmax[1] << ") < hk_g1;\n";
+        // loads at a fixed row / column: inside the grid and within the radius of every cell
+        for (int d = 0; d < 2; ++d)
+            for (const Lin& f : m.fixed[d]) {
+                const std::string lo = d == 0 ? "hk_tb.rlo[" + ks + "]" : "hk_tb.clo[" + ks + "]";
+                const std::string hi = d == 0 ? "hk_tb.rhi[" + ks + "]" : "hk_tb.chi[" + ks + "]";
+                const std::string g = d == 0 ? "hk_g0" : "hk_g1";
+                o << "            { const int64_t hk_v = " << lin_host(f, P) << ";\n";
+                o << "              hk_ok = hk_ok && hk_v >= 0 && hk_v < " << g << " && hk_v - " << lo << " <= " << R
+                  << " && " << lo << " - hk_v <= " << R << " && hk_v - (" << hi << " - 1) <= " << R << " && (" << hi
+                  << " - 1) - hk_v <= " << R << "; }\n";
+            }
+        o << "            if (hk_tb.rhi[" << ks << "] > hk_tb.rlo[" << ks << "] && hk_tb.chi[" << ks << "] > hk_tb.clo[" << ks
+          << "]) hk_area += (hk_tb.rhi[" << ks << "] - hk_tb.rlo[" << ks << "]) * (hk_tb.chi[" << ks << "] - hk_tb.clo[" << ks
+          << "]);\n";
+        o << "        }\n";
+    }
+    // the boxes tile the grid: disjoint, and their areas add up to it
+    o << "        for (int hk_a = 0; hk_ok && hk_a < " << M << "; ++hk_a)\n";
+    o << "            for (int hk_b = hk_a + 1; hk_ok && hk_b < " << M << "; ++hk_b)\n";
+    o << "                hk_ok = !(hk_tb.rlo[hk_a] < hk_tb.rhi[hk_b] && hk_tb.rlo[hk_b] < hk_tb.rhi[hk_a] &&\n";
+    o << "                          hk_tb.clo[hk_a] < hk_tb.chi[hk_b] && hk_tb.clo[hk_b] < hk_tb.chi[hk_a] &&\n";
+    o << "                          hk_tb.rhi[hk_a] > hk_tb.rlo[hk_a] && hk_tb.chi[hk_a] > hk_tb.clo[hk_a] &&\n";
+    o << "                          hk_tb.rhi[hk_b] > hk_tb.rlo[hk_b] && hk_tb.chi[hk_b] > hk_tb.clo[hk_b]);\n";
+    o << "        hk_ok = hk_ok && hk_g0 <= 0x7fffffffLL && hk_g1 <= 0x7fffffffLL && hk_area == hk_g0 * hk_g1;\n";
+    o << "        const dim3 hk_grid(hk_ok ? (unsigned)((hk_g1 + " << TW - 1 << ") / " << TW << ") : 0u, hk_ok ? (unsigned)((hk_g0 + "
+      << kIterTH - 1 << ") / " << kIterTH << ") : 0u);\n";
+    o << "        std::vector<uint32_t> hk_edge;\n";
+    o << "        if (hk_ok) {\n";
+    // the tiles no loop owns whole (the kernel's own test, once per call)
+    o << "            for (uint32_t hk_by = 0; hk_by < hk_grid.y; ++hk_by)\n";
+    o << "                for (uint32_t hk_bx = 0; hk_bx < hk_grid.x; ++hk_bx) {\n";
+    o << "                    const int64_t hk_R0 = (int64_t)hk_by * " << kIterTH << " - " << H << ", hk_C0 = (int64_t)hk_bx * "
+      << TW << " - " << H << ";\n";
+    o << "                    const int64_t hk_a0 = std::max<int64_t>(hk_R0 + " << R << ", 0), hk_a1 = std::min<int64_t>(hk_R0 + "
+      << LH - R << ", hk_g0);\n";
+    o << "                    const int64_t hk_b0 = std::max<int64_t>(hk_C0 + " << R << ", 0), hk_b1 = std::min<int64_t>(hk_C0 + "
+      << kIterLW - R << ", hk_g1);\n";
+    o << "                    bool hk_whole = false;\n";
+    for (size_t k = 0; k < M; ++k) {
+        const IterMember& m = *ip.members[(size_t)leaf[k]];
+        if (!(m.fixed[0].empty() && m.fixed[1].empty())) continue;
+        const std::string ks = std::to_string(k);
+        o << "                    hk_whole = hk_whole || (hk_a0 >= hk_tb.rlo[" << ks << "] && hk_a1 <= hk_tb.rhi[" << ks
+          << "] && hk_b0 >= hk_tb.clo[" << ks << "] && hk_b1 <= hk_tb.chi[" << ks << "]);\n";
+    }
+    o << "                    if (!hk_whole) hk_edge.push_back(hk_by * hk_grid.x + hk_bx);\n";
+    o << "                }\n";
+    o << "            if (hk_edge.size() > 0x7fffffffu) hk_ok = false;\n";
+    o << "        }\n";
+    o << "        if (hk_ok) {\n";
+    o << "            uint32_t* hk_edge_d = (uint32_t*)hydk_scratch(3, (int64_t)std::max<size_t>(hk_edge.size(), 1) * 4, hk_s);\n";
+    o << "            if (!hk_edge.empty() && (hipMemcpyAsync(hk_edge_d, hk_edge.data(), hk_edge.size() * 4, hipMemcpyHostToDevice, hk_s) != hipSuccess ||\n";
+    o << "                                     hipStreamSynchronize(hk_s) != hipSuccess))\n";
+    o << "                hydk_fail(hk_site, \"uploading the edge tiles failed\");\n";
+    o << "            for (int64_t hk_left = hk_steps; hk_left > 0;) {\n";
+    o << "                const int hk_k = (int)std::min<int64_t>(" << K << ", hk_left);\n";
+    o << "                if (!hk_buf[hk_w]) hk_buf[hk_w] = hk_alloc(hk_shape, hk_site);\n";
+    for (int kind = 0; kind < 2; ++kind) {
+        if (kind)
+            o << "                if (!hk_edge.empty()) hk_tke_" << tag << "<<<(unsigned)hk_edge.size(), " << kIterBlock
+              << ", 0, hk_s>>>(";
+        else
+            o << "                hk_tk_" << tag << "<<<hk_grid, " << kIterBlock << ", 0, hk_s>>>(";
+        o << "(const " << T << "*)hydk_device_data(hk_src), (" << T << "*)hydk_device_data(hk_buf[hk_w]), hk_g0, hk_g1, hk_k, hk_tb";
+        for (size_t k = 0; k < M; ++k) o << ", hk_P" << k;
+        if (kind) o << ", hk_edge_d, hk_grid.x";
+        o << ");\n";
+        o << "                hydk_check_launch(hk_site);\n";
+    }
+    o << "                hydk_device_written(hk_buf[hk_w]);\n";
+    o << "                hk_src = hk_buf[hk_w];\n                hk_w ^= 1;\n                hk_left -= hk_k;\n";
+    o << "            }\n";
+    o << "            hk_done = true;\n";
+    o << "        }\n";
+    o << "    }\n";
+}
+
+void emit_iter_nodes(std::ostringstream& o, Printer& hostp, const std::vector<IterNode>& nodes, int ind) {
+    for (const IterNode& n : nodes) {
+        const std::string pad((size_t)ind * 4, ' ');
+        if (n.kind == IterNode::Host) {
+            const CType& t = hostp.p.locals.at(n.s->var);
+            o << pad << sanitize(n.s->var) << " = (" << c_type(t) << ")" << hostp.rhs(n.s->rhs, t) << ";\n";
+        } else if (n.kind == IterNode::Member) {
+            o << pad << "hk_act[hk_nact++] = " << n.member << ";\n";
+        } else {
+            o << pad << "if (" << sanitize(n.s->a0.var) << ") {\n";
+            emit_iter_nodes(o, hostp, n.then_, ind + 1);
+            o << pad << "} else {\n";
+            emit_iter_nodes(o, hostp, n.else_, ind + 1);
+            o << pad << "}\n";
+        }
+    }
+}
+
+void emit_iterate_site(std::ostringstream& o, const Site& site, const Options& opt) {
+    const IterPlan ip = make_iter_plan(site);
+    o << "// ---- " << site.name << ": iterate of " << ip.members.size() << " loop(s)";
+    for (size_t L = 0; L < ip.leaves.size(); ++L)
+        o << "; run " << L << " " << (ip.leaf_radius[L] >= 0 ? "temporal" : "in order");
+    o << "\n";
+    for (const auto& m : ip.members) emit_site(o, m->sub, opt);
+    for (size_t L = 0; L < ip.leaves.size(); ++L)
+        if (ip.leaf_radius[L] >= 0) emit_iter_temporal(o, site, ip, L);
+    const std::string T = c_type(ip.elem);
+    std::string lparams;
+    for (const auto& a : site.args) {
+        lparams += lparams.empty() ? "" : ", ";
+        const std::string nm = a.first == ip.cur ? "hk_cur_in" : sanitize(a.first);
+        lparams += a.second.is_array() ? "hyd_array_t* " + nm : std::string(c_type(a.second)) + " " + nm;
+    }
+    // host evaluation of the step's scalars: a printer over a plan whose
+    // locals are those scalars
+    Plan hp;
+    for (const auto& h : ip.host_vars) hp.locals[h.first] = h.second;
+    Printer hostp(site, hp, o);
+    hostp.host = true;
+    o << "static hyd_array_t* hk_alloc_" << site.name << "(hyd_tuple_t hk_shape, const char* hk_site) {\n";
+    o << "    hyd_array_t* hk_a = hyd_array_alloc_device(hk_shape, sizeof(" << T << "));\n";
+    o << "    if (!hk_a) hydk_fail(hk_site, \"no memory for the iterate's state\");\n";
+    o << "    return hk_a;\n}\n";
+    // which form the last call took (0: no step, 1: in order, 2: temporal),
+    // for tests and profiles
+    o << "static int hk_form_" << site.name << " = 0;\n";
+    o << "extern \"C\" int " << site.name << "__form(void) { return hk_form_" << site.name << "; }\n";
+    o << "extern \"C\" hyd_array_t* " << site.name << "(" << lparams << ") {\n";
+    o << "    const char* hk_site = \"" << site.name << "\";\n";
+    o << "    hipStream_t hk_s = hydk_stream();\n";
+    o << "    const int64_t hk_steps = " << hostp.index_expr(ip.loop->loop.bounds[0]) << ";\n";
+    o << "    hk_form_" << site.name << " = 0;\n";
+    o << "    if (hk_steps <= 0) return hk_cur_in;\n";
+    o << "    auto hk_alloc = [](hyd_tuple_t hk_sh, const char* hk_nm) { return hk_alloc_" << site.name
+      << "(hk_sh, hk_nm); };\n";
+    o << "    hyd_array_t* " << sanitize(ip.cur) << " = hk_cur_in;\n";
+    o << "    hyd_array_t* " << sanitize(ip.next) << " = nullptr;\n";
+    for (const std::string& v : ip.host_order) {
+        const CType& t = ip.host_vars.at(v);
+        o << "    " << c_type(t) << " " << sanitize(v) << (t.kind == CType::Tuple ? " = hydk_tuple(0);\n" : " = 0;\n");
+    }
+    o << "    int hk_act[" << std::max<size_t>(1, ip.members.size()) << "];\n    int hk_nact = 0;\n";
+    emit_iter_nodes(o, hostp, ip.nodes, 1);
+    o << "    const hyd_tuple_t hk_shape = hk_cur_in->shape;\n";
+    o << "    hyd_array_t* hk_buf[2] = {hk_alloc(hk_shape, hk_site), nullptr};\n";
+    o << "    hyd_array_t* hk_src = hk_cur_in;\n";
+    o << "    int hk_w = 0;\n";
+    o << "    bool hk_done = false;\n";
+    for (size_t L = 0; L < ip.leaves.size(); ++L)
+        if (ip.leaf_radius[L] >= 0) emit_iter_try(o, ip, L, site.name);
+    // in order, step by step, each step into a zeroed array
+    o << "    hk_form_" << site.name << " = hk_done ? 2 : 1;\n";
+    o << "    if (!hk_done) {\n";
+    o << "        const int64_t hk_bytes = hyd_shape_size(hk_shape) * (int64_t)sizeof(" << T << ");\n";
+    o << "        for (int64_t hk_t = 0; hk_t < hk_steps; ++hk_t) {\n";
+    o << "            if (!hk_buf[hk_w]) hk_buf[hk_w] = hk_alloc(hk_shape, hk_site);\n";
+    o << "            if (hk_bytes > 0 && hipMemsetAsync(hydk_device_data(hk_buf[hk_w]), 0, (size_t)hk_bytes, hk_s) != hipSuccess)\n";
+    o << "                hydk_fail(hk_site, \"clearing the new state failed\");\n";
+    o << "            hydk_device_written(hk_buf[hk_w]);\n";
+    o << "            " << sanitize(ip.cur) << " = hk_src;\n";
+    o << "            " << sanitize(ip.next) << " = hk_buf[hk_w];\n";
+    o << "            for (int hk_a = 0; hk_a < hk_nact; ++hk_a) {\n";
+    o << "                switch (hk_act[hk_a]) {\n";
+    for (size_t k = 0; k < ip.members.size(); ++k) {
+        const IterMember& m = *ip.members[k];
+        std::string args;
+        for (const auto& a : m.sub.args) args += (args.empty() ? "" : ", ") + sanitize(a.first);
+        o << "                    case " << k << ": (void)" << m.sub.name << "(" << args << "); break;\n";
+    }
+    o << "                    default: break;\n";
+    o << "                }\n";
+    o << "            }\n";
+    o << "            hk_src = hk_buf[hk_w];\n";
+    o << "            hk_w ^= 1;\n";
+    o << "        }\n";
+    o << "    }\n";
+    o << "    (void)" << sanitize(ip.cur) << ";\n    (void)" << sanitize(ip.next) << ";\n";
+    o << "    for (int hk_b = 0; hk_b < 2; ++hk_b)\n";
+    o << "        if (hk_buf[hk_b] && hk_buf[hk_b] != hk_src) hyd_array_free(hk_buf[hk_b]);\n";
+    o << "    return hk_src;\n}\n\n";
+}
+
+std::string describe_iterate(const Site& site) {
+    const IterPlan ip = make_iter_plan(site);
+    std::ostringstream o;
+    o << "form=iterate state=" << ip.cur << " next=" << ip.next << " loops=" << ip.members.size() << " runs=";
+    for (size_t L = 0; L < ip.leaves.size(); ++L) {
+        o << (L ? "," : "") << ip.leaves[L].size();
+        if (ip.leaf_radius[L] >= 0)
+            o << "(temporal radius=" << ip.leaf_radius[L] << " steps=" << iter_steps_per_launch(ip.leaf_radius[L])
+              << " tile=" << kIterTH << "x" << iter_tile_w(ip.leaf_radius[L]) << ")";
+        else
+            o << "(in order)";
+    }
+    for (size_t k = 0; k < ip.members.size(); ++k)
+        if (!ip.members[k]->near) o << " loop" << k << "=" << ip.members[k]->why;
+    return o.str();
+}
+
+}  // namespace
+
 std::string describe_plan(const Site& site, const Options& opt) {
+    if (is_iterate_site(site)) return describe_iterate(site);
     const Plan p = make_plan(site);
     std::ostringstream o;
     o << "form=" << form_name(p.form) << " iters=" << p.iters.size();
@@ -1716,7 +2633,10 @@ std::string emit_sites(const std::vector<Site>& sites, const std::vector<Group>&
     for (const Site& s : sites) {
         if (!names.insert(s.name).second) throw Error("two sites named " + s.name);
         by_name[s.name] = &s;
-        emit_site(o, s, opt);
+        if (is_iterate_site(s))
+            emit_iterate_site(o, s, opt);
+        else
+            emit_site(o, s, opt);
     }
     std::set<std::string> grouped;
     for (const Group& g : groups) {
@@ -1755,7 +2675,8 @@ std::string emit_header(const std::vector<Site>& sites) {
          "#ifndef HYDGEN_SITES_H\n#define HYDGEN_SITES_H\n#include <stdint.h>\n#include \"hydrangea_runtime.h\"\n"
          "#ifdef __cplusplus\nextern \"C\" {\n#endif\n";
     for (const Site& s : sites) {
-        const std::string ret = s.has_result ? c_type(s.result.second) : "int64_t";
+        const std::string ret =
+            !s.has_result ? "int64_t" : s.result.second.is_array() ? "hyd_array_t*" : c_type(s.result.second);
         std::string ps;
         for (const auto& a : s.args) {
             ps += ps.empty() ? "" : ", ";

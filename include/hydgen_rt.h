@@ -22,6 +22,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "hyd_exp.h"
 #include "hyd_hip_runtime.h"
@@ -164,6 +165,25 @@ struct hydk_add_f64_op { __device__ double operator()(double a, double b) const 
 struct hydk_mul_f64_op { __device__ double operator()(double a, double b) const { return a * b; } };
 struct hydk_add_i64_op { __device__ int64_t operator()(int64_t a, int64_t b) const { return hydk_add_i64(a, b); } };
 struct hydk_mul_i64_op { __device__ int64_t operator()(int64_t a, int64_t b) const { return hydk_mul_i64(a, b); } };
+
+/* The value lane + D of the wave holds, D = +-1, +-2, ... (|D| wave shifts
+ * through DPP: wave_shl:1 moves every value one lane down, so lane i reads
+ * lane i + 1; wave_shr:1 one lane up).  Lanes with no source lane (the last
+ * |D| for D > 0, the first |D| for D < 0) read 0 bits.  64-bit values move
+ * as two 32-bit halves. */
+template <int D, class T>
+__device__ static inline T hydk_lane_shift(T v) {
+    static_assert(sizeof(T) == 8, "64-bit values");
+    uint64_t b = __builtin_bit_cast(uint64_t, v);
+    int lo = (int)(uint32_t)b, hi = (int)(uint32_t)(b >> 32);
+    constexpr int ctrl = D > 0 ? 0x130 : 0x138; /* wave_shl:1 / wave_shr:1 */
+#pragma unroll
+    for (int k = 0; k < (D > 0 ? D : -D); ++k) {
+        lo = __builtin_amdgcn_update_dpp(0, lo, ctrl, 0xf, 0xf, false);
+        hi = __builtin_amdgcn_update_dpp(0, hi, ctrl, 0xf, 0xf, false);
+    }
+    return __builtin_bit_cast(T, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
 
 /* xor butterfly over the 64 lanes of a wave: every lane ends with the same
  * value (each step combines two equal-length partial sums, and IEEE + and *

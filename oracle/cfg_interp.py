@@ -131,10 +131,19 @@ def nd_to_flat(nd, shp):
 
 
 class Interp:
-    """Runs one HipSite's loop over `env` (name -> value)."""
+    """Runs one HipSite's loop over `env` (name -> value).  `types` (the
+    site's hsTypes) gives the element type of the arrays RArrayAlloc makes:
+    zero-filled, as hyd_array_alloc's calloc (runtime/hydrangea_runtime.h:
+    292-310)."""
 
-    def __init__(self, env):
+    def __init__(self, env, types=None):
         self.env = env
+        self.types = types or {}
+
+    def alloc(self, var, shape):
+        t = self.types.get(var)
+        elem = t.args[0].name if t is not None and t.name == "CTArray" else "CTInt64"
+        return np.zeros(tuple(shape), dtype=np.float64 if elem == "CTDouble" else np.int64)
 
     def atom(self, a):
         k = a.name
@@ -206,7 +215,10 @@ class Interp:
         for s in ss:
             k = s.name
             if k == "SAssign":
-                self.env[s.args[0]] = self.rhs(s.args[1])
+                if s.args[1].name == "RArrayAlloc":
+                    self.env[s.args[0]] = self.alloc(s.args[0], self.atom(s.args[1].args[0]))
+                else:
+                    self.env[s.args[0]] = self.rhs(s.args[1])
                 assigned.add(s.args[0])
             elif k == "SArrayWrite":
                 arr = self.atom(s.args[0])
@@ -253,8 +265,9 @@ class Interp:
 
 def run_site(site, env):
     """Executes site.hsLoop over env (arrays written in place).  Returns the
-    accumulator for a reduction site, else None."""
-    interp = Interp(env)
+    site's result (a reduction's accumulator, an iterate's final array), else
+    None."""
+    interp = Interp(env, site.get("hsTypes"))
     assigned = set(name for name, _ in site.get("hsArgs"))
     interp.stmts([site.get("hsLoop")], assigned)
     res = site.get("hsResult")

@@ -32,6 +32,8 @@ Programs (loop shapes from src/Language/Hydrangea/Lowering.hs):
                around a LoopReduction (:724-760)
   gather       out[i] = src[perm[i]] (:1860-1898)
   map3d        a 3-index generate over an N-D shape (RTuple + RNdToFlat)
+  heat_iterate iterate of the clamp heat step (the LoopIterate loop around
+               jacobi_step's region loops, a small-grid branch, u = out)
 """
 import os
 import sys
@@ -126,8 +128,9 @@ FLOAT_BIN = {"CAddF", "CSubF", "CMulF", "CDivF"}
 CMP = {"CEq", "CNeq", "CLt", "CLe", "CGt", "CGe", "CAnd", "COr", "CEqF", "CNeqF", "CLtF", "CLeF", "CGtF", "CGeF"}
 
 
-def infer_types(args, lp, omit=()):
+def infer_types(args, lp, omit=(), extra=()):
     env = dict(args)
+    env.update(dict(extra))  # arrays RArrayAlloc creates: their element type is not in the RHS
 
     def atype(a):
         if a.name == "AVar":
@@ -166,8 +169,8 @@ def infer_types(args, lp, omit=()):
     return {k: v for k, v in env.items() if k not in omit}
 
 
-def site(name, args, lp, result=None, omit_types=()):
-    types = infer_types(args, lp, omit_types)
+def site(name, args, lp, result=None, omit_types=(), extra_types=()):
+    types = infer_types(args, lp, omit_types, extra_types)
     return R("HipSite", [
         ("hsName", name),
         ("hsArgs", [(v, t) for v, t in args]),
@@ -425,10 +428,56 @@ def map3d():
                                   ("src", arr(T_F))], lp)]
 
 
+def heat_iterate():
+    """iterate steps (fn u => stencil clamp heat u) u0: Lowering.hs:840-857 lowers
+    the iterate to a serial LoopIterate loop whose step is the stencil's statements
+    followed by u = out.  The step: the input's shape, the new array (RArrayAlloc),
+    the extents and clamp bounds of lowerSplitStencil2D (:2562-2600), then a branch
+    on both extents being at least the footprint (2): the 9 region loops of
+    jacobi_step, or for smaller grids the generic flat loop over shape_size
+    (:1997-2030) with genBnd1DVar's BClamp index arithmetic per access
+    (:2760-2770)."""
+    regions = [s.get("hsLoop") for s in jacobi_step()]
+    small = [assign("i0", bop("CDiv", "i", "n1")), assign("itmp", bop("CMul", "i0", "n1")),
+             assign("i1", bop("CSub", "i", "itmp"))]
+    for (dr, dc), res in HEAT_ACCESSES:
+        comps = []
+        for dim, (ia, na, d) in enumerate((("i0", "n0", dr), ("i1", "n1", dc))):
+            raw, bi, lt0, nm1, gen = ("s%s_%d_%s" % (x, dim, res) for x in ("raw", "bi", "lt0", "nm1", "geN"))
+            small += [assign(raw, bop("CAdd", ia, d)), assign(bi, atom(raw)), assign(lt0, bop("CLt", raw, 0)),
+                      sif(lt0, [assign(bi, atom(0))]), assign(nm1, bop("CSub", na, 1)),
+                      assign(gen, bop("CGe", bi, na)), sif(gen, [assign(bi, atom(nm1))])]
+            comps.append(bi)
+        small += [assign("sidx_" + res, C("RTuple", [atomize(comps[0]), atomize(comps[1])])),
+                  assign("sflat_" + res, C("RNdToFlat", atomize("sidx_" + res), atomize("shp"))),
+                  assign(res, load("u", "sflat_" + res))]
+    small += heat_body("val") + [write("out", "i", "val")]
+    step = [
+        assign("shp", C("RArrayShape", atomize("u"))),
+        assign("out", C("RArrayAlloc", atomize("shp"))),
+        assign("n", C("RShapeSize", atomize("shp"))),
+        assign("n0", C("RProj", 0, atomize("shp"))),
+        assign("n1", C("RProj", 1, atomize("shp"))),
+        assign("n0m1", bop("CSub", "n0", 1)),
+        assign("n1m1", bop("CSub", "n1", 1)),
+        assign("bd_start_r", bop("CSub", "n0", 1)),
+        assign("bd_start_c", bop("CSub", "n1", 1)),
+        assign("rows_large", bop("CGe", "n0", 2)),
+        assign("cols_large", bop("CGe", "n1", 2)),
+        assign("large", bop("CAnd", "rows_large", "cols_large")),
+        sif("large", regions, [loop(["i"], ["n"], small, par(), role="LoopMap")]),
+        assign("u", atom("out")),
+    ]
+    lp = loop(["t"], ["steps"], step, SERIAL, role="LoopIterate")
+    return [site("hydk_heat_iterate", [("steps", T_I), ("u", arr(T_F))], lp, result=("u", arr(T_F)),
+                 extra_types=[("out", arr(T_F))])]
+
+
 PROGRAMS = {
     "map_chain": map_chain, "map_chain2": map_chain2, "dot": dot, "reduce_i64": reduce_i64,
     "jacobi_step": jacobi_step, "wrap_stencil": wrap_stencil, "mirror_1d": mirror_1d, "histogram": histogram,
     "splat_f64": splat_f64, "row_sums": row_sums, "gather": gather, "map3d": map3d,
+    "heat_iterate": heat_iterate,
 }
 
 

@@ -228,6 +228,45 @@ def test_jacobi_group_iterate(hyd, orc, prog, arrays):
     assert np.array_equal(bits(arrays.read(a, np.float64)), bits(orc.iterate_stencil2d(u, steps, 0, 0.1)))
 
 
+def run_heat_iterate(prog, arrays, u, steps):
+    """hydk_heat_iterate(steps, u) as the printed C calls it: the result (the
+    caller's to free), whether it is the input array itself, and which form
+    ran (0: no step, 1: the loops' launchers in order, 2: temporal)."""
+    import ctypes
+
+    inp = arrays.host(u)
+    res = prog.hydk_heat_iterate(steps, inp)
+    same = ctypes.addressof(res.contents) == ctypes.addressof(inp.contents)
+    if not same:
+        arrays.made.append(res)
+    form = prog._cdll.hydk_heat_iterate__form()
+    return arrays.read(res, np.float64), same, form
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (1, 9), (7, 1), (2, 2), (3, 5), (33, 65), (37, 91), (130, 70), (257, 1031)])
+@pytest.mark.parametrize("steps", [0, 1, 7, 8, 9, 17])
+def test_heat_iterate_generated(hyd, orc, prog, arrays, shape, steps):
+    # the whole iterate in one launcher: grids of at least 2 x 2 take the
+    # temporal kernel (8 steps per launch, tiles with a halo), smaller ones the
+    # small-grid loop's launcher step by step; zero steps return the input
+    print("heat iterate %dx%d steps=%d" % (shape + (steps,)), flush=True)
+    u = orc.splitmix_f64(shape[0] * shape[1], 4001 + 31 * shape[0] + shape[1]).reshape(shape)
+    got, same, form = run_heat_iterate(prog, arrays, u, steps)
+    assert same == (steps == 0)
+    assert form == (0 if steps == 0 else 2 if min(shape) >= 2 else 1)
+    assert np.array_equal(bits(got.reshape(shape)), bits(orc.iterate_stencil2d(u, steps, 0, 0.1)))
+
+
+def test_heat_iterate_generated_config_grid(hyd, orc, prog, arrays):
+    c = workloads.JACOBI
+    h, w, steps = c["h"], c["w"], 20
+    print("generated heat iterate %dx%d steps=%d" % (h, w, steps), flush=True)
+    u = orc.splitmix_f64(h * w, c["seed"]).reshape(h, w)
+    got, same, form = run_heat_iterate(prog, arrays, u, steps)
+    assert not same and form == 2
+    assert np.array_equal(bits(got.reshape(h, w)), bits(orc.iterate_stencil2d(u, steps, 0, c["alpha"])))
+
+
 @pytest.mark.parametrize("n,sh", [(1000, 10), (1000, 0), (1000, 1000), (5, 7), (100_003, 517)])
 def test_group_in_order_when_dependent(hyd, prog, arrays, n, sh):
     # out = xs + 1; out[sh:] = 2 xs[sh:] (overlapping writes); ys = out + xs

@@ -45,7 +45,7 @@ EXPECTED_FORM = {
     "map_chain": "form=map1d", "map_chain2": "form=map1d", "dot": "form=reduce", "reduce_i64": "form=reduce",
     "jacobi_step": "form=map2d", "wrap_stencil": "form=map1d", "mirror_1d": "form=map1d",
     "histogram": "form=scatter_i64", "splat_f64": "form=scatter_f64", "row_sums": "form=map1d",
-    "gather": "form=map1d", "map3d": "form=mapnd", "group_overlap": "form=map1d",
+    "gather": "form=map1d", "map3d": "form=mapnd", "group_overlap": "form=map1d", "heat_iterate": "form=iterate",
 }
 
 
@@ -105,6 +105,63 @@ def test_generated_header_compiles_as_c99(tmp_path):
 def test_signatures():
     r = hydgen(os.path.join(CFG_DIR, "dot.sites"), "--signatures", "/dev/stdout")
     assert r.stdout.strip() == "hydk_dot_0 f64 : f64:acc i64:red_dim array:xs array:ys"
+
+
+def test_iterate_plan_and_launcher(tmp_path):
+    # the 9 region loops are one run the temporal form takes (radius 1, 8
+    # steps per launch); the small-grid loop (a flat 1-index map) runs in order
+    p = hydgen(os.path.join(CFG_DIR, "heat_iterate.sites"), "--plan").stdout.strip()
+    assert p.startswith("hydk_heat_iterate: form=iterate state=u next=out loops=10 "), p
+    assert "runs=9(temporal radius=1 steps=8 tile=32x48),1(in order)" in p and "loop9=not a 2-index map" in p
+    r = hydgen(os.path.join(CFG_DIR, "heat_iterate.sites"), "--signatures", "/dev/stdout")
+    assert r.stdout.strip() == "hydk_heat_iterate array : i64:steps array:u"
+    hdr = tmp_path / "it.h"
+    assert hydgen(os.path.join(CFG_DIR, "heat_iterate.sites"), "--header", str(hdr)).returncode == 0
+    assert "hyd_array_t* hydk_heat_iterate(int64_t v_steps, hyd_array_t* v_u);" in hdr.read_text()
+    c = tmp_path / "use.c"
+    c.write_text('#include "it.h"\nint main(void) { return 0; }\n')
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"), "-I" + str(tmp_path),
+                        "-c", str(c), "-o", str(tmp_path / "use.o")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    src = tmp_path / "it.hip"
+    assert hydgen(os.path.join(CFG_DIR, "heat_iterate.sites"), "-o", str(src)).returncode == 0
+    text = src.read_text()
+    # one temporal kernel, the region loops' own launchers for the in-order form
+    assert text.count("__global__ __launch_bounds__(512) void hk_tk_hydk_heat_iterate_0(") == 1
+    assert 'extern "C" int64_t hydk_heat_iterate_m9(' in text and "hipMemsetAsync" in text
+
+
+def _iterate_with(step, args=(("steps", None), ("u", None)), result="u"):
+    P = H.cfg_programs
+    types = {"steps": P.T_I, "u": P.arr(P.T_F)}
+    lp = P.loop(["t"], ["steps"], step, P.SERIAL, role="LoopIterate")
+    return show(P.site("hydk_bad_iter", [(a, types[a]) for a, _ in args], lp, result=(result, P.arr(P.T_F)),
+                       extra_types=[("out", P.arr(P.T_F))])) + "\n"
+
+
+def _copy_loop(extra=()):
+    P = H.cfg_programs
+    return P.loop(["i"], ["n"], list(extra) + [P.assign("x", P.load("u", "i")), P.write("out", "i", "x")], P.par())
+
+
+_HEAD = [H.cfg_programs.assign("shp", C("RArrayShape", C("AVar", "u"))),
+         H.cfg_programs.assign("out", C("RArrayAlloc", C("AVar", "shp"))),
+         H.cfg_programs.assign("n", C("RShapeSize", C("AVar", "shp")))]
+
+
+@pytest.mark.parametrize("step,why", [
+    (_HEAD + [_copy_loop()], "does not end with"),
+    (_HEAD + [_copy_loop([H.cfg_programs.assign("k", H.cfg_programs.atom("t"))]),
+              H.cfg_programs.assign("u", H.cfg_programs.atom("out"))], "iteration counter"),
+    (_HEAD[:1] + [_copy_loop(), H.cfg_programs.assign("u", H.cfg_programs.atom("out"))], "allocate out once"),
+    (_HEAD + [H.cfg_programs.assign("y", H.cfg_programs.load("u", 0)), _copy_loop(),
+              H.cfg_programs.assign("u", H.cfg_programs.atom("out"))], "not one the launcher evaluates"),
+])
+def test_refused_iterates(tmp_path, step, why):
+    f = tmp_path / "bad.sites"
+    f.write_text(_iterate_with(step))
+    r = hydgen(str(f), "-o", str(tmp_path / "bad.hip"))
+    assert r.returncode == 1 and why in r.stderr, r.stderr
 
 
 # ---- refusals --------------------------------------------------------------------
@@ -205,6 +262,18 @@ def test_interp_jacobi_regions_equal_c_oracle(orc, shape):
     u = orc.splitmix_f64(shape[0] * shape[1], 21 + shape[1]).reshape(shape)
     got = H.jacobi_interp(u)
     assert np.array_equal(got.view(np.uint64), orc.stencil2d(u, 0, 0.1).view(np.uint64))
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (1, 6), (5, 1), (2, 2), (3, 5), (9, 13)])
+@pytest.mark.parametrize("steps", [0, 1, 3])
+def test_interp_heat_iterate_equals_c_oracle(orc, shape, steps):
+    # the iterate site as a whole: the branch picks the 9 region loops or the
+    # small-grid loop, u = out ping-pongs, zero steps leave the input itself
+    u = orc.splitmix_f64(shape[0] * shape[1], 41 + 7 * shape[0] + shape[1]).reshape(shape)
+    env = dict(steps=steps, u=u.copy())
+    got = H.run_interp(H.site("heat_iterate"), env)
+    assert got.shape == shape
+    assert np.array_equal(got.view(np.uint64), orc.iterate_stencil2d(u, steps, 0, 0.1).view(np.uint64))
 
 
 def test_planted_clamp_mistake_fails(orc):
