@@ -436,6 +436,22 @@ def bench_generated(steps, warmup, cpu=True):
     res["heat_iterate"] = dict(steps_per_call=iters, ms=ms, us_per_step=ms * 1e3 / iters,
                                form="temporal" if form == 2 else "in order" if form == 1 else "none",
                                gbs_algorithmic=16 * hh * ww * iters / ms / 1e6)
+    # the radius-2 iterate (9-point cross, 25 region loops; 4 steps per
+    # launch) on the same grid, 40 steps per call; its parity is the tests'
+    ms_w = []
+    for k in range(1 + steps):
+        e0, e1 = hydhip.Event(), hydhip.Event()
+        e0.record()
+        r = prog.hydk_wide_iterate(40, u)
+        e1.record()
+        hydhip.device_sync()
+        form = prog._cdll.hydk_wide_iterate__form()
+        if k >= 1:
+            ms_w.append(e0.elapsed_ms(e1))
+        L.hyd_array_free(r)
+    ms = float(np.mean(ms_w))
+    res["wide_iterate"] = dict(steps_per_call=40, ms=ms, us_per_step=ms * 1e3 / 40,
+                               form="temporal" if form == 2 else "in order" if form == 1 else "none")
     for a in made:
         L.hyd_array_free(a)
     return res

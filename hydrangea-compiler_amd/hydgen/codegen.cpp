@@ -1979,7 +1979,7 @@ struct IterAnalyzer {
             fail(ip.next + " is not allocated with the shape of " + ip.cur);
         for (const auto& leaf : ip.leaves) {
             int r = 0;
-            bool ok = leaf.size() <= 16;
+            bool ok = leaf.size() <= 64;
             for (int k : leaf) {
                 ok = ok && ip.members[(size_t)k]->near;
                 r = std::max(r, ip.members[(size_t)k]->radius);
@@ -2030,13 +2030,18 @@ void emit_iter_temporal(std::ostringstream& o, const Site& site, const IterPlan&
       << LW << ": a lane per column, a wave per row)\n";
     o << "struct hk_tb_" << tag << " {\n    int64_t rlo[" << M << "], rhi[" << M << "], clo[" << M << "], chi[" << M
       << "], wr[" << M << "], wc[" << M << "];\n};\n";
+    // the kernels' arguments beyond the arrays: the boxes and every loop's
+    // parameters, one block the launcher uploads once per call
+    o << "struct hk_ta_" << tag << " {\n    hk_tb_" << tag << " tb;\n";
+    for (size_t k = 0; k < M; ++k) o << "    hk_m_" << ip.members[(size_t)leaf[k]]->sub.name << " P" << k << ";\n";
+    o << "};\n";
     // one member's body for the cell (hk_r, hk_c), its value into hk_val
     auto body = [&](size_t k, int ind) {
         const IterMember& m = *ip.members[(size_t)leaf[k]];
         const std::string ks = std::to_string(k);
         const std::string pad((size_t)ind * 4, ' ');
         for (const Param& q : kernel_params(m.sub, m.plan))
-            o << pad << "const " << q.ctype << " " << q.name << " = hk_P" << k << "." << q.name << ";\n"
+            o << pad << "const " << q.ctype << " " << q.name << " = hk_A->P" << k << "." << q.name << ";\n"
               << pad << "(void)" << q.name << ";\n";
         Printer pr(m.sub, m.plan, o);
         pr.tile_array = ip.cur;
@@ -2053,8 +2058,8 @@ void emit_iter_temporal(std::ostringstream& o, const Site& site, const IterPlan&
     for (int kind = 0; kind < 2; ++kind) {
     o << "__global__ __launch_bounds__(" << kIterBlock << ") void hk_tk" << (kind ? "e" : "") << "_" << tag
       << "(const " << T << "* __restrict__ hk_in, " << T
-      << "* __restrict__ hk_out, int64_t hk_g0, int64_t hk_g1, int hk_steps, hk_tb_" << tag << " hk_tb";
-    for (size_t k = 0; k < M; ++k) o << ", hk_m_" << ip.members[(size_t)leaf[k]]->sub.name << " hk_P" << k;
+      << "* __restrict__ hk_out, int64_t hk_g0, int64_t hk_g1, int hk_steps, const hk_ta_" << tag
+      << "* __restrict__ hk_A";
     if (kind) {
         o << ", const uint32_t* __restrict__ hk_tiles, uint32_t hk_ntx) {\n";
         o << "    const uint32_t hk_tile = hk_tiles[blockIdx.x];\n";
@@ -2063,6 +2068,7 @@ void emit_iter_temporal(std::ostringstream& o, const Site& site, const IterPlan&
         o << ") {\n";
         o << "    const int64_t hk_bx = blockIdx.x, hk_by = blockIdx.y;\n";
     }
+    o << "    const hk_tb_" << tag << "& hk_tb = hk_A->tb;\n";
     o << "    constexpr int hk_TH = " << kIterTH << ", hk_TW = " << TW << ", hk_H = " << H << ", hk_RR = " << R << ";\n";
     o << "    constexpr int hk_LH = " << LH << ", hk_LW = " << LW << ", hk_NC = hk_LH * hk_LW, hk_WAVES = " << WAVES
       << ", hk_RPW = " << RPW << ";\n";
@@ -2155,7 +2161,7 @@ void emit_iter_temporal(std::ostringstream& o, const Site& site, const IterPlan&
             const std::string ks = std::to_string(k);
             const std::string pad((size_t)5 * 4, ' ');
             for (const Param& q : kernel_params(m.sub, m.plan))
-                o << pad << "const " << q.ctype << " " << q.name << " = hk_P" << k << "." << q.name << ";\n"
+                o << pad << "const " << q.ctype << " " << q.name << " = hk_A->P" << k << "." << q.name << ";\n"
                   << pad << "(void)" << q.name << ";\n";
             Printer pr(m.sub, m.plan, o);
             pr.tile_array = ip.cur;
@@ -2280,10 +2286,20 @@ void emit_iter_try(std::ostringstream& o, const IterPlan& ip, size_t L, const st
     o << "            if (hk_edge.size() > 0x7fffffffu) hk_ok = false;\n";
     o << "        }\n";
     o << "        if (hk_ok) {\n";
-    o << "            uint32_t* hk_edge_d = (uint32_t*)hydk_scratch(3, (int64_t)std::max<size_t>(hk_edge.size(), 1) * 4, hk_s);\n";
-    o << "            if (!hk_edge.empty() && (hipMemcpyAsync(hk_edge_d, hk_edge.data(), hk_edge.size() * 4, hipMemcpyHostToDevice, hk_s) != hipSuccess ||\n";
-    o << "                                     hipStreamSynchronize(hk_s) != hipSuccess))\n";
-    o << "                hydk_fail(hk_site, \"uploading the edge tiles failed\");\n";
+    // the argument block, then the edge list, in one scratch buffer
+    o << "            hk_ta_" << tag << " hk_ta;\n";
+    o << "            hk_ta.tb = hk_tb;\n";
+    for (size_t k = 0; k < M; ++k) o << "            hk_ta.P" << k << " = hk_P" << k << ";\n";
+    o << "            const size_t hk_ta_bytes = (sizeof hk_ta + 255) / 256 * 256;\n";
+    o << "            std::vector<unsigned char> hk_up(hk_ta_bytes + hk_edge.size() * 4);\n";
+    o << "            memcpy(hk_up.data(), &hk_ta, sizeof hk_ta);\n";
+    o << "            if (!hk_edge.empty()) memcpy(hk_up.data() + hk_ta_bytes, hk_edge.data(), hk_edge.size() * 4);\n";
+    o << "            unsigned char* hk_up_d = (unsigned char*)hydk_scratch(3, (int64_t)hk_up.size(), hk_s);\n";
+    o << "            if (hipMemcpyAsync(hk_up_d, hk_up.data(), hk_up.size(), hipMemcpyHostToDevice, hk_s) != hipSuccess ||\n";
+    o << "                hipStreamSynchronize(hk_s) != hipSuccess)\n";
+    o << "                hydk_fail(hk_site, \"uploading the kernel arguments failed\");\n";
+    o << "            const hk_ta_" << tag << "* hk_ta_d = (const hk_ta_" << tag << "*)hk_up_d;\n";
+    o << "            const uint32_t* hk_edge_d = (const uint32_t*)(hk_up_d + hk_ta_bytes);\n";
     o << "            for (int64_t hk_left = hk_steps; hk_left > 0;) {\n";
     o << "                const int hk_k = (int)std::min<int64_t>(" << K << ", hk_left);\n";
     o << "                if (!hk_buf[hk_w]) hk_buf[hk_w] = hk_alloc(hk_shape, hk_site);\n";
@@ -2293,8 +2309,7 @@ void emit_iter_try(std::ostringstream& o, const IterPlan& ip, size_t L, const st
               << ", 0, hk_s>>>(";
         else
             o << "                hk_tk_" << tag << "<<<hk_grid, " << kIterBlock << ", 0, hk_s>>>(";
-        o << "(const " << T << "*)hydk_device_data(hk_src), (" << T << "*)hydk_device_data(hk_buf[hk_w]), hk_g0, hk_g1, hk_k, hk_tb";
-        for (size_t k = 0; k < M; ++k) o << ", hk_P" << k;
+        o << "(const " << T << "*)hydk_device_data(hk_src), (" << T << "*)hydk_device_data(hk_buf[hk_w]), hk_g0, hk_g1, hk_k, hk_ta_d";
         if (kind) o << ", hk_edge_d, hk_grid.x";
         o << ");\n";
         o << "                hydk_check_launch(hk_site);\n";

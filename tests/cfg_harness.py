@@ -92,6 +92,25 @@ def wrap_numpy(u):
     return (0.2 * (n + s) + w * e) - u / 3.0
 
 
+def wide_step_numpy(u):
+    """One step of the radius-2 clamp heat (cfg_programs.wide_iterate) in
+    numpy: neighbours by edge padding, the body's IEEE operations in its
+    order."""
+    h, w = u.shape
+    p = np.pad(u, 2, mode="edge")
+    a = lambda dr, dc: p[2 + dr:2 + dr + h, 2 + dc:2 + dc + w]  # noqa: E731
+    q3 = (a(-1, 0) + a(1, 0)) + (a(0, -1) + a(0, 1))
+    q6 = (a(-2, 0) + a(2, 0)) + (a(0, -2) + a(0, 2))
+    q10 = (16.0 * q3 - q6) - 60.0 * u
+    return u + cfg_programs.WIDE_COEF * q10
+
+
+def wide_iterate_numpy(u, steps):
+    for _ in range(steps):
+        u = wide_step_numpy(u)
+    return u
+
+
 def mirror_numpy(v):
     p = np.pad(v, (1, 2), mode="reflect")  # mirror without repeating the edge
     left, centre, right = p[0:-3], p[1:-2], p[3:]

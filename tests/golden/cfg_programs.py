@@ -34,6 +34,8 @@ Programs (loop shapes from src/Language/Hydrangea/Lowering.hs):
   map3d        a 3-index generate over an N-D shape (RTuple + RNdToFlat)
   heat_iterate iterate of the clamp heat step (the LoopIterate loop around
                jacobi_step's region loops, a small-grid branch, u = out)
+  wide_iterate iterate of a radius-2 clamp stencil (4th-order heat on a
+               9-point cross): 25 region loops, a small-grid branch
 """
 import os
 import sys
@@ -473,11 +475,117 @@ def heat_iterate():
                  extra_types=[("out", arr(T_F))])]
 
 
+# The 4th-order heat body on a 9-point cross of radius 2:
+# a 0 0 +. 0.01/12 *. (16 *. (a(-1)0 +. a 1 0 +. a 0(-1) +. a 0 1) -. (a(-2)0 +. a 2 0 +. a 0(-2) +. a 0 2)
+#                      -. 60 *. a 0 0)
+WIDE_ACCESSES = [((-1, 0), "b_n1"), ((1, 0), "b_s1"), ((0, -1), "b_w1"), ((0, 1), "b_e1"), ((-2, 0), "b_n2"),
+                 ((2, 0), "b_s2"), ((0, -2), "b_w2"), ((0, 2), "b_e2"), ((0, 0), "b_c")]
+WIDE_COEF = 0.01 / 12.0
+
+
+def wide_body(res):
+    return [
+        assign("q1", bop("CAddF", "b_n1", "b_s1")), assign("q2", bop("CAddF", "b_w1", "b_e1")),
+        assign("q3", bop("CAddF", "q1", "q2")), assign("q4", bop("CAddF", "b_n2", "b_s2")),
+        assign("q5", bop("CAddF", "b_w2", "b_e2")), assign("q6", bop("CAddF", "q4", "q5")),
+        assign("q7", bop("CMulF", 16.0, "q3")), assign("q8", bop("CSubF", "q7", "q6")),
+        assign("q9", bop("CMulF", 60.0, "b_c")), assign("q10", bop("CSubF", "q8", "q9")),
+        assign("q11", bop("CMulF", WIDE_COEF, "q10")), assign(res, bop("CAddF", "b_c", "q11")),
+    ]
+
+
+def wide_iterate():
+    """iterate steps (fn u => stencil clamp heat4 u) u0 with a footprint of
+    radius 2: lowerSplitStencil2D (:2562-2665) splits rows and columns into a
+    2-row/column left edge (one segment per row, offsets 0 and 1), the interior
+    (offset 2, n - 4 iterations) and a 2-row/column right edge (offsets n - 2,
+    n - 1): 25 region loops, each access's clamp folded per segment as
+    specializeClampIndex1D does (an index past the edge becomes the edge,
+    any other stays iterator + offset); grids below 4 x 4 take the generic flat
+    loop with genBnd1DVar's BClamp arithmetic.  The step ends with u = out
+    (Lowering.hs:840-857)."""
+    segs0 = [("l0", 1, 0), ("l1", 1, 1), ("in", C("IAdd", C("IVar", "n0"), C("IConst", -4)), 2),
+             ("r0", 1, "bd_r0"), ("r1", 1, "bd_r1")]
+    segs1 = [("l0", 1, 0), ("l1", 1, 1), ("in", C("IAdd", C("IVar", "n1"), C("IConst", -4)), 2),
+             ("r0", 1, "bd_c0"), ("r1", 1, "bd_c1")]
+    regions = []
+    for rk, rbound, roff in segs0:
+        for ck, cbound, coff in segs1:
+            body = []
+
+            def shift(it, off, tag):
+                if off == 0:
+                    return it
+                v = "%s_%s" % (it, tag)
+                body.append(assign(v, bop("CAdd", it, off)))
+                return v
+
+            ra = shift("i", roff, "r")
+            ca = shift("j", coff, "c")
+
+            def fold(kind, base, off, nm1, tag):
+                # segment l0 / l1 is row (column) 0 / 1, r0 / r1 is n - 2 /
+                # n - 1: an access past the edge folds to the edge
+                if kind in ("l0", "l1") and int(kind[1]) + off < 0:
+                    return 0
+                if kind in ("r0", "r1") and off > 1 - int(kind[1]):
+                    return nm1
+                if off == 0:
+                    return base
+                v = "%s_%s" % (base, tag)
+                body.append(assign(v, bop("CAdd", base, off)))
+                return v
+
+            for (dr, dc), res in WIDE_ACCESSES:
+                tag = "%s%d%s%d" % ("m" if dr < 0 else "p", abs(dr), "m" if dc < 0 else "p", abs(dc))
+                r = fold(rk, ra, dr, "n0m1", "r" + tag)
+                c = fold(ck, ca, dc, "n1m1", "c" + tag)
+                body.append(assign("idx_" + res, C("RTuple", [atomize(r), atomize(c)])))
+                body.append(assign("flat_" + res, C("RNdToFlat", atomize("idx_" + res), atomize("shp"))))
+                body.append(assign(res, load("u", "flat_" + res)))
+            body += wide_body("val")
+            body.append(assign("idx_o", C("RTuple", [atomize(ra), atomize(ca)])))
+            body.append(assign("flat_o", C("RNdToFlat", atomize("idx_o"), atomize("shp"))))
+            body.append(write("out", "flat_o", "val"))
+            regions.append(loop(["i", "j"], [rbound, cbound], body, par(), role="LoopMap"))
+    small = [assign("i0", bop("CDiv", "i", "n1")), assign("itmp", bop("CMul", "i0", "n1")),
+             assign("i1", bop("CSub", "i", "itmp"))]
+    for (dr, dc), res in WIDE_ACCESSES:
+        comps = []
+        for dim, (ia, na, d) in enumerate((("i0", "n0", dr), ("i1", "n1", dc))):
+            raw, bi, lt0, nm1, gen = ("w%s_%d_%s" % (x, dim, res) for x in ("raw", "bi", "lt0", "nm1", "geN"))
+            small += [assign(raw, bop("CAdd", ia, d)), assign(bi, atom(raw)), assign(lt0, bop("CLt", raw, 0)),
+                      sif(lt0, [assign(bi, atom(0))]), assign(nm1, bop("CSub", na, 1)),
+                      assign(gen, bop("CGe", bi, na)), sif(gen, [assign(bi, atom(nm1))])]
+            comps.append(bi)
+        small += [assign("widx_" + res, C("RTuple", [atomize(comps[0]), atomize(comps[1])])),
+                  assign("wflat_" + res, C("RNdToFlat", atomize("widx_" + res), atomize("shp"))),
+                  assign(res, load("u", "wflat_" + res))]
+    small += wide_body("val") + [write("out", "i", "val")]
+    step = [
+        assign("shp", C("RArrayShape", atomize("u"))),
+        assign("out", C("RArrayAlloc", atomize("shp"))),
+        assign("n", C("RShapeSize", atomize("shp"))),
+        assign("n0", C("RProj", 0, atomize("shp"))),
+        assign("n1", C("RProj", 1, atomize("shp"))),
+        assign("n0m1", bop("CSub", "n0", 1)), assign("n1m1", bop("CSub", "n1", 1)),
+        assign("bd_r0", bop("CSub", "n0", 2)), assign("bd_r1", bop("CSub", "n0", 1)),
+        assign("bd_c0", bop("CSub", "n1", 2)), assign("bd_c1", bop("CSub", "n1", 1)),
+        assign("rows_large", bop("CGe", "n0", 4)), assign("cols_large", bop("CGe", "n1", 4)),
+        assign("large", bop("CAnd", "rows_large", "cols_large")),
+        sif("large", regions, [loop(["i"], ["n"], small, par(), role="LoopMap")]),
+        assign("u", atom("out")),
+    ]
+    lp = loop(["t"], ["steps"], step, SERIAL, role="LoopIterate")
+    return [site("hydk_wide_iterate", [("steps", T_I), ("u", arr(T_F))], lp, result=("u", arr(T_F)),
+                 extra_types=[("out", arr(T_F))])]
+
+
 PROGRAMS = {
     "map_chain": map_chain, "map_chain2": map_chain2, "dot": dot, "reduce_i64": reduce_i64,
     "jacobi_step": jacobi_step, "wrap_stencil": wrap_stencil, "mirror_1d": mirror_1d, "histogram": histogram,
     "splat_f64": splat_f64, "row_sums": row_sums, "gather": gather, "map3d": map3d,
-    "heat_iterate": heat_iterate,
+    "heat_iterate": heat_iterate, "wide_iterate": wide_iterate,
 }
 
 

@@ -228,19 +228,23 @@ def test_jacobi_group_iterate(hyd, orc, prog, arrays):
     assert np.array_equal(bits(arrays.read(a, np.float64)), bits(orc.iterate_stencil2d(u, steps, 0, 0.1)))
 
 
-def run_heat_iterate(prog, arrays, u, steps):
-    """hydk_heat_iterate(steps, u) as the printed C calls it: the result (the
-    caller's to free), whether it is the input array itself, and which form
-    ran (0: no step, 1: the loops' launchers in order, 2: temporal)."""
+def run_iterate(prog, arrays, name, u, steps):
+    """<name>(steps, u) as the printed C calls it: the result (the caller's to
+    free), whether it is the input array itself, and which form ran (0: no
+    step, 1: the loops' launchers in order, 2: temporal)."""
     import ctypes
 
     inp = arrays.host(u)
-    res = prog.hydk_heat_iterate(steps, inp)
+    res = getattr(prog, name)(steps, inp)
     same = ctypes.addressof(res.contents) == ctypes.addressof(inp.contents)
     if not same:
         arrays.made.append(res)
-    form = prog._cdll.hydk_heat_iterate__form()
+    form = getattr(prog._cdll, name + "__form")()
     return arrays.read(res, np.float64), same, form
+
+
+def run_heat_iterate(prog, arrays, u, steps):
+    return run_iterate(prog, arrays, "hydk_heat_iterate", u, steps)
 
 
 @pytest.mark.parametrize("shape", [(1, 1), (1, 9), (7, 1), (2, 2), (3, 5), (33, 65), (37, 91), (130, 70), (257, 1031)])
@@ -265,6 +269,28 @@ def test_heat_iterate_generated_config_grid(hyd, orc, prog, arrays):
     got, same, form = run_heat_iterate(prog, arrays, u, steps)
     assert not same and form == 2
     assert np.array_equal(bits(got.reshape(h, w)), bits(orc.iterate_stencil2d(u, steps, 0, c["alpha"])))
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (2, 7), (3, 3), (4, 4), (5, 9), (33, 65), (130, 70), (257, 1031)])
+@pytest.mark.parametrize("steps", [0, 1, 3, 4, 5, 9])
+def test_wide_iterate_generated(hyd, prog, arrays, shape, steps):
+    # radius 2: 4 steps per launch, 25 region loops in the temporal run from
+    # 4 x 4 up, the small-grid loop below; against numpy's restatement
+    print("wide iterate %dx%d steps=%d" % (shape + (steps,)), flush=True)
+    u = np.random.default_rng(7 * shape[0] + shape[1] + steps).uniform(0, 1, shape)
+    got, same, form = run_iterate(prog, arrays, "hydk_wide_iterate", u, steps)
+    assert same == (steps == 0)
+    assert form == (0 if steps == 0 else 2 if min(shape) >= 4 else 1)
+    assert np.array_equal(bits(got.reshape(shape)), bits(H.wide_iterate_numpy(u, steps)))
+
+
+def test_wide_iterate_generated_large(hyd, prog, arrays):
+    shape, steps = (2048, 3000), 9
+    print("wide iterate %dx%d steps=%d" % (shape + (steps,)), flush=True)
+    u = np.random.default_rng(99).uniform(0, 1, shape)
+    got, same, form = run_iterate(prog, arrays, "hydk_wide_iterate", u, steps)
+    assert form == 2
+    assert np.array_equal(bits(got.reshape(shape)), bits(H.wide_iterate_numpy(u, steps)))
 
 
 @pytest.mark.parametrize("n,sh", [(1000, 10), (1000, 0), (1000, 1000), (5, 7), (100_003, 517)])

@@ -46,6 +46,7 @@ EXPECTED_FORM = {
     "jacobi_step": "form=map2d", "wrap_stencil": "form=map1d", "mirror_1d": "form=map1d",
     "histogram": "form=scatter_i64", "splat_f64": "form=scatter_f64", "row_sums": "form=map1d",
     "gather": "form=map1d", "map3d": "form=mapnd", "group_overlap": "form=map1d", "heat_iterate": "form=iterate",
+    "wide_iterate": "form=iterate",
 }
 
 
@@ -129,6 +130,13 @@ def test_iterate_plan_and_launcher(tmp_path):
     # one temporal kernel, the region loops' own launchers for the in-order form
     assert text.count("__global__ __launch_bounds__(512) void hk_tk_hydk_heat_iterate_0(") == 1
     assert 'extern "C" int64_t hydk_heat_iterate_m9(' in text and "hipMemsetAsync" in text
+
+
+def test_wide_iterate_plan():
+    # radius 2: 4 steps per launch, the same 8-cell halo; 25 region loops in
+    # one temporal run
+    p = hydgen(os.path.join(CFG_DIR, "wide_iterate.sites"), "--plan").stdout.strip()
+    assert "loops=26 runs=25(temporal radius=2 steps=4 tile=32x48),1(in order)" in p, p
 
 
 def _iterate_with(step, args=(("steps", None), ("u", None)), result="u"):
@@ -274,6 +282,34 @@ def test_interp_heat_iterate_equals_c_oracle(orc, shape, steps):
     got = H.run_interp(H.site("heat_iterate"), env)
     assert got.shape == shape
     assert np.array_equal(got.view(np.uint64), orc.iterate_stencil2d(u, steps, 0, 0.1).view(np.uint64))
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (2, 7), (3, 3), (4, 4), (5, 9), (9, 13), (12, 5)])
+@pytest.mark.parametrize("steps", [0, 1, 3])
+def test_interp_wide_iterate_equals_numpy(shape, steps):
+    # the radius-2 iterate (25 region loops, the small-grid loop below 4 x 4)
+    # against numpy's edge-padded restatement
+    u = np.random.default_rng(31 * shape[0] + shape[1]).uniform(0, 1, shape)
+    got = H.run_interp(H.site("wide_iterate"), dict(steps=steps, u=u.copy()))
+    assert np.array_equal(got.view(np.uint64), H.wide_iterate_numpy(u, steps).view(np.uint64))
+
+
+def test_planted_wide_fold_mistake_fails():
+    # a right-edge segment folding its +2 row access to n0m1 one row too early
+    # (at offset +1) changes the result
+    u = np.random.default_rng(5).uniform(0, 1, (9, 8))
+    site = copy.deepcopy(H.site("wide_iterate"))
+    step = site.get("hsLoop").args[1]
+    regions = [st for st in step if st.name == "SIf"][0].args[1]
+    bad = regions[15]  # segment r0 x l0: row n - 2
+    body = bad.args[1]
+    for k, st in enumerate(body):
+        if st.name == "SAssign" and st.args[0] == "idx_b_s1":
+            body[k] = C("SAssign", "idx_b_s1", C("RTuple", [C("AVar", "n0m1"), st.args[1].args[0][1]]))
+            body.insert(k, C("SAssign", "n0m1", C("RBinOp", C("CSub"), C("AVar", "n0"), C("AInt", 2))))
+            break
+    got = H.run_interp(site, dict(steps=1, u=u.copy()))
+    assert not np.array_equal(got, H.wide_iterate_numpy(u, 1))
 
 
 def test_planted_clamp_mistake_fails(orc):
