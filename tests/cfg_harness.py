@@ -111,6 +111,19 @@ def wide_iterate_numpy(u, steps):
     return u
 
 
+def interior_iterate_numpy(u, steps):
+    """cfg_programs.interior_iterate: each step a zero array whose interior
+    is the heat step of the previous state (the body's operations in order)."""
+    for _ in range(steps):
+        out = np.zeros_like(u)
+        if u.shape[0] > 2 and u.shape[1] > 2:
+            c = u[1:-1, 1:-1]
+            t3 = ((u[:-2, 1:-1] + u[2:, 1:-1]) + u[1:-1, :-2]) + u[1:-1, 2:]
+            out[1:-1, 1:-1] = c + 0.1 * (t3 - 4.0 * c)
+        u = out
+    return u
+
+
 def mirror_numpy(v):
     p = np.pad(v, (1, 2), mode="reflect")  # mirror without repeating the edge
     left, centre, right = p[0:-3], p[1:-2], p[3:]

@@ -36,6 +36,8 @@ Programs (loop shapes from src/Language/Hydrangea/Lowering.hs):
                jacobi_step's region loops, a small-grid branch, u = out)
   wide_iterate iterate of a radius-2 clamp stencil (4th-order heat on a
                9-point cross): 25 region loops, a small-grid branch
+  interior_iterate  iterate of a step that writes only the interior of its
+               fresh (zeroed) array
 """
 import os
 import sys
@@ -581,11 +583,48 @@ def wide_iterate():
                  extra_types=[("out", arr(T_F))])]
 
 
+def interior_iterate():
+    """iterate steps (fn u => a heat step that writes only the interior of a
+    fresh array) u0: the step's one loop writes out[(i + 1, j + 1)] for the
+    (n0 - 2) x (n1 - 2) interior, so each step's boundary keeps the zero the
+    new array is allocated with (calloc, runtime/hydrangea_runtime.h:292-310).
+    Its loop qualifies for the temporal form when hydgen plans it, but its box
+    does not tile the grid: the launcher must find that at run time and run
+    the steps in order, each into a zeroed array."""
+    body = [assign("i_r", bop("CAdd", "i", 1)), assign("j_c", bop("CAdd", "j", 1))]
+    for (dr, dc), res in HEAT_ACCESSES:
+        r = "i_r" if dr == 0 else "ir_%s" % res
+        c = "j_c" if dc == 0 else "jc_%s" % res
+        if dr:
+            body.append(assign(r, bop("CAdd", "i_r", dr)))
+        if dc:
+            body.append(assign(c, bop("CAdd", "j_c", dc)))
+        body += [assign("idx_" + res, C("RTuple", [atomize(r), atomize(c)])),
+                 assign("flat_" + res, C("RNdToFlat", atomize("idx_" + res), atomize("shp"))),
+                 assign(res, load("u", "flat_" + res))]
+    body += heat_body("val")
+    body += [assign("idx_o", C("RTuple", [atomize("i_r"), atomize("j_c")])),
+             assign("flat_o", C("RNdToFlat", atomize("idx_o"), atomize("shp"))), write("out", "flat_o", "val")]
+    inner = loop(["i", "j"], ["rows", "cols"], body, par(), role="LoopMap")
+    step = [
+        assign("shp", C("RArrayShape", atomize("u"))),
+        assign("out", C("RArrayAlloc", atomize("shp"))),
+        assign("n0", C("RProj", 0, atomize("shp"))),
+        assign("n1", C("RProj", 1, atomize("shp"))),
+        assign("rows", bop("CSub", "n0", 2)), assign("cols", bop("CSub", "n1", 2)),
+        inner,
+        assign("u", atom("out")),
+    ]
+    lp = loop(["t"], ["steps"], step, SERIAL, role="LoopIterate")
+    return [site("hydk_interior_iterate", [("steps", T_I), ("u", arr(T_F))], lp, result=("u", arr(T_F)),
+                 extra_types=[("out", arr(T_F))])]
+
+
 PROGRAMS = {
     "map_chain": map_chain, "map_chain2": map_chain2, "dot": dot, "reduce_i64": reduce_i64,
     "jacobi_step": jacobi_step, "wrap_stencil": wrap_stencil, "mirror_1d": mirror_1d, "histogram": histogram,
     "splat_f64": splat_f64, "row_sums": row_sums, "gather": gather, "map3d": map3d,
-    "heat_iterate": heat_iterate, "wide_iterate": wide_iterate,
+    "heat_iterate": heat_iterate, "wide_iterate": wide_iterate, "interior_iterate": interior_iterate,
 }
 
 

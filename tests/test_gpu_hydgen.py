@@ -293,6 +293,17 @@ def test_wide_iterate_generated_large(hyd, prog, arrays):
     assert np.array_equal(bits(got.reshape(shape)), bits(H.wide_iterate_numpy(u, steps)))
 
 
+@pytest.mark.parametrize("shape,steps", [((1, 1), 2), ((3, 3), 3), ((37, 91), 9), ((513, 1031), 17), ((2048, 2048), 5)])
+def test_interior_iterate_falls_back_in_order(hyd, prog, arrays, shape, steps):
+    # the loop's box leaves the boundary unwritten: the launcher's tiling check
+    # rejects the temporal form, and each step runs into a zeroed array
+    print("interior iterate %dx%d steps=%d" % (shape + (steps,)), flush=True)
+    u = np.random.default_rng(shape[0] + steps).uniform(0, 1, shape)
+    got, same, form = run_iterate(prog, arrays, "hydk_interior_iterate", u, steps)
+    assert not same and form == 1
+    assert np.array_equal(bits(got.reshape(shape)), bits(H.interior_iterate_numpy(u, steps)))
+
+
 @pytest.mark.parametrize("n,sh", [(1000, 10), (1000, 0), (1000, 1000), (5, 7), (100_003, 517)])
 def test_group_in_order_when_dependent(hyd, prog, arrays, n, sh):
     # out = xs + 1; out[sh:] = 2 xs[sh:] (overlapping writes); ys = out + xs

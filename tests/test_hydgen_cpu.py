@@ -46,7 +46,7 @@ EXPECTED_FORM = {
     "jacobi_step": "form=map2d", "wrap_stencil": "form=map1d", "mirror_1d": "form=map1d",
     "histogram": "form=scatter_i64", "splat_f64": "form=scatter_f64", "row_sums": "form=map1d",
     "gather": "form=map1d", "map3d": "form=mapnd", "group_overlap": "form=map1d", "heat_iterate": "form=iterate",
-    "wide_iterate": "form=iterate",
+    "wide_iterate": "form=iterate", "interior_iterate": "form=iterate",
 }
 
 
@@ -292,6 +292,21 @@ def test_interp_wide_iterate_equals_numpy(shape, steps):
     u = np.random.default_rng(31 * shape[0] + shape[1]).uniform(0, 1, shape)
     got = H.run_interp(H.site("wide_iterate"), dict(steps=steps, u=u.copy()))
     assert np.array_equal(got.view(np.uint64), H.wide_iterate_numpy(u, steps).view(np.uint64))
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (2, 5), (3, 3), (6, 9)])
+@pytest.mark.parametrize("steps", [0, 1, 3])
+def test_interp_interior_iterate_equals_numpy(shape, steps):
+    u = np.random.default_rng(3 * shape[0] + shape[1]).uniform(0, 1, shape)
+    got = H.run_interp(H.site("interior_iterate"), dict(steps=steps, u=u.copy()))
+    assert np.array_equal(got.view(np.uint64), H.interior_iterate_numpy(u, steps).view(np.uint64))
+
+
+def test_interior_iterate_plan():
+    # statically its one loop qualifies (the launcher rejects it at run time:
+    # its box does not tile the grid)
+    p = hydgen(os.path.join(CFG_DIR, "interior_iterate.sites"), "--plan").stdout.strip()
+    assert "loops=1 runs=1(temporal radius=1 steps=8 tile=32x48)" in p, p
 
 
 def test_planted_wide_fold_mistake_fails():
