@@ -124,6 +124,21 @@ def interior_iterate_numpy(u, steps):
     return u
 
 
+def int_iterate_numpy(u, steps):
+    """cfg_programs.int_iterate for non-negative int64 grids (numpy's % then
+    equals C's): clamp neighbours by edge padding; grids below 2 x 2 become
+    zero after a step."""
+    for _ in range(steps):
+        if u.shape[0] < 2 or u.shape[1] < 2:
+            u = np.zeros_like(u)
+            continue
+        p = np.pad(u, 1, mode="edge")
+        h, w = u.shape
+        k3 = ((p[0:h, 1:w + 1] + p[2:h + 2, 1:w + 1]) + p[1:h + 1, 0:w]) + p[1:h + 1, 2:w + 2]
+        u = (k3 + 3 * u) % 1000003
+    return u
+
+
 def mirror_numpy(v):
     p = np.pad(v, (1, 2), mode="reflect")  # mirror without repeating the edge
     left, centre, right = p[0:-3], p[1:-2], p[3:]

@@ -38,6 +38,7 @@ Programs (loop shapes from src/Language/Hydrangea/Lowering.hs):
                9-point cross): 25 region loops, a small-grid branch
   interior_iterate  iterate of a step that writes only the interior of its
                fresh (zeroed) array
+  int_iterate  iterate of an int64 clamp stencil (wrapping sums, C %)
 """
 import os
 import sys
@@ -264,7 +265,16 @@ def heat_body(res):
     ]
 
 
-def jacobi_step():
+def int_body(res):
+    """(a(-1)0 + a 1 0 + a 0(-1) + a 0 1 + 3 * a 0 0) % 1000003 on int64"""
+    return [
+        assign("k1", bop("CAdd", "a_n", "a_s")), assign("k2", bop("CAdd", "k1", "a_w")),
+        assign("k3", bop("CAdd", "k2", "a_e")), assign("k4", bop("CMul", 3, "a_c")),
+        assign("k5", bop("CAdd", "k3", "k4")), assign(res, bop("CMod", "k5", 1000003)),
+    ]
+
+
+def jacobi_step(body_fn=None):
     """lowerSplitStencil2D's 'large' case (n0, n1 >= 2) of the clamp heat step:
     rows and columns each split into interior (offset 1, n - 2 iterations),
     left (offset 0, one iteration) and right (offset bd_start = n - 1, one
@@ -313,7 +323,7 @@ def jacobi_step():
                 body.append(assign("idx_" + res, C("RTuple", [atomize(r), atomize(c)])))
                 body.append(assign("flat_" + res, C("RNdToFlat", atomize("idx_" + res), atomize("shp"))))
                 body.append(assign(res, load("u", "flat_" + res)))
-            body += heat_body("val")
+            body += (body_fn or heat_body)("val")
             body.append(assign("idx_o", C("RTuple", [atomize(ra), atomize(ca)])))
             body.append(assign("flat_o", C("RNdToFlat", atomize("idx_o"), atomize("shp"))))
             body.append(write("out", "flat_o", "val"))
@@ -430,6 +440,30 @@ def map3d():
     lp = loop(["a", "b", "c"], ["n0", "n1", "n2"], body, par(), role="LoopMap")
     return [site("hydk_map3d_0", [("n0", T_I), ("n1", T_I), ("n2", T_I), ("out", arr(T_F)), ("shp", T_T),
                                   ("src", arr(T_F))], lp)]
+
+
+def int_iterate():
+    """iterate steps (fn u => stencil clamp f u) u0 on an int64 grid, f = the
+    5-point sum with the centre tripled, C % 1000003: the split step's 9 region
+    loops (jacobi_step's shape with this body) under the large-grid branch; no
+    small-grid loop (grids below 2 x 2 run no loop and keep the fresh array's
+    zeros)."""
+    regions = [s.get("hsLoop") for s in jacobi_step(int_body)]
+    step = [
+        assign("shp", C("RArrayShape", atomize("u"))),
+        assign("out", C("RArrayAlloc", atomize("shp"))),
+        assign("n0", C("RProj", 0, atomize("shp"))),
+        assign("n1", C("RProj", 1, atomize("shp"))),
+        assign("n0m1", bop("CSub", "n0", 1)), assign("n1m1", bop("CSub", "n1", 1)),
+        assign("bd_start_r", bop("CSub", "n0", 1)), assign("bd_start_c", bop("CSub", "n1", 1)),
+        assign("rows_large", bop("CGe", "n0", 2)), assign("cols_large", bop("CGe", "n1", 2)),
+        assign("large", bop("CAnd", "rows_large", "cols_large")),
+        sif("large", regions, []),
+        assign("u", atom("out")),
+    ]
+    lp = loop(["t"], ["steps"], step, SERIAL, role="LoopIterate")
+    return [site("hydk_int_iterate", [("steps", T_I), ("u", arr(T_I))], lp, result=("u", arr(T_I)),
+                 extra_types=[("out", arr(T_I))])]
 
 
 def heat_iterate():
@@ -625,6 +659,7 @@ PROGRAMS = {
     "jacobi_step": jacobi_step, "wrap_stencil": wrap_stencil, "mirror_1d": mirror_1d, "histogram": histogram,
     "splat_f64": splat_f64, "row_sums": row_sums, "gather": gather, "map3d": map3d,
     "heat_iterate": heat_iterate, "wide_iterate": wide_iterate, "interior_iterate": interior_iterate,
+    "int_iterate": int_iterate,
 }
 
 

@@ -240,7 +240,7 @@ def run_iterate(prog, arrays, name, u, steps):
     if not same:
         arrays.made.append(res)
     form = getattr(prog._cdll, name + "__form")()
-    return arrays.read(res, np.float64), same, form
+    return arrays.read(res, u.dtype), same, form
 
 
 def run_heat_iterate(prog, arrays, u, steps):
@@ -291,6 +291,17 @@ def test_wide_iterate_generated_large(hyd, prog, arrays):
     got, same, form = run_iterate(prog, arrays, "hydk_wide_iterate", u, steps)
     assert form == 2
     assert np.array_equal(bits(got.reshape(shape)), bits(H.wide_iterate_numpy(u, steps)))
+
+
+@pytest.mark.parametrize("shape,steps", [((1, 1), 2), ((1, 9), 1), ((2, 2), 9), ((37, 91), 9), ((513, 1031), 17),
+                                         ((2048, 2000), 10)])
+def test_int_iterate_generated(hyd, prog, arrays, shape, steps):
+    # int64 state through the temporal form (from 2 x 2 up): wrapping sums and C %
+    print("int iterate %dx%d steps=%d" % (shape + (steps,)), flush=True)
+    u = np.random.default_rng(shape[1] + steps).integers(0, 1 << 40, shape, dtype=np.int64)
+    got, same, form = run_iterate(prog, arrays, "hydk_int_iterate", u, steps)
+    assert form == (2 if min(shape) >= 2 else 1)
+    assert np.array_equal(got.reshape(shape), H.int_iterate_numpy(u, steps))
 
 
 @pytest.mark.parametrize("shape,steps", [((1, 1), 2), ((3, 3), 3), ((37, 91), 9), ((513, 1031), 17), ((2048, 2048), 5)])

@@ -47,6 +47,7 @@ EXPECTED_FORM = {
     "histogram": "form=scatter_i64", "splat_f64": "form=scatter_f64", "row_sums": "form=map1d",
     "gather": "form=map1d", "map3d": "form=mapnd", "group_overlap": "form=map1d", "heat_iterate": "form=iterate",
     "wide_iterate": "form=iterate", "interior_iterate": "form=iterate",
+    "int_iterate": "form=iterate",
 }
 
 
@@ -300,6 +301,14 @@ def test_interp_interior_iterate_equals_numpy(shape, steps):
     u = np.random.default_rng(3 * shape[0] + shape[1]).uniform(0, 1, shape)
     got = H.run_interp(H.site("interior_iterate"), dict(steps=steps, u=u.copy()))
     assert np.array_equal(got.view(np.uint64), H.interior_iterate_numpy(u, steps).view(np.uint64))
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (1, 5), (2, 2), (3, 7), (9, 4)])
+@pytest.mark.parametrize("steps", [0, 1, 3])
+def test_interp_int_iterate_equals_numpy(shape, steps):
+    u = np.random.default_rng(5 * shape[0] + shape[1]).integers(0, 1 << 40, shape, dtype=np.int64)
+    got = H.run_interp(H.site("int_iterate"), dict(steps=steps, u=u.copy()))
+    assert got.dtype == np.int64 and np.array_equal(got, H.int_iterate_numpy(u, steps))
 
 
 def test_interior_iterate_plan():
