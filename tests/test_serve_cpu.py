@@ -128,7 +128,9 @@ def test_injected_failure_fails_one_job_only(fake):
 
 def test_continuous_batching_groups_waiting_requests(fake):
     s = fake(1 << 30, workers=1)
-    first = s.submit(stencil(1024, 100))  # occupies the only worker for a while
+    # occupies the only worker for ~27 ms of modelled device time (16 B per
+    # cell and step at 4 TB/s), far longer than submitting the 30 below
+    first = s.submit(stencil(4096, 400))
     ids = [s.submit(hist(1 << 14)) for _ in range(20)] + [s.submit(dot(1 << 14)) for _ in range(10)]
     assert s.wait_all(30_000)
     res = [s.wait(r) for r in ids]
